@@ -1,0 +1,147 @@
+/*
+ * tailor.h -- C ABI of the MI355X BiGlobal assemble-and-solve path.
+ *
+ * One context owns one GPU's device memory and one HIP stream; every call is
+ * stream-ordered on that stream.  A context is not reentrant: use one per
+ * host thread.  Host buffers are always owned by the caller; device buffers
+ * are owned by the context.  Every entry point returns a tailor_status
+ * (0 = success); tailor_last_error() returns the message of the last
+ * failure, which carries the reference's error text where the reference has
+ * one.
+ *
+ * The entry points replace the reference's stage interfaces, which are C++
+ * classes over PETSc objects (Stage(MPI_Comm)::run(const Recipe&,
+ * ProblemData&, Diagnostics&) -> PetscErrorCode):
+ *
+ *   tailor_load_case     Prepare::initialize      Prepare.hpp:20-47,
+ *                                                 Prepare.cpp:670-687
+ *   tailor_assemble      Metrics::compute         Metrics.hpp:59-87
+ *                        BaseFlowDerivatives::compute
+ *                                                 BaseFlowDerivatives.hpp:86-111
+ *                        GlobalOperator::assemble GlobalOperator.hpp:41-71
+ *                        BoundaryConditions::apply
+ *                                                 BoundaryConditions.hpp:46-65
+ *   tailor_export_*      MatGetValues / VecGetArray reads of the PETSc
+ *                        objects held in ProblemData (ProblemData.hpp:104-159)
+ */
+#ifndef TAILOR_H
+#define TAILOR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tailor_ctx tailor_ctx;
+
+/* Status codes mirror the PETSc error classes the reference raises. */
+typedef enum {
+  TAILOR_OK = 0,
+  TAILOR_ERR_FILE_UNEXPECTED = 1, /* PETSC_ERR_FILE_UNEXPECTED: bad rule data */
+  TAILOR_ERR_FP = 2,              /* PETSC_ERR_FP: non-finite values          */
+  TAILOR_ERR_ARG_OUTOFRANGE = 3,  /* PETSC_ERR_ARG_OUTOFRANGE                 */
+  TAILOR_ERR_SUP = 4,             /* PETSC_ERR_SUP: complex grid/base flow    */
+  TAILOR_ERR_PLIB = 5,            /* PETSC_ERR_PLIB: internal inconsistency   */
+  TAILOR_ERR_NOT_CONVERGED = 6,   /* PETSC_ERR_NOT_CONVERGED                  */
+  TAILOR_ERR_USER_INPUT = 7,      /* PETSC_ERR_USER_INPUT                     */
+  TAILOR_ERR_ARG_WRONGSTATE = 8,  /* PETSC_ERR_ARG_WRONGSTATE                 */
+  TAILOR_ERR_MEM = 9,             /* device allocation failed                 */
+  TAILOR_ERR_HIP = 10             /* HIP runtime error                        */
+} tailor_status;
+
+/* One-dimensional FD-q rule, row-major over (node, slot)
+ * (FDQRuleData, ProblemData.hpp:26-96). */
+typedef struct {
+  int32_t node_count;
+  int32_t q;                        /* stencil size is q + 1 */
+  int32_t periodic;                 /* 0 bounded, 1 periodic half-open */
+  double period;                    /* computational period (2 for [-1,1)) */
+  const double *nodes;              /* node_count */
+  const int64_t *stencil_indices;   /* node_count * (q+1) */
+  const int64_t *stencil_offsets;   /* node_count * (q+1) */
+  const double *weights;            /* 3 * node_count * (q+1): d0, d1, d2 */
+} tailor_rule;
+
+/* Everything the reference reads from config.yaml and the FD-q cache
+ * (Recipe.hpp:17-61, Prepare.cpp:443-668).  Complex arrays are interleaved
+ * (re, im) doubles in natural k_j_dof order: node = j*ny + i with i the xi
+ * (wall-normal) index and j the eta (spanwise) index. */
+typedef struct {
+  int32_t ny, nz;
+  tailor_rule xi, eta;
+  const double *grid;       /* ny*nz*3 complex: x, y, z */
+  const double *baseflow;   /* ny*nz*5 complex: rho, u, v, w, T */
+  double spanwise_period;
+  double reynolds, mach, prandtl, gamma;
+  double gas_constant, reference_mu;   /* validated, not used by the physics */
+  double reference_temperature, sutherland_constant;
+  double alpha_re, alpha_im;
+} tailor_case;
+
+/* Diagnostics printed by the reference (Process.cpp:62-152). */
+typedef struct {
+  double min_jacobian, max_jacobian, max_grid_imag;
+  double metric_norms[10];          /* xi_y xi_z eta_y eta_z xi_yy xi_zz xi_yz eta_yy eta_zz eta_yz */
+  double max_baseflow_imag;
+  double derivative_norms[25];      /* derivative-major [dy,dz,dyy,dzz,dyz] x [rho,u,v,w,T] */
+  int64_t rows, block_size, mass_blocks, spatial_blocks;
+  double mass_fro, spatial_fro;     /* ||M_Gamma||F, ||L||F */
+  int32_t wall_nodes, farfield_nodes, incoming_modes, neutral_modes;
+  int32_t min_incoming_per_node, max_incoming_per_node;
+  double max_characteristic_imag, max_eigenvector_condition;
+  double a_fro, b_fro;              /* ||A_bc||F, ||B_bc||F */
+} tailor_assembly_diag;
+
+/* Which operator an export or a re-assembly refers to. */
+enum { TAILOR_OP_L = 0, TAILOR_OP_M = 1, TAILOR_OP_A = 2, TAILOR_OP_B = 3 };
+
+/* Create a context on HIP device `device`. */
+int tailor_create(tailor_ctx **ctx, int device);
+void tailor_destroy(tailor_ctx *ctx);
+const char *tailor_last_error(const tailor_ctx *ctx);
+
+/* Validate rules (Prepare.cpp:328-437) and physics (Parser.cpp:110-145),
+ * copy the case to the device and build the block-CSR structure:
+ * 77 = (q_y+1)(q_z+1) block columns per block row, sorted ascending. */
+int tailor_load_case(tailor_ctx *ctx, const tailor_case *c);
+
+/* Change the streamwise wavenumber of the loaded case (alpha sweeps). */
+int tailor_set_alpha(tailor_ctx *ctx, double alpha_re, double alpha_im);
+
+/* Metrics -> base-flow derivatives -> A_bc = -L and B_bc = M with wall and
+ * far-field rows replaced, all on the device.  diag may be NULL (then no
+ * norms are computed and nothing but a status word is read back). */
+int tailor_assemble(tailor_ctx *ctx, tailor_assembly_diag *diag);
+
+/* Overwrite the device value buffers with L and M_Gamma (no boundary rows,
+ * no sign change), the operators GlobalOperator::assemble leaves in
+ * ProblemData.  Structure is the same as A_bc's. */
+int tailor_assemble_unconstrained(tailor_ctx *ctx);
+
+/* Block-CSR sizes: block rows (= nodes) and stored blocks of A_bc / L. */
+int tailor_bsr_shape(const tailor_ctx *ctx, int64_t *block_rows, int64_t *nnzb);
+
+/* Copy an operator to caller buffers.  For TAILOR_OP_L / TAILOR_OP_A:
+ * rowptr (block_rows+1), colidx (nnzb), vals (nnzb*25 complex, 5x5
+ * row-major blocks).  For TAILOR_OP_M / TAILOR_OP_B (block diagonal):
+ * rowptr[i] = i, colidx[i] = i, vals (block_rows*25 complex).  Any pointer
+ * may be NULL to skip it.  The value buffer holds whichever of L/A_bc
+ * (resp. M/B_bc) was assembled last; asking for the other is an error. */
+int tailor_export_bsr(tailor_ctx *ctx, int which, int32_t *rowptr, int32_t *colidx,
+                      double *vals);
+
+/* Copy the metric (N*10) and base-flow derivative (N*25) fields. */
+int tailor_export_fields(tailor_ctx *ctx, double *metrics, double *derivatives);
+
+/* Time `reps` launches of the value-writing assembly kernel on the
+ * context's stream with HIP events; reports the mean per launch.  Also
+ * reports the mean time of the full assembly (all kernels). */
+int tailor_time_assembly(tailor_ctx *ctx, int reps, double *write_kernel_ms,
+                         double *full_assembly_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TAILOR_H */

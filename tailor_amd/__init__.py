@@ -1,0 +1,262 @@
+"""tailor_amd -- MI355X-native BiGlobal assemble-and-solve path of Tailor.
+
+The package is a thin Python host over ``libtailor.so`` (HIP kernels for
+gfx950 plus C++20 host code behind the C ABI declared in include/tailor.h).
+Python only builds inputs, marshals them across the C ABI and formats the
+diagnostics; every arithmetic step of the hot path runs in the library.
+
+The stage objects mirror the reference's stage classes
+(Metrics / BaseFlowDerivatives / GlobalOperator / BoundaryConditions /
+EigenSolver, src/CPlusPlus/*.hpp): ``Context.assemble`` is Metrics::compute
++ BaseFlowDerivatives::compute + GlobalOperator::assemble +
+BoundaryConditions::apply (Process.cpp:53-56).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from .case import Case, Physics, synthetic_case, uniform_case  # noqa: F401
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libtailor.so"
+
+OP_L, OP_M, OP_A, OP_B = 0, 1, 2, 3
+
+STATUS_NAMES = {
+    0: "OK", 1: "FILE_UNEXPECTED", 2: "FP", 3: "ARG_OUTOFRANGE", 4: "SUP", 5: "PLIB",
+    6: "NOT_CONVERGED", 7: "USER_INPUT", 8: "ARG_WRONGSTATE", 9: "MEM", 10: "HIP",
+}
+
+
+class TailorError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[{STATUS_NAMES.get(status, status)}] {message}")
+        self.status = status
+        self.message = message
+
+
+class _Rule(C.Structure):
+    _fields_ = [("node_count", C.c_int32), ("q", C.c_int32), ("periodic", C.c_int32),
+                ("period", C.c_double), ("nodes", C.POINTER(C.c_double)),
+                ("stencil_indices", C.POINTER(C.c_int64)),
+                ("stencil_offsets", C.POINTER(C.c_int64)), ("weights", C.POINTER(C.c_double))]
+
+
+class _Case(C.Structure):
+    _fields_ = [("ny", C.c_int32), ("nz", C.c_int32), ("xi", _Rule), ("eta", _Rule),
+                ("grid", C.POINTER(C.c_double)), ("baseflow", C.POINTER(C.c_double)),
+                ("spanwise_period", C.c_double), ("reynolds", C.c_double), ("mach", C.c_double),
+                ("prandtl", C.c_double), ("gamma", C.c_double), ("gas_constant", C.c_double),
+                ("reference_mu", C.c_double), ("reference_temperature", C.c_double),
+                ("sutherland_constant", C.c_double), ("alpha_re", C.c_double),
+                ("alpha_im", C.c_double)]
+
+
+class AssemblyDiag(C.Structure):
+    _fields_ = [("min_jacobian", C.c_double), ("max_jacobian", C.c_double),
+                ("max_grid_imag", C.c_double), ("metric_norms", C.c_double * 10),
+                ("max_baseflow_imag", C.c_double), ("derivative_norms", C.c_double * 25),
+                ("rows", C.c_int64), ("block_size", C.c_int64), ("mass_blocks", C.c_int64),
+                ("spatial_blocks", C.c_int64), ("mass_fro", C.c_double),
+                ("spatial_fro", C.c_double), ("wall_nodes", C.c_int32),
+                ("farfield_nodes", C.c_int32), ("incoming_modes", C.c_int32),
+                ("neutral_modes", C.c_int32), ("min_incoming_per_node", C.c_int32),
+                ("max_incoming_per_node", C.c_int32), ("max_characteristic_imag", C.c_double),
+                ("max_eigenvector_condition", C.c_double), ("a_fro", C.c_double),
+                ("b_fro", C.c_double)]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for name, _t in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
+_lib = None
+
+
+def library() -> C.CDLL:
+    """Load libtailor.so; raises if the HIP library has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m tailor_amd.build` "
+                           "(the HIP library is required; there is no CPU fallback)")
+    lib = C.CDLL(str(LIB_PATH))
+    P = C.POINTER
+    lib.tailor_create.argtypes = [P(C.c_void_p), C.c_int]
+    lib.tailor_destroy.argtypes = [C.c_void_p]
+    lib.tailor_destroy.restype = None
+    lib.tailor_last_error.argtypes = [C.c_void_p]
+    lib.tailor_last_error.restype = C.c_char_p
+    lib.tailor_load_case.argtypes = [C.c_void_p, P(_Case)]
+    lib.tailor_set_alpha.argtypes = [C.c_void_p, C.c_double, C.c_double]
+    lib.tailor_assemble.argtypes = [C.c_void_p, P(AssemblyDiag)]
+    lib.tailor_assemble_unconstrained.argtypes = [C.c_void_p]
+    lib.tailor_bsr_shape.argtypes = [C.c_void_p, P(C.c_int64), P(C.c_int64)]
+    lib.tailor_export_bsr.argtypes = [C.c_void_p, C.c_int, P(C.c_int32), P(C.c_int32),
+                                      P(C.c_double)]
+    lib.tailor_export_fields.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
+    lib.tailor_time_assembly.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double)]
+    _lib = lib
+    return lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _rule_struct(rule, keep: list) -> _Rule:
+    nodes = np.ascontiguousarray(rule.nodes, dtype=np.float64)
+    idx = np.ascontiguousarray(rule.stencil_indices, dtype=np.int64)
+    off = np.ascontiguousarray(rule.stencil_offsets, dtype=np.int64)
+    w = np.ascontiguousarray(rule.weights, dtype=np.float64)
+    keep += [nodes, idx, off, w]
+    return _Rule(rule.node_count, rule.q, 1 if rule.topology == "periodic" else 0,
+                 float(rule.period), _dptr(nodes), idx.ctypes.data_as(C.POINTER(C.c_int64)),
+                 off.ctypes.data_as(C.POINTER(C.c_int64)), _dptr(w))
+
+
+class Context:
+    """One GPU context: device buffers of one case and its operators."""
+
+    def __init__(self, device: int = 0):
+        self.lib = library()
+        self.ptr = C.c_void_p()
+        st = self.lib.tailor_create(C.byref(self.ptr), device)
+        if st:
+            raise TailorError(st, "tailor_create failed")
+        self.case: Case | None = None
+
+    def close(self):
+        if self.ptr:
+            self.lib.tailor_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st: int):
+        if st:
+            raise TailorError(st, self.lib.tailor_last_error(self.ptr).decode())
+
+    # -- case --------------------------------------------------------------
+    def load_case(self, case: Case) -> None:
+        keep: list = []
+        grid = np.ascontiguousarray(case.grid, dtype=np.complex128).view(np.float64)
+        base = np.ascontiguousarray(case.baseflow, dtype=np.complex128).view(np.float64)
+        ph = case.physics
+        cs = _Case(case.ny, case.nz, _rule_struct(case.xi_rule, keep),
+                   _rule_struct(case.eta_rule, keep), _dptr(grid), _dptr(base),
+                   float(case.spanwise_period), ph.reynolds, ph.mach, ph.prandtl, ph.gamma,
+                   ph.gas_constant, ph.reference_mu, ph.reference_temperature,
+                   ph.sutherland_constant, case.alpha.real, case.alpha.imag)
+        self._check(self.lib.tailor_load_case(self.ptr, C.byref(cs)))
+        self.case = case
+
+    def set_alpha(self, alpha: complex) -> None:
+        alpha = complex(alpha)
+        self._check(self.lib.tailor_set_alpha(self.ptr, alpha.real, alpha.imag))
+
+    # -- assembly ----------------------------------------------------------
+    def assemble(self, diagnostics: bool = True) -> dict | None:
+        if diagnostics:
+            d = AssemblyDiag()
+            self._check(self.lib.tailor_assemble(self.ptr, C.byref(d)))
+            return d.as_dict()
+        self._check(self.lib.tailor_assemble(self.ptr, None))
+        return None
+
+    def assemble_unconstrained(self) -> None:
+        self._check(self.lib.tailor_assemble_unconstrained(self.ptr))
+
+    def shape(self) -> tuple[int, int]:
+        rows, nnzb = C.c_int64(), C.c_int64()
+        self._check(self.lib.tailor_bsr_shape(self.ptr, C.byref(rows), C.byref(nnzb)))
+        return rows.value, nnzb.value
+
+    def export_bsr(self, which: int):
+        rows, nnzb = self.shape()
+        if which in (OP_L, OP_A):
+            rowptr = np.zeros(rows + 1, dtype=np.int32)
+            colidx = np.zeros(nnzb, dtype=np.int32)
+            vals = np.zeros(nnzb * 50)
+        else:
+            rowptr = np.zeros(rows + 1, dtype=np.int32)
+            colidx = np.zeros(rows, dtype=np.int32)
+            vals = np.zeros(rows * 50)
+        self._check(self.lib.tailor_export_bsr(
+            self.ptr, which, rowptr.ctypes.data_as(C.POINTER(C.c_int32)),
+            colidx.ctypes.data_as(C.POINTER(C.c_int32)), _dptr(vals)))
+        return rowptr, colidx, vals.view(np.complex128).reshape(-1, 5, 5)
+
+    def export_fields(self):
+        rows, _ = self.shape()
+        m = np.zeros((rows, 10))
+        dd = np.zeros((rows, 25))
+        self._check(self.lib.tailor_export_fields(self.ptr, _dptr(m), _dptr(dd)))
+        return m, dd
+
+    def time_assembly(self, reps: int = 10) -> tuple[float, float]:
+        w, f = C.c_double(), C.c_double()
+        self._check(self.lib.tailor_time_assembly(self.ptr, reps, C.byref(w), C.byref(f)))
+        return w.value, f.value
+
+
+def format_assembly_report(case: Case, d: dict) -> str:
+    """The stdout lines of Process.cpp:62-152 (regex-parsed by tests)."""
+    grid_norm = float(np.linalg.norm(case.grid))
+    base_norm = float(np.linalg.norm(case.baseflow))
+    names = ["xi_y", "xi_z", "eta_y", "eta_z", "xi_yy", "xi_zz", "xi_yz", "eta_yy", "eta_zz",
+             "eta_yz"]
+    dn = ["dy", "dz", "dyy", "dzz", "dyz"]
+    width = max(int(np.abs(case.xi_rule.stencil_offsets).max()),
+                int(np.abs(case.eta_rule.stencil_offsets).max()))
+    lines = [
+        f"Prepared case '{case.title}'",
+        f"  Ny x Nz: {case.ny} x {case.nz}",
+        f"  alpha: {case.alpha.real:.16g} {case.alpha.imag:+.16g}i",
+        f"  FD-q degrees: y={case.xi_rule.q}, z={case.eta_rule.q}; DMDA stencil width: {width}",
+        f"  xi rule: {case.ny} nodes, {case.xi_rule.stencil_size}-point stencil",
+        f"  eta rule: {case.nz} nodes, {case.eta_rule.stencil_size}-point stencil",
+        f"  ||baseflow||2: {base_norm:.16g}, ||grid||2: {grid_norm:.16g}",
+        f"  Jacobian range: [{d['min_jacobian']:.16g}, {d['max_jacobian']:.16g}]",
+        f"  max |Im(grid y/z)|: {d['max_grid_imag']:.16g}",
+    ]
+    for k, nm in enumerate(names):
+        lines.append(f"  metric norm {nm}: {d['metric_norms'][k]:.16g}")
+    lines.append(f"  base-flow derivative components: 25; max |Im(baseflow)|: "
+                 f"{d['max_baseflow_imag']:.16g}")
+    for k, nm in enumerate(dn):
+        vals = " ".join(f"{v:.16g}" for v in d["derivative_norms"][5 * k:5 * k + 5])
+        lines.append(f"  base-flow derivative norm {nm}: {vals}")
+    lines += [
+        f"  global matrices: {d['rows']} x {d['rows']}, block size {d['block_size']}",
+        f"  M_Gamma blocks used/allocated: {d['mass_blocks']} / {d['mass_blocks']}; "
+        f"||M_Gamma||F: {d['mass_fro']:.16g}",
+        f"  L blocks used/allocated: {d['spatial_blocks']} / {d['spatial_blocks']}; "
+        f"||L||F: {d['spatial_fro']:.16g}",
+        "  eigenvalue convention: (-L) q = lambda M_Gamma q, lambda = -i omega",
+        f"  boundaries: wall nodes={d['wall_nodes']}, far-field nodes={d['farfield_nodes']}",
+        f"  far-field incoming modes={d['incoming_modes']} (per node "
+        f"{d['min_incoming_per_node']}..{d['max_incoming_per_node']}), "
+        f"neutral modes={d['neutral_modes']}",
+        f"  max characteristic |imag(c)|: {d['max_characteristic_imag']:.16g}; "
+        f"max cond(R): {d['max_eigenvector_condition']:.16g}",
+        f"  ||A_bc||F: {d['a_fro']:.16g}; ||B_bc||F: {d['b_fro']:.16g}",
+    ]
+    return "\n".join(lines)

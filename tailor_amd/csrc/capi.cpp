@@ -1,0 +1,533 @@
+// capi.cpp -- the C ABI of libtailor: context, case loading and validation,
+// assembly driver and exports.  See include/tailor.h for the contract.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+using tailor::fail;
+
+namespace tailor {
+
+int fail(tailor_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->error = msg;
+  return code;
+}
+
+int hip_fail(tailor_ctx *ctx, hipError_t e, const char *what) {
+  return fail(ctx, e == hipErrorOutOfMemory ? TAILOR_ERR_MEM : TAILOR_ERR_HIP,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace tailor
+
+namespace {
+
+#define HIP_TRY(ctx, expr)                                     \
+  do {                                                         \
+    hipError_t e_ = (expr);                                    \
+    if (e_ != hipSuccess) return tailor::hip_fail(ctx, e_, #expr); \
+  } while (0)
+
+template <class T>
+hipError_t dalloc(T **p, size_t count) {
+  return hipMalloc(reinterpret_cast<void **>(p), std::max<size_t>(count, 1) * sizeof(T));
+}
+
+template <class T>
+void dfree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+void free_case(tailor_ctx *ctx) {
+  dfree(ctx->d_xoff);
+  dfree(ctx->d_xw);
+  dfree(ctx->d_eoff);
+  dfree(ctx->d_ew);
+  dfree(ctx->d_grid);
+  dfree(ctx->d_base);
+  dfree(ctx->d_metrics);
+  dfree(ctx->d_deriv);
+  dfree(ctx->d_rowptr);
+  dfree(ctx->d_colidx);
+  dfree(ctx->d_vals);
+  dfree(ctx->d_mvals);
+  dfree(ctx->d_bcdata);
+  ctx->loaded = false;
+  ctx->assembled = false;
+  ctx->vals_kind = ctx->mvals_kind = -1;
+}
+
+double poly_derivative(int power, int derivative, double x) {
+  if (derivative > power) return 0.0;
+  double coef = 1.0;
+  for (int k = 0; k < derivative; ++k) coef *= static_cast<double>(power - k);
+  return coef * std::pow(x, power - derivative);
+}
+
+// Rule validation of Prepare.cpp:328-437 (same checks and messages).
+int validate_rule(tailor_ctx *ctx, const char *name, const tailor_rule &r) {
+  char buf[256];
+  const int n = r.node_count, ss = r.q + 1;
+  if (!(n >= 3 && r.q >= 2 && r.q < n)) {
+    std::snprintf(buf, sizeof buf, "%s FD-q metadata must satisfy node_count > q >= 2", name);
+    return fail(ctx, TAILOR_ERR_FILE_UNEXPECTED, buf);
+  }
+  if (!std::isfinite(r.period) || r.period <= 0.0) {
+    std::snprintf(buf, sizeof buf, "%s FD-q period must be finite and positive", name);
+    return fail(ctx, TAILOR_ERR_FILE_UNEXPECTED, buf);
+  }
+  if (r.periodic && r.q % 2 != 0) {
+    std::snprintf(buf, sizeof buf, "%s periodic FD-q degree must be even", name);
+    return fail(ctx, TAILOR_ERR_FILE_UNEXPECTED, buf);
+  }
+  if (!r.nodes || !r.stencil_indices || !r.stencil_offsets || !r.weights) {
+    std::snprintf(buf, sizeof buf, "%s FD-q arrays are missing", name);
+    return fail(ctx, TAILOR_ERR_FILE_UNEXPECTED, buf);
+  }
+  const size_t plane = static_cast<size_t>(n) * ss;
+  for (int row = 0; row < n; ++row) {
+    if (!std::isfinite(r.nodes[row])) {
+      std::snprintf(buf, sizeof buf, "%s FD-q node %d is not finite", name, row);
+      return fail(ctx, TAILOR_ERR_FILE_UNEXPECTED, buf);
+    }
+    if (row > 0 && !(r.nodes[row] > r.nodes[row - 1])) {
+      std::snprintf(buf, sizeof buf, "%s FD-q nodes must be strictly increasing", name);
+      return fail(ctx, TAILOR_ERR_FILE_UNEXPECTED, buf);
+    }
+    bool contains = false;
+    for (int s = 0; s < ss; ++s) {
+      const int64_t col = r.stencil_indices[static_cast<size_t>(row) * ss + s];
+      const int64_t off = r.stencil_offsets[static_cast<size_t>(row) * ss + s];
+      if (col < 0 || col >= n) {
+        std::snprintf(buf, sizeof buf, "%s FD-q row %d contains out-of-range index %lld", name,
+                      row, static_cast<long long>(col));
+        return fail(ctx, TAILOR_ERR_FILE_UNEXPECTED, buf);
+      }
+      if (r.periodic) {
+        int64_t w = (row + off) % n;
+        if (w < 0) w += n;
+        if (w != col) {
+          std::snprintf(buf, sizeof buf, "%s periodic row %d has inconsistent stencil index/offset",
+                        name, row);
+          return fail(ctx, TAILOR_ERR_FILE_UNEXPECTED, buf);
+        }
+      } else if (row + off != col) {
+        std::snprintf(buf, sizeof buf, "%s bounded row %d has inconsistent stencil index/offset",
+                      name, row);
+        return fail(ctx, TAILOR_ERR_FILE_UNEXPECTED, buf);
+      }
+      if (col == row) contains = true;
+      for (int m = 0; m < 3; ++m) {
+        if (!std::isfinite(r.weights[m * plane + static_cast<size_t>(row) * ss + s])) {
+          std::snprintf(buf, sizeof buf, "%s FD-q weight is not finite at row %d", name, row);
+          return fail(ctx, TAILOR_ERR_FILE_UNEXPECTED, buf);
+        }
+      }
+    }
+    if (!contains) {
+      std::snprintf(buf, sizeof buf, "%s FD-q stencil row %d does not contain its evaluation node",
+                    name, row);
+      return fail(ctx, TAILOR_ERR_FILE_UNEXPECTED, buf);
+    }
+    for (int m = 0; m < 3; ++m) {
+      for (int power = 0; power <= 2; ++power) {
+        double approx = 0.0;
+        for (int s = 0; s < ss; ++s) {
+          const size_t k = static_cast<size_t>(row) * ss + s;
+          const double x = r.periodic ? r.nodes[row] + static_cast<double>(r.stencil_offsets[k]) *
+                                                           r.period / static_cast<double>(n)
+                                      : r.nodes[r.stencil_indices[k]];
+          approx += r.weights[m * plane + k] * std::pow(x, power);
+        }
+        const double expected = poly_derivative(power, m, r.nodes[row]);
+        if (std::fabs(approx - expected) > 5.0e-9 * std::max(1.0, std::fabs(expected))) {
+          std::snprintf(buf, sizeof buf,
+                        "%s FD-q order-%d weights fail degree-%d polynomial check at row %d", name,
+                        m, power, row);
+          return fail(ctx, TAILOR_ERR_FILE_UNEXPECTED, buf);
+        }
+      }
+    }
+  }
+  return TAILOR_OK;
+}
+
+int check_finite(tailor_ctx *ctx, double v, const char *name) {
+  if (!std::isfinite(v))
+    return fail(ctx, TAILOR_ERR_USER_INPUT, std::string(name) + " must be finite");
+  return TAILOR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tailor_create(tailor_ctx **out, int device) {
+  if (!out) return TAILOR_ERR_USER_INPUT;
+  *out = nullptr;
+  auto *ctx = new tailor_ctx();
+  ctx->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = dalloc(&ctx->d_diag, 1);
+  if (e == hipSuccess) e = dalloc(&ctx->d_recipe, 8);
+  if (e == hipSuccess)
+    e = dalloc(&ctx->d_partials,
+               static_cast<size_t>(tailor::kPartialBlocks) * tailor::kPartialComps);
+  if (e != hipSuccess) {
+    int code = e == hipErrorOutOfMemory ? TAILOR_ERR_MEM : TAILOR_ERR_HIP;
+    tailor_destroy(ctx);
+    return code;
+  }
+  *out = ctx;
+  return TAILOR_OK;
+}
+
+void tailor_destroy(tailor_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  free_case(ctx);
+  dfree(ctx->d_diag);
+  dfree(ctx->d_recipe);
+  dfree(ctx->d_partials);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char *tailor_last_error(const tailor_ctx *ctx) {
+  return ctx ? ctx->error.c_str() : "null context";
+}
+
+int tailor_load_case(tailor_ctx *ctx, const tailor_case *c) {
+  if (!ctx || !c) return TAILOR_ERR_USER_INPUT;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  // physics validation (Parser.cpp:118-140)
+  const char *names[] = {"Stability.Alpha.Real", "Stability.Alpha.Imag", "ReynoldsNumber",
+                         "MachNumber", "PrandtlNumber", "GasConstant", "RatioOfSpecificHeats",
+                         "ReferenceMu", "ReferenceTemperature", "SutherlandConstant"};
+  const double vals[] = {c->alpha_re, c->alpha_im, c->reynolds, c->mach, c->prandtl,
+                         c->gas_constant, c->gamma, c->reference_mu, c->reference_temperature,
+                         c->sutherland_constant};
+  for (int k = 0; k < 10; ++k)
+    if (int st = check_finite(ctx, vals[k], names[k])) return st;
+  if (c->reynolds <= 0.0 || c->mach <= 0.0 || c->prandtl <= 0.0 || c->gas_constant <= 0.0 ||
+      c->gamma <= 1.0 || c->reference_mu <= 0.0 || c->reference_temperature <= 0.0 ||
+      c->sutherland_constant < 0.0)
+    return fail(ctx, TAILOR_ERR_USER_INPUT,
+                "The physical parameters in config.yaml are outside their valid ranges");
+  if (c->ny <= 0 || c->nz <= 0)
+    return fail(ctx, TAILOR_ERR_FILE_UNEXPECTED, "HDF5 attributes Ny and Nz must be positive");
+  if (!std::isfinite(c->spanwise_period) || c->spanwise_period <= 0.0)
+    return fail(ctx, TAILOR_ERR_FILE_UNEXPECTED,
+                "FD-q spanwise_period must be finite and positive");
+  if (c->xi.periodic || !c->eta.periodic)
+    return fail(ctx, TAILOR_ERR_FILE_UNEXPECTED,
+                "xi rule must be bounded and eta rule periodic");
+  if (int st = validate_rule(ctx, "xi", c->xi)) return st;
+  if (int st = validate_rule(ctx, "eta", c->eta)) return st;
+  if (c->xi.node_count != c->ny)
+    return fail(ctx, TAILOR_ERR_FILE_UNEXPECTED, "xi rule node count does not match Ny");
+  if (c->eta.node_count != c->nz)
+    return fail(ctx, TAILOR_ERR_FILE_UNEXPECTED, "eta rule node count does not match Nz");
+  if (c->ny < 2)
+    return fail(ctx, TAILOR_ERR_ARG_OUTOFRANGE,
+                "At least two xi planes are required for wall/far-field boundaries");
+  if (!c->grid || !c->baseflow) return fail(ctx, TAILOR_ERR_USER_INPUT, "grid/baseflow missing");
+
+  free_case(ctx);
+  const int ny = c->ny, nz = c->nz, ssx = c->xi.q + 1, sse = c->eta.q + 1;
+  tailor::CaseDims &d = ctx->dims;
+  d.ny = ny;
+  d.nz = nz;
+  d.ssx = ssx;
+  d.sse = sse;
+  d.nodes = static_cast<long long>(ny) * nz;
+  d.nnzb = d.nodes * ssx * sse;
+  ctx->period = c->spanwise_period;
+  ctx->recipe[0] = c->reynolds;
+  ctx->recipe[1] = c->mach;
+  ctx->recipe[2] = c->prandtl;
+  ctx->recipe[3] = c->gamma;
+  ctx->recipe[4] = c->reference_temperature;
+  ctx->recipe[5] = c->sutherland_constant;
+  ctx->alpha_re = c->alpha_re;
+  ctx->alpha_im = c->alpha_im;
+
+  // structure: sorted block columns of the tensor-product stencil
+  std::vector<int> xoff(static_cast<size_t>(ny) * ssx), eoff(static_cast<size_t>(nz) * sse);
+  for (size_t k = 0; k < xoff.size(); ++k) xoff[k] = static_cast<int>(c->xi.stencil_offsets[k]);
+  for (size_t k = 0; k < eoff.size(); ++k) eoff[k] = static_cast<int>(c->eta.stencil_offsets[k]);
+  for (int j = 0; j < nz; ++j) {
+    std::vector<int> cols;
+    for (int t = 0; t < sse; ++t) cols.push_back(static_cast<int>(c->eta.stencil_indices[j * sse + t]));
+    std::sort(cols.begin(), cols.end());
+    if (std::adjacent_find(cols.begin(), cols.end()) != cols.end())
+      return fail(ctx, TAILOR_ERR_ARG_OUTOFRANGE,
+                  "periodic eta stencil wraps onto itself: Nz must exceed q_z");
+  }
+  ctx->h_rowptr.resize(d.nodes + 1);
+  ctx->h_colidx.resize(d.nnzb);
+  for (int j = 0; j < nz; ++j) {
+    std::vector<int> jcols;
+    for (int t = 0; t < sse; ++t) jcols.push_back(static_cast<int>(c->eta.stencil_indices[j * sse + t]));
+    std::sort(jcols.begin(), jcols.end());
+    for (int i = 0; i < ny; ++i) {
+      const long long row = static_cast<long long>(j) * ny + i;
+      ctx->h_rowptr[row] = static_cast<int>(row * ssx * sse);
+      int *dst = ctx->h_colidx.data() + row * ssx * sse;
+      for (int tr = 0; tr < sse; ++tr)
+        for (int s = 0; s < ssx; ++s)
+          dst[tr * ssx + s] = jcols[tr] * ny + static_cast<int>(c->xi.stencil_indices[i * ssx + s]);
+    }
+  }
+  ctx->h_rowptr[d.nodes] = static_cast<int>(d.nnzb);
+
+  HIP_TRY(ctx, dalloc(&ctx->d_xoff, xoff.size()));
+  HIP_TRY(ctx, dalloc(&ctx->d_eoff, eoff.size()));
+  HIP_TRY(ctx, dalloc(&ctx->d_xw, 3 * xoff.size()));
+  HIP_TRY(ctx, dalloc(&ctx->d_ew, 3 * eoff.size()));
+  HIP_TRY(ctx, dalloc(&ctx->d_grid, static_cast<size_t>(d.nodes) * 6));
+  HIP_TRY(ctx, dalloc(&ctx->d_base, static_cast<size_t>(d.nodes) * 10));
+  HIP_TRY(ctx, dalloc(&ctx->d_metrics, static_cast<size_t>(d.nodes) * 10));
+  HIP_TRY(ctx, dalloc(&ctx->d_deriv, static_cast<size_t>(d.nodes) * 25));
+  HIP_TRY(ctx, dalloc(&ctx->d_rowptr, static_cast<size_t>(d.nodes) + 1));
+  HIP_TRY(ctx, dalloc(&ctx->d_colidx, static_cast<size_t>(d.nnzb)));
+  HIP_TRY(ctx, dalloc(&ctx->d_vals, static_cast<size_t>(d.nnzb) * 50));
+  HIP_TRY(ctx, dalloc(&ctx->d_mvals, static_cast<size_t>(d.nodes) * 50));
+  HIP_TRY(ctx, dalloc(&ctx->d_bcdata, static_cast<size_t>(nz) * 106));
+  auto up = [&](void *dst, const void *src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+  };
+  HIP_TRY(ctx, up(ctx->d_xoff, xoff.data(), xoff.size() * sizeof(int)));
+  HIP_TRY(ctx, up(ctx->d_eoff, eoff.data(), eoff.size() * sizeof(int)));
+  HIP_TRY(ctx, up(ctx->d_xw, c->xi.weights, 3 * xoff.size() * sizeof(double)));
+  HIP_TRY(ctx, up(ctx->d_ew, c->eta.weights, 3 * eoff.size() * sizeof(double)));
+  HIP_TRY(ctx, up(ctx->d_grid, c->grid, static_cast<size_t>(d.nodes) * 6 * sizeof(double)));
+  HIP_TRY(ctx, up(ctx->d_base, c->baseflow, static_cast<size_t>(d.nodes) * 10 * sizeof(double)));
+  HIP_TRY(ctx, up(ctx->d_rowptr, ctx->h_rowptr.data(), ctx->h_rowptr.size() * sizeof(int)));
+  HIP_TRY(ctx, up(ctx->d_colidx, ctx->h_colidx.data(), ctx->h_colidx.size() * sizeof(int)));
+  HIP_TRY(ctx, up(ctx->d_recipe, ctx->recipe, 6 * sizeof(double)));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->loaded = true;
+  return TAILOR_OK;
+}
+
+int tailor_set_alpha(tailor_ctx *ctx, double alpha_re, double alpha_im) {
+  if (!ctx) return TAILOR_ERR_USER_INPUT;
+  if (!std::isfinite(alpha_re) || !std::isfinite(alpha_im))
+    return fail(ctx, TAILOR_ERR_USER_INPUT, "Stability.Alpha must be finite");
+  ctx->alpha_re = alpha_re;
+  ctx->alpha_im = alpha_im;
+  ctx->assembled = false;
+  return TAILOR_OK;
+}
+
+static int check_assembly_flags(tailor_ctx *ctx, const tailor::DeviceDiag &g) {
+  char buf[256];
+  using namespace tailor;
+  if (g.flags & kGridNonFinite)
+    return fail(ctx, TAILOR_ERR_FP, "Grid coordinates or computed first-order metrics are not finite");
+  if (g.flags & kJacobian) {
+    std::snprintf(buf, sizeof buf,
+                  "Grid Jacobian must be positive and nonsingular; global range is [%.16g, %.16g]",
+                  g.min_jac, g.max_jac);
+    return fail(ctx, TAILOR_ERR_ARG_OUTOFRANGE, buf);
+  }
+  if (g.max_grid_imag > 100.0 * 2.220446049250313e-16) {
+    std::snprintf(buf, sizeof buf,
+                  "Grid y/z coordinates must be real; maximum imaginary magnitude is %.16g",
+                  g.max_grid_imag);
+    return fail(ctx, TAILOR_ERR_SUP, buf);
+  }
+  if (g.flags & kSecondNonFinite)
+    return fail(ctx, TAILOR_ERR_FP, "Computed second-order metrics are not finite");
+  if (g.flags & kBaseNonFinite) return fail(ctx, TAILOR_ERR_FP, "Base-flow values are not finite");
+  if (g.flags & kBasePositive)
+    return fail(ctx, TAILOR_ERR_ARG_OUTOFRANGE,
+                "Base-flow density and temperature must both be positive");
+  if (g.flags & kDerivNonFinite)
+    return fail(ctx, TAILOR_ERR_FP, "Computed base-flow derivatives are not finite");
+  if (g.max_base_imag > 100.0 * 2.220446049250313e-16) {
+    std::snprintf(buf, sizeof buf,
+                  "Base-flow values must be real; maximum imaginary magnitude is %.16g",
+                  g.max_base_imag);
+    return fail(ctx, TAILOR_ERR_SUP, buf);
+  }
+  if (g.flags & kCoefficient)
+    return fail(ctx, TAILOR_ERR_FP,
+                "At least one node produced invalid curvilinear LNS coefficients during global "
+                "operator validation");
+  if (g.flags & kCharacteristic) {
+    std::snprintf(buf, sizeof buf, "Far-field characteristic decomposition failed (status %d)",
+                  g.char_status);
+    return fail(ctx, TAILOR_ERR_FP, buf);
+  }
+  return TAILOR_OK;
+}
+
+static int run_assembly(tailor_ctx *ctx, bool constrained, tailor::DeviceDiag *host_diag) {
+  if (!ctx) return TAILOR_ERR_USER_INPUT;
+  if (!ctx->loaded)
+    return fail(ctx, TAILOR_ERR_ARG_WRONGSTATE,
+                "Prepared field data are required before matrix assembly");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, tailor::launch_reset_diag(ctx));
+  HIP_TRY(ctx, tailor::launch_metrics(ctx));
+  HIP_TRY(ctx, tailor::launch_derivatives(ctx));
+  if (constrained) HIP_TRY(ctx, tailor::launch_characteristics(ctx));
+  HIP_TRY(ctx, tailor::launch_operator(ctx, constrained));
+  tailor::DeviceDiag g{};
+  HIP_TRY(ctx, hipMemcpyAsync(&g, ctx->d_diag, sizeof g, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (host_diag) *host_diag = g;
+  ctx->vals_kind = constrained ? TAILOR_OP_A : TAILOR_OP_L;
+  ctx->mvals_kind = constrained ? TAILOR_OP_B : TAILOR_OP_M;
+  if (int st = check_assembly_flags(ctx, g)) return st;
+  ctx->assembled = true;
+  return TAILOR_OK;
+}
+
+int tailor_assemble(tailor_ctx *ctx, tailor_assembly_diag *diag) {
+  tailor::DeviceDiag g{};
+  if (int st = run_assembly(ctx, true, &g)) return st;
+  if (!diag) return TAILOR_OK;
+  std::memset(diag, 0, sizeof *diag);
+  const auto &d = ctx->dims;
+  diag->min_jacobian = g.min_jac;
+  diag->max_jacobian = g.max_jac;
+  diag->max_grid_imag = g.max_grid_imag;
+  diag->max_baseflow_imag = g.max_base_imag;
+  double s[32];
+  HIP_TRY(ctx, tailor::launch_sumsq(ctx, ctx->d_metrics, d.nodes, 10, 10, s));
+  for (int k = 0; k < 10; ++k) diag->metric_norms[k] = std::sqrt(s[k]);
+  HIP_TRY(ctx, tailor::launch_sumsq(ctx, ctx->d_deriv, d.nodes, 25, 25, s));
+  for (int k = 0; k < 25; ++k) diag->derivative_norms[k] = std::sqrt(s[k]);
+  diag->rows = 5 * d.nodes;
+  diag->block_size = 5;
+  diag->mass_blocks = d.nodes;
+  diag->spatial_blocks = d.nnzb;
+  diag->wall_nodes = d.nz;
+  diag->farfield_nodes = d.nz;
+  diag->incoming_modes = g.incoming;
+  diag->neutral_modes = g.neutral;
+  diag->min_incoming_per_node = g.min_incoming;
+  diag->max_incoming_per_node = g.max_incoming;
+  diag->max_characteristic_imag = g.max_char_imag;
+  diag->max_eigenvector_condition = g.max_cond;
+  HIP_TRY(ctx, tailor::launch_sumsq(ctx, ctx->d_vals, d.nnzb * 50, 1, 1, s));
+  diag->a_fro = std::sqrt(s[0]);
+  HIP_TRY(ctx, tailor::launch_sumsq(ctx, ctx->d_mvals, d.nodes * 50, 1, 1, s));
+  diag->b_fro = std::sqrt(s[0]);
+  // ||L||F and ||M||F need the unconstrained operators: evaluate them in
+  // place, then restore A_bc / B_bc.
+  if (int st = run_assembly(ctx, false, nullptr)) return st;
+  HIP_TRY(ctx, tailor::launch_sumsq(ctx, ctx->d_vals, d.nnzb * 50, 1, 1, s));
+  diag->spatial_fro = std::sqrt(s[0]);
+  HIP_TRY(ctx, tailor::launch_sumsq(ctx, ctx->d_mvals, d.nodes * 50, 1, 1, s));
+  diag->mass_fro = std::sqrt(s[0]);
+  return run_assembly(ctx, true, nullptr);
+}
+
+int tailor_assemble_unconstrained(tailor_ctx *ctx) { return run_assembly(ctx, false, nullptr); }
+
+int tailor_bsr_shape(const tailor_ctx *ctx, int64_t *block_rows, int64_t *nnzb) {
+  if (!ctx || !ctx->loaded) return TAILOR_ERR_ARG_WRONGSTATE;
+  if (block_rows) *block_rows = ctx->dims.nodes;
+  if (nnzb) *nnzb = ctx->dims.nnzb;
+  return TAILOR_OK;
+}
+
+int tailor_export_bsr(tailor_ctx *ctx, int which, int32_t *rowptr, int32_t *colidx, double *vals) {
+  if (!ctx) return TAILOR_ERR_USER_INPUT;
+  if (!ctx->loaded) return fail(ctx, TAILOR_ERR_ARG_WRONGSTATE, "no case loaded");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const auto &d = ctx->dims;
+  if (which == TAILOR_OP_L || which == TAILOR_OP_A) {
+    if (vals && ctx->vals_kind != which)
+      return fail(ctx, TAILOR_ERR_ARG_WRONGSTATE, "requested operator is not the one assembled");
+    if (rowptr) std::memcpy(rowptr, ctx->h_rowptr.data(), ctx->h_rowptr.size() * sizeof(int));
+    if (colidx) std::memcpy(colidx, ctx->h_colidx.data(), ctx->h_colidx.size() * sizeof(int));
+    if (vals)
+      HIP_TRY(ctx, hipMemcpyAsync(vals, ctx->d_vals, static_cast<size_t>(d.nnzb) * 50 * sizeof(double),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+  } else if (which == TAILOR_OP_M || which == TAILOR_OP_B) {
+    if (vals && ctx->mvals_kind != which)
+      return fail(ctx, TAILOR_ERR_ARG_WRONGSTATE, "requested operator is not the one assembled");
+    if (rowptr)
+      for (long long k = 0; k <= d.nodes; ++k) rowptr[k] = static_cast<int>(k);
+    if (colidx)
+      for (long long k = 0; k < d.nodes; ++k) colidx[k] = static_cast<int>(k);
+    if (vals)
+      HIP_TRY(ctx, hipMemcpyAsync(vals, ctx->d_mvals, static_cast<size_t>(d.nodes) * 50 * sizeof(double),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+  } else {
+    return fail(ctx, TAILOR_ERR_USER_INPUT, "unknown operator");
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return TAILOR_OK;
+}
+
+int tailor_export_fields(tailor_ctx *ctx, double *metrics, double *derivatives) {
+  if (!ctx) return TAILOR_ERR_USER_INPUT;
+  if (!ctx->loaded) return fail(ctx, TAILOR_ERR_ARG_WRONGSTATE, "no case loaded");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const auto &d = ctx->dims;
+  if (metrics)
+    HIP_TRY(ctx, hipMemcpyAsync(metrics, ctx->d_metrics, static_cast<size_t>(d.nodes) * 10 * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream));
+  if (derivatives)
+    HIP_TRY(ctx, hipMemcpyAsync(derivatives, ctx->d_deriv,
+                                static_cast<size_t>(d.nodes) * 25 * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return TAILOR_OK;
+}
+
+int tailor_time_assembly(tailor_ctx *ctx, int reps, double *write_kernel_ms,
+                         double *full_assembly_ms) {
+  if (!ctx || reps <= 0) return TAILOR_ERR_USER_INPUT;
+  if (!ctx->loaded) return fail(ctx, TAILOR_ERR_ARG_WRONGSTATE, "no case loaded");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipEvent_t e0, e1;
+  HIP_TRY(ctx, hipEventCreate(&e0));
+  HIP_TRY(ctx, hipEventCreate(&e1));
+  float ms = 0.0f;
+  // full assembly: diag reset, metrics, derivatives, characteristics, operator
+  HIP_TRY(ctx, hipEventRecord(e0, ctx->stream));
+  for (int r = 0; r < reps; ++r) {
+    HIP_TRY(ctx, tailor::launch_reset_diag(ctx));
+    HIP_TRY(ctx, tailor::launch_metrics(ctx));
+    HIP_TRY(ctx, tailor::launch_derivatives(ctx));
+    HIP_TRY(ctx, tailor::launch_characteristics(ctx));
+    HIP_TRY(ctx, tailor::launch_operator(ctx, true));
+  }
+  HIP_TRY(ctx, hipEventRecord(e1, ctx->stream));
+  HIP_TRY(ctx, hipEventSynchronize(e1));
+  HIP_TRY(ctx, hipEventElapsedTime(&ms, e0, e1));
+  if (full_assembly_ms) *full_assembly_ms = static_cast<double>(ms) / reps;
+  // operator writer alone (k_operator; its inputs are resident from above)
+  HIP_TRY(ctx, hipEventRecord(e0, ctx->stream));
+  for (int r = 0; r < reps; ++r) HIP_TRY(ctx, tailor::launch_operator(ctx, true));
+  HIP_TRY(ctx, hipEventRecord(e1, ctx->stream));
+  HIP_TRY(ctx, hipEventSynchronize(e1));
+  HIP_TRY(ctx, hipEventElapsedTime(&ms, e0, e1));
+  if (write_kernel_ms) *write_kernel_ms = static_cast<double>(ms) / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->vals_kind = TAILOR_OP_A;
+  ctx->mvals_kind = TAILOR_OP_B;
+  tailor::DeviceDiag g{};
+  HIP_TRY(ctx, hipMemcpy(&g, ctx->d_diag, sizeof g, hipMemcpyDeviceToHost));
+  return check_assembly_flags(ctx, g);
+}
+
+}  // extern "C"

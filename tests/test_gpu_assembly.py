@@ -1,0 +1,126 @@
+"""GPU assembly parity: libtailor (HIP, gfx950) vs the CPU oracle.
+
+The bar: assembled block entries within 1e-12
+relative.  The kernels follow the oracle's operation order without FP
+contraction, so most entries are expected to be bit-identical; the test
+reports the bit-exact fraction and enforces the 1e-12 bar.
+"""
+import numpy as np
+import pytest
+
+from conftest import small_case
+import tailor_amd as ta
+from tailor_amd import case as cs
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(got, ref):
+    return np.max(np.abs(got - ref) / np.maximum(1.0, np.abs(ref)))
+
+
+def assemble_both(c, oracle_lib):
+    ref = oracle_lib.assemble(c)
+    with ta.Context(0) as ctx:
+        ctx.load_case(c)
+        diag = ctx.assemble(diagnostics=True)
+        m, d = ctx.export_fields()
+        rp, ci, A = ctx.export_bsr(ta.OP_A)
+        _, _, B = ctx.export_bsr(ta.OP_B)
+        ctx.assemble_unconstrained()
+        _, _, L = ctx.export_bsr(ta.OP_L)
+        _, _, M = ctx.export_bsr(ta.OP_M)
+    return ref, diag, dict(metrics=m, deriv=d, rowptr=rp, colidx=ci, A=A, B=B, L=L, M=M)
+
+
+@pytest.mark.parametrize("shape", [(24, 12), (100, 50)])
+def test_assembly_matches_oracle(oracle_lib, shape):
+    ny, nz = shape
+    c = small_case(ny, nz) if ny < 100 else cs.synthetic_case(100, 50)
+    ref, diag, got = assemble_both(c, oracle_lib)
+    print(f"case {ny}x{nz}", flush=True)
+    assert np.array_equal(got["rowptr"], ref.rowptr)
+    assert np.array_equal(got["colidx"], ref.colidx)
+    assert rel_err(got["metrics"], ref.metrics) <= 1e-13
+    assert rel_err(got["deriv"], ref.deriv) <= 1e-13
+    for key in ("L", "M", "A", "B"):
+        g, r = got[key], getattr(ref, key)
+        err = rel_err(g, r)
+        exact = np.mean(g == r)
+        print(f"{key}: max rel err {err:.3e}, bit-exact fraction {exact:.6f}")
+        assert err <= 1e-12, key
+    assert diag["wall_nodes"] == nz and diag["farfield_nodes"] == nz
+    assert diag["incoming_modes"] == int(ref.bc_diag[2])
+    assert diag["neutral_modes"] == int(ref.bc_diag[3])
+    assert abs(diag["a_fro"] - np.linalg.norm(ref.A)) <= 1e-12 * np.linalg.norm(ref.A)
+    assert abs(diag["spatial_fro"] - np.linalg.norm(ref.L)) <= 1e-12 * np.linalg.norm(ref.L)
+    assert abs(diag["b_fro"] - np.linalg.norm(ref.B)) <= 1e-12 * np.linalg.norm(ref.B)
+
+
+def test_full_size_assembly_matches_oracle(oracle_lib):
+    c = cs.synthetic_case(200, 100, omega_shift=0.02 + 0j)
+    ref = oracle_lib.assemble(c)
+    with ta.Context(0) as ctx:
+        ctx.load_case(c)
+        ctx.assemble(diagnostics=False)
+        _, _, A = ctx.export_bsr(ta.OP_A)
+        _, _, B = ctx.export_bsr(ta.OP_B)
+    assert A.shape[0] == 200 * 100 * 77
+    assert rel_err(A, ref.A) <= 1e-12
+    assert rel_err(B, ref.B) <= 1e-12
+    print(f"200x100 bit-exact fraction A {np.mean(A == ref.A):.6f}")
+
+
+def test_alpha_change_reassembles(oracle_lib):
+    c = small_case(24, 12)
+    with ta.Context(0) as ctx:
+        ctx.load_case(c)
+        ctx.set_alpha(0.7 + 0.1j)
+        ctx.assemble(diagnostics=False)
+        _, _, A = ctx.export_bsr(ta.OP_A)
+    ref = oracle_lib.assemble(c.with_alpha(0.7 + 0.1j))
+    assert rel_err(A, ref.A) <= 1e-12
+
+
+def test_uniform_flow_counts():
+    c = cs.uniform_case(24, 8, q_y=4, q_z=2)
+    with ta.Context(0) as ctx:
+        ctx.load_case(c)
+        d = ctx.assemble()
+    assert (d["incoming_modes"], d["neutral_modes"]) == (8, 24)
+    assert (d["min_incoming_per_node"], d["max_incoming_per_node"]) == (1, 1)
+
+
+@pytest.mark.parametrize("mutation,status,message", [
+    ("nan_grid", ta.TailorError, "Grid coordinates or computed first-order metrics are not finite"),
+    ("complex_grid", ta.TailorError, "Grid y/z coordinates must be real"),
+    ("degenerate_grid", ta.TailorError, "Grid Jacobian must be positive and nonsingular"),
+    ("negative_T", ta.TailorError, "density and temperature must both be positive"),
+    ("complex_base", ta.TailorError, "Base-flow values must be real"),
+])
+def test_invalid_cases_fail_with_reference_messages(mutation, status, message):
+    c = small_case(24, 12)
+    if mutation == "nan_grid":
+        c.grid = c.grid.copy(); c.grid[5, 1] = np.nan
+    elif mutation == "complex_grid":
+        c.grid = c.grid.copy(); c.grid[:, 1] += 1e-6j
+    elif mutation == "degenerate_grid":
+        c.grid = c.grid.copy(); c.grid[:, 1:] = 0.0
+    elif mutation == "negative_T":
+        c.baseflow = c.baseflow.copy(); c.baseflow[7, 4] = -1.0
+    elif mutation == "complex_base":
+        c.baseflow = c.baseflow.copy(); c.baseflow[:, 1] += 1e-6j
+    with ta.Context(0) as ctx:
+        ctx.load_case(c)
+        with pytest.raises(status) as e:
+            ctx.assemble(diagnostics=False)
+    assert message in str(e.value)
+
+
+def test_physics_validation_message():
+    c = small_case(24, 12)
+    c.physics = cs.Physics(reynolds=-1.0)
+    with ta.Context(0) as ctx:
+        with pytest.raises(ta.TailorError) as e:
+            ctx.load_case(c)
+    assert "outside their valid ranges" in str(e.value)
