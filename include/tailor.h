@@ -140,6 +140,64 @@ int tailor_export_fields(tailor_ctx *ctx, double *metrics, double *derivatives);
 int tailor_time_assembly(tailor_ctx *ctx, int reps, double *write_kernel_ms,
                          double *full_assembly_ms);
 
+/* ------------------------------------------------------------------------
+ * Shift-invert eigen-solve (replaces EigenSolver::solve, EigenSolver.cpp:31-89:
+ * EPS GNHEP Krylov-Schur, ST SINVERT, KSP PREONLY, PC LU).
+ * ---------------------------------------------------------------------- */
+
+/* Sparse LU of K = A_bc - sigma B_bc (replaces the PCLU factorization inside
+ * STSetUp).  Diagnostics of the symbolic and numeric phases. */
+typedef struct {
+  int32_t fronts, levels, max_front, perturbed_pivots;
+  int64_t arena_bytes;     /* dense front storage in HBM */
+  int64_t factor_entries;  /* complex entries of L and U */
+  double flops;            /* real flops of the partial factorizations */
+  double symbolic_ms, factor_ms;
+} tailor_factor_diag;
+
+int tailor_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, tailor_factor_diag *diag);
+
+/* x = K^-1 b for natural-order interleaved complex vectors of length 5*N,
+ * with up to refine_max steps of iterative refinement towards a relative
+ * residual refine_tol; relres/steps report the outcome (may be NULL). */
+int tailor_linear_solve(tailor_ctx *ctx, const double *b, double *x, int32_t refine_max,
+                        double refine_tol, double *relres, int32_t *steps);
+
+/* Settings of EigenSolver in config.yaml (Parser.cpp:62-83). */
+typedef struct {
+  int32_t nev;             /* NumberOfEigenvalues */
+  int32_t ncv;             /* subspace size; 0 selects max(2 nev, nev + 15) */
+  int32_t maxit;           /* MaximumIterations (restarts) */
+  double tol;              /* Tolerance: accepted when ||A q - lambda B q|| / ||q|| <= tol */
+  double omega_re, omega_im;  /* SearchCenterOmega; lambda_s = Im(omega) - i Re(omega) */
+  int32_t refine_max;      /* iterative refinement steps per shift-invert solve */
+  double refine_tol;       /* relative residual target of a refined solve */
+} tailor_solve_opts;
+
+typedef struct {
+  int32_t requested, converged, iterations, solves, refinement_steps, perturbed_pivots;
+  double lambda_target_re, lambda_target_im;
+  double max_residual;        /* max ||A q - lambda B q|| / ||q|| over the requested pairs */
+  double max_relative_error;  /* the same divided by |lambda| */
+  double symbolic_ms, factor_ms, solve_ms;
+  int64_t fronts, factor_bytes;
+  double factor_flops;
+} tailor_solve_diag;
+
+/* Factor (if the shift changed) and run Krylov-Schur on the assembled
+ * A_bc, B_bc.  TAILOR_ERR_NOT_CONVERGED if fewer than nev pairs meet tol. */
+int tailor_solve(tailor_ctx *ctx, const tailor_solve_opts *opts, tailor_solve_diag *diag);
+
+/* Number of Ritz pairs available from the last solve (sorted by distance
+ * of lambda to the target). */
+int tailor_eigenpair_count(const tailor_ctx *ctx);
+
+/* Pair k: lambda (2 doubles), mode q (natural k_j_dof order, 5N complex,
+ * unit 2-norm, largest entry real positive; may be NULL), residual
+ * ||A q - lambda B q|| / ||q|| and relative error residual / |lambda|. */
+int tailor_get_eigenpair(tailor_ctx *ctx, int32_t k, double *lambda, double *q, double *residual,
+                         double *relative_error);
+
 #ifdef __cplusplus
 }
 #endif

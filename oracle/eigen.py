@@ -1,0 +1,176 @@
+"""CPU oracle of the shift-invert eigen-solve (TEST INFRASTRUCTURE ONLY).
+
+The reference solves A_bc q = lambda B_bc q with SLEPc (EigenSolver.cpp:31-89):
+EPS GNHEP, Krylov-Schur, nev, tol, maxit, target lambda_s = Im(omega_s) -
+i Re(omega_s) (EigenSolver.cpp:46-48), EPS_TARGET_MAGNITUDE, ST SINVERT with
+shift lambda_s, KSP PREONLY, PC LU (PETSc native or MUMPS).  SLEPc (>= 3.25,
+CMakeLists.txt:92) and MUMPS are not in this tree or this image; this module
+restates their published algorithms:
+
+* the LU of K = A - sigma B is scipy's SuperLU (any exact LU gives the same
+  shift-invert operator up to rounding);
+* Krylov-Schur (G. W. Stewart, SIAM J. Matrix Anal. Appl. 23 (2001)):
+  Arnoldi with classical Gram-Schmidt and one reorthogonalisation, complex
+  Schur form of the projected matrix sorted by |theta|, restart keeping
+  the converged pairs plus half of the rest (SLEPc's restart ratio 0.5),
+  ncv = max(2 nev, nev + 15) (SLEPc's default for nev < 500).
+
+Pinned by the known 10x10 GNHEP of test_eigen_solver.cpp:17-57 (lambda in
+{-0.5+0.7i, -1.2-0.4i} nearest lambda_s = -0.45+0.65i within 2e-10) and by
+dense LAPACK brute force on small grids (tests/test_oracle_eigen.py).
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.linalg as sla
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+
+def target_lambda(omega: complex) -> complex:
+    """lambda_s = Im(omega_s) - i Re(omega_s)  (EigenSolver.cpp:46-48)."""
+    omega = complex(omega)
+    return complex(omega.imag, -omega.real)
+
+
+def krylov_schur(A, B, sigma: complex, nev: int = 8, tol: float = 1e-10, maxit: int = 1000,
+                 ncv: int | None = None, seed: int = 0):
+    """Eigenpairs of A q = lambda B q nearest sigma.
+
+    Returns (lambdas, vectors (n, k), residuals ||Aq - lambda Bq|| / ||q||,
+    restarts).  Convergence: Ritz estimate / |theta| <= tol for the nev
+    leading pairs (EPS_CONV_REL on the transformed problem).
+    """
+    A = sp.csc_matrix(A)
+    B = sp.csc_matrix(B)
+    n = A.shape[0]
+    if ncv is None:
+        ncv = max(2 * nev, nev + 15)
+    ncv = min(ncv, n - 1)
+    lu = spla.splu((A - sigma * B).tocsc())
+
+    def op(v):
+        return lu.solve(B @ v)
+
+    rng = np.random.default_rng(seed)
+    V = np.zeros((n, ncv + 1), dtype=np.complex128)
+    v0 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    V[:, 0] = v0 / np.linalg.norm(v0)
+    S = np.zeros((ncv + 1, ncv), dtype=np.complex128)
+    k = 0
+    for restart in range(maxit):
+        for j in range(k, ncv):
+            w = op(V[:, j])
+            h = V[:, :j + 1].conj().T @ w
+            w = w - V[:, :j + 1] @ h
+            h2 = V[:, :j + 1].conj().T @ w
+            w = w - V[:, :j + 1] @ h2
+            h = h + h2
+            beta = np.linalg.norm(w)
+            S[:j + 1, j] = h
+            S[j + 1, j] = beta
+            V[:, j + 1] = w / beta
+        T, Q = sla.schur(S[:ncv, :ncv], output="complex")
+        # sort by |theta| descending (selection sort with Schur swaps)
+        for pos in range(ncv):
+            dd = np.abs(np.diag(T))
+            best = pos + int(np.argmax(dd[pos:]))
+            if best != pos:
+                sel = np.zeros(ncv, dtype=bool)
+                sel[:pos] = True
+                sel[best] = True
+                T, Q, _ = _reorder(T, Q, sel)
+        b = S[ncv, :ncv] @ Q
+        theta = np.diag(T)
+        conv = 0
+        ests = []
+        for i in range(ncv - 1):
+            y = _tri_eigvec(T, i)
+            est = abs(b @ y) / abs(theta[i])
+            ests.append(est)
+        while conv < len(ests) and ests[conv] <= tol:
+            conv += 1
+        if conv >= nev or restart == maxit - 1:
+            lams, vecs, res = [], [], []
+            for i in range(min(conv, ncv - 1)):
+                y = _tri_eigvec(T, i)
+                x = V[:, :ncv] @ (Q @ y)
+                lam = sigma + 1.0 / theta[i]
+                r = np.linalg.norm(A @ x - lam * (B @ x)) / np.linalg.norm(x)
+                lams.append(lam)
+                vecs.append(x / np.linalg.norm(x))
+                res.append(r)
+            return np.array(lams), np.array(vecs).T, np.array(res), restart + 1
+        keep = conv + max(1, (ncv - conv) // 2)
+        keep = min(keep, ncv - 1)
+        V[:, :keep] = V[:, :ncv] @ Q[:, :keep]
+        V[:, keep] = V[:, ncv]
+        S = np.zeros((ncv + 1, ncv), dtype=np.complex128)
+        S[:keep, :keep] = np.triu(T[:keep, :keep])
+        S[keep, :keep] = b[:keep]
+        k = keep
+    raise RuntimeError("unreachable")
+
+
+def _tri_eigvec(T, i):
+    n = T.shape[0]
+    y = np.zeros(n, dtype=np.complex128)
+    y[i] = 1.0
+    lam = T[i, i]
+    small = np.finfo(float).eps * max(np.max(np.abs(np.diag(T)[:i + 1])), 1e-300)
+    for kk in range(i - 1, -1, -1):
+        s = T[kk, kk + 1:i + 1] @ y[kk + 1:i + 1]
+        den = T[kk, kk] - lam
+        if abs(den) < small:
+            den = small
+        y[kk] = -s / den
+    return y / np.linalg.norm(y)
+
+
+def _reorder(T, Q, select):
+    """Move the selected diagonal entries to the front (adjacent swaps)."""
+    T = T.copy()
+    Q = Q.copy()
+    n = T.shape[0]
+    target = 0
+    for kk in range(n):
+        if not select[kk]:
+            continue
+        for j in range(kk - 1, target - 1, -1):
+            t11, t22 = T[j, j], T[j + 1, j + 1]
+            f, g = T[j, j + 1], t22 - t11
+            c, s = _givens(f, g)
+            G = np.array([[c, s], [-np.conj(s), c]])
+            T[j:j + 2, j:] = G @ T[j:j + 2, j:]
+            T[:j + 2, j:j + 2] = T[:j + 2, j:j + 2] @ G.conj().T
+            Q[:, j:j + 2] = Q[:, j:j + 2] @ G.conj().T
+            T[j + 1, j] = 0.0
+            T[j, j], T[j + 1, j + 1] = t22, t11
+        target += 1
+    return T, Q, None
+
+
+def _givens(f, g):
+    af, ag = abs(f), abs(g)
+    if ag == 0:
+        return 1.0, 0.0
+    if af == 0:
+        return 0.0, np.conj(g) / ag
+    nrm = np.hypot(af, ag)
+    return af / nrm, (f / af) * np.conj(g) / nrm
+
+
+def known_problem():
+    """The 10x10 GNHEP of test_eigen_solver.cpp:17-57 (A, B as dense)."""
+    lam = np.array([1.0 + 0.2j, -0.5 + 0.7j, 2.0 - 0.3j, -1.2 - 0.4j, 3.0 + 1.0j, -3.0 - 1.0j,
+                    4.0 + 2.0j, -4.0 - 2.0j, 5.0 + 3.0j, -5.0 - 3.0j])
+    mass = np.array([1.0, 2.0, 1.5, 1.0, 0.8, 1.2, 1.1, 0.9, 1.4, 1.3])
+    A = np.diag(mass * lam).astype(np.complex128)
+    A[0, 1] = 0.7 - 0.2j
+    A[0, 3] = -0.3 + 0.5j
+    A[1, 2] = 1.1 + 0.4j
+    A[2, 3] = 0.2 - 0.8j
+    for r in range(3, 9):
+        A[r, r + 1] = complex(0.05 * r, -0.03 * r)
+    B = np.diag(mass).astype(np.complex128)
+    return A, B

@@ -76,6 +76,33 @@ class AssemblyDiag(C.Structure):
         return out
 
 
+class FactorDiag(C.Structure):
+    _fields_ = [("fronts", C.c_int32), ("levels", C.c_int32), ("max_front", C.c_int32),
+                ("perturbed_pivots", C.c_int32), ("arena_bytes", C.c_int64),
+                ("factor_entries", C.c_int64), ("flops", C.c_double),
+                ("symbolic_ms", C.c_double), ("factor_ms", C.c_double)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("nev", C.c_int32), ("ncv", C.c_int32), ("maxit", C.c_int32),
+                ("tol", C.c_double), ("omega_re", C.c_double), ("omega_im", C.c_double),
+                ("refine_max", C.c_int32), ("refine_tol", C.c_double)]
+
+
+class SolveDiag(C.Structure):
+    _fields_ = [("requested", C.c_int32), ("converged", C.c_int32), ("iterations", C.c_int32),
+                ("solves", C.c_int32), ("refinement_steps", C.c_int32),
+                ("perturbed_pivots", C.c_int32), ("lambda_target_re", C.c_double),
+                ("lambda_target_im", C.c_double), ("max_residual", C.c_double),
+                ("max_relative_error", C.c_double), ("symbolic_ms", C.c_double),
+                ("factor_ms", C.c_double), ("solve_ms", C.c_double), ("fronts", C.c_int64),
+                ("factor_bytes", C.c_int64), ("factor_flops", C.c_double)]
+
+
+def _sdict(st) -> dict:
+    return {name: getattr(st, name) for name, _t in st._fields_}
+
+
 _lib = None
 
 
@@ -103,6 +130,13 @@ def library() -> C.CDLL:
                                       P(C.c_double)]
     lib.tailor_export_fields.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
     lib.tailor_time_assembly.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double)]
+    lib.tailor_factor.argtypes = [C.c_void_p, C.c_double, C.c_double, P(FactorDiag)]
+    lib.tailor_linear_solve.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), C.c_int32,
+                                        C.c_double, P(C.c_double), P(C.c_int32)]
+    lib.tailor_solve.argtypes = [C.c_void_p, P(SolveOpts), P(SolveDiag)]
+    lib.tailor_eigenpair_count.argtypes = [C.c_void_p]
+    lib.tailor_get_eigenpair.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double),
+                                         P(C.c_double), P(C.c_double)]
     _lib = lib
     return lib
 
@@ -215,6 +249,63 @@ class Context:
         w, f = C.c_double(), C.c_double()
         self._check(self.lib.tailor_time_assembly(self.ptr, reps, C.byref(w), C.byref(f)))
         return w.value, f.value
+
+
+def _factor(self, sigma: complex) -> dict:
+    d = FactorDiag()
+    sigma = complex(sigma)
+    self._check(self.lib.tailor_factor(self.ptr, sigma.real, sigma.imag, C.byref(d)))
+    return _sdict(d)
+
+
+def _linear_solve(self, b: np.ndarray, refine_max: int = 0, refine_tol: float = 1e-14):
+    """x = (A_bc - sigma B_bc)^-1 b for a natural-order complex vector."""
+    b = np.ascontiguousarray(b, dtype=np.complex128)
+    x = np.zeros_like(b)
+    rel = C.c_double(-1.0)
+    steps = C.c_int32(0)
+    self._check(self.lib.tailor_linear_solve(self.ptr, _dptr(b.view(np.float64)),
+                                             _dptr(x.view(np.float64)), refine_max, refine_tol,
+                                             C.byref(rel), C.byref(steps)))
+    return x, rel.value, steps.value
+
+
+def _solve(self, nev: int = 8, omega: complex = 0j, tol: float = 1e-10, maxit: int = 1000,
+           ncv: int = 0, refine_max: int = 2, refine_tol: float = 1e-13,
+           raise_on_failure: bool = True) -> dict:
+    """Shift-invert Krylov-Schur (EigenSolver::solve)."""
+    omega = complex(omega)
+    o = SolveOpts(nev, ncv, maxit, tol, omega.real, omega.imag, refine_max, refine_tol)
+    d = SolveDiag()
+    st = self.lib.tailor_solve(self.ptr, C.byref(o), C.byref(d))
+    out = _sdict(d)
+    out["status"] = st
+    if st and raise_on_failure:
+        self._check(st)
+    return out
+
+
+def _eigenpairs(self, vectors: bool = False) -> list:
+    cnt = self.lib.tailor_eigenpair_count(self.ptr)
+    rows, _ = self.shape()
+    out = []
+    for k in range(cnt):
+        lam = np.zeros(2)
+        res, rel = C.c_double(), C.c_double()
+        q = np.zeros(2 * 5 * rows) if vectors else None
+        self._check(self.lib.tailor_get_eigenpair(self.ptr, k, _dptr(lam),
+                                                  _dptr(q) if vectors else None,
+                                                  C.byref(res), C.byref(rel)))
+        out.append({"lambda": complex(lam[0], lam[1]), "omega": 1j * complex(lam[0], lam[1]),
+                    "residual": res.value, "relative_error": rel.value,
+                    "mode": q.view(np.complex128) if vectors else None})
+    return out
+
+
+Context.factor = _factor
+Context.linear_solve = _linear_solve
+Context.solve = _solve
+Context.eigenpairs = _eigenpairs
 
 
 def format_assembly_report(case: Case, d: dict) -> str:

@@ -23,8 +23,10 @@ COMMON = ["-O3", "-std=c++20", "-fPIC", "--offload-arch=gfx950",
           "-Wall", "-Wno-unused-function", f"-I{PKG.parent / 'include'}"]
 PER_FILE = {
     "assembly.hip": ["-ffp-contract=off"],
+    "lu.hip": ["-munsafe-fp-atomics"],
 }
-SOURCES = ["assembly.hip", "capi.cpp"]
+SOURCES = ["assembly.hip", "lu.hip", "vec.hip", "capi.cpp", "eigen.cpp", "symbolic.cpp",
+           "lu_plan.cpp", "dense.cpp"]
 
 
 def _compile(src: str) -> Path:

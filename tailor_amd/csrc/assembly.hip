@@ -176,100 +176,131 @@ __global__ __launch_bounds__(kThreads) void k_metrics_first(
 
 // ---------------------------------------------------------------------------
 // Second metrics (chain rule of first metrics, Metrics.cpp:255-337) and the 25
-// physical base-flow derivatives (BaseFlowDerivatives.cpp:85-218), one thread
-// per node.
-__global__ __launch_bounds__(kThreads) void k_metrics2_derivatives(
+// physical base-flow derivatives (BaseFlowDerivatives.cpp:85-218).
+//
+// One workgroup owns a DTX x DTY tile of nodes and stages the tile plus its
+// stencil halo (hx nodes each side in xi, clipped at the walls; hy rows each
+// side in eta, wrapped periodically) of the base flow and of the first
+// metrics in LDS.  Thread (node, field) then evaluates its 11 + 7 + 77 taps
+// from LDS; the threads of field 0 first produce the node's second metrics.
+constexpr int DTX = 16, DTY = 8, DNODES = DTX * DTY, DTHREADS = 5 * DNODES;
+
+__global__ __launch_bounds__(DTHREADS) void k_metrics2_derivatives(
     CaseDims d, const int *__restrict__ xoff, const double *__restrict__ xw,
     const int *__restrict__ eoff, const double *__restrict__ ew,
     const double *__restrict__ base, double *__restrict__ metrics, double *__restrict__ deriv,
     DeviceDiag *diag) {
-  const long long node = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
+  extern __shared__ double lds[];
+  __shared__ double m2[DNODES][6];
+  __shared__ unsigned s_flags;
+  __shared__ unsigned long long s_imax;
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * DTX, j0 = blockIdx.y * DTY;
+  const int xlo = max(0, i0 - d.hx), xhi = min(d.ny, i0 + DTX + d.hx);
+  const int EX = xhi - xlo, EY = DTY + 2 * d.hy;
+  const int plane = EX * EY;
+  double *sb = lds;                 // [5][EY][EX] base flow (real parts)
+  double *sm = lds + 5 * plane;     // [4][EY][EX] first metrics
+  if (tid == 0) {
+    s_flags = 0;
+    s_imax = 0;
+  }
+  __syncthreads();
   unsigned flags = 0;
   double imax = 0.0;
-  if (node < d.nodes) {
-    const int i = static_cast<int>(node % d.ny), j = static_cast<int>(node / d.ny);
-    const double *w1x = xw + static_cast<size_t>(d.ny) * d.ssx + static_cast<size_t>(i) * d.ssx;
-    const double *w2x = w1x + static_cast<size_t>(d.ny) * d.ssx;
-    const double *w1e = ew + static_cast<size_t>(d.nz) * d.sse + static_cast<size_t>(j) * d.sse;
-    const double *w2e = w1e + static_cast<size_t>(d.nz) * d.sse;
-    const int *ox = xoff + i * d.ssx;
-    const int *oe = eoff + j * d.sse;
+  for (int idx = tid; idx < plane; idx += DTHREADS) {
+    const int ly = idx / EX, lx = idx - EX * (idx / EX);
+    const int gy = wrap_mod(j0 - d.hy + ly, d.nz);
+    const size_t node = static_cast<size_t>(gy) * d.ny + xlo + lx;
+    const double *b = base + 10 * node;
+    for (int f = 0; f < 5; ++f) {
+      const double re = b[2 * f], im = b[2 * f + 1];
+      if (!finite(re) || !finite(im)) flags |= kBaseNonFinite;
+      imax = fmax(imax, fabs(im));
+      sb[f * plane + idx] = re;
+    }
+    const double *mm = metrics + 10 * node;
+    for (int c = 0; c < 4; ++c) sm[c * plane + idx] = mm[c];
+  }
+  __syncthreads();
 
+  const int ln = tid % DNODES, f = tid / DNODES;
+  const int i = i0 + ln % DTX, j = j0 + ln / DTX;
+  const bool active = i < d.ny && j < d.nz;
+  const double *w1x = xw + static_cast<size_t>(d.ny) * d.ssx + static_cast<size_t>(i) * d.ssx;
+  const double *w2x = w1x + static_cast<size_t>(d.ny) * d.ssx;
+  const double *w1e = ew + static_cast<size_t>(d.nz) * d.sse + static_cast<size_t>(j) * d.sse;
+  const double *w2e = w1e + static_cast<size_t>(d.nz) * d.sse;
+  const int *ox = xoff + i * d.ssx;
+  const int *oe = eoff + j * d.sse;
+  const int cy = j - j0 + d.hy, cx = i - xlo;  // local coordinates of the node
+  if (active && f == 0) {
     double dXi[4] = {0.0, 0.0, 0.0, 0.0}, dEta[4] = {0.0, 0.0, 0.0, 0.0};
     for (int s = 0; s < d.ssx; ++s) {
-      const double *mm = metrics + 10 * (static_cast<size_t>(j) * d.ny + i + ox[s]);
+      const int l = cy * EX + cx + ox[s];
       const double w = w1x[s];
-      for (int c = 0; c < 4; ++c) dXi[c] += w * mm[c];
+      for (int c = 0; c < 4; ++c) dXi[c] += w * sm[c * plane + l];
     }
     for (int t = 0; t < d.sse; ++t) {
-      const double *mm = metrics + 10 * (static_cast<size_t>(wrap_mod(j + oe[t], d.nz)) * d.ny + i);
+      const int l = (cy + oe[t]) * EX + cx;
       const double w = w1e[t];
-      for (int c = 0; c < 4; ++c) dEta[c] += w * mm[c];
+      for (int c = 0; c < 4; ++c) dEta[c] += w * sm[c * plane + l];
     }
-    double *m = metrics + 10 * node;
-    const double xiY = m[0], xiZ = m[1], etaY = m[2], etaZ = m[3];
-    const double xiYY = xiY * dXi[0] + etaY * dEta[0];
-    const double xiZZ = xiZ * dXi[1] + etaZ * dEta[1];
-    const double xiYZ = xiZ * dXi[0] + etaZ * dEta[0];
-    const double etaYY = xiY * dXi[2] + etaY * dEta[2];
-    const double etaZZ = xiZ * dXi[3] + etaZ * dEta[3];
-    const double etaYZ = xiZ * dXi[2] + etaZ * dEta[2];
-    if (!finite(xiYY) || !finite(xiZZ) || !finite(xiYZ) || !finite(etaYY) || !finite(etaZZ) ||
-        !finite(etaYZ))
-      flags |= kSecondNonFinite;
-    m[4] = xiYY;
-    m[5] = xiZZ;
-    m[6] = xiYZ;
-    m[7] = etaYY;
-    m[8] = etaZZ;
-    m[9] = etaYZ;
-
-    const double *b0 = base + 10 * node;
-    if (!finite(b0[0]) || !finite(b0[1]) || !finite(b0[8]) || !finite(b0[9]))
-      flags |= kBaseNonFinite;
-    else if (b0[0] <= 0.0 || b0[8] <= 0.0)
-      flags |= kBasePositive;
-
-    for (int f = 0; f < 5; ++f) {
-      double vXi = 0.0, vEta = 0.0, vXiXi = 0.0, vEtaEta = 0.0, vXiEta = 0.0;
-      for (int s = 0; s < d.ssx; ++s) {
-        const double *b = base + 10 * (static_cast<size_t>(j) * d.ny + i + ox[s]) + 2 * f;
-        if (!finite(b[0]) || !finite(b[1])) flags |= kBaseNonFinite;
-        imax = fmax(imax, fabs(b[1]));
-        vXi += w1x[s] * b[0];
-        vXiXi += w2x[s] * b[0];
-      }
-      for (int t = 0; t < d.sse; ++t) {
-        const double *b =
-            base + 10 * (static_cast<size_t>(wrap_mod(j + oe[t], d.nz)) * d.ny + i) + 2 * f;
-        if (!finite(b[0]) || !finite(b[1])) flags |= kBaseNonFinite;
-        imax = fmax(imax, fabs(b[1]));
-        vEta += w1e[t] * b[0];
-        vEtaEta += w2e[t] * b[0];
-      }
-      for (int t = 0; t < d.sse; ++t) {
-        const double we = w1e[t];
-        const double *brow = base + 10 * (static_cast<size_t>(wrap_mod(j + oe[t], d.nz)) * d.ny + i) + 2 * f;
-        for (int s = 0; s < d.ssx; ++s) {
-          const double *b = brow + 10 * ox[s];
-          if (!finite(b[0]) || !finite(b[1])) flags |= kBaseNonFinite;
-          imax = fmax(imax, fabs(b[1]));
-          vXiEta += we * w1x[s] * b[0];
-        }
-      }
-      const double vY = xiY * vXi + etaY * vEta;
-      const double vZ = xiZ * vXi + etaZ * vEta;
-      const double vYY = xiY * xiY * vXiXi + 2.0 * xiY * etaY * vXiEta + etaY * etaY * vEtaEta +
-                         xiYY * vXi + etaYY * vEta;
-      const double vZZ = xiZ * xiZ * vXiXi + 2.0 * xiZ * etaZ * vXiEta + etaZ * etaZ * vEtaEta +
-                         xiZZ * vXi + etaZZ * vEta;
-      const double vYZ = xiY * xiZ * vXiXi + (xiY * etaZ + etaY * xiZ) * vXiEta +
-                         etaY * etaZ * vEtaEta + xiYZ * vXi + etaYZ * vEta;
-      if (!finite(vY) || !finite(vZ) || !finite(vYY) || !finite(vZZ) || !finite(vYZ)) {
-        flags |= kDerivNonFinite;
-        continue;
-      }
-      double *dd = deriv + 25 * node;
+    const int l0 = cy * EX + cx;
+    const double xiY = sm[l0], xiZ = sm[plane + l0], etaY = sm[2 * plane + l0],
+                 etaZ = sm[3 * plane + l0];
+    double v[6];
+    v[0] = xiY * dXi[0] + etaY * dEta[0];
+    v[1] = xiZ * dXi[1] + etaZ * dEta[1];
+    v[2] = xiZ * dXi[0] + etaZ * dEta[0];
+    v[3] = xiY * dXi[2] + etaY * dEta[2];
+    v[4] = xiZ * dXi[3] + etaZ * dEta[3];
+    v[5] = xiZ * dXi[2] + etaZ * dEta[2];
+    double *m = metrics + 10 * (static_cast<size_t>(j) * d.ny + i);
+    for (int c = 0; c < 6; ++c) {
+      if (!finite(v[c])) flags |= kSecondNonFinite;
+      m2[ln][c] = v[c];
+      m[4 + c] = v[c];
+    }
+    const double rho = sb[l0], T = sb[4 * plane + l0];
+    if (finite(rho) && finite(T) && (rho <= 0.0 || T <= 0.0)) flags |= kBasePositive;
+  }
+  __syncthreads();
+  if (active) {
+    const double *sf = sb + f * plane;
+    double vXi = 0.0, vEta = 0.0, vXiXi = 0.0, vEtaEta = 0.0, vXiEta = 0.0;
+    for (int s = 0; s < d.ssx; ++s) {
+      const double b = sf[cy * EX + cx + ox[s]];
+      vXi += w1x[s] * b;
+      vXiXi += w2x[s] * b;
+    }
+    for (int t = 0; t < d.sse; ++t) {
+      const double b = sf[(cy + oe[t]) * EX + cx];
+      vEta += w1e[t] * b;
+      vEtaEta += w2e[t] * b;
+    }
+    for (int t = 0; t < d.sse; ++t) {
+      const double we = w1e[t];
+      const double *row = sf + (cy + oe[t]) * EX + cx;
+      for (int s = 0; s < d.ssx; ++s) vXiEta += we * w1x[s] * row[ox[s]];
+    }
+    const int l0 = cy * EX + cx;
+    const double xiY = sm[l0], xiZ = sm[plane + l0], etaY = sm[2 * plane + l0],
+                 etaZ = sm[3 * plane + l0];
+    const double xiYY = m2[ln][0], xiZZ = m2[ln][1], xiYZ = m2[ln][2];
+    const double etaYY = m2[ln][3], etaZZ = m2[ln][4], etaYZ = m2[ln][5];
+    const double vY = xiY * vXi + etaY * vEta;
+    const double vZ = xiZ * vXi + etaZ * vEta;
+    const double vYY = xiY * xiY * vXiXi + 2.0 * xiY * etaY * vXiEta + etaY * etaY * vEtaEta +
+                       xiYY * vXi + etaYY * vEta;
+    const double vZZ = xiZ * xiZ * vXiXi + 2.0 * xiZ * etaZ * vXiEta + etaZ * etaZ * vEtaEta +
+                       xiZZ * vXi + etaZZ * vEta;
+    const double vYZ = xiY * xiZ * vXiXi + (xiY * etaZ + etaY * xiZ) * vXiEta +
+                       etaY * etaZ * vEtaEta + xiYZ * vXi + etaYZ * vEta;
+    if (!finite(vY) || !finite(vZ) || !finite(vYY) || !finite(vZZ) || !finite(vYZ)) {
+      flags |= kDerivNonFinite;
+    } else {
+      double *dd = deriv + 25 * (static_cast<size_t>(j) * d.ny + i);
       dd[0 * 5 + f] = vY;
       dd[1 * 5 + f] = vZ;
       dd[2 * 5 + f] = vYY;
@@ -277,7 +308,13 @@ __global__ __launch_bounds__(kThreads) void k_metrics2_derivatives(
       dd[4 * 5 + f] = vYZ;
     }
   }
-  reduce_publish(flags, 0.0, 0.0, imax, &diag->flags, nullptr, nullptr, &diag->max_base_imag);
+  if (flags) atomicOr(&s_flags, flags);
+  if (imax > 0.0) atomicMax(&s_imax, static_cast<unsigned long long>(__double_as_longlong(imax)));
+  __syncthreads();
+  if (tid == 0) {
+    if (s_flags) atomicOr(&diag->flags, s_flags);
+    if (s_imax) atomic_max_d(&diag->max_base_imag, __longlong_as_double(static_cast<long long>(s_imax)));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -285,6 +322,8 @@ __global__ __launch_bounds__(kThreads) void k_metrics2_derivatives(
 // x-derivatives are zero in the locally parallel BiGlobal model
 // (LNSCoefficients.cpp:75-110), so the x terms of the reference formulas
 // drop out exactly.
+typedef double double2v __attribute__((ext_vector_type(2)));
+
 struct Point {
   double rho, U, V, W, T;
   double rhoy, Uy, Vy, Wy, Ty;
@@ -511,65 +550,64 @@ __device__ void physical_part(int part, const Point &p, const Transport &q, doub
 // used as columns of R: acoustic [rho, 0, -+n_y a, -+n_z a, (gamma-1) T],
 // streamwise e_1, tangential [0, 0, n_z, -n_y, 0], entropy [rho, 0, 0, 0, -T].
 
-__device__ void cdiv(double a, double b, double c, double dd, double &qr, double &qi) {
-  if (fabs(c) >= fabs(dd)) {
-    const double r = dd / c, den = c + dd * r;
-    qr = (a + b * r) / den;
-    qi = (b - a * r) / den;
-  } else {
-    const double r = c / dd, den = c * r + dd;
-    qr = (a * r + b) / den;
-    qi = (b * r - a) / den;
-  }
-}
-
-// Pivoted Gauss-Jordan inverse of a complex 5x5 (BoundaryConditions.cpp:92-140).
-__device__ bool invert5(dcx *a, dcx *inv) {
-  for (int k = 0; k < 25; ++k) inv[k] = dcx{0.0, 0.0};
-  for (int k = 0; k < 5; ++k) inv[k * 5 + k].re = 1.0;
+// Pivoted Gauss-Jordan inverse of a 5x5 (BoundaryConditions.cpp:92-140).
+// The closed-form eigenbasis and Gamma are real, and for real operands the
+// reference's complex steps (|z| by hypot, division, multiply-subtract)
+// reduce exactly to these real ones, so the inverse is computed in real
+// arithmetic: fully unrolled, register resident.
+__device__ __forceinline__ bool invert5(double (&a)[25], double (&inv)[25]) {
+#pragma unroll
+  for (int k = 0; k < 25; ++k) inv[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) inv[k * 5 + k] = 1.0;
+#pragma unroll
   for (int pc = 0; pc < 5; ++pc) {
     int pr = pc;
-    double pm = hypot(a[pr * 5 + pc].re, a[pr * 5 + pc].im);
+    double pm = fabs(a[pc * 5 + pc]);
+#pragma unroll
     for (int r = pc + 1; r < 5; ++r) {
-      const double mg = hypot(a[r * 5 + pc].re, a[r * 5 + pc].im);
+      const double mg = fabs(a[r * 5 + pc]);
       if (mg > pm) {
         pm = mg;
         pr = r;
       }
     }
     if (!finite(pm) || pm <= 100.0 * kEps) return false;
-    if (pr != pc) {
-      for (int c = 0; c < 5; ++c) {
-        dcx t = a[pr * 5 + c];
-        a[pr * 5 + c] = a[pc * 5 + c];
-        a[pc * 5 + c] = t;
-        t = inv[pr * 5 + c];
-        inv[pr * 5 + c] = inv[pc * 5 + c];
-        inv[pc * 5 + c] = t;
+#pragma unroll
+    for (int r = pc + 1; r < 5; ++r) {
+      if (r == pr) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+          double t = a[r * 5 + c];
+          a[r * 5 + c] = a[pc * 5 + c];
+          a[pc * 5 + c] = t;
+          t = inv[r * 5 + c];
+          inv[r * 5 + c] = inv[pc * 5 + c];
+          inv[pc * 5 + c] = t;
+        }
       }
     }
-    const dcx piv = a[pc * 5 + pc];
+    const double piv = a[pc * 5 + pc];
+#pragma unroll
     for (int c = 0; c < 5; ++c) {
-      cdiv(a[pc * 5 + c].re, a[pc * 5 + c].im, piv.re, piv.im, a[pc * 5 + c].re, a[pc * 5 + c].im);
-      cdiv(inv[pc * 5 + c].re, inv[pc * 5 + c].im, piv.re, piv.im, inv[pc * 5 + c].re,
-           inv[pc * 5 + c].im);
+      a[pc * 5 + c] = a[pc * 5 + c] / piv;
+      inv[pc * 5 + c] = inv[pc * 5 + c] / piv;
     }
+#pragma unroll
     for (int r = 0; r < 5; ++r) {
       if (r == pc) continue;
-      const dcx f = a[r * 5 + pc];
+      const double f = a[r * 5 + pc];
+#pragma unroll
       for (int c = 0; c < 5; ++c) {
-        const dcx y = a[pc * 5 + c], z = inv[pc * 5 + c];
-        a[r * 5 + c].re -= f.re * y.re - f.im * y.im;
-        a[r * 5 + c].im -= f.re * y.im + f.im * y.re;
-        inv[r * 5 + c].re -= f.re * z.re - f.im * z.im;
-        inv[r * 5 + c].im -= f.re * z.im + f.im * z.re;
+        a[r * 5 + c] -= f * a[pc * 5 + c];
+        inv[r * 5 + c] -= f * inv[pc * 5 + c];
       }
     }
   }
   return true;
 }
 
-__global__ void k_characteristics(CaseDims d, const double *__restrict__ recipe,
+__global__ __launch_bounds__(64) void k_characteristics(CaseDims d, const double *__restrict__ recipe,
                                   const double *__restrict__ base,
                                   const double *__restrict__ deriv,
                                   const double *__restrict__ metrics, double *__restrict__ bcdata,
@@ -581,8 +619,9 @@ __global__ void k_characteristics(CaseDims d, const double *__restrict__ recipe,
   Point p;
   int st = load_point(base, deriv, node, p) ? 1 : 0;
   const double *m = metrics + 10 * node;
-  for (int k = 0; k < 10 && !st; ++k)
-    if (!finite(m[k])) st = 1;
+#pragma unroll
+  for (int k = 0; k < 10; ++k)
+    if (!finite(m[k])) st = st ? st : 1;
   const double Ma = recipe[1], gam = recipe[3];
   const double Pe = 1.0 / (gam * Ma * Ma), g2 = 1.0 / ((gam - 1.0) * Ma * Ma);
   double nm = 0.0;
@@ -590,7 +629,7 @@ __global__ void k_characteristics(CaseDims d, const double *__restrict__ recipe,
     nm = sqrt(m[0] * m[0] + m[1] * m[1]);
     if (!finite(nm) || nm <= 100.0 * kEps) st = 2;
   }
-  dcx R[25], Ri[25], G[25], Gi[25], tmp[25];
+  double R[25], Ri[25], G[25], Gi[25], tmp[25];
   double c[5] = {0, 0, 0, 0, 0};
   double cond = 0.0;
   if (!st) {
@@ -598,26 +637,33 @@ __global__ void k_characteristics(CaseDims d, const double *__restrict__ recipe,
     const double un = ny * p.V + nz * p.W;
     const double a = sqrt(p.T) / Ma;
     c[0] = un - a; c[1] = un; c[2] = un; c[3] = un; c[4] = un + a;
-    for (int k = 0; k < 25; ++k) R[k] = dcx{0.0, 0.0};
-    R[0 * 5 + 0].re = p.rho; R[2 * 5 + 0].re = -ny * a; R[3 * 5 + 0].re = -nz * a;
-    R[4 * 5 + 0].re = (gam - 1.0) * p.T;
-    R[1 * 5 + 1].re = 1.0;
-    R[2 * 5 + 2].re = nz; R[3 * 5 + 2].re = -ny;
-    R[0 * 5 + 3].re = p.rho; R[4 * 5 + 3].re = -p.T;
-    R[0 * 5 + 4].re = p.rho; R[2 * 5 + 4].re = ny * a; R[3 * 5 + 4].re = nz * a;
-    R[4 * 5 + 4].re = (gam - 1.0) * p.T;
-    for (int k = 0; k < 25; ++k) { tmp[k] = R[k]; G[k] = dcx{0.0, 0.0}; }
-    G[0].re = 1.0; G[6].re = p.rho; G[12].re = p.rho; G[18].re = p.rho;
-    G[20].re = -Pe * p.T; G[24].re = p.rho * g2 - Pe * p.rho;
+#pragma unroll
+    for (int k = 0; k < 25; ++k) {
+      R[k] = 0.0;
+      G[k] = 0.0;
+    }
+    R[0 * 5 + 0] = p.rho; R[2 * 5 + 0] = -ny * a; R[3 * 5 + 0] = -nz * a;
+    R[4 * 5 + 0] = (gam - 1.0) * p.T;
+    R[1 * 5 + 1] = 1.0;
+    R[2 * 5 + 2] = nz; R[3 * 5 + 2] = -ny;
+    R[0 * 5 + 3] = p.rho; R[4 * 5 + 3] = -p.T;
+    R[0 * 5 + 4] = p.rho; R[2 * 5 + 4] = ny * a; R[3 * 5 + 4] = nz * a;
+    R[4 * 5 + 4] = (gam - 1.0) * p.T;
+#pragma unroll
+    for (int k = 0; k < 25; ++k) tmp[k] = R[k];
+    G[0] = 1.0; G[6] = p.rho; G[12] = p.rho; G[18] = p.rho;
+    G[20] = -Pe * p.T; G[24] = p.rho * g2 - Pe * p.rho;
     if (!invert5(tmp, Ri)) st = 6;
     if (!st && !invert5(G, Gi)) st = 6;
     if (!st) {
       double nR = 0.0, nRi = 0.0;
+#pragma unroll
       for (int r = 0; r < 5; ++r) {
         double s1 = 0.0, s2 = 0.0;
+#pragma unroll
         for (int cc = 0; cc < 5; ++cc) {
-          s1 += hypot(R[r * 5 + cc].re, R[r * 5 + cc].im);
-          s2 += hypot(Ri[r * 5 + cc].re, Ri[r * 5 + cc].im);
+          s1 += fabs(R[r * 5 + cc]);
+          s2 += fabs(Ri[r * 5 + cc]);
         }
         nR = fmax(nR, s1);
         nRi = fmax(nRi, s2);
@@ -633,27 +679,28 @@ __global__ void k_characteristics(CaseDims d, const double *__restrict__ recipe,
     return;
   }
   double maxSpeed = 1.0;
+#pragma unroll
   for (int k = 0; k < 5; ++k) maxSpeed = fmax(maxSpeed, fabs(c[k]));
   const double tol = 1000.0 * kEps * maxSpeed;
   int inc = 0, neu = 0;
+#pragma unroll
   for (int k = 0; k < 5; ++k) {
     const bool in = c[k] < -tol;
     out[100 + k] = in ? 1.0 : 0.0;
     inc += in;
     neu += fabs(c[k]) <= tol;
   }
+#pragma unroll
   for (int r = 0; r < 5; ++r)
+#pragma unroll
     for (int cc = 0; cc < 5; ++cc) {
-      double sr = 0.0, si = 0.0;
-      for (int k = 0; k < 5; ++k) {
-        const dcx x = Ri[r * 5 + k], y = Gi[k * 5 + cc];
-        sr += x.re * y.re - x.im * y.im;
-        si += x.re * y.im + x.im * y.re;
-      }
+      double sr = 0.0;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) sr += Ri[r * 5 + k] * Gi[k * 5 + cc];
       out[2 * (r * 5 + cc)] = sr;
-      out[2 * (r * 5 + cc) + 1] = si;
-      out[50 + 2 * (r * 5 + cc)] = Ri[r * 5 + cc].re;
-      out[50 + 2 * (r * 5 + cc) + 1] = Ri[r * 5 + cc].im;
+      out[2 * (r * 5 + cc) + 1] = 0.0;
+      out[50 + 2 * (r * 5 + cc)] = Ri[r * 5 + cc];
+      out[50 + 2 * (r * 5 + cc) + 1] = 0.0;
     }
   out[105] = 0.0;
   atomicAdd(&diag->incoming, inc);
@@ -837,49 +884,87 @@ __global__ __launch_bounds__(kThreads) void k_operator(
     return;
   }
 
-  // phase 2: stream the NPW block rows out (contiguous in the value array)
-  const long long nodes_here = d.nodes - first < NPW ? d.nodes - first : NPW;
-  const int total = static_cast<int>(nodes_here) * PER * 25;
-  dcx *out = reinterpret_cast<dcx *>(vals) + first * PER * 25;
-  for (int v = tid; v < total; v += kThreads) {
-    const int n = v / (PER * 25);
-    const int rem = v - n * (PER * 25);
-    const int b = rem / 25, e = rem - 25 * (rem / 25);
-    const int tr = b / SSX, s = b - SSX * (b / SSX);
-    const NodeRec<SSX, SSE> &R = rec[n];
-    dcx val;
-    if (!constrained) {
-      val = l_entry(R, tr, s, e);
-    } else {
+  // phase 2: wave w streams block row `first + w` out.  Lanes 0..49 each own
+  // one entry e = (r, c) of a 5x5 block and walk the row two blocks at a time
+  // (lanes 0..24 the even blocks, 25..49 the odd ones), so each store
+  // instruction writes 50 x 16 B contiguous bytes and the entry's coefficient
+  // values stay in registers for the whole row.
+  static_assert(kThreads / 64 == NPW, "one wave per block row");
+  {
+    const int w = tid >> 6, lane = tid & 63;
+    const long long node = first + w;
+    if (node < d.nodes && lane < 50) {
+      const NodeRec<SSX, SSE> &R = rec[w];
+      const int half = lane >= 25 ? 1 : 0;
+      const int e = lane - 25 * half;
       const int r = e / 5, c = e - 5 * (e / 5);
-      const bool diagblk = R.slot_of_rank[tr] == R.t0 && s == R.s0;
-      if (R.kind == 1 && r >= 1) {
-        // isothermal no-slip wall: u, v, w, T rows become identity rows
-        val = dcx{(diagblk && r == c) ? 1.0 : 0.0, 0.0};
-      } else if (R.kind == 2) {
-        if (R.incoming[r]) {
-          // incoming characteristic amplitude R^-1 q = 0 on the node itself
-          val = diagblk ? R.Ri[r * 5 + c] : dcx{0.0, 0.0};
-        } else {
-          // projection P = R^-1 Gamma^-1 of the five rows of -L
-          double sr = 0.0, si = 0.0;
-          for (int k = 0; k < 5; ++k) {
-            const dcx pk = R.P[r * 5 + k];
-            if (pk.re == 0.0 && pk.im == 0.0) continue;
-            const dcx lv = l_entry(R, tr, s, k * 5 + c);
-            const double ar = lv.re * -1.0, ai = lv.im * -1.0;
-            sr += pk.re * ar - pk.im * ai;
-            si += pk.re * ai + pk.im * ar;
+      const dcx K0 = R.K0[e], Kx = R.Kxi[e], Ke = R.Keta[e];
+      const double Vxx = R.Vxx[e], Vxe = R.Vxe[e], Vee = R.Vee[e];
+      const int kind = constrained ? R.kind : 0;
+      const int t0 = R.t0, s0 = R.s0;
+      double2v *out = reinterpret_cast<double2v *>(vals) + node * PER * 25;
+      for (int k = 0; k < (PER + 1) / 2; ++k) {
+        const int braw = 2 * k + half;
+        const bool valid = braw < PER;
+        const int b = valid ? braw : PER - 1;
+        const int tr = b / SSX, s = b - SSX * (b / SSX);
+        const int t = R.slot_of_rank[tr];
+        // L entry, accumulation order K0, xi, eta, mixed (GlobalOperator.cpp:323-365)
+        double vr = 0.0, vi = 0.0;
+        if (t == t0) {
+          if (s == s0) {
+            vr = vr + K0.re;
+            vi = vi + K0.im;
           }
-          val = dcx{sr, si};
+          const double wv = R.w1x[s];
+          double ar = Kx.re * wv;
+          const double ai = Kx.im * wv;
+          ar = ar + Vxx * R.mw2x[s];
+          vr = vr + ar;
+          vi = vi + ai;
         }
-      } else {
-        const dcx lv = l_entry(R, tr, s, e);
-        val = dcx{lv.re * -1.0, lv.im * -1.0};
+        if (s == s0) {
+          const double wv = R.w1e[t];
+          double ar = Ke.re * wv;
+          const double ai = Ke.im * wv;
+          ar = ar + Vee * R.mw2e[t];
+          vr = vr + ar;
+          vi = vi + ai;
+        }
+        vr = vr + Vxe * (-R.w1e[t] * R.w1x[s]);
+        double2v val;
+        if (!constrained) {
+          val = double2v{vr, vi};
+        } else {
+          const double ar = vr * -1.0, ai = vi * -1.0;  // A_bc = -L
+          const bool diagblk = t == t0 && s == s0;
+          if (kind == 1 && r >= 1) {
+            // isothermal no-slip wall: u, v, w, T rows become identity rows
+            val = double2v{(diagblk && r == c) ? 1.0 : 0.0, 0.0};
+          } else if (kind == 2) {
+            // far field: row r = P[r,:] (rows of -L) unless mode r is incoming
+            double sr = 0.0, si = 0.0;
+            for (int q = 0; q < 5; ++q) {
+              const int src = half * 25 + q * 5 + c;
+              const double xr = __shfl(ar, src), xi = __shfl(ai, src);
+              const dcx pk = R.P[r * 5 + q];
+              if (pk.re == 0.0 && pk.im == 0.0) continue;
+              sr += pk.re * xr - pk.im * xi;
+              si += pk.re * xi + pk.im * xr;
+            }
+            if (R.incoming[r]) {
+              const dcx ri = R.Ri[r * 5 + c];
+              val = diagblk ? double2v{ri.re, ri.im} : double2v{0.0, 0.0};
+            } else {
+              val = double2v{sr, si};
+            }
+          } else {
+            val = double2v{ar, ai};
+          }
+        }
+        if (valid) __builtin_nontemporal_store(val, out + b * 25 + e);
       }
     }
-    __builtin_nontemporal_store(val.re, &out[v].re);
-    __builtin_nontemporal_store(val.im, &out[v].im);
   }
 
   // the block-diagonal M_Gamma / B_bc
@@ -970,8 +1055,9 @@ hipError_t launch_metrics(tailor_ctx *ctx) {
 
 hipError_t launch_derivatives(tailor_ctx *ctx) {
   const CaseDims &d = ctx->dims;
-  const unsigned grid = static_cast<unsigned>((d.nodes + kThreads - 1) / kThreads);
-  hipLaunchKernelGGL(k_metrics2_derivatives, dim3(grid), dim3(kThreads), 0, ctx->stream, d,
+  const dim3 grid((d.ny + DTX - 1) / DTX, (d.nz + DTY - 1) / DTY);
+  const size_t lds = sizeof(double) * 9 * static_cast<size_t>(DTX + 2 * d.hx) * (DTY + 2 * d.hy);
+  hipLaunchKernelGGL(k_metrics2_derivatives, grid, dim3(DTHREADS), lds, ctx->stream, d,
                      ctx->d_xoff, ctx->d_xw, ctx->d_eoff, ctx->d_ew, ctx->d_base, ctx->d_metrics,
                      ctx->d_deriv, ctx->d_diag);
   return hipGetLastError();

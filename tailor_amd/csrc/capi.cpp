@@ -10,6 +10,8 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "solver.hpp"
+#include "vec.hpp"
 
 using tailor::fail;
 
@@ -47,6 +49,8 @@ void dfree(T *&p) {
 }
 
 void free_case(tailor_ctx *ctx) {
+  tailor::solver_release(ctx->solver);
+  ctx->solver = nullptr;
   dfree(ctx->d_xoff);
   dfree(ctx->d_xw);
   dfree(ctx->d_eoff);
@@ -266,6 +270,11 @@ int tailor_load_case(tailor_ctx *ctx, const tailor_case *c) {
   std::vector<int> xoff(static_cast<size_t>(ny) * ssx), eoff(static_cast<size_t>(nz) * sse);
   for (size_t k = 0; k < xoff.size(); ++k) xoff[k] = static_cast<int>(c->xi.stencil_offsets[k]);
   for (size_t k = 0; k < eoff.size(); ++k) eoff[k] = static_cast<int>(c->eta.stencil_offsets[k]);
+  d.hx = d.hy = 0;
+  for (int v : xoff) d.hx = std::max(d.hx, std::abs(v));
+  for (int v : eoff) d.hy = std::max(d.hy, std::abs(v));
+  if (static_cast<size_t>(9 * 8) * (16 + 2 * d.hx) * (8 + 2 * d.hy) > 48 * 1024)
+    return fail(ctx, TAILOR_ERR_ARG_OUTOFRANGE, "FD-q stencil halo too wide for the LDS tile");
   for (int j = 0; j < nz; ++j) {
     std::vector<int> cols;
     for (int t = 0; t < sse; ++t) cols.push_back(static_cast<int>(c->eta.stencil_indices[j * sse + t]));
@@ -290,6 +299,8 @@ int tailor_load_case(tailor_ctx *ctx, const tailor_case *c) {
     }
   }
   ctx->h_rowptr[d.nodes] = static_cast<int>(d.nnzb);
+  ctx->h_xoff = xoff;
+  ctx->h_eoff = eoff;
 
   HIP_TRY(ctx, dalloc(&ctx->d_xoff, xoff.size()));
   HIP_TRY(ctx, dalloc(&ctx->d_eoff, eoff.size()));
@@ -328,6 +339,7 @@ int tailor_set_alpha(tailor_ctx *ctx, double alpha_re, double alpha_im) {
   ctx->alpha_re = alpha_re;
   ctx->alpha_im = alpha_im;
   ctx->assembled = false;
+  if (ctx->solver) ctx->solver->factored = false;
   return TAILOR_OK;
 }
 
@@ -391,6 +403,7 @@ static int run_assembly(tailor_ctx *ctx, bool constrained, tailor::DeviceDiag *h
   if (host_diag) *host_diag = g;
   ctx->vals_kind = constrained ? TAILOR_OP_A : TAILOR_OP_L;
   ctx->mvals_kind = constrained ? TAILOR_OP_B : TAILOR_OP_M;
+  if (ctx->solver) ctx->solver->factored = false;
   if (int st = check_assembly_flags(ctx, g)) return st;
   ctx->assembled = true;
   return TAILOR_OK;
@@ -528,6 +541,106 @@ int tailor_time_assembly(tailor_ctx *ctx, int reps, double *write_kernel_ms,
   tailor::DeviceDiag g{};
   HIP_TRY(ctx, hipMemcpy(&g, ctx->d_diag, sizeof g, hipMemcpyDeviceToHost));
   return check_assembly_flags(ctx, g);
+}
+
+int tailor_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, tailor_factor_diag *diag) {
+  if (!ctx) return TAILOR_ERR_USER_INPUT;
+  if (!std::isfinite(sigma_re) || !std::isfinite(sigma_im))
+    return fail(ctx, TAILOR_ERR_USER_INPUT, "shift must be finite");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (int st = tailor::solver_factor(ctx, tailor::cplx(sigma_re, sigma_im))) return st;
+  if (diag) {
+    const tailor::SolverState &S = *ctx->solver;
+    diag->fronts = S.sym.nsn;
+    diag->levels = static_cast<int32_t>(S.sym.levels.size());
+    int maxm = 0;
+    for (int m : S.sym.m) maxm = std::max(maxm, m);
+    diag->max_front = maxm;
+    diag->perturbed_pivots = S.perturbed;
+    diag->arena_bytes = S.plan.arena_doubles * 8;
+    diag->factor_entries = S.sym.factor_entries;
+    diag->flops = static_cast<double>(S.sym.flops);
+    diag->symbolic_ms = S.symbolic_ms;
+    diag->factor_ms = S.factor_ms;
+  }
+  return TAILOR_OK;
+}
+
+int tailor_linear_solve(tailor_ctx *ctx, const double *b, double *x, int32_t refine_max,
+                        double refine_tol, double *relres, int32_t *steps) {
+  if (!ctx || !b || !x) return TAILOR_ERR_USER_INPUT;
+  if (!ctx->solver || !ctx->solver->factored)
+    return fail(ctx, TAILOR_ERR_ARG_WRONGSTATE, "tailor_factor must precede tailor_linear_solve");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  int st_steps = 0;
+  const int st = tailor::solver_linear_solve(ctx, b, x, refine_max, refine_tol, relres, &st_steps);
+  if (steps) *steps = st_steps;
+  return st;
+}
+
+int tailor_solve(tailor_ctx *ctx, const tailor_solve_opts *opts, tailor_solve_diag *diag) {
+  if (!ctx || !opts) return TAILOR_ERR_USER_INPUT;
+  if (opts->nev <= 0 || !(opts->tol > 0.0) || opts->maxit <= 0)
+    return fail(ctx, TAILOR_ERR_USER_INPUT,
+                "EigenSolver counts, tolerance, and iteration limit must be positive");
+  if (!std::isfinite(opts->omega_re) || !std::isfinite(opts->omega_im))
+    return fail(ctx, TAILOR_ERR_USER_INPUT, "EigenSolver.SearchCenterOmega must be finite");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  // lambda_s = Im(omega_s) - i Re(omega_s)  (EigenSolver.cpp:46-48)
+  const tailor::cplx sigma(opts->omega_im, -opts->omega_re);
+  if (!ctx->solver || !ctx->solver->factored || ctx->solver->sigma != sigma)
+    if (int st = tailor::solver_factor(ctx, sigma)) return st;
+  tailor_solve_diag local{};
+  const int st = tailor::solver_krylov_schur(ctx, *opts, local);
+  if (diag) *diag = local;
+  return st;
+}
+
+int tailor_eigenpair_count(const tailor_ctx *ctx) {
+  return ctx && ctx->solver ? ctx->solver->nritz : 0;
+}
+
+int tailor_get_eigenpair(tailor_ctx *ctx, int32_t k, double *lambda, double *q, double *residual,
+                         double *relative_error) {
+  if (!ctx) return TAILOR_ERR_USER_INPUT;
+  if (!ctx->solver || k < 0 || k >= ctx->solver->nritz)
+    return fail(ctx, TAILOR_ERR_ARG_OUTOFRANGE, "eigenpair index out of range");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  tailor::SolverState &S = *ctx->solver;
+  if (lambda) {
+    lambda[0] = S.lambda[k].real();
+    lambda[1] = S.lambda[k].imag();
+  }
+  if (residual) *residual = S.resid[k];
+  if (relative_error) *relative_error = S.relerr[k];
+  if (q) {
+    const long long n = S.n;
+    double *nat = S.work + 10 * n;
+    double *Rr = S.ritz + k * n, *Ri = S.ritz + n * S.ritz_ld + k * n;
+    HIP_TRY(ctx, tailor::vec_elim_to_nat(ctx->stream, n, S.plan.d_perm, Rr, Ri, nat));
+    HIP_TRY(ctx, hipMemcpyAsync(q, nat, 2 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    // unit 2-norm, largest entry (lowest index on ties) rotated to the
+    // positive real axis (EigenOutput.cpp:36-82)
+    double nrm = 0.0, best = -1.0;
+    long long at = 0;
+    for (long long i = 0; i < n; ++i) {
+      const double a = std::hypot(q[2 * i], q[2 * i + 1]);
+      nrm += a * a;
+      if (a > best) {
+        best = a;
+        at = i;
+      }
+    }
+    nrm = std::sqrt(nrm);
+    const double pr = q[2 * at] / best, pi = q[2 * at + 1] / best;  // phase of the pivot
+    for (long long i = 0; i < n; ++i) {
+      const double xr = q[2 * i], xi = q[2 * i + 1];
+      q[2 * i] = (xr * pr + xi * pi) / nrm;
+      q[2 * i + 1] = (xi * pr - xr * pi) / nrm;
+    }
+  }
+  return TAILOR_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,436 @@
+// eigen.cpp -- shift-invert Krylov-Schur for A_bc q = lambda B_bc q on the
+// GPU, replacing SLEPc's EPSKRYLOVSCHUR + STSINVERT + PCLU path
+// (EigenSolver.cpp:31-89).
+//
+// Operator: Op = (A - sigma B)^-1 B with sigma = lambda_s = Im(omega_s) -
+// i Re(omega_s) (EigenSolver.cpp:46-48); Ritz values theta of Op map to
+// lambda = sigma + 1/theta, and the wanted pairs are those of largest
+// |theta| (nearest sigma, EPS_TARGET_MAGNITUDE).  Krylov-Schur restarts
+// (G. W. Stewart, SIAM J. Matrix Anal. Appl. 23 (2001) 601-614): Arnoldi with
+// classical Gram-Schmidt and one reorthogonalisation, complex Schur form of
+// the projected matrix, sorting by |theta|, restart keeping the converged
+// pairs plus half of the rest (SLEPc's default restart ratio 0.5).  Pairs
+// are accepted by their true residual ||A q - lambda B q|| / ||q||.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "dense.hpp"
+#include "internal.hpp"
+#include "lu.hpp"
+#include "solver.hpp"
+#include "vec.hpp"
+
+namespace tailor {
+
+#define HIP_OK(expr)                                     \
+  do {                                                   \
+    hipError_t e_ = (expr);                              \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+  } while (0)
+
+namespace {
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+template <class T>
+hipError_t grow(T **p, size_t &cap, size_t need) {
+  if (*p && cap >= need) return hipSuccess;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  cap = 0;
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(p), need * sizeof(T));
+  if (e == hipSuccess) cap = need;
+  return e;
+}
+
+}  // namespace
+
+void solver_release(SolverState *S) {
+  if (!S) return;
+  lu_plan_destroy(S->plan);
+  double *bufs[] = {S->V, S->work, S->ritz, S->partials, S->small};
+  for (double *b : bufs)
+    if (b) (void)hipFree(b);
+  int *ib[] = {S->d_order, S->d_pos};
+  for (int *b : ib)
+    if (b) (void)hipFree(b);
+  delete S;
+}
+
+// Symbolic analysis and device plan for the loaded case (once per case).
+int solver_prepare(tailor_ctx *ctx) {
+  if (!ctx->solver) ctx->solver = new SolverState();
+  SolverState &S = *ctx->solver;
+  if (S.ready) return TAILOR_OK;
+  const double t0 = now_ms();
+  const auto &d = ctx->dims;
+  const std::vector<int> &xoff = ctx->h_xoff, &eoff = ctx->h_eoff;
+  try {
+    S.sym = analyse(d.ny, d.nz, d.ssx, d.sse, xoff, eoff, 64);
+  } catch (const std::exception &ex) {
+    return fail(ctx, TAILOR_ERR_PLIB, std::string("symbolic analysis failed: ") + ex.what());
+  }
+  hipError_t e = lu_plan_create(S.sym, ctx->h_colidx, d.ssx * d.sse, S.plan, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "lu_plan_create");
+  std::vector<int> order(S.sym.order), pos(S.sym.pos);
+  HIP_OK(hipMalloc(reinterpret_cast<void **>(&S.d_order), order.size() * sizeof(int)));
+  HIP_OK(hipMalloc(reinterpret_cast<void **>(&S.d_pos), pos.size() * sizeof(int)));
+  HIP_OK(hipMemcpy(S.d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(S.d_pos, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice));
+  S.n = 5LL * d.nodes;
+  S.symbolic_ms = now_ms() - t0;
+  S.ready = true;
+  return TAILOR_OK;
+}
+
+int solver_factor(tailor_ctx *ctx, cplx sigma) {
+  if (!ctx->assembled || ctx->vals_kind != TAILOR_OP_A || ctx->mvals_kind != TAILOR_OP_B)
+    return fail(ctx, TAILOR_ERR_ARG_WRONGSTATE,
+                "Boundary-constrained matrices are required before EPSSolve");
+  if (int st = solver_prepare(ctx)) return st;
+  SolverState &S = *ctx->solver;
+  const auto &d = ctx->dims;
+  LuInputs in;
+  in.nnzb = d.nnzb;
+  in.per = d.ssx * d.sse;
+  in.avals = ctx->d_vals;
+  in.colidx = ctx->d_colidx;
+  in.bvals = ctx->d_mvals;
+  // pivot threshold and static perturbation, scaled by the RMS row norm of K
+  double ss[2];
+  HIP_OK(launch_sumsq(ctx, ctx->d_vals, d.nnzb * 50, 1, 1, ss));
+  HIP_OK(launch_sumsq(ctx, ctx->d_mvals, d.nodes * 50, 1, 1, ss + 1));
+  const double rows = 5.0 * static_cast<double>(d.nodes);
+  const double scale = std::max(1e-300, std::sqrt(ss[0] / rows) + std::abs(sigma) * std::sqrt(ss[1] / rows));
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  HIP_OK(hipEventRecord(e0, ctx->stream));
+  HIP_OK(lu_factor(S.plan, in, sigma.real(), sigma.imag(), 1e-14 * scale, 1.4901161193847656e-08 * scale,
+                   ctx->stream));
+  HIP_OK(hipEventRecord(e1, ctx->stream));
+  HIP_OK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  int counters[8];
+  HIP_OK(hipMemcpy(counters, S.plan.d_counters, sizeof counters, hipMemcpyDeviceToHost));
+  S.perturbed = counters[0];
+  S.factor_ms = ms;
+  S.sigma = sigma;
+  S.factored = true;
+  return TAILOR_OK;
+}
+
+namespace {
+
+// y = K^-1 b with optional iterative refinement (b, y: elimination order,
+// split planes of length n; scratch: 3 vectors).
+int refined_solve(tailor_ctx *ctx, const double *br, const double *bi, double *yr, double *yi,
+                  int refine_max, double refine_tol, int *steps, double *relres) {
+  SolverState &S = *ctx->solver;
+  const long long n = S.n;
+  double *rr = S.work + 6 * n, *ri = rr + n;  // residual
+  double *dr = S.work + 8 * n, *di = dr + n;  // correction
+  HIP_OK(lu_solve(S.plan, br, bi, yr, yi, ctx->stream));
+  if (steps) *steps = 0;
+  if (refine_max <= 0 && !relres) return TAILOR_OK;
+  const auto &d = ctx->dims;
+  double bnorm = -1.0;
+  for (int it = 0; it <= refine_max; ++it) {
+    // r = b - K y
+    HIP_OK(vec_kmul(ctx->stream, d.nodes, d.ssx * d.sse, ctx->d_vals, ctx->d_colidx, ctx->d_mvals,
+                    S.d_pos, S.sigma.real(), S.sigma.imag(), yr, yi, rr, ri));
+    HIP_OK(vec_axpby(ctx->stream, n, 1.0, 0.0, br, bi, -1.0, 0.0, rr, ri));
+    double nr = 0.0, nb = 0.0;
+    if (int st = solver_norm2(ctx, rr, ri, &nr)) return st;
+    if (bnorm < 0.0) {
+      if (int st = solver_norm2(ctx, br, bi, &nb)) return st;
+      bnorm = nb;
+    }
+    const double rel = bnorm > 0.0 ? nr / bnorm : nr;
+    if (relres) *relres = rel;
+    if (it == refine_max || rel <= refine_tol) break;
+    HIP_OK(lu_solve(S.plan, rr, ri, dr, di, ctx->stream));
+    HIP_OK(vec_axpby(ctx->stream, n, 1.0, 0.0, dr, di, 1.0, 0.0, yr, yi));
+    if (steps) ++*steps;
+  }
+  return TAILOR_OK;
+}
+
+}  // namespace
+
+int solver_norm2(tailor_ctx *ctx, const double *xr, const double *xi, double *out) {
+  SolverState &S = *ctx->solver;
+  std::vector<cplx> h(1);
+  if (int st = solver_dots(ctx, xr, xi, S.n, 1, xr, xi, h.data())) return st;
+  *out = std::sqrt(std::max(0.0, h[0].real()));
+  return TAILOR_OK;
+}
+
+int solver_dots(tailor_ctx *ctx, const double *Vr, const double *Vi, long long ldv, int ncols,
+                const double *wr, const double *wi, cplx *h) {
+  SolverState &S = *ctx->solver;
+  HIP_OK(vec_dots(ctx->stream, S.n, ncols, Vr, Vi, ldv, wr, wi, S.partials, kDotBlocks));
+  std::vector<double> part(2 * static_cast<size_t>(kDotBlocks) * ncols);
+  HIP_OK(hipMemcpyAsync(part.data(), S.partials, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  for (int j = 0; j < ncols; ++j) {
+    double sr = 0.0, si = 0.0;
+    for (int b = 0; b < kDotBlocks; ++b) {
+      sr += part[2 * (static_cast<size_t>(b) * ncols + j)];
+      si += part[2 * (static_cast<size_t>(b) * ncols + j) + 1];
+    }
+    h[j] = cplx(sr, si);
+  }
+  return TAILOR_OK;
+}
+
+int solver_linear_solve(tailor_ctx *ctx, const double *b_nat, double *x_nat, int refine_max,
+                        double refine_tol, double *relres, int *steps) {
+  SolverState &S = *ctx->solver;
+  const long long n = S.n;
+  HIP_OK(grow(&S.work, S.work_cap, static_cast<size_t>(12 * n)));
+  HIP_OK(grow(&S.partials, S.partials_cap, static_cast<size_t>(2 * kDotBlocks * 64)));
+  double *nat = S.work + 10 * n;  // interleaved natural vector (2n)
+  double *br = S.work, *bi = br + n, *xr = S.work + 2 * n, *xi = xr + n;
+  HIP_OK(hipMemcpyAsync(nat, b_nat, 2 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  HIP_OK(vec_nat_to_elim(ctx->stream, n, S.plan.d_perm, nat, br, bi));
+  if (int st = refined_solve(ctx, br, bi, xr, xi, refine_max, refine_tol, steps, relres)) return st;
+  HIP_OK(vec_elim_to_nat(ctx->stream, n, S.plan.d_perm, xr, xi, nat));
+  HIP_OK(hipMemcpyAsync(x_nat, nat, 2 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  return TAILOR_OK;
+}
+
+// ----------------------------------------------------------- Krylov-Schur
+
+int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solve_diag &diag) {
+  SolverState &S = *ctx->solver;
+  const long long n = S.n;
+  const int nev = o.nev;
+  int ncv = o.ncv > 0 ? o.ncv : std::max(2 * nev, nev + 15);
+  ncv = std::min<long long>(ncv, n - 1);
+  if (ncv > 60) ncv = 60;
+  if (nev <= 0 || nev >= ncv) return fail(ctx, TAILOR_ERR_USER_INPUT, "invalid nev / ncv");
+  HIP_OK(grow(&S.V, S.V_cap, static_cast<size_t>(2 * n * (ncv + 1))));
+  HIP_OK(grow(&S.work, S.work_cap, static_cast<size_t>(12 * n)));
+  HIP_OK(grow(&S.partials, S.partials_cap, static_cast<size_t>(2 * kDotBlocks * 64)));
+  HIP_OK(grow(&S.small, S.small_cap, static_cast<size_t>(2 * 64 * 64)));
+  HIP_OK(grow(&S.ritz, S.ritz_cap, static_cast<size_t>(2 * n * ncv)));
+  double *Vr = S.V, *Vi = S.V + n * (ncv + 1);
+  auto col_r = [&](int j) { return Vr + static_cast<long long>(j) * n; };
+  auto col_i = [&](int j) { return Vi + static_cast<long long>(j) * n; };
+  double *wr = S.work, *wi = wr + n;     // Arnoldi vector
+  double *tr = S.work + 2 * n, *ti = tr + n;  // B v
+  double *yr = S.work + 4 * n, *yi = yr + n;  // residual checks
+  const auto &d = ctx->dims;
+
+  // deterministic start vector (SplitMix64 stream, elimination order)
+  {
+    std::vector<double> v0(2 * n);
+    unsigned long long st = 0x5eed1234ULL;
+    for (long long k = 0; k < 2 * n; ++k) {
+      st += 0x9E3779B97F4A7C15ULL;
+      unsigned long long z = st;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+      z ^= z >> 31;
+      v0[k] = static_cast<double>(z >> 11) * 0x1.0p-53 - 0.5;
+    }
+    HIP_OK(hipMemcpyAsync(col_r(0), v0.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_OK(hipMemcpyAsync(col_i(0), v0.data() + n, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    double nv;
+    if (int s2 = solver_norm2(ctx, col_r(0), col_i(0), &nv)) return s2;
+    HIP_OK(vec_axpby(ctx->stream, n, 0.0, 0.0, col_r(0), col_i(0), 1.0 / nv, 0.0, col_r(0), col_i(0)));
+  }
+
+  CMat Sm(ncv + 1, ncv);  // projected matrix (rows 0..ncv)
+  int k = 0;
+  int solves = 0, refine_total = 0, restarts = 0;
+  std::vector<cplx> theta;
+  CMat T, Q;
+  int nconv = 0;
+  std::vector<double> resid(ncv, 1.0), relerr(ncv, 1.0);
+  const double t0 = now_ms();
+  const double tol = o.tol;
+  double best_res = 1e300;
+  int stagnant = 0;
+  for (restarts = 0; restarts < o.maxit; ++restarts) {
+    for (int j = k; j < ncv; ++j) {
+      // w = Op v_j
+      HIP_OK(vec_bmul(ctx->stream, d.nodes, S.d_order, ctx->d_mvals, col_r(j), col_i(j), tr, ti));
+      int steps = 0;
+      if (int st = refined_solve(ctx, tr, ti, wr, wi, o.refine_max, o.refine_tol, &steps, nullptr))
+        return st;
+      ++solves;
+      refine_total += steps;
+      // CGS with one reorthogonalisation
+      std::vector<cplx> h(j + 1), h2(j + 1);
+      if (int st = solver_dots(ctx, Vr, Vi, n, j + 1, wr, wi, h.data())) return st;
+      HIP_OK(hipMemcpyAsync(S.small, h.data(), (j + 1) * sizeof(cplx), hipMemcpyHostToDevice, ctx->stream));
+      HIP_OK(vec_update(ctx->stream, n, j + 1, Vr, Vi, n, S.small, wr, wi));
+      if (int st = solver_dots(ctx, Vr, Vi, n, j + 1, wr, wi, h2.data())) return st;
+      HIP_OK(hipMemcpyAsync(S.small, h2.data(), (j + 1) * sizeof(cplx), hipMemcpyHostToDevice, ctx->stream));
+      HIP_OK(vec_update(ctx->stream, n, j + 1, Vr, Vi, n, S.small, wr, wi));
+      double beta;
+      if (int st = solver_norm2(ctx, wr, wi, &beta)) return st;
+      for (int i = 0; i <= j; ++i) Sm(i, j) = h[i] + h2[i];
+      Sm(j + 1, j) = beta;
+      if (beta == 0.0) return fail(ctx, TAILOR_ERR_NOT_CONVERGED, "Krylov subspace broke down");
+      HIP_OK(vec_axpby(ctx->stream, n, 1.0 / beta, 0.0, wr, wi, 0.0, 0.0, col_r(j + 1), col_i(j + 1)));
+    }
+    // Schur form of the projected matrix, sorted by |theta| descending
+    T = CMat(ncv, ncv);
+    for (int jj = 0; jj < ncv; ++jj)
+      for (int ii = 0; ii < ncv; ++ii) T(ii, jj) = Sm(ii, jj);
+    if (!schur_decompose(T, Q)) return fail(ctx, TAILOR_ERR_NOT_CONVERGED, "projected Schur form failed");
+    for (int pos = 0; pos < ncv; ++pos) {
+      int best = pos;
+      for (int q = pos + 1; q < ncv; ++q)
+        if (std::abs(T(q, q)) > std::abs(T(best, best))) best = q;
+      if (best != pos) {
+        std::vector<bool> sel(ncv, false);
+        // move `best` to `pos`: select it together with the already placed ones
+        for (int q = 0; q < pos; ++q) sel[q] = true;
+        sel[best] = true;
+        schur_reorder(T, Q, sel);
+      }
+    }
+    // coupling row b = S(ncv, :) Q
+    std::vector<cplx> b(ncv, 0.0);
+    for (int jj = 0; jj < ncv; ++jj) {
+      cplx s = 0.0;
+      for (int q = 0; q < ncv; ++q) s += Sm(ncv, q) * Q(q, jj);
+      b[jj] = s;
+    }
+    theta.assign(ncv, 0.0);
+    for (int i = 0; i < ncv; ++i) theta[i] = T(i, i);
+    // Ritz estimates and true residuals of the leading pairs
+    nconv = 0;
+    const int ncheck = std::min(ncv - 1, nev + 2);
+    std::vector<std::vector<cplx>> Y(ncheck);
+    bool all_est_ok = true;
+    for (int i = 0; i < ncheck; ++i) {
+      Y[i] = triangular_eigenvector(T, i);
+      cplx s = 0.0;
+      for (int q = 0; q < ncv; ++q) s += b[q] * Y[i][q];
+      const double est = std::abs(s) / std::abs(theta[i]);
+      if (i < nev && est > tol) all_est_ok = false;
+    }
+    if (all_est_ok || restarts == o.maxit - 1) {
+      // Ritz vectors x_i = V Q y_i and their true residuals
+      CMat QY(ncv, ncheck);
+      for (int i = 0; i < ncheck; ++i)
+        for (int q = 0; q < ncv; ++q) {
+          cplx s = 0.0;
+          for (int r = 0; r < ncv; ++r) s += Q(q, r) * Y[i][r];
+          QY(q, i) = s;
+        }
+      HIP_OK(hipMemcpyAsync(S.small, QY.a.data(), QY.a.size() * sizeof(cplx), hipMemcpyHostToDevice, ctx->stream));
+      double *Rr = S.ritz, *Ri = S.ritz + n * ncv;
+      HIP_OK(vec_combine(ctx->stream, n, ncv, ncheck, Vr, Vi, n, S.small, Rr, Ri, n));
+      for (int i = 0; i < ncheck; ++i) {
+        const cplx lam = S.sigma + 1.0 / theta[i];
+        HIP_OK(vec_kmul(ctx->stream, d.nodes, d.ssx * d.sse, ctx->d_vals, ctx->d_colidx, ctx->d_mvals,
+                        S.d_pos, lam.real(), lam.imag(), Rr + i * n, Ri + i * n, yr, yi));
+        double nr, nx;
+        if (int st = solver_norm2(ctx, yr, yi, &nr)) return st;
+        if (int st = solver_norm2(ctx, Rr + i * n, Ri + i * n, &nx)) return st;
+        resid[i] = nr / nx;
+        relerr[i] = resid[i] / std::max(std::abs(lam), 1e-300);
+      }
+      nconv = 0;
+      while (nconv < ncheck && resid[nconv] <= tol) ++nconv;
+      if (nconv >= nev || restarts == o.maxit - 1) break;
+      // the Ritz estimates are below tol but the true residuals are not:
+      // stop when they no longer improve (accuracy floor of the solves)
+      double worst = 0.0;
+      for (int i = 0; i < nev; ++i) worst = std::max(worst, resid[i]);
+      if (worst < 0.5 * best_res) {
+        best_res = worst;
+        stagnant = 0;
+      } else if (++stagnant >= 3) {
+        break;
+      }
+    }
+    // restart: keep the leading l Schur vectors
+    int conv_est = 0;
+    for (int i = 0; i < ncheck; ++i) {
+      cplx s = 0.0;
+      for (int q = 0; q < ncv; ++q) s += b[q] * Y[i][q];
+      if (std::abs(s) / std::abs(theta[i]) <= tol) ++conv_est;
+      else break;
+    }
+    int l = conv_est + std::max(1, (ncv - conv_est) / 2);
+    l = std::min(l, ncv - 1);
+    {
+      CMat Ql(ncv, l);
+      for (int jj = 0; jj < l; ++jj)
+        for (int q = 0; q < ncv; ++q) Ql(q, jj) = Q(q, jj);
+      HIP_OK(hipMemcpyAsync(S.small, Ql.a.data(), Ql.a.size() * sizeof(cplx), hipMemcpyHostToDevice, ctx->stream));
+      double *Rr = S.ritz, *Ri = S.ritz + n * ncv;
+      HIP_OK(vec_combine(ctx->stream, n, ncv, l, Vr, Vi, n, S.small, Rr, Ri, n));
+      HIP_OK(hipMemcpyAsync(Vr, Rr, l * n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      HIP_OK(hipMemcpyAsync(Vi, Ri, l * n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      HIP_OK(hipMemcpyAsync(col_r(l), col_r(ncv), n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      HIP_OK(hipMemcpyAsync(col_i(l), col_i(ncv), n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    Sm = CMat(ncv + 1, ncv);
+    for (int jj = 0; jj < l; ++jj) {
+      for (int ii = 0; ii <= jj; ++ii) Sm(ii, jj) = T(ii, jj);
+      Sm(l, jj) = b[jj];
+    }
+    k = l;
+  }
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  // results: leading pairs in order of |theta|
+  S.lambda.clear();
+  S.resid.clear();
+  S.relerr.clear();
+  const int nout = std::min(ncv - 1, std::max(nconv, 0));
+  for (int i = 0; i < nout; ++i) {
+    S.lambda.push_back(S.sigma + 1.0 / theta[i]);
+    S.resid.push_back(resid[i]);
+    S.relerr.push_back(relerr[i]);
+  }
+  S.nritz = nout;
+  S.ritz_ld = ncv;
+  diag.requested = nev;
+  diag.converged = nconv;
+  diag.iterations = restarts + 1;
+  diag.solves = solves;
+  diag.refinement_steps = refine_total;
+  diag.perturbed_pivots = S.perturbed;
+  diag.lambda_target_re = S.sigma.real();
+  diag.lambda_target_im = S.sigma.imag();
+  diag.max_residual = 0.0;
+  diag.max_relative_error = 0.0;
+  for (int i = 0; i < std::min(nconv, nev); ++i) {
+    diag.max_residual = std::max(diag.max_residual, resid[i]);
+    diag.max_relative_error = std::max(diag.max_relative_error, relerr[i]);
+  }
+  diag.solve_ms = now_ms() - t0;
+  diag.factor_ms = S.factor_ms;
+  diag.symbolic_ms = S.symbolic_ms;
+  diag.fronts = S.sym.nsn;
+  diag.factor_bytes = static_cast<int64_t>(S.plan.arena_doubles) * 8;
+  diag.factor_flops = static_cast<double>(S.sym.flops);
+  if (nconv < nev)
+    return fail(ctx, TAILOR_ERR_NOT_CONVERGED,
+                "Krylov-Schur failed to converge (" + std::to_string(nconv) + " eigenpairs after " +
+                    std::to_string(restarts + 1) + " iterations)");
+  return TAILOR_OK;
+}
+
+}  // namespace tailor

@@ -11,6 +11,8 @@
 
 namespace tailor {
 
+struct SolverState;
+
 struct dcx {
   double re, im;
 };
@@ -42,6 +44,7 @@ struct DeviceDiag {
 
 struct CaseDims {
   int ny, nz, ssx, sse;
+  int hx, hy;  // largest |stencil offset| in xi and eta
   long long nodes, nnzb;
 };
 
@@ -81,8 +84,11 @@ struct tailor_ctx {
   double *d_recipe = nullptr;    // Re, Ma, Pr, gamma, Tref, S
   double *d_partials = nullptr;  // reduction scratch, kPartials doubles
 
-  // host copy of the structure
+  // host copy of the structure and of the rule offsets
   std::vector<int> h_rowptr, h_colidx;
+  std::vector<int> h_xoff, h_eoff;
+  // factorization and eigen-solve state (solver.hpp)
+  tailor::SolverState *solver = nullptr;
 };
 
 namespace tailor {

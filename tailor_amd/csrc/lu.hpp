@@ -1,0 +1,81 @@
+// lu.hpp -- device-side plan of the multifrontal LU and its launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "symbolic.hpp"
+
+namespace tailor {
+
+// Per-front descriptor read by the kernels.
+struct FrontDev {
+  long long off;   // arena offset (doubles)
+  long long woff;  // workspace offset (per-front vector of m entries)
+  int m, p, c;
+  int piv_off;     // offset into the pivot array
+  int dof0;        // first elimination-order dof of the front's pivots
+  int pad;
+};
+
+struct LevelPlan {
+  int list_off = 0, count = 0;
+  int pmax = 0, cmax = 0, mmax = 0;
+  int child_slot_off[2] = {0, 0};
+  int child_slot_count[2] = {0, 0};
+};
+
+// Everything the numeric phase needs, resident on the device.
+struct LuPlan {
+  int nsn = 0;
+  int nb = 32;
+  long long n = 0;              // unknowns
+  long long arena_doubles = 0;
+  long long w_len = 0;          // per-plane length of the front workspace
+  std::vector<LevelPlan> levels;
+  FrontDev *d_front = nullptr;
+  int *d_level_list = nullptr;
+  int *d_child_slots = nullptr;
+  int *d_parent = nullptr;
+  int *d_rel = nullptr;
+  long long *d_rel_off = nullptr;
+  int *d_child_ptr = nullptr;
+  int *d_child_list = nullptr;
+  int *d_upd_pos = nullptr;
+  long long *d_upd_off = nullptr;
+  int *d_amap = nullptr;        // 3 ints per BSR block
+  int *d_ipiv = nullptr;
+  int *d_counters = nullptr;    // [0] perturbed pivots
+  int *d_perm = nullptr;        // elimination dof -> natural dof (n entries)
+  double *arena = nullptr;
+  double *W = nullptr;          // 2 * w_len
+  double *Y = nullptr;          // 2 * n
+  double *T = nullptr;          // 2 * n
+};
+
+struct LuInputs {
+  long long nnzb = 0;
+  int per = 0;                  // blocks per block row
+  const double *avals = nullptr;  // A_bc, interleaved complex, 25 per block
+  const int *colidx = nullptr;
+  const double *bvals = nullptr;  // B_bc block diagonal, interleaved complex
+};
+
+constexpr int kLuNb = 32;
+
+// Build the device plan from the symbolic analysis and the BSR structure
+// (host arrays).  Allocates device memory; returns hipSuccess or an error.
+hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int per,
+                          LuPlan &P, hipStream_t s);
+void lu_plan_destroy(LuPlan &P);
+
+hipError_t lu_fill(hipStream_t s, double *p, long long n, double v);
+hipError_t lu_factor(const LuPlan &P, const LuInputs &in, double sig_re, double sig_im,
+                     double tiny, double perturb, hipStream_t s);
+// x = K^-1 b, vectors in elimination dof order, split planes (re, im)
+hipError_t lu_solve(const LuPlan &P, const double *br, const double *bi, double *xr, double *xi,
+                    hipStream_t s);
+
+}  // namespace tailor

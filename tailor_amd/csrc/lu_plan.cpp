@@ -1,0 +1,155 @@
+// lu_plan.cpp -- turn the symbolic analysis into device-resident maps for
+// the multifrontal factorization and solves.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "lu.hpp"
+
+namespace tailor {
+
+namespace {
+
+template <class T>
+hipError_t upload(T **dst, const std::vector<T> &src, hipStream_t s) {
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(dst), std::max<size_t>(1, src.size()) * sizeof(T));
+  if (e != hipSuccess) return e;
+  if (src.empty()) return hipSuccess;
+  return hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+template <class T>
+void release(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+}  // namespace
+
+hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int per, LuPlan &P,
+                          hipStream_t s) {
+  lu_plan_destroy(P);
+  P.nsn = S.nsn;
+  P.nb = kLuNb;
+  P.n = 5LL * S.nodes;
+  std::vector<FrontDev> fr(S.nsn);
+  long long off = 0, woff = 0;
+  int poff = 0;
+  for (int f = 0; f < S.nsn; ++f) {
+    FrontDev &d = fr[f];
+    d.m = S.m[f];
+    d.p = S.p[f];
+    d.c = S.c[f];
+    d.off = off;
+    d.woff = woff;
+    d.piv_off = poff;
+    d.dof0 = 5 * S.sn_start[f];
+    d.pad = 0;
+    off += 2LL * d.m * d.m + 4LL * d.p * d.p;
+    woff += d.m;
+    poff += d.p;
+  }
+  P.arena_doubles = off;
+  P.w_len = woff;
+
+  // level lists and child slots
+  std::vector<int> level_list, child_slots;
+  P.levels.assign(S.levels.size(), {});
+  for (size_t l = 0; l < S.levels.size(); ++l) {
+    LevelPlan &L = P.levels[l];
+    L.list_off = static_cast<int>(level_list.size());
+    L.count = static_cast<int>(S.levels[l].size());
+    for (int f : S.levels[l]) {
+      level_list.push_back(f);
+      L.pmax = std::max(L.pmax, S.p[f]);
+      L.cmax = std::max(L.cmax, S.c[f]);
+      L.mmax = std::max(L.mmax, S.m[f]);
+    }
+    for (int slot = 0; slot < 2; ++slot) {
+      L.child_slot_off[slot] = static_cast<int>(child_slots.size());
+      for (int f : S.levels[l])
+        if (static_cast<int>(S.children[f].size()) > slot) child_slots.push_back(S.children[f][slot]);
+      L.child_slot_count[slot] = static_cast<int>(child_slots.size()) - L.child_slot_off[slot];
+    }
+  }
+  std::vector<int> parent = S.parent, rel, child_ptr(S.nsn + 1, 0), child_list, upd_pos;
+  std::vector<long long> rel_off(S.nsn, 0), upd_off(S.nsn, 0);
+  for (int f = 0; f < S.nsn; ++f) {
+    if (S.children[f].size() > 2) return hipErrorInvalidValue;
+    rel_off[f] = static_cast<long long>(rel.size());
+    rel.insert(rel.end(), S.rel[f].begin(), S.rel[f].end());
+    child_ptr[f + 1] = child_ptr[f] + static_cast<int>(S.children[f].size());
+    child_list.insert(child_list.end(), S.children[f].begin(), S.children[f].end());
+    upd_off[f] = static_cast<long long>(upd_pos.size());
+    for (int v : S.strct[f]) upd_pos.push_back(S.pos[v]);
+  }
+  // assembly map: every stored block of A goes to the front of whichever of
+  // its row / column node is eliminated first
+  std::vector<int> sn_of_pos(S.nodes);
+  for (int f = 0; f < S.nsn; ++f)
+    for (int q = S.sn_start[f]; q < S.sn_start[f + 1]; ++q) sn_of_pos[q] = f;
+  const long long nnzb = static_cast<long long>(colidx.size());
+  std::vector<int> amap(3 * nnzb);
+  for (long long k = 0; k < nnzb; ++k) {
+    const int u = static_cast<int>(k / per), v = colidx[k];
+    const int f = sn_of_pos[std::min(S.pos[u], S.pos[v])];
+    const int lu = S.local_index(f, u), lv = S.local_index(f, v);
+    if (lu < 0 || lv < 0) return hipErrorInvalidValue;
+    amap[3 * k] = f;
+    amap[3 * k + 1] = lu;
+    amap[3 * k + 2] = lv;
+  }
+  std::vector<int> perm(P.n);
+  for (int q = 0; q < S.nodes; ++q)
+    for (int dof = 0; dof < 5; ++dof) perm[5 * q + dof] = 5 * S.order[q] + dof;
+
+  hipError_t e;
+  if ((e = upload(&P.d_front, fr, s)) != hipSuccess) return e;
+  if ((e = upload(&P.d_level_list, level_list, s)) != hipSuccess) return e;
+  if ((e = upload(&P.d_child_slots, child_slots, s)) != hipSuccess) return e;
+  if ((e = upload(&P.d_parent, parent, s)) != hipSuccess) return e;
+  if ((e = upload(&P.d_rel, rel, s)) != hipSuccess) return e;
+  if ((e = upload(&P.d_rel_off, rel_off, s)) != hipSuccess) return e;
+  if ((e = upload(&P.d_child_ptr, child_ptr, s)) != hipSuccess) return e;
+  if ((e = upload(&P.d_child_list, child_list, s)) != hipSuccess) return e;
+  if ((e = upload(&P.d_upd_pos, upd_pos, s)) != hipSuccess) return e;
+  if ((e = upload(&P.d_upd_off, upd_off, s)) != hipSuccess) return e;
+  if ((e = upload(&P.d_amap, amap, s)) != hipSuccess) return e;
+  if ((e = upload(&P.d_perm, perm, s)) != hipSuccess) return e;
+  std::vector<int> zeros(std::max(1, poff), 0);
+  if ((e = upload(&P.d_ipiv, zeros, s)) != hipSuccess) return e;
+  if ((e = upload(&P.d_counters, std::vector<int>(8, 0), s)) != hipSuccess) return e;
+  if ((e = hipMalloc(reinterpret_cast<void **>(&P.arena), P.arena_doubles * sizeof(double))) != hipSuccess)
+    return e;
+  if ((e = hipMalloc(reinterpret_cast<void **>(&P.W), 2 * P.w_len * sizeof(double))) != hipSuccess)
+    return e;
+  if ((e = hipMalloc(reinterpret_cast<void **>(&P.Y), 2 * P.n * sizeof(double))) != hipSuccess)
+    return e;
+  if ((e = hipMalloc(reinterpret_cast<void **>(&P.T), 2 * P.n * sizeof(double))) != hipSuccess)
+    return e;
+  return hipStreamSynchronize(s);
+}
+
+void lu_plan_destroy(LuPlan &P) {
+  release(P.d_front);
+  release(P.d_level_list);
+  release(P.d_child_slots);
+  release(P.d_parent);
+  release(P.d_rel);
+  release(P.d_rel_off);
+  release(P.d_child_ptr);
+  release(P.d_child_list);
+  release(P.d_upd_pos);
+  release(P.d_upd_off);
+  release(P.d_amap);
+  release(P.d_ipiv);
+  release(P.d_counters);
+  release(P.d_perm);
+  release(P.arena);
+  release(P.W);
+  release(P.Y);
+  release(P.T);
+  P.levels.clear();
+}
+
+}  // namespace tailor

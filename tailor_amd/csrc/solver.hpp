@@ -1,0 +1,43 @@
+// solver.hpp -- state of the factorization and eigen-solve of one context.
+#pragma once
+
+#include <complex>
+#include <vector>
+
+#include "../../include/tailor.h"
+#include "dense.hpp"
+#include "lu.hpp"
+#include "symbolic.hpp"
+
+namespace tailor {
+
+struct SolverState {
+  bool ready = false;     // symbolic analysis + device plan built
+  bool factored = false;
+  Symbolic sym;
+  LuPlan plan;
+  long long n = 0;
+  cplx sigma = 0.0;
+  int perturbed = 0;
+  double factor_ms = 0.0, symbolic_ms = 0.0;
+  int *d_order = nullptr, *d_pos = nullptr;
+  double *V = nullptr, *work = nullptr, *ritz = nullptr, *partials = nullptr, *small = nullptr;
+  size_t V_cap = 0, work_cap = 0, ritz_cap = 0, partials_cap = 0, small_cap = 0;
+  // results
+  int nritz = 0;
+  int ritz_ld = 0;  // columns per plane of the Ritz vector buffer
+  std::vector<cplx> lambda;
+  std::vector<double> resid, relerr;
+};
+
+void solver_release(SolverState *S);
+int solver_prepare(tailor_ctx *ctx);
+int solver_factor(tailor_ctx *ctx, cplx sigma);
+int solver_linear_solve(tailor_ctx *ctx, const double *b_nat, double *x_nat, int refine_max,
+                        double refine_tol, double *relres, int *steps);
+int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solve_diag &diag);
+int solver_norm2(tailor_ctx *ctx, const double *xr, const double *xi, double *out);
+int solver_dots(tailor_ctx *ctx, const double *Vr, const double *Vi, long long ldv, int ncols,
+                const double *wr, const double *wi, cplx *h);
+
+}  // namespace tailor

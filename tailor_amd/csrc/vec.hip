@@ -1,0 +1,270 @@
+// vec.hip -- vector kernels of the shift-invert Krylov-Schur iteration:
+// permutations between natural (k_j_dof, interleaved complex) and
+// elimination (split planes) order, the block-diagonal B product, the BSR
+// product K x = A x - mu B x, and the tall-skinny dot / update / combine
+// kernels of classical Gram-Schmidt with reorthogonalisation.
+#include "vec.hpp"
+
+namespace tailor {
+namespace {
+
+constexpr int VT = 256;
+
+__global__ void k_nat_to_elim(long long n, const int *__restrict__ perm, const double *__restrict__ src,
+                              double *__restrict__ dr, double *__restrict__ di) {
+  const long long k = static_cast<long long>(blockIdx.x) * VT + threadIdx.x;
+  if (k >= n) return;
+  const long long g = perm[k];
+  dr[k] = src[2 * g];
+  di[k] = src[2 * g + 1];
+}
+
+__global__ void k_elim_to_nat(long long n, const int *__restrict__ perm, const double *__restrict__ sr,
+                              const double *__restrict__ si, double *__restrict__ dst) {
+  const long long k = static_cast<long long>(blockIdx.x) * VT + threadIdx.x;
+  if (k >= n) return;
+  const long long g = perm[k];
+  dst[2 * g] = sr[k];
+  dst[2 * g + 1] = si[k];
+}
+
+// y = B x, elimination order; order[q] = node at position q.
+__global__ void k_bmul(long long nodes, const int *__restrict__ order, const double *__restrict__ bvals,
+                       const double *__restrict__ xr, const double *__restrict__ xi,
+                       double *__restrict__ yr, double *__restrict__ yi) {
+  const long long t = static_cast<long long>(blockIdx.x) * VT + threadIdx.x;
+  if (t >= nodes * 5) return;
+  const long long q = t / 5;
+  const int r = static_cast<int>(t - 5 * q);
+  const double *b = bvals + 50 * static_cast<long long>(order[q]) + 10 * r;
+  double sr = 0.0, si = 0.0;
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    const double br = b[2 * c], bi = b[2 * c + 1];
+    const double x_r = xr[5 * q + c], x_i = xi[5 * q + c];
+    sr += br * x_r - bi * x_i;
+    si += br * x_i + bi * x_r;
+  }
+  yr[t] = sr;
+  yi[t] = si;
+}
+
+// y = A x - mu B x in elimination order.  One wave per block row: the row's
+// 77 x 25 entries are contiguous, each lane accumulates per block-row entry
+// r, then the wave reduces the five row sums.
+__global__ __launch_bounds__(VT) void k_kmul(long long nodes, int per, const double *__restrict__ avals,
+                                             const int *__restrict__ colidx,
+                                             const double *__restrict__ bvals,
+                                             const int *__restrict__ pos, double mu_re, double mu_im,
+                                             const double *__restrict__ xr,
+                                             const double *__restrict__ xi, double *__restrict__ yr,
+                                             double *__restrict__ yi) {
+  const long long u = static_cast<long long>(blockIdx.x) * (VT / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  if (u >= nodes) return;
+  double ar[5] = {0, 0, 0, 0, 0}, ai[5] = {0, 0, 0, 0, 0};
+  const long long base = u * per * 25;
+  const int total = per * 25;
+  for (int idx = lane; idx < total; idx += 64) {
+    const int k = idx / 25, e = idx - 25 * (idx / 25);
+    const int r = e / 5, c = e - 5 * (e / 5);
+    const double a_r = avals[2 * (base + idx)], a_i = avals[2 * (base + idx) + 1];
+    const long long xc = 5LL * pos[colidx[u * per + k]] + c;
+    const double x_r = xr[xc], x_i = xi[xc];
+    const double pr = a_r * x_r - a_i * x_i, pi = a_r * x_i + a_i * x_r;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      if (q == r) {
+        ar[q] += pr;
+        ai[q] += pi;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    for (int off = 32; off > 0; off >>= 1) {
+      ar[q] += __shfl_xor(ar[q], off);
+      ai[q] += __shfl_xor(ai[q], off);
+    }
+  }
+  if (lane < 5) {
+    const int r = lane;
+    double sr = 0.0, si = 0.0;
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+      if (q == r) {
+        sr = ar[q];
+        si = ai[q];
+      }
+    const long long pu = 5LL * pos[u];
+    // - mu B x on the diagonal block
+    const double *b = bvals + 50 * u + 10 * r;
+    double bxr = 0.0, bxi = 0.0;
+    for (int c = 0; c < 5; ++c) {
+      const double br = b[2 * c], bi = b[2 * c + 1];
+      bxr += br * xr[pu + c] - bi * xi[pu + c];
+      bxi += br * xi[pu + c] + bi * xr[pu + c];
+    }
+    sr -= mu_re * bxr - mu_im * bxi;
+    si -= mu_re * bxi + mu_im * bxr;
+    yr[pu + r] = sr;
+    yi[pu + r] = si;
+  }
+}
+
+// partial h_j = sum_i conj(V_ij) w_i over this block's rows; ncols <= 64.
+__global__ __launch_bounds__(VT) void k_dots(long long n, int ncols, const double *__restrict__ Vr,
+                                             const double *__restrict__ Vi, long long ldv,
+                                             const double *__restrict__ wr,
+                                             const double *__restrict__ wi,
+                                             double *__restrict__ partials) {
+  __shared__ double red_r[VT], red_i[VT];
+  for (int j = 0; j < ncols; ++j) {
+    double sr = 0.0, si = 0.0;
+    for (long long i = static_cast<long long>(blockIdx.x) * VT + threadIdx.x; i < n;
+         i += static_cast<long long>(gridDim.x) * VT) {
+      const double vr = Vr[j * ldv + i], vi = Vi[j * ldv + i];
+      const double x_r = wr[i], x_i = wi[i];
+      sr += vr * x_r + vi * x_i;
+      si += vr * x_i - vi * x_r;
+    }
+    red_r[threadIdx.x] = sr;
+    red_i[threadIdx.x] = si;
+    __syncthreads();
+    for (int w = VT / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) {
+        red_r[threadIdx.x] += red_r[threadIdx.x + w];
+        red_i[threadIdx.x] += red_i[threadIdx.x + w];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      partials[2 * (static_cast<long long>(blockIdx.x) * ncols + j)] = red_r[0];
+      partials[2 * (static_cast<long long>(blockIdx.x) * ncols + j) + 1] = red_i[0];
+    }
+    __syncthreads();
+  }
+}
+
+// w -= V h (h complex, ncols <= 64, interleaved in device memory)
+__global__ void k_update(long long n, int ncols, const double *__restrict__ Vr,
+                         const double *__restrict__ Vi, long long ldv, const double *__restrict__ h,
+                         double *__restrict__ wr, double *__restrict__ wi) {
+  __shared__ double hr[64], hi[64];
+  if (threadIdx.x < ncols) {
+    hr[threadIdx.x] = h[2 * threadIdx.x];
+    hi[threadIdx.x] = h[2 * threadIdx.x + 1];
+  }
+  __syncthreads();
+  const long long i = static_cast<long long>(blockIdx.x) * VT + threadIdx.x;
+  if (i >= n) return;
+  double sr = 0.0, si = 0.0;
+  for (int j = 0; j < ncols; ++j) {
+    const double vr = Vr[j * ldv + i], vi = Vi[j * ldv + i];
+    sr += vr * hr[j] - vi * hi[j];
+    si += vr * hi[j] + vi * hr[j];
+  }
+  wr[i] -= sr;
+  wi[i] -= si;
+}
+
+// Out[:, 0:k] = V[:, 0:m] C, C m x k column-major complex (interleaved).
+__global__ void k_combine(long long n, int m, int k, const double *__restrict__ Vr,
+                          const double *__restrict__ Vi, long long ldv, const double *__restrict__ C,
+                          double *__restrict__ Or, double *__restrict__ Oi, long long ldo) {
+  __shared__ double cr[64 * 64], ci[64 * 64];
+  for (int t = threadIdx.x; t < m * k; t += VT) {
+    cr[t] = C[2 * t];
+    ci[t] = C[2 * t + 1];
+  }
+  __syncthreads();
+  const long long i = static_cast<long long>(blockIdx.x) * VT + threadIdx.x;
+  if (i >= n) return;
+  double vr[64], vi[64];
+  for (int a = 0; a < m; ++a) {
+    vr[a] = Vr[a * ldv + i];
+    vi[a] = Vi[a * ldv + i];
+  }
+  for (int b = 0; b < k; ++b) {
+    double sr = 0.0, si = 0.0;
+    for (int a = 0; a < m; ++a) {
+      const double c_r = cr[b * m + a], c_i = ci[b * m + a];
+      sr += vr[a] * c_r - vi[a] * c_i;
+      si += vr[a] * c_i + vi[a] * c_r;
+    }
+    Or[b * ldo + i] = sr;
+    Oi[b * ldo + i] = si;
+  }
+}
+
+// y = alpha x + beta y (complex scalars)
+__global__ void k_axpby(long long n, double a_r, double a_i, const double *__restrict__ xr,
+                        const double *__restrict__ xi, double b_r, double b_i, double *__restrict__ yr,
+                        double *__restrict__ yi) {
+  const long long i = static_cast<long long>(blockIdx.x) * VT + threadIdx.x;
+  if (i >= n) return;
+  const double x_r = xr[i], x_i = xi[i], y_r = yr[i], y_i = yi[i];
+  yr[i] = a_r * x_r - a_i * x_i + b_r * y_r - b_i * y_i;
+  yi[i] = a_r * x_i + a_i * x_r + b_r * y_i + b_i * y_r;
+}
+
+}  // namespace
+
+unsigned vblocks(long long n) { return static_cast<unsigned>((n + VT - 1) / VT); }
+
+hipError_t vec_nat_to_elim(hipStream_t s, long long n, const int *perm, const double *src, double *dr,
+                           double *di) {
+  hipLaunchKernelGGL(k_nat_to_elim, dim3(vblocks(n)), dim3(VT), 0, s, n, perm, src, dr, di);
+  return hipGetLastError();
+}
+
+hipError_t vec_elim_to_nat(hipStream_t s, long long n, const int *perm, const double *sr,
+                           const double *si, double *dst) {
+  hipLaunchKernelGGL(k_elim_to_nat, dim3(vblocks(n)), dim3(VT), 0, s, n, perm, sr, si, dst);
+  return hipGetLastError();
+}
+
+hipError_t vec_bmul(hipStream_t s, long long nodes, const int *order, const double *bvals,
+                    const double *xr, const double *xi, double *yr, double *yi) {
+  hipLaunchKernelGGL(k_bmul, dim3(vblocks(nodes * 5)), dim3(VT), 0, s, nodes, order, bvals, xr, xi,
+                     yr, yi);
+  return hipGetLastError();
+}
+
+hipError_t vec_kmul(hipStream_t s, long long nodes, int per, const double *avals, const int *colidx,
+                    const double *bvals, const int *pos, double mu_re, double mu_im,
+                    const double *xr, const double *xi, double *yr, double *yi) {
+  const unsigned g = static_cast<unsigned>((nodes + VT / 64 - 1) / (VT / 64));
+  hipLaunchKernelGGL(k_kmul, dim3(g), dim3(VT), 0, s, nodes, per, avals, colidx, bvals, pos, mu_re,
+                     mu_im, xr, xi, yr, yi);
+  return hipGetLastError();
+}
+
+hipError_t vec_dots(hipStream_t s, long long n, int ncols, const double *Vr, const double *Vi,
+                    long long ldv, const double *wr, const double *wi, double *partials,
+                    int nblocks) {
+  hipLaunchKernelGGL(k_dots, dim3(nblocks), dim3(VT), 0, s, n, ncols, Vr, Vi, ldv, wr, wi, partials);
+  return hipGetLastError();
+}
+
+hipError_t vec_update(hipStream_t s, long long n, int ncols, const double *Vr, const double *Vi,
+                      long long ldv, const double *h, double *wr, double *wi) {
+  hipLaunchKernelGGL(k_update, dim3(vblocks(n)), dim3(VT), 0, s, n, ncols, Vr, Vi, ldv, h, wr, wi);
+  return hipGetLastError();
+}
+
+hipError_t vec_combine(hipStream_t s, long long n, int m, int k, const double *Vr, const double *Vi,
+                       long long ldv, const double *C, double *Or, double *Oi, long long ldo) {
+  hipLaunchKernelGGL(k_combine, dim3(vblocks(n)), dim3(VT), 0, s, n, m, k, Vr, Vi, ldv, C, Or, Oi,
+                     ldo);
+  return hipGetLastError();
+}
+
+hipError_t vec_axpby(hipStream_t s, long long n, double a_r, double a_i, const double *xr,
+                     const double *xi, double b_r, double b_i, double *yr, double *yi) {
+  hipLaunchKernelGGL(k_axpby, dim3(vblocks(n)), dim3(VT), 0, s, n, a_r, a_i, xr, xi, b_r, b_i, yr,
+                     yi);
+  return hipGetLastError();
+}
+
+}  // namespace tailor
