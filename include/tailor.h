@@ -163,6 +163,12 @@ int tailor_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, tailor_fact
 int tailor_linear_solve(tailor_ctx *ctx, const double *b, double *x, int32_t refine_max,
                         double refine_tol, double *relres, int32_t *steps);
 
+/* Time `reps` unrefined solves x = K^-1 b (device-resident vectors) with
+ * HIP events on the context stream: mean ms per solve and the bytes of
+ * factor data one solve reads (E, L21, U12 and Z of every front). */
+int tailor_time_linear_solve(tailor_ctx *ctx, int32_t reps, double *ms_per_solve,
+                             double *bytes_per_solve);
+
 /* Settings of EigenSolver in config.yaml (Parser.cpp:62-83). */
 typedef struct {
   int32_t nev;             /* NumberOfEigenvalues */

@@ -134,6 +134,7 @@ def library() -> C.CDLL:
     lib.tailor_linear_solve.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), C.c_int32,
                                         C.c_double, P(C.c_double), P(C.c_int32)]
     lib.tailor_solve.argtypes = [C.c_void_p, P(SolveOpts), P(SolveDiag)]
+    lib.tailor_time_linear_solve.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double)]
     lib.tailor_eigenpair_count.argtypes = [C.c_void_p]
     lib.tailor_get_eigenpair.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double),
                                          P(C.c_double), P(C.c_double)]
@@ -302,6 +303,13 @@ def _eigenpairs(self, vectors: bool = False) -> list:
     return out
 
 
+def _time_linear_solve(self, reps: int = 20):
+    ms, nbytes = C.c_double(), C.c_double()
+    self._check(self.lib.tailor_time_linear_solve(self.ptr, reps, C.byref(ms), C.byref(nbytes)))
+    return ms.value, nbytes.value
+
+
+Context.time_linear_solve = _time_linear_solve
 Context.factor = _factor
 Context.linear_solve = _linear_solve
 Context.solve = _solve

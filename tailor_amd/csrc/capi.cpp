@@ -578,6 +578,27 @@ int tailor_linear_solve(tailor_ctx *ctx, const double *b, double *x, int32_t ref
   return st;
 }
 
+int tailor_time_linear_solve(tailor_ctx *ctx, int32_t reps, double *ms_per_solve,
+                             double *bytes_per_solve) {
+  if (!ctx || reps <= 0) return TAILOR_ERR_USER_INPUT;
+  if (!ctx->solver || !ctx->solver->factored)
+    return fail(ctx, TAILOR_ERR_ARG_WRONGSTATE, "tailor_factor must precede tailor_time_linear_solve");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  double ms = 0.0;
+  if (int st = tailor::solver_time_solves(ctx, reps, &ms)) return st;
+  if (ms_per_solve) *ms_per_solve = ms;
+  if (bytes_per_solve) {
+    const tailor::Symbolic &S = ctx->solver->sym;
+    double b = 0.0;
+    for (int f = 0; f < S.nsn; ++f) {
+      const double p = S.p[f], c = S.c[f];
+      b += 16.0 * (2.0 * p * p + 2.0 * p * c);
+    }
+    *bytes_per_solve = b;
+  }
+  return TAILOR_OK;
+}
+
 int tailor_solve(tailor_ctx *ctx, const tailor_solve_opts *opts, tailor_solve_diag *diag) {
   if (!ctx || !opts) return TAILOR_ERR_USER_INPUT;
   if (opts->nev <= 0 || !(opts->tol > 0.0) || opts->maxit <= 0)

@@ -213,6 +213,28 @@ int solver_linear_solve(tailor_ctx *ctx, const double *b_nat, double *x_nat, int
   return TAILOR_OK;
 }
 
+int solver_time_solves(tailor_ctx *ctx, int reps, double *ms) {
+  SolverState &S = *ctx->solver;
+  const long long n = S.n;
+  HIP_OK(grow(&S.work, S.work_cap, static_cast<size_t>(12 * n)));
+  double *br = S.work, *bi = br + n, *xr = S.work + 2 * n, *xi = xr + n;
+  HIP_OK(lu_fill(ctx->stream, br, 2 * n, 1.0));
+  HIP_OK(lu_solve(S.plan, br, bi, xr, xi, ctx->stream));  // warm-up
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  HIP_OK(hipEventRecord(e0, ctx->stream));
+  for (int r = 0; r < reps; ++r) HIP_OK(lu_solve(S.plan, br, bi, xr, xi, ctx->stream));
+  HIP_OK(hipEventRecord(e1, ctx->stream));
+  HIP_OK(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIP_OK(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = static_cast<double>(t) / reps;
+  return TAILOR_OK;
+}
+
 // ----------------------------------------------------------- Krylov-Schur
 
 int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solve_diag &diag) {

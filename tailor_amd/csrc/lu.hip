@@ -102,11 +102,13 @@ __global__ void k_extend_add(const int *__restrict__ tasks, int ntasks,
 
 // ------------------------------------------------------------ factorization
 
-constexpr int kPanelThreads = 512;
+constexpr int kPanelThreads = 1024;
 
-// Unblocked LU of the panel columns [k0, k0+kb) of each front (rows k0..m),
-// pivot rows chosen among the fully summed rows [j, p).  Row exchanges are
-// applied to the whole front row and to the E row.
+// Unblocked LU of the panel columns [k0, k0+kb) over the fully summed rows
+// [k0, p) of each front; pivot rows are chosen among those rows.  Row
+// exchanges touch only the panel columns here (k_trsm_lower applies them to
+// the columns right of the panel and to E); the update rows [p, m) of the
+// panel are finished by k_l21.
 __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__ list, int nlist,
                                                          const FrontDev *__restrict__ fr, int k0,
                                                          int nb, double tiny, double perturb,
@@ -119,7 +121,6 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__
   const int kb = min(nb, d.p - k0);
   const long long mm = static_cast<long long>(d.m) * d.m;
   double *Fr = arena + d.off, *Fi = Fr + mm;
-  double *Er = Fi + mm, *Ei = Er + static_cast<long long>(d.p) * d.p;
   __shared__ double sv[kPanelThreads];
   __shared__ int si[kPanelThreads];
   __shared__ double prow_r[kLuNb], prow_i[kLuNb];
@@ -153,17 +154,10 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__
     const int piv = si[0];
     const bool small = !(sv[0] > tiny);
     __syncthreads();
-    if (!small && piv != j) {
-      for (int col = tid; col < d.m; col += kPanelThreads) {
-        const long long a = static_cast<long long>(col) * d.m + j, b = static_cast<long long>(col) * d.m + piv;
-        double t = Fr[a]; Fr[a] = Fr[b]; Fr[b] = t;
-        t = Fi[a]; Fi[a] = Fi[b]; Fi[b] = t;
-      }
-      for (int col = tid; col < d.p; col += kPanelThreads) {
-        const long long a = static_cast<long long>(col) * d.p + j, b = static_cast<long long>(col) * d.p + piv;
-        double t = Er[a]; Er[a] = Er[b]; Er[b] = t;
-        t = Ei[a]; Ei[a] = Ei[b]; Ei[b] = t;
-      }
+    if (!small && piv != j && tid < kb) {
+      const long long col = static_cast<long long>(k0 + tid) * d.m;
+      double t = Fr[col + j]; Fr[col + j] = Fr[col + piv]; Fr[col + piv] = t;
+      t = Fi[col + j]; Fi[col + j] = Fi[col + piv]; Fi[col + piv] = t;
     }
     __syncthreads();
     if (tid == 0) {
@@ -189,7 +183,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__
     __syncthreads();
     const double ir = inv_r, ii = inv_i;
     const int ncol = k0 + kb - j - 1;
-    for (int r = j + 1 + tid; r < d.m; r += kPanelThreads) {
+    for (int r = j + 1 + tid; r < d.p; r += kPanelThreads) {
       const double ar = Fr[cj + r], ai = Fi[cj + r];
       const double lr = ar * ir - ai * ii, li = ar * ii + ai * ir;
       Fr[cj + r] = lr;
@@ -205,10 +199,62 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__
   }
 }
 
+// Update rows [p, m) of the panel columns: X <- X U_D^-1 with U_D the upper
+// kb x kb pivot block (row-wise forward substitution, thread per row).
+__global__ __launch_bounds__(256) void k_l21(const int *__restrict__ list, int nlist,
+                                             const FrontDev *__restrict__ fr, int k0, int nb,
+                                             double *__restrict__ arena) {
+  const int f = list[blockIdx.y];
+  const FrontDev d = fr[f];
+  if (k0 >= d.p || d.c == 0) return;
+  const int kb = min(nb, d.p - k0);
+  const long long mm = static_cast<long long>(d.m) * d.m;
+  double *Fr = arena + d.off, *Fi = Fr + mm;
+  __shared__ double Ur[kLuNb * kLuNb], Ui[kLuNb * kLuNb], Dr[kLuNb], Di[kLuNb];
+  for (int k = threadIdx.x; k < kb * kb; k += blockDim.x) {
+    const int r = k % kb, s = k / kb;
+    const long long idx = static_cast<long long>(k0 + s) * d.m + k0 + r;
+    Ur[s * kLuNb + r] = Fr[idx];
+    Ui[s * kLuNb + r] = Fi[idx];
+  }
+  __syncthreads();
+  if (threadIdx.x < kb) {
+    const double pr = Ur[threadIdx.x * kLuNb + threadIdx.x], pi = Ui[threadIdx.x * kLuNb + threadIdx.x];
+    const double den = pr * pr + pi * pi;
+    Dr[threadIdx.x] = pr / den;
+    Di[threadIdx.x] = -pi / den;
+  }
+  __syncthreads();
+  const int row = d.p + blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= d.m) return;
+  double xr[kLuNb], xi[kLuNb];
+#pragma unroll
+  for (int j = 0; j < kLuNb; ++j) {
+    if (j < kb) {
+      const long long idx = static_cast<long long>(k0 + j) * d.m + row;
+      double ar = Fr[idx], ai = Fi[idx];
+#pragma unroll
+      for (int i = 0; i < kLuNb; ++i) {
+        if (i < j) {
+          const double ur = Ur[j * kLuNb + i], ui = Ui[j * kLuNb + i];
+          ar -= xr[i] * ur - xi[i] * ui;
+          ai -= xr[i] * ui + xi[i] * ur;
+        }
+      }
+      xr[j] = ar * Dr[j] - ai * Di[j];
+      xi[j] = ar * Di[j] + ai * Dr[j];
+      Fr[idx] = xr[j];
+      Fi[idx] = xi[j];
+    }
+  }
+}
+
 // Rows [k0, k0+kb) of the F columns right of the panel and of all E columns:
-// X <- L_D^-1 X with L_D the unit lower kb x kb pivot block.
+// apply the panel's row exchanges to the column, then X <- L_D^-1 X with L_D
+// the unit lower kb x kb pivot block.
 __global__ __launch_bounds__(256) void k_trsm_lower(const int *__restrict__ list, int nlist,
                                                     const FrontDev *__restrict__ fr, int k0, int nb,
+                                                    const int *__restrict__ ipiv,
                                                     double *__restrict__ arena) {
   const int f = list[blockIdx.y];
   const FrontDev d = fr[f];
@@ -218,26 +264,37 @@ __global__ __launch_bounds__(256) void k_trsm_lower(const int *__restrict__ list
   double *Fr = arena + d.off, *Fi = Fr + mm;
   double *Er = Fi + mm, *Ei = Er + static_cast<long long>(d.p) * d.p;
   __shared__ double Lr[kLuNb * kLuNb], Li[kLuNb * kLuNb];
+  __shared__ int pv[kLuNb];
   for (int k = threadIdx.x; k < kb * kb; k += blockDim.x) {
     const int r = k % kb, s = k / kb;
     const long long idx = static_cast<long long>(k0 + s) * d.m + k0 + r;
     Lr[s * kLuNb + r] = Fr[idx];
     Li[s * kLuNb + r] = Fi[idx];
   }
+  if (threadIdx.x < kb) pv[threadIdx.x] = ipiv[d.piv_off + k0 + threadIdx.x];
   __syncthreads();
   const int nF = d.m - k0 - kb;
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= nF + d.p) return;
   double *Xr, *Xi;
   if (col < nF) {
-    const long long base = static_cast<long long>(k0 + kb + col) * d.m + k0;
+    const long long base = static_cast<long long>(k0 + kb + col) * d.m;
     Xr = Fr + base;
     Xi = Fi + base;
   } else {
-    const long long base = static_cast<long long>(col - nF) * d.p + k0;
+    const long long base = static_cast<long long>(col - nF) * d.p;
     Xr = Er + base;
     Xi = Ei + base;
   }
+  for (int j = 0; j < kb; ++j) {
+    const int pr = pv[j], r = k0 + j;
+    if (pr != r) {
+      double t = Xr[r]; Xr[r] = Xr[pr]; Xr[pr] = t;
+      t = Xi[r]; Xi[r] = Xi[pr]; Xi[pr] = t;
+    }
+  }
+  Xr += k0;
+  Xi += k0;
   double xr[kLuNb], xi[kLuNb];
 #pragma unroll
   for (int r = 0; r < kLuNb; ++r) {
@@ -459,13 +516,18 @@ __global__ void k_fwd_init(const int *__restrict__ list, int nlist, const FrontD
                            const int *__restrict__ child_ptr, const int *__restrict__ child_list,
                            const int *__restrict__ rel, const long long *__restrict__ rel_off,
                            const double *__restrict__ br, const double *__restrict__ bi,
-                           double *__restrict__ Wr, double *__restrict__ Wi) {
+                           double *__restrict__ Wr, double *__restrict__ Wi,
+                           double *__restrict__ Yr, double *__restrict__ Yi) {
   const int f = list[blockIdx.x];
   const FrontDev d = fr[f];
   double *wr = Wr + d.woff, *wi = Wi + d.woff;
   for (int k = threadIdx.x; k < d.m; k += blockDim.x) {
     wr[k] = k < d.p ? br[d.dof0 + k] : 0.0;
     wi[k] = k < d.p ? bi[d.dof0 + k] : 0.0;
+    if (k < d.p) {
+      Yr[d.dof0 + k] = 0.0;  // output of the E product
+      Yi[d.dof0 + k] = 0.0;
+    }
   }
   __syncthreads();
   for (int q = child_ptr[f]; q < child_ptr[f + 1]; ++q) {
@@ -482,11 +544,14 @@ __global__ void k_fwd_init(const int *__restrict__ list, int nlist, const FrontD
   }
 }
 
-// Backward gather of a level: G_f = x[update rows of f] (length c).
+// Backward set-up of a level: W_f[p:] = x[update rows of f] (ancestors,
+// already solved), T_f = y_f, and x_f = 0 (accumulated by the Z product).
 __global__ void k_bwd_gather(const int *__restrict__ list, int nlist, const FrontDev *__restrict__ fr,
                              const int *__restrict__ upd_pos, const long long *__restrict__ upd_off,
-                             const double *__restrict__ xr, const double *__restrict__ xi,
-                             double *__restrict__ Wr, double *__restrict__ Wi) {
+                             double *__restrict__ xr, double *__restrict__ xi,
+                             double *__restrict__ Wr, double *__restrict__ Wi,
+                             const double *__restrict__ Yr, const double *__restrict__ Yi,
+                             double *__restrict__ Tr, double *__restrict__ Ti) {
   const int f = list[blockIdx.x];
   const FrontDev d = fr[f];
   const int *up = upd_pos + upd_off[f];
@@ -494,6 +559,12 @@ __global__ void k_bwd_gather(const int *__restrict__ list, int nlist, const Fron
     const int g = 5 * up[k / 5] + k % 5;
     Wr[d.woff + d.p + k] = xr[g];
     Wi[d.woff + d.p + k] = xi[g];
+  }
+  for (int k = threadIdx.x; k < d.p; k += blockDim.x) {
+    Tr[d.dof0 + k] = Yr[d.dof0 + k];
+    Ti[d.dof0 + k] = Yi[d.dof0 + k];
+    xr[d.dof0 + k] = 0.0;
+    xi[d.dof0 + k] = 0.0;
   }
 }
 
@@ -506,9 +577,8 @@ __global__ void k_bwd_gather(const int *__restrict__ list, int nlist, const Fron
 // Output rows are split over blockIdx.x (GVR rows per workgroup) and the
 // reduction dimension over blockIdx.z (GVC columns); partial sums are added
 // with FP64 atomics into an output that ops 0, 2, 3 first initialise
-// (blockIdx.z == 0 does not own the init: k_gemv_init does).
-constexpr int GVR = 256, GVC = 512;
-
+// (k_fwd_init / k_bwd_gather initialise them).
+constexpr int GVR = 64, GVC = 64, GVW = 4;  // rows per WG, columns per wave, waves per WG
 struct GemvView {
   const double *Ar, *Ai;
   long long lda;
@@ -542,55 +612,70 @@ __device__ inline bool gemv_view(int op, const FrontDev &d, double *arena, doubl
   }
 }
 
-__global__ __launch_bounds__(GVR) void k_gemv(const int *__restrict__ list, int nlist,
-                                              const FrontDev *__restrict__ fr, int op,
-                                              double *__restrict__ arena, double *Wr, double *Wi,
-                                              double *Sr, double *Si, double *Xr, double *Xi) {
+// One workgroup: GVR output rows x (GVW * GVC) reduction columns; each wave
+// takes GVC of the columns and the four partial sums are combined in LDS
+// before one FP64 atomic per row.
+__global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list, int nlist,
+                                                    const FrontDev *__restrict__ fr, int op,
+                                                    double *__restrict__ arena, double *Wr, double *Wi,
+                                                    double *Sr, double *Si, double *Xr, double *Xi) {
   const int f = list[blockIdx.y];
   const FrontDev d = fr[f];
   GemvView v;
   if (!gemv_view(op, d, arena, Wr, Wi, Sr, Si, Xr, Xi, v)) return;
-  const int row = blockIdx.x * GVR + threadIdx.x;
-  const int c0 = blockIdx.z * GVC;
-  if (blockIdx.x * GVR >= v.M || c0 >= v.N) return;
+  const int lane = threadIdx.x % GVR, wave = threadIdx.x / GVR;
+  const int row = blockIdx.x * GVR + lane;
+  const int cw = blockIdx.z * GVW * GVC;  // first column of this workgroup
+  if (blockIdx.x * GVR >= v.M || cw >= v.N) return;
+  const int c0 = cw + wave * GVC;
   const int c1 = min(v.N, c0 + GVC);
-  __shared__ double sxr[GVC], sxi[GVC];
-  for (int k = c0 + threadIdx.x; k < c1; k += GVR) {
-    sxr[k - c0] = v.xr[k];
-    sxi[k - c0] = v.xi[k];
+  __shared__ double sxr[GVW * GVC], sxi[GVW * GVC];
+  __shared__ double pr[GVW][GVR], pi[GVW][GVR];
+  for (int k = threadIdx.x; k < GVW * GVC; k += GVR * GVW) {
+    const int col = cw + k;
+    sxr[k] = col < v.N ? v.xr[col] : 0.0;
+    sxi[k] = col < v.N ? v.xi[col] : 0.0;
   }
   __syncthreads();
-  if (row >= v.M) return;
   double accr = 0.0, acci = 0.0;
-  const double *ar = v.Ar + row, *ai = v.Ai + row;
-  for (int k = c0; k < c1; ++k) {
-    const long long idx = static_cast<long long>(k) * v.lda;
-    const double a_r = ar[idx], a_i = ai[idx];
-    const double x_r = sxr[k - c0], x_i = sxi[k - c0];
-    accr += a_r * x_r - a_i * x_i;
-    acci += a_r * x_i + a_i * x_r;
-  }
-  atomicAdd(v.yr + row, v.sign * accr);
-  atomicAdd(v.yi + row, v.sign * acci);
-}
-
-// Output initialisation for ops 0 (y = 0), 2 (T = y) and 3 (x = 0).
-__global__ void k_gemv_init(const int *__restrict__ list, int nlist, const FrontDev *__restrict__ fr,
-                            int op, double *Sr, double *Si, double *Xr, double *Xi,
-                            const double *Yr, const double *Yi) {
-  const int f = list[blockIdx.x];
-  const FrontDev d = fr[f];
-  for (int k = threadIdx.x; k < d.p; k += blockDim.x) {
-    if (op == 0) {
-      Sr[d.dof0 + k] = 0.0;
-      Si[d.dof0 + k] = 0.0;
-    } else if (op == 2) {
-      Sr[d.dof0 + k] = Yr[d.dof0 + k];
-      Si[d.dof0 + k] = Yi[d.dof0 + k];
-    } else {
-      Xr[d.dof0 + k] = 0.0;
-      Xi[d.dof0 + k] = 0.0;
+  if (row < v.M) {
+    const double *ar = v.Ar + row, *ai = v.Ai + row;
+    int k = c0;
+    // 16 columns per step so that 32 loads per lane are in flight
+    for (; k + 16 <= c1; k += 16) {
+      double a_r[16], a_i[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const long long idx = static_cast<long long>(k + u) * v.lda;
+        a_r[u] = ar[idx];
+        a_i[u] = ai[idx];
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const double x_r = sxr[k + u - cw], x_i = sxi[k + u - cw];
+        accr += a_r[u] * x_r - a_i[u] * x_i;
+        acci += a_r[u] * x_i + a_i[u] * x_r;
+      }
     }
+    for (; k < c1; ++k) {
+      const long long idx = static_cast<long long>(k) * v.lda;
+      const double a_r = ar[idx], a_i = ai[idx];
+      const double x_r = sxr[k - cw], x_i = sxi[k - cw];
+      accr += a_r * x_r - a_i * x_i;
+      acci += a_r * x_i + a_i * x_r;
+    }
+  }
+  pr[wave][lane] = accr;
+  pi[wave][lane] = acci;
+  __syncthreads();
+  if (wave == 0 && row < v.M) {
+    double sr = 0.0, si = 0.0;
+    for (int w = 0; w < GVW; ++w) {
+      sr += pr[w][lane];
+      si += pi[w][lane];
+    }
+    atomicAdd(v.yr + row, v.sign * sr);
+    atomicAdd(v.yi + row, v.sign * si);
   }
 }
 
@@ -633,9 +718,12 @@ hipError_t lu_factor(const LuPlan &P, const LuInputs &in, double sig_re, double 
     for (int k0 = 0; k0 < L.pmax; k0 += P.nb) {
       hipLaunchKernelGGL(k_panel, dim3(L.count), dim3(kPanelThreads), 0, s, list, L.count,
                          P.d_front, k0, P.nb, tiny, perturb, P.arena, P.d_ipiv, P.d_counters);
+      if (L.cmax > 0)
+        hipLaunchKernelGGL(k_l21, dim3(cdivu(L.cmax, 256), L.count), dim3(256), 0, s, list,
+                           L.count, P.d_front, k0, P.nb, P.arena);
       const int cols = L.mmax + L.pmax;
       hipLaunchKernelGGL(k_trsm_lower, dim3(cdivu(cols, 256), L.count), dim3(256), 0, s, list,
-                         L.count, P.d_front, k0, P.nb, P.arena);
+                         L.count, P.d_front, k0, P.nb, P.d_ipiv, P.arena);
       const int tm = (L.mmax + GT - 1) / GT;
       hipLaunchKernelGGL(k_gemm_update, dim3(static_cast<unsigned>(tm * tm), L.count), dim3(256), 0, s, list, L.count,
                          P.d_front, k0, P.nb, 0, P.arena);
@@ -671,28 +759,22 @@ hipError_t lu_solve(const LuPlan &P, const double *br, const double *bi, double 
     const LevelPlan &L = P.levels[lev];
     const int *list = P.d_level_list + L.list_off;
     hipLaunchKernelGGL(k_fwd_init, dim3(L.count), dim3(256), 0, s, list, L.count, P.d_front,
-                       P.d_child_ptr, P.d_child_list, P.d_rel, P.d_rel_off, br, bi, Wr, Wi);
-    hipLaunchKernelGGL(k_gemv_init, dim3(L.count), dim3(256), 0, s, list, L.count, P.d_front, 0,
-                       Yr, Yi, xr, xi, Yr, Yi);
-    hipLaunchKernelGGL(k_gemv, dim3(cdivu(L.pmax, GVR), L.count, cdivu(L.pmax, GVC)),
-                       dim3(GVR), 0, s, list, L.count, P.d_front, 0, P.arena, Wr, Wi, Yr, Yi, xr, xi);
-    hipLaunchKernelGGL(k_gemv, dim3(cdivu(L.cmax, GVR), L.count, cdivu(L.pmax, GVC)),
-                       dim3(GVR), 0, s, list, L.count, P.d_front, 1, P.arena, Wr, Wi, Yr, Yi, xr, xi);
+                       P.d_child_ptr, P.d_child_list, P.d_rel, P.d_rel_off, br, bi, Wr, Wi, Yr, Yi);
+    hipLaunchKernelGGL(k_gemv, dim3(cdivu(L.pmax, GVR), L.count, cdivu(L.pmax, GVC * GVW)),
+                       dim3(GVR * GVW), 0, s, list, L.count, P.d_front, 0, P.arena, Wr, Wi, Yr, Yi, xr, xi);
+    hipLaunchKernelGGL(k_gemv, dim3(cdivu(L.cmax, GVR), L.count, cdivu(L.pmax, GVC * GVW)),
+                       dim3(GVR * GVW), 0, s, list, L.count, P.d_front, 1, P.arena, Wr, Wi, Yr, Yi, xr, xi);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   for (int lev = static_cast<int>(P.levels.size()) - 1; lev >= 0; --lev) {
     const LevelPlan &L = P.levels[lev];
     const int *list = P.d_level_list + L.list_off;
     hipLaunchKernelGGL(k_bwd_gather, dim3(L.count), dim3(256), 0, s, list, L.count, P.d_front,
-                       P.d_upd_pos, P.d_upd_off, xr, xi, Wr, Wi);
-    hipLaunchKernelGGL(k_gemv_init, dim3(L.count), dim3(256), 0, s, list, L.count, P.d_front, 2,
-                       Tr, Ti, xr, xi, Yr, Yi);
-    hipLaunchKernelGGL(k_gemv, dim3(cdivu(L.pmax, GVR), L.count, cdivu(L.cmax, GVC)),
-                       dim3(GVR), 0, s, list, L.count, P.d_front, 2, P.arena, Wr, Wi, Tr, Ti, xr, xi);
-    hipLaunchKernelGGL(k_gemv_init, dim3(L.count), dim3(256), 0, s, list, L.count, P.d_front, 3,
-                       Tr, Ti, xr, xi, Yr, Yi);
-    hipLaunchKernelGGL(k_gemv, dim3(cdivu(L.pmax, GVR), L.count, cdivu(L.pmax, GVC)),
-                       dim3(GVR), 0, s, list, L.count, P.d_front, 3, P.arena, Wr, Wi, Tr, Ti, xr, xi);
+                       P.d_upd_pos, P.d_upd_off, xr, xi, Wr, Wi, Yr, Yi, Tr, Ti);
+    hipLaunchKernelGGL(k_gemv, dim3(cdivu(L.pmax, GVR), L.count, cdivu(L.cmax, GVC * GVW)),
+                       dim3(GVR * GVW), 0, s, list, L.count, P.d_front, 2, P.arena, Wr, Wi, Tr, Ti, xr, xi);
+    hipLaunchKernelGGL(k_gemv, dim3(cdivu(L.pmax, GVR), L.count, cdivu(L.pmax, GVC * GVW)),
+                       dim3(GVR * GVW), 0, s, list, L.count, P.d_front, 3, P.arena, Wr, Wi, Tr, Ti, xr, xi);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   return hipSuccess;

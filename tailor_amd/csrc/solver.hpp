@@ -36,6 +36,7 @@ int solver_factor(tailor_ctx *ctx, cplx sigma);
 int solver_linear_solve(tailor_ctx *ctx, const double *b_nat, double *x_nat, int refine_max,
                         double refine_tol, double *relres, int *steps);
 int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solve_diag &diag);
+int solver_time_solves(tailor_ctx *ctx, int reps, double *ms);
 int solver_norm2(tailor_ctx *ctx, const double *xr, const double *xi, double *out);
 int solver_dots(tailor_ctx *ctx, const double *Vr, const double *Vi, long long ldv, int ncols,
                 const double *wr, const double *wi, cplx *h);
