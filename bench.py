@@ -28,7 +28,7 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 NEV, TOL, OMEGA = 8, 1e-10, 0.02 + 0j
-NCV = 32  # Krylov subspace size (the -eps_ncv of the reference's options database)
+NCV = 64  # Krylov subspace size (the -eps_ncv of the reference's options database)
 
 
 def algorithmic_assembly_bytes(ny: int, nz: int, ssx: int, sse: int) -> int:

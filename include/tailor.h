@@ -165,14 +165,15 @@ int tailor_linear_solve(tailor_ctx *ctx, const double *b, double *x, int32_t ref
 
 /* Time `reps` unrefined solves x = K^-1 b (device-resident vectors) with
  * HIP events on the context stream: mean ms per solve and the bytes of
- * factor data one solve reads (E, L21, U12 and Z of every front). */
+ * factor data one solve must read: E, L21 and U12 of every front and the
+ * upper triangle of Z. */
 int tailor_time_linear_solve(tailor_ctx *ctx, int32_t reps, double *ms_per_solve,
                              double *bytes_per_solve);
 
 /* Settings of EigenSolver in config.yaml (Parser.cpp:62-83). */
 typedef struct {
   int32_t nev;             /* NumberOfEigenvalues */
-  int32_t ncv;             /* subspace size; 0 selects max(2 nev, nev + 15) */
+  int32_t ncv;             /* subspace size (at most 128); 0 selects max(2 nev, nev + 15) */
   int32_t maxit;           /* MaximumIterations (restarts) */
   double tol;              /* Tolerance: accepted when ||A q - lambda B q|| / ||q|| <= tol */
   double omega_re, omega_im;  /* SearchCenterOmega; lambda_s = Im(omega) - i Re(omega) */

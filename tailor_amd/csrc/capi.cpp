@@ -592,7 +592,8 @@ int tailor_time_linear_solve(tailor_ctx *ctx, int32_t reps, double *ms_per_solve
     double b = 0.0;
     for (int f = 0; f < S.nsn; ++f) {
       const double p = S.p[f], c = S.c[f];
-      b += 16.0 * (2.0 * p * p + 2.0 * p * c);
+      // E (p x p), Z (upper triangle), L21 and U12 (p x c each)
+      b += 16.0 * (p * p + 0.5 * p * (p + 1.0) + 2.0 * p * c);
     }
     *bytes_per_solve = b;
   }

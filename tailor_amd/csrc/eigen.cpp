@@ -243,7 +243,7 @@ int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solv
   const int nev = o.nev;
   int ncv = o.ncv > 0 ? o.ncv : std::max(2 * nev, nev + 15);
   ncv = std::min<long long>(ncv, n - 1);
-  if (ncv > 60) ncv = 60;
+  if (ncv > 128) ncv = 128;
   if (nev <= 0 || nev >= ncv) return fail(ctx, TAILOR_ERR_USER_INPUT, "invalid nev / ncv");
   HIP_OK(grow(&S.V, S.V_cap, static_cast<size_t>(2 * n * (ncv + 1))));
   HIP_OK(grow(&S.work, S.work_cap, static_cast<size_t>(12 * n)));

@@ -20,6 +20,7 @@
 //
 // Arena layout of front f at offset off (doubles):
 //   F_re[m*m] F_im[m*m] E_re[p*p] E_im[p*p] Z_re[p*p] Z_im[p*p]
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 
@@ -101,37 +102,60 @@ __global__ void k_extend_add(const int *__restrict__ tasks, int ntasks,
 }
 
 // ------------------------------------------------------------ factorization
+//
+// Blocked right-looking LU per front.  The fully summed columns are taken in
+// outer blocks of kLuNbOuter columns, each factored as kLuNb-column panels:
+//
+//   for each block [K0, K1):
+//     for each panel [k0, k0+kb) of the block:
+//       k_panel       LU of the panel over the fully summed rows [k0, p),
+//                     row exchanges applied across the block's columns
+//       k_l21         rows [p, m) of the panel:  X <- X U_D^-1
+//       k_trsm_lower  (mode 0) panel rows of the block columns right of it
+//       k_gemm INNER  F[k0+kb:m, k0+kb:K1] -= L_panel U_panel
+//     k_trsm_lower    (mode 1) all row exchanges of the block, then the
+//                     panel rows of the trailing F columns and of E
+//     k_gemm BLKROWS  remaining block rows of those columns, panel by panel
+//     k_gemm TRAIL    F[K1:m, K1:m] -= L21_blk U12_blk and
+//                     E[K1:p, :]    -= L21_blk E[K0:K1, :]   (K = K1 - K0)
+//
+// Row exchanges of a block are applied to the block's own columns at once
+// but never to earlier blocks: L11 is stored block by block in the row order
+// of its own block.  The solves never read L11 (they use E = L11^-1 P, which
+// records every exchange), so this ordering is invisible outside the
+// factorization.
 
 constexpr int kPanelThreads = 1024;
 
 // Unblocked LU of the panel columns [k0, k0+kb) over the fully summed rows
-// [k0, p) of each front; pivot rows are chosen among those rows.  Row
-// exchanges touch only the panel columns here (k_trsm_lower applies them to
-// the columns right of the panel and to E); the update rows [p, m) of the
-// panel are finished by k_l21.
+// [k0, p) of each front; pivot rows are chosen among those rows and the
+// exchange is applied to the columns [K0, K1) of the current block.  The
+// update rows [p, m) of the panel are finished by k_l21.
+template <typename T>
 __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__ list, int nlist,
                                                          const FrontDev *__restrict__ fr, int k0,
-                                                         int nb, double tiny, double perturb,
-                                                         double *__restrict__ arena,
+                                                         int nb, int K0, int NB, double tiny,
+                                                         double perturb, T *__restrict__ arena,
                                                          int *__restrict__ ipiv,
                                                          int *__restrict__ nperturbed) {
   const int f = list[blockIdx.x];
   const FrontDev d = fr[f];
   if (k0 >= d.p) return;
   const int kb = min(nb, d.p - k0);
+  const int K1 = min(K0 + NB, d.p);
   const long long mm = static_cast<long long>(d.m) * d.m;
-  double *Fr = arena + d.off, *Fi = Fr + mm;
+  T *Fr = arena + d.off, *Fi = Fr + mm;
   __shared__ double sv[kPanelThreads];
   __shared__ int si[kPanelThreads];
-  __shared__ double prow_r[kLuNb], prow_i[kLuNb];
-  __shared__ double inv_r, inv_i;
+  __shared__ T prow_r[kLuNb], prow_i[kLuNb];
+  __shared__ T inv_r, inv_i;
   const int tid = threadIdx.x;
   for (int j = k0; j < k0 + kb; ++j) {
     const long long cj = static_cast<long long>(j) * d.m;
     double best = -1.0;
     int bi = j;
     for (int r = j + tid; r < d.p; r += kPanelThreads) {
-      const double a = cabs1(Fr[cj + r], Fi[cj + r]);
+      const double a = cabs1(static_cast<double>(Fr[cj + r]), static_cast<double>(Fi[cj + r]));
       if (a > best) {
         best = a;
         bi = r;
@@ -154,9 +178,9 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__
     const int piv = si[0];
     const bool small = !(sv[0] > tiny);
     __syncthreads();
-    if (!small && piv != j && tid < kb) {
-      const long long col = static_cast<long long>(k0 + tid) * d.m;
-      double t = Fr[col + j]; Fr[col + j] = Fr[col + piv]; Fr[col + piv] = t;
+    if (!small && piv != j && tid < K1 - K0) {
+      const long long col = static_cast<long long>(K0 + tid) * d.m;
+      T t = Fr[col + j]; Fr[col + j] = Fr[col + piv]; Fr[col + piv] = t;
       t = Fi[col + j]; Fi[col + j] = Fi[col + piv]; Fi[col + piv] = t;
     }
     __syncthreads();
@@ -167,13 +191,13 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__
         // static pivot perturbation: refinement recovers the accuracy
         pr = pr >= 0.0 ? perturb : -perturb;
         pi = 0.0;
-        Fr[cj + j] = pr;
-        Fi[cj + j] = pi;
+        Fr[cj + j] = static_cast<T>(pr);
+        Fi[cj + j] = static_cast<T>(pi);
         atomicAdd(nperturbed, 1);
       }
       const double den = pr * pr + pi * pi;
-      inv_r = pr / den;
-      inv_i = -pi / den;
+      inv_r = static_cast<T>(pr / den);
+      inv_i = static_cast<T>(-pi / den);
     }
     if (tid < k0 + kb - j - 1) {
       const long long cc = static_cast<long long>(j + 1 + tid) * d.m + j;
@@ -181,16 +205,16 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__
       prow_i[tid] = Fi[cc];
     }
     __syncthreads();
-    const double ir = inv_r, ii = inv_i;
+    const T ir = inv_r, ii = inv_i;
     const int ncol = k0 + kb - j - 1;
     for (int r = j + 1 + tid; r < d.p; r += kPanelThreads) {
-      const double ar = Fr[cj + r], ai = Fi[cj + r];
-      const double lr = ar * ir - ai * ii, li = ar * ii + ai * ir;
+      const T ar = Fr[cj + r], ai = Fi[cj + r];
+      const T lr = ar * ir - ai * ii, li = ar * ii + ai * ir;
       Fr[cj + r] = lr;
       Fi[cj + r] = li;
       for (int q = 0; q < ncol; ++q) {
         const long long cq = static_cast<long long>(j + 1 + q) * d.m + r;
-        const double ur = prow_r[q], ui = prow_i[q];
+        const T ur = prow_r[q], ui = prow_i[q];
         Fr[cq] -= lr * ur - li * ui;
         Fi[cq] -= lr * ui + li * ur;
       }
@@ -201,16 +225,17 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__
 
 // Update rows [p, m) of the panel columns: X <- X U_D^-1 with U_D the upper
 // kb x kb pivot block (row-wise forward substitution, thread per row).
+template <typename T>
 __global__ __launch_bounds__(256) void k_l21(const int *__restrict__ list, int nlist,
                                              const FrontDev *__restrict__ fr, int k0, int nb,
-                                             double *__restrict__ arena) {
+                                             T *__restrict__ arena) {
   const int f = list[blockIdx.y];
   const FrontDev d = fr[f];
   if (k0 >= d.p || d.c == 0) return;
   const int kb = min(nb, d.p - k0);
   const long long mm = static_cast<long long>(d.m) * d.m;
-  double *Fr = arena + d.off, *Fi = Fr + mm;
-  __shared__ double Ur[kLuNb * kLuNb], Ui[kLuNb * kLuNb], Dr[kLuNb], Di[kLuNb];
+  T *Fr = arena + d.off, *Fi = Fr + mm;
+  __shared__ T Ur[kLuNb * kLuNb], Ui[kLuNb * kLuNb], Dr[kLuNb], Di[kLuNb];
   for (int k = threadIdx.x; k < kb * kb; k += blockDim.x) {
     const int r = k % kb, s = k / kb;
     const long long idx = static_cast<long long>(k0 + s) * d.m + k0 + r;
@@ -221,22 +246,22 @@ __global__ __launch_bounds__(256) void k_l21(const int *__restrict__ list, int n
   if (threadIdx.x < kb) {
     const double pr = Ur[threadIdx.x * kLuNb + threadIdx.x], pi = Ui[threadIdx.x * kLuNb + threadIdx.x];
     const double den = pr * pr + pi * pi;
-    Dr[threadIdx.x] = pr / den;
-    Di[threadIdx.x] = -pi / den;
+    Dr[threadIdx.x] = static_cast<T>(pr / den);
+    Di[threadIdx.x] = static_cast<T>(-pi / den);
   }
   __syncthreads();
   const int row = d.p + blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= d.m) return;
-  double xr[kLuNb], xi[kLuNb];
+  T xr[kLuNb], xi[kLuNb];
 #pragma unroll
   for (int j = 0; j < kLuNb; ++j) {
     if (j < kb) {
       const long long idx = static_cast<long long>(k0 + j) * d.m + row;
-      double ar = Fr[idx], ai = Fi[idx];
+      T ar = Fr[idx], ai = Fi[idx];
 #pragma unroll
       for (int i = 0; i < kLuNb; ++i) {
         if (i < j) {
-          const double ur = Ur[j * kLuNb + i], ui = Ui[j * kLuNb + i];
+          const T ur = Ur[j * kLuNb + i], ui = Ui[j * kLuNb + i];
           ar -= xr[i] * ur - xi[i] * ui;
           ai -= xr[i] * ui + xi[i] * ur;
         }
@@ -249,53 +274,68 @@ __global__ __launch_bounds__(256) void k_l21(const int *__restrict__ list, int n
   }
 }
 
-// Rows [k0, k0+kb) of the F columns right of the panel and of all E columns:
-// apply the panel's row exchanges to the column, then X <- L_D^-1 X with L_D
-// the unit lower kb x kb pivot block.
+// Panel rows [k0, k0+kb) of a set of columns: X <- L_D^-1 X with L_D the
+// unit lower kb x kb pivot block.  Thread per column.
+//   mode 0: F columns [k0+kb, K1) of the current block (exchanges already
+//           applied by k_panel);
+//   mode 1: F columns [K1, m) and all p columns of E; with `swaps`, the
+//           row exchanges of rows [K0, K1) are applied first, in order.
+template <typename T>
 __global__ __launch_bounds__(256) void k_trsm_lower(const int *__restrict__ list, int nlist,
                                                     const FrontDev *__restrict__ fr, int k0, int nb,
+                                                    int K0, int NB, int mode, int swaps,
                                                     const int *__restrict__ ipiv,
-                                                    double *__restrict__ arena) {
+                                                    T *__restrict__ arena) {
   const int f = list[blockIdx.y];
   const FrontDev d = fr[f];
   if (k0 >= d.p) return;
   const int kb = min(nb, d.p - k0);
+  const int K1 = min(K0 + NB, d.p);
   const long long mm = static_cast<long long>(d.m) * d.m;
-  double *Fr = arena + d.off, *Fi = Fr + mm;
-  double *Er = Fi + mm, *Ei = Er + static_cast<long long>(d.p) * d.p;
-  __shared__ double Lr[kLuNb * kLuNb], Li[kLuNb * kLuNb];
-  __shared__ int pv[kLuNb];
+  T *Fr = arena + d.off, *Fi = Fr + mm;
+  T *Er = Fi + mm, *Ei = Er + static_cast<long long>(d.p) * d.p;
+  __shared__ T Lr[kLuNb * kLuNb], Li[kLuNb * kLuNb];
+  __shared__ int pv[kLuNbOuter];
   for (int k = threadIdx.x; k < kb * kb; k += blockDim.x) {
     const int r = k % kb, s = k / kb;
     const long long idx = static_cast<long long>(k0 + s) * d.m + k0 + r;
     Lr[s * kLuNb + r] = Fr[idx];
     Li[s * kLuNb + r] = Fi[idx];
   }
-  if (threadIdx.x < kb) pv[threadIdx.x] = ipiv[d.piv_off + k0 + threadIdx.x];
+  if (swaps && threadIdx.x < K1 - K0) pv[threadIdx.x] = ipiv[d.piv_off + K0 + threadIdx.x];
   __syncthreads();
-  const int nF = d.m - k0 - kb;
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= nF + d.p) return;
-  double *Xr, *Xi;
-  if (col < nF) {
+  T *Xr, *Xi;
+  if (mode == 0) {
+    if (col >= K1 - (k0 + kb)) return;
     const long long base = static_cast<long long>(k0 + kb + col) * d.m;
     Xr = Fr + base;
     Xi = Fi + base;
   } else {
-    const long long base = static_cast<long long>(col - nF) * d.p;
-    Xr = Er + base;
-    Xi = Ei + base;
-  }
-  for (int j = 0; j < kb; ++j) {
-    const int pr = pv[j], r = k0 + j;
-    if (pr != r) {
-      double t = Xr[r]; Xr[r] = Xr[pr]; Xr[pr] = t;
-      t = Xi[r]; Xi[r] = Xi[pr]; Xi[pr] = t;
+    const int nF = d.m - K1;
+    if (col >= nF + d.p) return;
+    if (col < nF) {
+      const long long base = static_cast<long long>(K1 + col) * d.m;
+      Xr = Fr + base;
+      Xi = Fi + base;
+    } else {
+      const long long base = static_cast<long long>(col - nF) * d.p;
+      Xr = Er + base;
+      Xi = Ei + base;
+    }
+    if (swaps) {
+      for (int r = K0; r < K1; ++r) {
+        const int pr = pv[r - K0];
+        if (pr != r) {
+          T t = Xr[r]; Xr[r] = Xr[pr]; Xr[pr] = t;
+          t = Xi[r]; Xi[r] = Xi[pr]; Xi[pr] = t;
+        }
+      }
     }
   }
   Xr += k0;
   Xi += k0;
-  double xr[kLuNb], xi[kLuNb];
+  T xr[kLuNb], xi[kLuNb];
 #pragma unroll
   for (int r = 0; r < kLuNb; ++r) {
     if (r < kb) {
@@ -306,11 +346,11 @@ __global__ __launch_bounds__(256) void k_trsm_lower(const int *__restrict__ list
 #pragma unroll
   for (int r = 1; r < kLuNb; ++r) {
     if (r < kb) {
-      double ar = xr[r], ai = xi[r];
+      T ar = xr[r], ai = xi[r];
 #pragma unroll
       for (int s = 0; s < kLuNb; ++s) {
         if (s < r) {
-          const double lr = Lr[s * kLuNb + r], li = Li[s * kLuNb + r];
+          const T lr = Lr[s * kLuNb + r], li = Li[s * kLuNb + r];
           ar -= lr * xr[s] - li * xi[s];
           ai -= lr * xi[s] + li * xr[s];
         }
@@ -330,18 +370,19 @@ __global__ __launch_bounds__(256) void k_trsm_lower(const int *__restrict__ list
 
 // Rows [k0, k0+kb) of Z columns [k0, p): X <- U_D^-1 X with U_D the upper
 // (non-unit) kb x kb pivot block.
+template <typename T>
 __global__ __launch_bounds__(256) void k_trsm_upper(const int *__restrict__ list, int nlist,
                                                     const FrontDev *__restrict__ fr, int k0, int nb,
-                                                    double *__restrict__ arena) {
+                                                    T *__restrict__ arena) {
   const int f = list[blockIdx.y];
   const FrontDev d = fr[f];
   if (k0 >= d.p) return;
   const int kb = min(nb, d.p - k0);
   const long long mm = static_cast<long long>(d.m) * d.m;
-  const double *Fr = arena + d.off, *Fi = Fr + mm;
-  double *Zr = arena + d.off + 2 * mm + 2LL * d.p * d.p;
-  double *Zi = Zr + static_cast<long long>(d.p) * d.p;
-  __shared__ double Ur[kLuNb * kLuNb], Ui[kLuNb * kLuNb];
+  const T *Fr = arena + d.off, *Fi = Fr + mm;
+  T *Zr = arena + d.off + 2 * mm + 2LL * d.p * d.p;
+  T *Zi = Zr + static_cast<long long>(d.p) * d.p;
+  __shared__ T Ur[kLuNb * kLuNb], Ui[kLuNb * kLuNb];
   for (int k = threadIdx.x; k < kb * kb; k += blockDim.x) {
     const int r = k % kb, s = k / kb;
     const long long idx = static_cast<long long>(k0 + s) * d.m + k0 + r;
@@ -351,9 +392,9 @@ __global__ __launch_bounds__(256) void k_trsm_upper(const int *__restrict__ list
   __syncthreads();
   const int col = k0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= d.p) return;
-  double *Xr = Zr + static_cast<long long>(col) * d.p + k0;
-  double *Xi = Zi + static_cast<long long>(col) * d.p + k0;
-  double xr[kLuNb], xi[kLuNb];
+  T *Xr = Zr + static_cast<long long>(col) * d.p + k0;
+  T *Xi = Zi + static_cast<long long>(col) * d.p + k0;
+  T xr[kLuNb], xi[kLuNb];
 #pragma unroll
   for (int r = 0; r < kLuNb; ++r) {
     if (r < kb) {
@@ -364,17 +405,17 @@ __global__ __launch_bounds__(256) void k_trsm_upper(const int *__restrict__ list
 #pragma unroll
   for (int r = kLuNb - 1; r >= 0; --r) {
     if (r < kb) {
-      double ar = xr[r], ai = xi[r];
+      T ar = xr[r], ai = xi[r];
 #pragma unroll
       for (int s = 0; s < kLuNb; ++s) {
         if (s > r && s < kb) {
-          const double ur = Ur[s * kLuNb + r], ui = Ui[s * kLuNb + r];
+          const T ur = Ur[s * kLuNb + r], ui = Ui[s * kLuNb + r];
           ar -= ur * xr[s] - ui * xi[s];
           ai -= ur * xi[s] + ui * xr[s];
         }
       }
-      const double dr = Ur[r * kLuNb + r], di = Ui[r * kLuNb + r];
-      const double den = dr * dr + di * di;
+      const T dr = Ur[r * kLuNb + r], di = Ui[r * kLuNb + r];
+      const T den = dr * dr + di * di;
       xr[r] = (ar * dr + ai * di) / den;
       xi[r] = (ai * dr - ar * di) / den;
     }
@@ -388,226 +429,302 @@ __global__ __launch_bounds__(256) void k_trsm_upper(const int *__restrict__ list
   }
 }
 
-// Batched complex GEMM  C -= A B  on the update regions of one panel step:
-//   region 0: F[k0+kb:m, k0+kb:m] -= F[k0+kb:m, panel] F[panel, k0+kb:m]
-//   region 1: E[k0+kb:p, :]       -= F[k0+kb:p, panel] E[panel, :]
-//   region 2: Z[0:k0, k0:p]       -= F[0:k0, panel]    Z[panel, k0:p]
-// 64x64 output tile per workgroup, 4x4 complex outputs per thread, K staged
-// through LDS in chunks of 16.
-constexpr int GT = 64, GK = 16;
+// ------------------------------------------------------- MFMA complex GEMM
+//
+// C -= A B on split (re, im) column-major planes, batched over the fronts
+// of a level.  A workgroup computes a 64 x 64 complex tile with four waves
+// (2 x 2, 32 x 32 each = 2 x 2 MFMA 16 x 16 blocks, real and imaginary
+// accumulators), K staged through LDS in chunks of 16 with the next chunk's
+// global loads in flight during the MFMAs.
+//
+// The 16x16x4 MFMAs are issued as D^T = B^T A^T: lane l supplies
+// B[k = l/16][j = l%16] as the first operand and A[i = l%16][k = l/16] as
+// the second, which leaves each lane holding outputs of one row i = l%16,
+// so each accumulator element stores 16 consecutive rows of a column.
+// Output column of accumulator element v (lane l):
+//   f64 16x16x4: 4 v + l/16        f32 16x16x4: 4 (l/16) + v
+// (lane layout checked on gfx950 with value-coded operands).
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  static __device__ inline acc_t mma(double x, double y, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+  }
+  static __device__ inline int ocol(int l, int v) { return 4 * v + l / 16; }
+};
+template <> struct Mfma<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  static __device__ inline acc_t mma(float x, float y, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0);
+  }
+  static __device__ inline int ocol(int l, int v) { return 4 * (l / 16) + v; }
+};
 
-__global__ __launch_bounds__(256) void k_gemm_update(const int *__restrict__ list, int nlist,
-                                                     const FrontDev *__restrict__ fr, int k0, int nb,
-                                                     int region, double *__restrict__ arena) {
+enum GemmRegion { kGemmInner = 0, kGemmBlkRows = 1, kGemmTrail = 2, kGemmZ = 3 };
+
+// Operands of one front's update.  Columns [0, N1) of B and C come from the
+// first segment (rows [0, M) of C), columns [N1, N) from the second (rows
+// [0, M2)); A is shared.
+template <typename T> struct GemmView {
+  const T *Ar, *Ai;
+  long long lda;
+  const T *Br, *Bi, *B2r, *B2i;
+  long long ldb, ldb2;
+  T *Cr, *Ci, *C2r, *C2i;
+  long long ldc, ldc2;
+  int M, M2, N, N1, K;
+};
+
+template <typename T>
+__device__ inline bool gemm_view(int region, const FrontDev &d, int k0, int nb, int K0, int NB,
+                                 T *arena, GemmView<T> &v) {
+  if (region == kGemmTrail ? K0 >= d.p : k0 >= d.p) return false;
+  const long long m = d.m, p = d.p, mm = m * m, pp = p * p;
+  T *Fr = arena + d.off, *Fi = Fr + mm;
+  T *Er = Fi + mm, *Ei = Er + pp;
+  T *Zr = Ei + pp, *Zi = Zr + pp;
+  const int kb = min(nb, d.p - k0);
+  const int K1 = min(K0 + NB, d.p);
+  switch (region) {
+    case kGemmInner: {
+      const long long a = k0 * m + k0 + kb, b = (k0 + kb) * m + k0, c = (k0 + kb) * m + k0 + kb;
+      v = {Fr + a, Fi + a, m, Fr + b, Fi + b, nullptr, nullptr, m, 0,
+           Fr + c, Fi + c, nullptr, nullptr, m, 0,
+           d.m - k0 - kb, d.m - k0 - kb, K1 - k0 - kb, K1 - k0 - kb, kb};
+      break;
+    }
+    case kGemmBlkRows: {
+      const long long a = k0 * m + k0 + kb, b = K1 * m + k0, c = K1 * m + k0 + kb;
+      v = {Fr + a, Fi + a, m, Fr + b, Fi + b, Er + k0, Ei + k0, m, p,
+           Fr + c, Fi + c, Er + k0 + kb, Ei + k0 + kb, m, p,
+           K1 - k0 - kb, K1 - k0 - kb, d.m - K1 + d.p, d.m - K1, kb};
+      break;
+    }
+    case kGemmTrail: {
+      const long long a = K0 * m + K1, b = K1 * m + K0, c = K1 * m + K1;
+      v = {Fr + a, Fi + a, m, Fr + b, Fi + b, Er + K0, Ei + K0, m, p,
+           Fr + c, Fi + c, Er + K1, Ei + K1, m, p,
+           d.m - K1, d.p - K1, d.m - K1 + d.p, d.m - K1, K1 - K0};
+      break;
+    }
+    default: {
+      const long long a = k0 * m, b = k0 * p + k0, c = k0 * p;
+      v = {Fr + a, Fi + a, m, Zr + b, Zi + b, nullptr, nullptr, p, 0,
+           Zr + c, Zi + c, nullptr, nullptr, p, 0,
+           k0, k0, d.p - k0, d.p - k0, kb};
+      break;
+    }
+  }
+  return v.M > 0 && v.N > 0 && v.K > 0;
+}
+
+constexpr int GT = 64;   // output tile (complex) per workgroup
+constexpr int GKC = 16;  // K chunk staged in LDS
+constexpr int GLD = 80;  // LDS row stride: 64 + 16 keeps the k rows of a wave's reads in different banks
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm(const int *__restrict__ list, int nlist,
+                                              const FrontDev *__restrict__ fr, int region, int k0,
+                                              int nb, int K0, int NB, T *__restrict__ arena) {
   const int f = list[blockIdx.y];
   const FrontDev d = fr[f];
-  if (k0 >= d.p) return;
-  const int kb = min(nb, d.p - k0);
-  const long long mm = static_cast<long long>(d.m) * d.m, pp = static_cast<long long>(d.p) * d.p;
-  double *Fr = arena + d.off, *Fi = Fr + mm;
-  double *Er = Fi + mm, *Ei = Er + pp;
-  double *Zr = Ei + pp, *Zi = Zr + pp;
-  int M, N;
-  const double *Ar, *Ai, *Br, *Bi;
-  double *Cr, *Ci;
-  long long lda, ldb, ldc;
-  if (region == 0) {
-    M = N = d.m - k0 - kb;
-    Ar = Fr + static_cast<long long>(k0) * d.m + k0 + kb;
-    Ai = Fi + static_cast<long long>(k0) * d.m + k0 + kb;
-    Br = Fr + static_cast<long long>(k0 + kb) * d.m + k0;
-    Bi = Fi + static_cast<long long>(k0 + kb) * d.m + k0;
-    Cr = Fr + static_cast<long long>(k0 + kb) * d.m + k0 + kb;
-    Ci = Fi + static_cast<long long>(k0 + kb) * d.m + k0 + kb;
-    lda = ldb = ldc = d.m;
-  } else if (region == 1) {
-    M = d.p - k0 - kb;
-    N = d.p;
-    Ar = Fr + static_cast<long long>(k0) * d.m + k0 + kb;
-    Ai = Fi + static_cast<long long>(k0) * d.m + k0 + kb;
-    Br = Er + k0;
-    Bi = Ei + k0;
-    Cr = Er + k0 + kb;
-    Ci = Ei + k0 + kb;
-    lda = d.m;
-    ldb = ldc = d.p;
-  } else {
-    M = k0;
-    N = d.p - k0;
-    Ar = Fr + static_cast<long long>(k0) * d.m;
-    Ai = Fi + static_cast<long long>(k0) * d.m;
-    Br = Zr + static_cast<long long>(k0) * d.p + k0;
-    Bi = Zi + static_cast<long long>(k0) * d.p + k0;
-    Cr = Zr + static_cast<long long>(k0) * d.p;
-    Ci = Zi + static_cast<long long>(k0) * d.p;
-    lda = d.m;
-    ldb = ldc = d.p;
-  }
-  if (M <= 0 || N <= 0) return;
-  const int tiles_m = (M + GT - 1) / GT, tiles_n = (N + GT - 1) / GT;
+  GemmView<T> v;
+  if (!gemm_view(region, d, k0, nb, K0, NB, arena, v)) return;
+  const int tiles_m = (v.M + GT - 1) / GT, tiles_n = (v.N + GT - 1) / GT;
   const int tile = blockIdx.x;
   if (tile >= tiles_m * tiles_n) return;
-  const int tm = tile % tiles_m, tn = tile / tiles_m;
-  const int row0 = tm * GT, col0 = tn * GT;
-  __shared__ double sAr[GK][GT + 1], sAi[GK][GT + 1], sBr[GK][GT + 1], sBi[GK][GT + 1];
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  double cr[4][4], ci[4][4];
+  const int row0 = (tile % tiles_m) * GT, col0 = (tile / tiles_m) * GT;
+  __shared__ T sAr[GKC][GLD], sAi[GKC][GLD], sBr[GKC][GLD], sBi[GKC][GLD];
+  const int t = threadIdx.x, l = t % 64, w = t / 64;
+  const int wm = w % 2, wn = w / 2;
+  // global -> register staging: A (row t%64, k t/64 + 4s), B (k t%16, col t/16 + 16s)
+  const int ar_ = t % 64, ak_ = t / 64;
+  const int bk_ = t % 16, bc_ = t / 16;
+  const bool arow_ok = row0 + ar_ < v.M;
+  const T *bcol_r[4], *bcol_i[4];
+  bool bcol_ok[4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int s = 0; s < 4; ++s) {
+    const int col = col0 + bc_ + 16 * s;
+    bcol_ok[s] = col < v.N;
+    if (col < v.N1) {
+      bcol_r[s] = v.Br + col * v.ldb;
+      bcol_i[s] = v.Bi + col * v.ldb;
+    } else {
+      const long long c2 = bcol_ok[s] ? (col - v.N1) * v.ldb2 : 0;
+      bcol_r[s] = v.B2r + c2;
+      bcol_i[s] = v.B2i + c2;
+    }
+  }
+  T ga_r[4], ga_i[4], gb_r[4], gb_i[4];
+  auto fetch = [&](int kc) {
 #pragma unroll
-    for (int b = 0; b < 4; ++b) cr[a][b] = ci[a][b] = 0.0;
-  for (int kk = 0; kk < kb; kk += GK) {
-    for (int e = threadIdx.x; e < GT * GK; e += 256) {
-      const int r = e % GT, k = e / GT;
-      const bool ok = row0 + r < M && kk + k < kb;
-      const long long ia = static_cast<long long>(kk + k) * lda + row0 + r;
-      sAr[k][r] = ok ? Ar[ia] : 0.0;
-      sAi[k][r] = ok ? Ai[ia] : 0.0;
-      const int cb = e / GK, kb2 = e % GK;
-      const bool okb = col0 + cb < N && kk + kb2 < kb;
-      const long long ib = static_cast<long long>(col0 + cb) * ldb + kk + kb2;
-      sBr[kb2][cb] = okb ? Br[ib] : 0.0;
-      sBi[kb2][cb] = okb ? Bi[ib] : 0.0;
+    for (int s = 0; s < 4; ++s) {
+      const int ka = kc + ak_ + 4 * s;
+      const bool oka = arow_ok && ka < v.K;
+      const long long ia = static_cast<long long>(ka) * v.lda + row0 + ar_;
+      ga_r[s] = oka ? v.Ar[ia] : T(0);
+      ga_i[s] = oka ? v.Ai[ia] : T(0);
+      const int kbb = kc + bk_;
+      const bool okb = bcol_ok[s] && kbb < v.K;
+      gb_r[s] = okb ? bcol_r[s][kbb] : T(0);
+      gb_i[s] = okb ? bcol_i[s][kbb] : T(0);
+    }
+  };
+  using acc_t = typename Mfma<T>::acc_t;
+  acc_t cr[2][2], ci[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      cr[a][b] = acc_t{0, 0, 0, 0};
+      ci[a][b] = acc_t{0, 0, 0, 0};
+    }
+  fetch(0);
+  for (int kc = 0; kc < v.K; kc += GKC) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      sAr[ak_ + 4 * s][ar_] = ga_r[s];
+      sAi[ak_ + 4 * s][ar_] = ga_i[s];
+      sBr[bk_][bc_ + 16 * s] = gb_r[s];
+      sBi[bk_][bc_ + 16 * s] = gb_i[s];
     }
     __syncthreads();
+    if (kc + GKC < v.K) fetch(kc + GKC);
 #pragma unroll
-    for (int k = 0; k < GK; ++k) {
-      double ar[4], ai[4], br[4], bi[4];
+    for (int kk = 0; kk < GKC; kk += 4) {
+      const int kr = kk + l / 16;
+      T a_r[2], a_i[2], b_r[2], b_i[2];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        ar[a] = sAr[k][ty + 16 * a];
-        ai[a] = sAi[k][ty + 16 * a];
-        br[a] = sBr[k][tx + 16 * a];
-        bi[a] = sBi[k][tx + 16 * a];
+      for (int q = 0; q < 2; ++q) {
+        a_r[q] = sAr[kr][wm * 32 + q * 16 + l % 16];
+        a_i[q] = sAi[kr][wm * 32 + q * 16 + l % 16];
+        b_r[q] = sBr[kr][wn * 32 + q * 16 + l % 16];
+        b_i[q] = sBi[kr][wn * 32 + q * 16 + l % 16];
       }
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          cr[a][b] += ar[a] * br[b] - ai[a] * bi[b];
-          ci[a][b] += ar[a] * bi[b] + ai[a] * br[b];
+        for (int b = 0; b < 2; ++b) {
+          cr[a][b] = Mfma<T>::mma(b_r[b], a_r[a], cr[a][b]);
+          cr[a][b] = Mfma<T>::mma(b_i[b], -a_i[a], cr[a][b]);
+          ci[a][b] = Mfma<T>::mma(b_i[b], a_r[a], ci[a][b]);
+          ci[a][b] = Mfma<T>::mma(b_r[b], a_i[a], ci[a][b]);
         }
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int r = row0 + ty + 16 * a;
-    if (r >= M) continue;
+  for (int a = 0; a < 2; ++a) {
+    const int row = row0 + wm * 32 + a * 16 + l % 16;
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int cidx = col0 + tx + 16 * b;
-      if (cidx >= N) continue;
-      const long long ic = static_cast<long long>(cidx) * ldc + r;
-      Cr[ic] -= cr[a][b];
-      Ci[ic] -= ci[a][b];
+    for (int b = 0; b < 2; ++b) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int col = col0 + wn * 32 + b * 16 + Mfma<T>::ocol(l, e);
+        if (col >= v.N) continue;
+        if (col < v.N1) {
+          if (row >= v.M) continue;
+          const long long ic = static_cast<long long>(col) * v.ldc + row;
+          v.Cr[ic] -= cr[a][b][e];
+          v.Ci[ic] -= ci[a][b][e];
+        } else {
+          if (row >= v.M2) continue;
+          const long long ic = static_cast<long long>(col - v.N1) * v.ldc2 + row;
+          v.C2r[ic] -= cr[a][b][e];
+          v.C2i[ic] -= ci[a][b][e];
+        }
+      }
     }
   }
 }
 
 // ------------------------------------------------------------------ solves
 //
-// Vectors are split complex planes in elimination (dof) order:
-// position 5*pos[node] + dof.  Per-front workspace W (m entries per front).
+// Vectors are split complex planes in elimination (dof) order: position
+// 5*pos[node] + dof.  The per-front workspace W (m entries per front)
+// collects the right-hand side of the front's pivots (W[0:p]) and the
+// contributions passing through its update rows (W[p:m]):
+//
+//   forward, levels bottom-up   op 0  Y_f = E_f W_f[0:p]
+//                               op 1  W_parent[rel_f] += W_f[p:m] - L21_f Y_f
+//   backward, levels top-down   op 2  Y_f -= U12_f x[update rows of f]
+//                               op 3  x_f  = Z_f Y_f
+//
+// k_solve_init sets W = [b; 0], Y = 0 and x = 0 for every front in one
+// launch; all GEMV outputs are accumulated with FP64 atomics.
 
-// Forward set-up of a level: W_f = [b_f; 0] + sum over children of their
-// update vectors (extend-add).
-__global__ void k_fwd_init(const int *__restrict__ list, int nlist, const FrontDev *__restrict__ fr,
-                           const int *__restrict__ child_ptr, const int *__restrict__ child_list,
-                           const int *__restrict__ rel, const long long *__restrict__ rel_off,
-                           const double *__restrict__ br, const double *__restrict__ bi,
-                           double *__restrict__ Wr, double *__restrict__ Wi,
-                           double *__restrict__ Yr, double *__restrict__ Yi) {
-  const int f = list[blockIdx.x];
-  const FrontDev d = fr[f];
-  double *wr = Wr + d.woff, *wi = Wi + d.woff;
-  for (int k = threadIdx.x; k < d.m; k += blockDim.x) {
-    wr[k] = k < d.p ? br[d.dof0 + k] : 0.0;
-    wi[k] = k < d.p ? bi[d.dof0 + k] : 0.0;
-    if (k < d.p) {
-      Yr[d.dof0 + k] = 0.0;  // output of the E product
-      Yi[d.dof0 + k] = 0.0;
-    }
-  }
-  __syncthreads();
-  for (int q = child_ptr[f]; q < child_ptr[f + 1]; ++q) {
-    const int ch = child_list[q];
-    const FrontDev dc = fr[ch];
-    const int *rl = rel + rel_off[ch];
-    const double *ur = Wr + dc.woff + dc.p, *ui = Wi + dc.woff + dc.p;
-    for (int k = threadIdx.x; k < dc.c; k += blockDim.x) {
-      const int t = 5 * rl[k / 5] + k % 5;
-      wr[t] += ur[k];
-      wi[t] += ui[k];
-    }
-    __syncthreads();
-  }
-}
-
-// Backward set-up of a level: W_f[p:] = x[update rows of f] (ancestors,
-// already solved), T_f = y_f, and x_f = 0 (accumulated by the Z product).
-__global__ void k_bwd_gather(const int *__restrict__ list, int nlist, const FrontDev *__restrict__ fr,
-                             const int *__restrict__ upd_pos, const long long *__restrict__ upd_off,
-                             double *__restrict__ xr, double *__restrict__ xi,
+__global__ void k_solve_init(const FrontDev *__restrict__ fr, const double *br, const double *bi,
                              double *__restrict__ Wr, double *__restrict__ Wi,
-                             const double *__restrict__ Yr, const double *__restrict__ Yi,
-                             double *__restrict__ Tr, double *__restrict__ Ti) {
-  const int f = list[blockIdx.x];
-  const FrontDev d = fr[f];
-  const int *up = upd_pos + upd_off[f];
-  for (int k = threadIdx.x; k < d.c; k += blockDim.x) {
-    const int g = 5 * up[k / 5] + k % 5;
-    Wr[d.woff + d.p + k] = xr[g];
-    Wi[d.woff + d.p + k] = xi[g];
-  }
-  for (int k = threadIdx.x; k < d.p; k += blockDim.x) {
-    Tr[d.dof0 + k] = Yr[d.dof0 + k];
-    Ti[d.dof0 + k] = Yi[d.dof0 + k];
-    xr[d.dof0 + k] = 0.0;
-    xi[d.dof0 + k] = 0.0;
+                             double *__restrict__ Yr, double *__restrict__ Yi, double *xr,
+                             double *xi) {
+  const FrontDev d = fr[blockIdx.x];
+  for (int k = threadIdx.x; k < d.m; k += blockDim.x) {
+    const bool piv = k < d.p;
+    // b may alias x: each dof is read and then cleared by the same thread
+    const double vr = piv ? br[d.dof0 + k] : 0.0, vi = piv ? bi[d.dof0 + k] : 0.0;
+    Wr[d.woff + k] = vr;
+    Wi[d.woff + k] = vi;
+    if (piv) {
+      Yr[d.dof0 + k] = 0.0;
+      Yi[d.dof0 + k] = 0.0;
+      xr[d.dof0 + k] = 0.0;
+      xi[d.dof0 + k] = 0.0;
+    }
   }
 }
 
-// Batched complex GEMV over the fronts of a level, y (+)= alpha * A x.
-//   op 0: y = b0 + E W[0:p]          (E = L11^-1 P; y written to the
-//                                     solution plane at the front's dofs)
-//   op 1: W[p:m] -= L21 y            (y read from the solution plane)
-//   op 2: T = y - U12 W[p:m]         (T in the scratch plane at dof0)
-//   op 3: x = Z T                    (x written at the front's dofs)
-// Output rows are split over blockIdx.x (GVR rows per workgroup) and the
-// reduction dimension over blockIdx.z (GVC columns); partial sums are added
-// with FP64 atomics into an output that ops 0, 2, 3 first initialise
-// (k_fwd_init / k_bwd_gather initialise them).
 constexpr int GVR = 64, GVC = 64, GVW = 4;  // rows per WG, columns per wave, waves per WG
+
+// One batched GEMV y (+)= add + sign * A x.  xmap / ymap, when set, gather x
+// and scatter y through node maps (entry j -> 5 * map[j / 5] + j % 5).
 struct GemvView {
   const double *Ar, *Ai;
   long long lda;
   int M, N;
   const double *xr, *xi;
+  const int *xmap;
   double *yr, *yi;
+  const int *ymap;
+  const double *addr, *addi;
   double sign;
+  bool upper;  // A is upper triangular (skip tiles below the diagonal)
 };
 
-__device__ inline bool gemv_view(int op, const FrontDev &d, double *arena, double *Wr, double *Wi,
-                                 double *Sr, double *Si, double *Xr, double *Xi, GemvView &v) {
+struct SolveMaps {
+  const int *parent, *rel, *upd_pos;
+  const long long *rel_off, *upd_off;
+};
+
+__device__ inline bool gemv_view(int op, const FrontDev *fr, int f, const FrontDev &d,
+                                 const SolveMaps &mp, double *arena, double *Wr, double *Wi,
+                                 double *Yr, double *Yi, double *Xr, double *Xi, GemvView &v) {
   const long long mm = static_cast<long long>(d.m) * d.m, pp = static_cast<long long>(d.p) * d.p;
   double *Fr = arena + d.off, *Fi = Fr + mm;
   double *Er = Fi + mm, *Ei = Er + pp;
   double *Zr = Ei + pp, *Zi = Zr + pp;
   switch (op) {
     case 0:
-      v = {Er, Ei, d.p, d.p, d.p, Wr + d.woff, Wi + d.woff, Sr + d.dof0, Si + d.dof0, 1.0};
+      v = {Er, Ei, d.p, d.p, d.p, Wr + d.woff, Wi + d.woff, nullptr, Yr + d.dof0, Yi + d.dof0,
+           nullptr, nullptr, nullptr, 1.0, false};
       return d.p > 0;
-    case 1:
-      v = {Fr + d.p, Fi + d.p, d.m, d.c, d.p, Sr + d.dof0, Si + d.dof0, Wr + d.woff + d.p,
-           Wi + d.woff + d.p, -1.0};
-      return d.c > 0 && d.p > 0;
-    case 2:
-      v = {Fr + static_cast<long long>(d.p) * d.m, Fi + static_cast<long long>(d.p) * d.m,
-           d.m, d.p, d.c, Wr + d.woff + d.p, Wi + d.woff + d.p, Sr + d.dof0, Si + d.dof0, -1.0};
-      return d.c > 0 && d.p > 0;
+    case 1: {
+      if (d.c == 0 || d.p == 0) return false;
+      const long long pw = fr[mp.parent[f]].woff;
+      v = {Fr + d.p, Fi + d.p, d.m, d.c, d.p, Yr + d.dof0, Yi + d.dof0, nullptr, Wr + pw, Wi + pw,
+           mp.rel + mp.rel_off[f], Wr + d.woff + d.p, Wi + d.woff + d.p, -1.0, false};
+      return true;
+    }
+    case 2: {
+      if (d.c == 0 || d.p == 0) return false;
+      const long long a = static_cast<long long>(d.p) * d.m;
+      v = {Fr + a, Fi + a, d.m, d.p, d.c, Xr, Xi, mp.upd_pos + mp.upd_off[f], Yr + d.dof0,
+           Yi + d.dof0, nullptr, nullptr, nullptr, -1.0, false};
+      return true;
+    }
     default:
-      v = {Zr, Zi, d.p, d.p, d.p, Sr + d.dof0, Si + d.dof0, Xr + d.dof0, Xi + d.dof0, 1.0};
+      v = {Zr, Zi, d.p, d.p, d.p, Yr + d.dof0, Yi + d.dof0, nullptr, Xr + d.dof0, Xi + d.dof0,
+           nullptr, nullptr, nullptr, 1.0, true};
       return d.p > 0;
   }
 }
@@ -616,29 +733,38 @@ __device__ inline bool gemv_view(int op, const FrontDev &d, double *arena, doubl
 // takes GVC of the columns and the four partial sums are combined in LDS
 // before one FP64 atomic per row.
 __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list, int nlist,
-                                                    const FrontDev *__restrict__ fr, int op,
-                                                    double *__restrict__ arena, double *Wr, double *Wi,
-                                                    double *Sr, double *Si, double *Xr, double *Xi) {
+                                                    const FrontDev *__restrict__ fr, SolveMaps mp,
+                                                    int op, double *__restrict__ arena, double *Wr,
+                                                    double *Wi, double *Yr, double *Yi, double *Xr,
+                                                    double *Xi) {
   const int f = list[blockIdx.y];
   const FrontDev d = fr[f];
   GemvView v;
-  if (!gemv_view(op, d, arena, Wr, Wi, Sr, Si, Xr, Xi, v)) return;
+  if (!gemv_view(op, fr, f, d, mp, arena, Wr, Wi, Yr, Yi, Xr, Xi, v)) return;
   const int lane = threadIdx.x % GVR, wave = threadIdx.x / GVR;
-  const int row = blockIdx.x * GVR + lane;
+  const int r0 = blockIdx.x * GVR;
+  const int row = r0 + lane;
   const int cw = blockIdx.z * GVW * GVC;  // first column of this workgroup
-  if (blockIdx.x * GVR >= v.M || cw >= v.N) return;
+  if (r0 >= v.M || cw >= v.N) return;
+  if (v.upper && cw + GVW * GVC <= r0) return;  // every entry of the tile is below the diagonal
   const int c0 = cw + wave * GVC;
   const int c1 = min(v.N, c0 + GVC);
   __shared__ double sxr[GVW * GVC], sxi[GVW * GVC];
   __shared__ double pr[GVW][GVR], pi[GVW][GVR];
   for (int k = threadIdx.x; k < GVW * GVC; k += GVR * GVW) {
     const int col = cw + k;
-    sxr[k] = col < v.N ? v.xr[col] : 0.0;
-    sxi[k] = col < v.N ? v.xi[col] : 0.0;
+    double xr = 0.0, xi = 0.0;
+    if (col < v.N) {
+      const int g = v.xmap ? 5 * v.xmap[col / 5] + col % 5 : col;
+      xr = v.xr[g];
+      xi = v.xi[g];
+    }
+    sxr[k] = xr;
+    sxi[k] = xi;
   }
   __syncthreads();
   double accr = 0.0, acci = 0.0;
-  if (row < v.M) {
+  if (row < v.M && !(v.upper && c1 <= r0)) {
     const double *ar = v.Ar + row, *ai = v.Ai + row;
     int k = c0;
     // 16 columns per step so that 32 loads per lane are in flight
@@ -674,8 +800,15 @@ __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list
       sr += pr[w][lane];
       si += pi[w][lane];
     }
-    atomicAdd(v.yr + row, v.sign * sr);
-    atomicAdd(v.yi + row, v.sign * si);
+    sr *= v.sign;
+    si *= v.sign;
+    if (v.addr && blockIdx.z == 0) {
+      sr += v.addr[row];
+      si += v.addi[row];
+    }
+    const int o = v.ymap ? 5 * v.ymap[row / 5] + row % 5 : row;
+    atomicAdd(v.yr + o, sr);
+    atomicAdd(v.yi + o, si);
   }
 }
 
@@ -704,6 +837,7 @@ hipError_t lu_factor(const LuPlan &P, const LuInputs &in, double sig_re, double 
                      sig_im, P.d_amap, P.d_front, P.arena);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, s, reinterpret_cast<double *>(P.d_counters), 1LL, 0.0);
+  const int nb = P.nb, NB = kLuNbOuter;
   for (size_t lev = 0; lev < P.levels.size(); ++lev) {
     const LevelPlan &L = P.levels[lev];
     for (int slot = 0; slot < 2; ++slot) {
@@ -715,66 +849,78 @@ hipError_t lu_factor(const LuPlan &P, const LuInputs &in, double sig_re, double 
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     const int *list = P.d_level_list + L.list_off;
-    for (int k0 = 0; k0 < L.pmax; k0 += P.nb) {
-      hipLaunchKernelGGL(k_panel, dim3(L.count), dim3(kPanelThreads), 0, s, list, L.count,
-                         P.d_front, k0, P.nb, tiny, perturb, P.arena, P.d_ipiv, P.d_counters);
-      if (L.cmax > 0)
-        hipLaunchKernelGGL(k_l21, dim3(cdivu(L.cmax, 256), L.count), dim3(256), 0, s, list,
-                           L.count, P.d_front, k0, P.nb, P.arena);
-      const int cols = L.mmax + L.pmax;
-      hipLaunchKernelGGL(k_trsm_lower, dim3(cdivu(cols, 256), L.count), dim3(256), 0, s, list,
-                         L.count, P.d_front, k0, P.nb, P.d_ipiv, P.arena);
-      const int tm = (L.mmax + GT - 1) / GT;
-      hipLaunchKernelGGL(k_gemm_update, dim3(static_cast<unsigned>(tm * tm), L.count), dim3(256), 0, s, list, L.count,
-                         P.d_front, k0, P.nb, 0, P.arena);
-      const int tp = (L.pmax + GT - 1) / GT;
-      hipLaunchKernelGGL(k_gemm_update, dim3(tp * tp, L.count), dim3(256), 0, s, list, L.count,
-                         P.d_front, k0, P.nb, 1, P.arena);
+    const unsigned tm_m = cdivu(L.mmax, GT), tm_mp = cdivu(L.mmax + L.pmax, GT);
+    for (int K0 = 0; K0 < L.pmax; K0 += NB) {
+      const int Kend = std::min(K0 + NB, L.pmax);
+      for (int k0 = K0; k0 < Kend; k0 += nb) {
+        hipLaunchKernelGGL(k_panel<double>, dim3(L.count), dim3(kPanelThreads), 0, s, list,
+                           L.count, P.d_front, k0, nb, K0, NB, tiny, perturb, P.arena, P.d_ipiv,
+                           P.d_counters);
+        if (L.cmax > 0)
+          hipLaunchKernelGGL(k_l21<double>, dim3(cdivu(L.cmax, 256), L.count), dim3(256), 0, s,
+                             list, L.count, P.d_front, k0, nb, P.arena);
+        if (k0 + nb < Kend) {
+          hipLaunchKernelGGL(k_trsm_lower<double>, dim3(1, L.count), dim3(256), 0, s, list,
+                             L.count, P.d_front, k0, nb, K0, NB, 0, 0, P.d_ipiv, P.arena);
+          hipLaunchKernelGGL(k_gemm<double>, dim3(tm_m * cdivu(NB - nb, GT), L.count), dim3(256),
+                             0, s, list, L.count, P.d_front, static_cast<int>(kGemmInner), k0, nb,
+                             K0, NB, P.arena);
+        }
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+      }
+      for (int k0 = K0; k0 < Kend; k0 += nb) {
+        hipLaunchKernelGGL(k_trsm_lower<double>, dim3(cdivu(L.mmax + L.pmax, 256), L.count),
+                           dim3(256), 0, s, list, L.count, P.d_front, k0, nb, K0, NB, 1,
+                           k0 == K0 ? 1 : 0, P.d_ipiv, P.arena);
+        if (k0 + nb < Kend)
+          hipLaunchKernelGGL(k_gemm<double>, dim3(cdivu(NB - nb, GT) * tm_mp, L.count),
+                             dim3(256), 0, s, list, L.count, P.d_front,
+                             static_cast<int>(kGemmBlkRows), k0, nb, K0, NB, P.arena);
+      }
+      hipLaunchKernelGGL(k_gemm<double>, dim3(tm_m * tm_mp, L.count), dim3(256), 0, s, list,
+                         L.count, P.d_front, static_cast<int>(kGemmTrail), K0, nb, K0, NB,
+                         P.arena);
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     // Z = U11^-1 by backward Jordan steps on the pivot blocks
-    for (int k0 = ((L.pmax - 1) / P.nb) * P.nb; k0 >= 0; k0 -= P.nb) {
-      hipLaunchKernelGGL(k_trsm_upper, dim3(cdivu(L.pmax, 256), L.count), dim3(256), 0, s, list,
-                         L.count, P.d_front, k0, P.nb, P.arena);
-      if (k0 > 0) {
-        const int tp = (L.pmax + GT - 1) / GT;
-        hipLaunchKernelGGL(k_gemm_update, dim3(tp * tp, L.count), dim3(256), 0, s, list, L.count,
-                           P.d_front, k0, P.nb, 2, P.arena);
-      }
+    const unsigned tp = cdivu(L.pmax, GT);
+    for (int k0 = ((L.pmax - 1) / nb) * nb; k0 >= 0; k0 -= nb) {
+      hipLaunchKernelGGL(k_trsm_upper<double>, dim3(cdivu(L.pmax, 256), L.count), dim3(256), 0,
+                         s, list, L.count, P.d_front, k0, nb, P.arena);
+      if (k0 > 0)
+        hipLaunchKernelGGL(k_gemm<double>, dim3(tp * tp, L.count), dim3(256), 0, s, list,
+                           L.count, P.d_front, static_cast<int>(kGemmZ), k0, nb, 0, NB, P.arena);
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
   }
   return hipSuccess;
 }
 
-// x = K^-1 b for vectors in elimination order (split planes).  y and t are
-// solver scratch planes of the same length.
+// x = K^-1 b for vectors in elimination order (split planes).  b may alias x.
 hipError_t lu_solve(const LuPlan &P, const double *br, const double *bi, double *xr, double *xi,
                     hipStream_t s) {
   double *Wr = P.W, *Wi = P.W + P.w_len;
   double *Yr = P.Y, *Yi = P.Y + P.n;
-  double *Tr = P.T, *Ti = P.T + P.n;
-  hipError_t e;
+  const SolveMaps mp{P.d_parent, P.d_rel, P.d_upd_pos, P.d_rel_off, P.d_upd_off};
+  hipLaunchKernelGGL(k_solve_init, dim3(P.nsn), dim3(256), 0, s, P.d_front, br, bi, Wr, Wi, Yr,
+                     Yi, xr, xi);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  auto gemv = [&](const LevelPlan &L, int op, int rows, int cols) {
+    hipLaunchKernelGGL(k_gemv, dim3(cdivu(rows, GVR), L.count, cdivu(cols, GVC * GVW)),
+                       dim3(GVR * GVW), 0, s, P.d_level_list + L.list_off, L.count, P.d_front, mp,
+                       op, P.arena, Wr, Wi, Yr, Yi, xr, xi);
+  };
   for (size_t lev = 0; lev < P.levels.size(); ++lev) {
     const LevelPlan &L = P.levels[lev];
-    const int *list = P.d_level_list + L.list_off;
-    hipLaunchKernelGGL(k_fwd_init, dim3(L.count), dim3(256), 0, s, list, L.count, P.d_front,
-                       P.d_child_ptr, P.d_child_list, P.d_rel, P.d_rel_off, br, bi, Wr, Wi, Yr, Yi);
-    hipLaunchKernelGGL(k_gemv, dim3(cdivu(L.pmax, GVR), L.count, cdivu(L.pmax, GVC * GVW)),
-                       dim3(GVR * GVW), 0, s, list, L.count, P.d_front, 0, P.arena, Wr, Wi, Yr, Yi, xr, xi);
-    hipLaunchKernelGGL(k_gemv, dim3(cdivu(L.cmax, GVR), L.count, cdivu(L.pmax, GVC * GVW)),
-                       dim3(GVR * GVW), 0, s, list, L.count, P.d_front, 1, P.arena, Wr, Wi, Yr, Yi, xr, xi);
+    gemv(L, 0, L.pmax, L.pmax);
+    if (L.cmax > 0) gemv(L, 1, L.cmax, L.pmax);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   for (int lev = static_cast<int>(P.levels.size()) - 1; lev >= 0; --lev) {
     const LevelPlan &L = P.levels[lev];
-    const int *list = P.d_level_list + L.list_off;
-    hipLaunchKernelGGL(k_bwd_gather, dim3(L.count), dim3(256), 0, s, list, L.count, P.d_front,
-                       P.d_upd_pos, P.d_upd_off, xr, xi, Wr, Wi, Yr, Yi, Tr, Ti);
-    hipLaunchKernelGGL(k_gemv, dim3(cdivu(L.pmax, GVR), L.count, cdivu(L.cmax, GVC * GVW)),
-                       dim3(GVR * GVW), 0, s, list, L.count, P.d_front, 2, P.arena, Wr, Wi, Tr, Ti, xr, xi);
-    hipLaunchKernelGGL(k_gemv, dim3(cdivu(L.pmax, GVR), L.count, cdivu(L.pmax, GVC * GVW)),
-                       dim3(GVR * GVW), 0, s, list, L.count, P.d_front, 3, P.arena, Wr, Wi, Tr, Ti, xr, xi);
+    if (L.cmax > 0) gemv(L, 2, L.pmax, L.cmax);
+    gemv(L, 3, L.pmax, L.pmax);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   return hipSuccess;

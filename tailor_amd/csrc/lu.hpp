@@ -52,7 +52,6 @@ struct LuPlan {
   double *arena = nullptr;
   double *W = nullptr;          // 2 * w_len
   double *Y = nullptr;          // 2 * n
-  double *T = nullptr;          // 2 * n
 };
 
 struct LuInputs {
@@ -63,7 +62,8 @@ struct LuInputs {
   const double *bvals = nullptr;  // B_bc block diagonal, interleaved complex
 };
 
-constexpr int kLuNb = 32;
+constexpr int kLuNb = 32;        // panel width
+constexpr int kLuNbOuter = 128;  // outer block width (K of the trailing update)
 
 // Build the device plan from the symbolic analysis and the BSR structure
 // (host arrays).  Allocates device memory; returns hipSuccess or an error.

@@ -125,8 +125,6 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
     return e;
   if ((e = hipMalloc(reinterpret_cast<void **>(&P.Y), 2 * P.n * sizeof(double))) != hipSuccess)
     return e;
-  if ((e = hipMalloc(reinterpret_cast<void **>(&P.T), 2 * P.n * sizeof(double))) != hipSuccess)
-    return e;
   return hipStreamSynchronize(s);
 }
 
@@ -148,7 +146,6 @@ void lu_plan_destroy(LuPlan &P) {
   release(P.arena);
   release(P.W);
   release(P.Y);
-  release(P.T);
   P.levels.clear();
 }
 
