@@ -157,6 +157,15 @@ typedef struct {
 
 int tailor_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, tailor_factor_diag *diag);
 
+/* Precision of the factorization used by tailor_factor / tailor_solve.
+ * FP64: complex FP64 fronts (the plain form).  MIXED: complex FP32 fronts
+ * factored on FP32 MFMA, with FP64 right-hand sides, FP64 residuals and
+ * FP64 iterative refinement of every solve (refine_max / refine_tol of the
+ * calls); Krylov-Schur itself stays FP64.  No reference counterpart: PETSc
+ * factors in the scalar type of the Mat (EigenSolver.cpp:61-67). */
+enum { TAILOR_PRECISION_FP64 = 0, TAILOR_PRECISION_MIXED = 1 };
+int tailor_set_precision(tailor_ctx *ctx, int32_t precision);
+
 /* x = K^-1 b for natural-order interleaved complex vectors of length 5*N,
  * with up to refine_max steps of iterative refinement towards a relative
  * residual refine_tol; relres/steps report the outcome (may be NULL). */

@@ -134,6 +134,7 @@ def library() -> C.CDLL:
     lib.tailor_linear_solve.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), C.c_int32,
                                         C.c_double, P(C.c_double), P(C.c_int32)]
     lib.tailor_solve.argtypes = [C.c_void_p, P(SolveOpts), P(SolveDiag)]
+    lib.tailor_set_precision.argtypes = [C.c_void_p, C.c_int32]
     lib.tailor_time_linear_solve.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double)]
     lib.tailor_eigenpair_count.argtypes = [C.c_void_p]
     lib.tailor_get_eigenpair.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double),
@@ -252,6 +253,15 @@ class Context:
         return w.value, f.value
 
 
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+
+
+def _set_precision(self, precision: int) -> None:
+    """PRECISION_FP64 (plain form) or PRECISION_MIXED (complex FP32 fronts,
+    FP64 refinement of every solve)."""
+    self._check(self.lib.tailor_set_precision(self.ptr, int(precision)))
+
+
 def _factor(self, sigma: complex) -> dict:
     d = FactorDiag()
     sigma = complex(sigma)
@@ -311,6 +321,7 @@ def _time_linear_solve(self, reps: int = 20):
 
 Context.time_linear_solve = _time_linear_solve
 Context.factor = _factor
+Context.set_precision = _set_precision
 Context.linear_solve = _linear_solve
 Context.solve = _solve
 Context.eigenpairs = _eigenpairs

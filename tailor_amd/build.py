@@ -20,7 +20,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 COMMON = ["-O3", "-std=c++20", "-fPIC", "--offload-arch=gfx950",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-Wall", "-Wno-unused-function", f"-I{PKG.parent / 'include'}"]
+          "-Wall", "-Wno-unused-function", "-Werror=uninitialized",
+          f"-I{PKG.parent / 'include'}"]
 PER_FILE = {
     "assembly.hip": ["-ffp-contract=off"],
     "lu.hip": ["-munsafe-fp-atomics"],

@@ -557,12 +557,22 @@ int tailor_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, tailor_fact
     for (int m : S.sym.m) maxm = std::max(maxm, m);
     diag->max_front = maxm;
     diag->perturbed_pivots = S.perturbed;
-    diag->arena_bytes = S.plan.arena_doubles * 8;
+    diag->arena_bytes = S.plan.arena_doubles * (S.fp32 ? 4 : 8);
     diag->factor_entries = S.sym.factor_entries;
     diag->flops = static_cast<double>(S.sym.flops);
     diag->symbolic_ms = S.symbolic_ms;
     diag->factor_ms = S.factor_ms;
   }
+  return TAILOR_OK;
+}
+
+int tailor_set_precision(tailor_ctx *ctx, int32_t precision) {
+  if (!ctx) return TAILOR_ERR_USER_INPUT;
+  if (precision != TAILOR_PRECISION_FP64 && precision != TAILOR_PRECISION_MIXED)
+    return fail(ctx, TAILOR_ERR_ARG_OUTOFRANGE, "unknown precision mode");
+  ctx->precision = precision;
+  if (ctx->solver && ctx->solver->fp32 != (precision == TAILOR_PRECISION_MIXED))
+    ctx->solver->factored = false;
   return TAILOR_OK;
 }
 
@@ -593,7 +603,7 @@ int tailor_time_linear_solve(tailor_ctx *ctx, int32_t reps, double *ms_per_solve
     for (int f = 0; f < S.nsn; ++f) {
       const double p = S.p[f], c = S.c[f];
       // E (p x p), Z (upper triangle), L21 and U12 (p x c each)
-      b += 16.0 * (p * p + 0.5 * p * (p + 1.0) + 2.0 * p * c);
+      b += (ctx->solver->fp32 ? 8.0 : 16.0) * (p * p + 0.5 * p * (p + 1.0) + 2.0 * p * c);
     }
     *bytes_per_solve = b;
   }
@@ -610,7 +620,8 @@ int tailor_solve(tailor_ctx *ctx, const tailor_solve_opts *opts, tailor_solve_di
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   // lambda_s = Im(omega_s) - i Re(omega_s)  (EigenSolver.cpp:46-48)
   const tailor::cplx sigma(opts->omega_im, -opts->omega_re);
-  if (!ctx->solver || !ctx->solver->factored || ctx->solver->sigma != sigma)
+  if (!ctx->solver || !ctx->solver->factored || ctx->solver->sigma != sigma ||
+      ctx->solver->fp32 != (ctx->precision == TAILOR_PRECISION_MIXED))
     if (int st = tailor::solver_factor(ctx, sigma)) return st;
   tailor_solve_diag local{};
   const int st = tailor::solver_krylov_schur(ctx, *opts, local);

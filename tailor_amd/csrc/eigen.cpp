@@ -114,8 +114,12 @@ int solver_factor(tailor_ctx *ctx, cplx sigma) {
   HIP_OK(hipEventCreate(&e0));
   HIP_OK(hipEventCreate(&e1));
   HIP_OK(hipEventRecord(e0, ctx->stream));
-  HIP_OK(lu_factor(S.plan, in, sigma.real(), sigma.imag(), 1e-14 * scale, 1.4901161193847656e-08 * scale,
-                   ctx->stream));
+  // pivots below tiny are replaced by +-perturb (sqrt(eps) of the front
+  // precision times the scale); refinement recovers the accuracy
+  const bool fp32 = ctx->precision == TAILOR_PRECISION_MIXED;
+  const double tiny = (fp32 ? 1e-7 : 1e-14) * scale;
+  const double perturb = (fp32 ? 3.4526698300124393e-04 : 1.4901161193847656e-08) * scale;
+  HIP_OK(lu_factor(S.plan, in, sigma.real(), sigma.imag(), tiny, perturb, fp32, ctx->stream));
   HIP_OK(hipEventRecord(e1, ctx->stream));
   HIP_OK(hipEventSynchronize(e1));
   float ms = 0.f;
@@ -127,6 +131,7 @@ int solver_factor(tailor_ctx *ctx, cplx sigma) {
   S.perturbed = counters[0];
   S.factor_ms = ms;
   S.sigma = sigma;
+  S.fp32 = fp32;
   S.factored = true;
   return TAILOR_OK;
 }
@@ -141,7 +146,7 @@ int refined_solve(tailor_ctx *ctx, const double *br, const double *bi, double *y
   const long long n = S.n;
   double *rr = S.work + 6 * n, *ri = rr + n;  // residual
   double *dr = S.work + 8 * n, *di = dr + n;  // correction
-  HIP_OK(lu_solve(S.plan, br, bi, yr, yi, ctx->stream));
+  HIP_OK(lu_solve(S.plan, br, bi, yr, yi, S.fp32, ctx->stream));
   if (steps) *steps = 0;
   if (refine_max <= 0 && !relres) return TAILOR_OK;
   const auto &d = ctx->dims;
@@ -160,7 +165,7 @@ int refined_solve(tailor_ctx *ctx, const double *br, const double *bi, double *y
     const double rel = bnorm > 0.0 ? nr / bnorm : nr;
     if (relres) *relres = rel;
     if (it == refine_max || rel <= refine_tol) break;
-    HIP_OK(lu_solve(S.plan, rr, ri, dr, di, ctx->stream));
+    HIP_OK(lu_solve(S.plan, rr, ri, dr, di, S.fp32, ctx->stream));
     HIP_OK(vec_axpby(ctx->stream, n, 1.0, 0.0, dr, di, 1.0, 0.0, yr, yi));
     if (steps) ++*steps;
   }
@@ -219,12 +224,12 @@ int solver_time_solves(tailor_ctx *ctx, int reps, double *ms) {
   HIP_OK(grow(&S.work, S.work_cap, static_cast<size_t>(12 * n)));
   double *br = S.work, *bi = br + n, *xr = S.work + 2 * n, *xi = xr + n;
   HIP_OK(lu_fill(ctx->stream, br, 2 * n, 1.0));
-  HIP_OK(lu_solve(S.plan, br, bi, xr, xi, ctx->stream));  // warm-up
+  HIP_OK(lu_solve(S.plan, br, bi, xr, xi, S.fp32, ctx->stream));  // warm-up
   hipEvent_t e0, e1;
   HIP_OK(hipEventCreate(&e0));
   HIP_OK(hipEventCreate(&e1));
   HIP_OK(hipEventRecord(e0, ctx->stream));
-  for (int r = 0; r < reps; ++r) HIP_OK(lu_solve(S.plan, br, bi, xr, xi, ctx->stream));
+  for (int r = 0; r < reps; ++r) HIP_OK(lu_solve(S.plan, br, bi, xr, xi, S.fp32, ctx->stream));
   HIP_OK(hipEventRecord(e1, ctx->stream));
   HIP_OK(hipEventSynchronize(e1));
   float t = 0.f;
@@ -247,8 +252,10 @@ int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solv
   if (nev <= 0 || nev >= ncv) return fail(ctx, TAILOR_ERR_USER_INPUT, "invalid nev / ncv");
   HIP_OK(grow(&S.V, S.V_cap, static_cast<size_t>(2 * n * (ncv + 1))));
   HIP_OK(grow(&S.work, S.work_cap, static_cast<size_t>(12 * n)));
-  HIP_OK(grow(&S.partials, S.partials_cap, static_cast<size_t>(2 * kDotBlocks * 64)));
-  HIP_OK(grow(&S.small, S.small_cap, static_cast<size_t>(2 * 64 * 64)));
+  // dot partials for up to ncv + 1 columns, and the small host-computed
+  // matrices (QY: ncv x ncheck, Q_l: ncv x l, both below (ncv + 1)^2)
+  HIP_OK(grow(&S.partials, S.partials_cap, static_cast<size_t>(2 * kDotBlocks * (ncv + 1))));
+  HIP_OK(grow(&S.small, S.small_cap, static_cast<size_t>(2 * (ncv + 1) * (ncv + 1))));
   HIP_OK(grow(&S.ritz, S.ritz_cap, static_cast<size_t>(2 * n * ncv)));
   double *Vr = S.V, *Vi = S.V + n * (ncv + 1);
   auto col_r = [&](int j) { return Vr + static_cast<long long>(j) * n; };
@@ -446,7 +453,7 @@ int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solv
   diag.factor_ms = S.factor_ms;
   diag.symbolic_ms = S.symbolic_ms;
   diag.fronts = S.sym.nsn;
-  diag.factor_bytes = static_cast<int64_t>(S.plan.arena_doubles) * 8;
+  diag.factor_bytes = static_cast<int64_t>(S.plan.arena_doubles) * (S.fp32 ? 4 : 8);
   diag.factor_flops = static_cast<double>(S.sym.flops);
   if (nconv < nev)
     return fail(ctx, TAILOR_ERR_NOT_CONVERGED,

@@ -89,6 +89,7 @@ struct tailor_ctx {
   std::vector<int> h_xoff, h_eoff;
   // factorization and eigen-solve state (solver.hpp)
   tailor::SolverState *solver = nullptr;
+  int precision = 0;  // TAILOR_PRECISION_FP64 or TAILOR_PRECISION_MIXED
 };
 
 namespace tailor {

@@ -40,24 +40,27 @@ __global__ void k_fill(double *__restrict__ p, long long n, double v) {
 }
 
 // E = I and Z = I for every front (the planes were zero-filled before).
-__global__ void k_init_identity(const FrontDev *__restrict__ fr, int nf, double *__restrict__ arena) {
+template <typename T>
+__global__ void k_init_identity(const FrontDev *__restrict__ fr, int nf, T *__restrict__ arena) {
   const int f = blockIdx.y;
   if (f >= nf) return;
   const FrontDev d = fr[f];
-  double *E = arena + d.off + 2LL * d.m * d.m;
-  double *Z = E + 2LL * d.p * d.p;
+  T *E = arena + d.off + 2LL * d.m * d.m;
+  T *Z = E + 2LL * d.p * d.p;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < d.p; k += gridDim.x * blockDim.x) {
-    E[static_cast<long long>(k) * d.p + k] = 1.0;
-    Z[static_cast<long long>(k) * d.p + k] = 1.0;
+    E[static_cast<long long>(k) * d.p + k] = T(1);
+    Z[static_cast<long long>(k) * d.p + k] = T(1);
   }
 }
 
 // Scatter K = A - sigma B into the fronts.  Thread per (block, entry).
 // amap[3k..3k+2] = front, local row node, local col node of BSR block k.
+// K is formed in FP64 and rounded once to the front precision T.
+template <typename T>
 __global__ void k_assemble_fronts(long long nnzb, int per, const double *__restrict__ avals,
                                   const int *__restrict__ colidx, const double *__restrict__ bvals,
                                   double sig_re, double sig_im, const int *__restrict__ amap,
-                                  const FrontDev *__restrict__ fr, double *__restrict__ arena) {
+                                  const FrontDev *__restrict__ fr, T *__restrict__ arena) {
   const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= nnzb * 25) return;
   const long long k = t / 25;
@@ -73,23 +76,24 @@ __global__ void k_assemble_fronts(long long nnzb, int per, const double *__restr
     vi -= sig_re * bi + sig_im * br;
   }
   const long long idx = static_cast<long long>(5 * lv + c) * d.m + 5 * lu + r;
-  arena[d.off + idx] = vr;
-  arena[d.off + static_cast<long long>(d.m) * d.m + idx] = vi;
+  arena[d.off + idx] = static_cast<T>(vr);
+  arena[d.off + static_cast<long long>(d.m) * d.m + idx] = static_cast<T>(vi);
 }
 
 // F_parent[rel, rel] += F_child[p:, p:] for the children listed in `tasks`.
+template <typename T>
 __global__ void k_extend_add(const int *__restrict__ tasks, int ntasks,
                              const FrontDev *__restrict__ fr, const int *__restrict__ par,
                              const int *__restrict__ rel, const long long *__restrict__ rel_off,
-                             double *__restrict__ arena) {
+                             T *__restrict__ arena) {
   const int ti = blockIdx.y;
   if (ti >= ntasks) return;
   const int ch = tasks[ti];
   const FrontDev dc = fr[ch], dp = fr[par[ch]];
   const int *rl = rel + rel_off[ch];
   const long long cc = static_cast<long long>(dc.c) * dc.c;
-  const double *Cr = arena + dc.off, *Ci = Cr + static_cast<long long>(dc.m) * dc.m;
-  double *Pr = arena + dp.off, *Pi = Pr + static_cast<long long>(dp.m) * dp.m;
+  const T *Cr = arena + dc.off, *Ci = Cr + static_cast<long long>(dc.m) * dc.m;
+  T *Pr = arena + dp.off, *Pi = Pr + static_cast<long long>(dp.m) * dp.m;
   for (long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; idx < cc;
        idx += static_cast<long long>(gridDim.x) * blockDim.x) {
     const int i = static_cast<int>(idx % dc.c), j = static_cast<int>(idx / dc.c);
@@ -678,8 +682,8 @@ constexpr int GVR = 64, GVC = 64, GVW = 4;  // rows per WG, columns per wave, wa
 
 // One batched GEMV y (+)= add + sign * A x.  xmap / ymap, when set, gather x
 // and scatter y through node maps (entry j -> 5 * map[j / 5] + j % 5).
-struct GemvView {
-  const double *Ar, *Ai;
+template <typename T> struct GemvView {
+  const T *Ar, *Ai;
   long long lda;
   int M, N;
   const double *xr, *xi;
@@ -696,13 +700,14 @@ struct SolveMaps {
   const long long *rel_off, *upd_off;
 };
 
+template <typename T>
 __device__ inline bool gemv_view(int op, const FrontDev *fr, int f, const FrontDev &d,
-                                 const SolveMaps &mp, double *arena, double *Wr, double *Wi,
-                                 double *Yr, double *Yi, double *Xr, double *Xi, GemvView &v) {
+                                 const SolveMaps &mp, const T *arena, double *Wr, double *Wi,
+                                 double *Yr, double *Yi, double *Xr, double *Xi, GemvView<T> &v) {
   const long long mm = static_cast<long long>(d.m) * d.m, pp = static_cast<long long>(d.p) * d.p;
-  double *Fr = arena + d.off, *Fi = Fr + mm;
-  double *Er = Fi + mm, *Ei = Er + pp;
-  double *Zr = Ei + pp, *Zi = Zr + pp;
+  const T *Fr = arena + d.off, *Fi = Fr + mm;
+  const T *Er = Fi + mm, *Ei = Er + pp;
+  const T *Zr = Ei + pp, *Zi = Zr + pp;
   switch (op) {
     case 0:
       v = {Er, Ei, d.p, d.p, d.p, Wr + d.woff, Wi + d.woff, nullptr, Yr + d.dof0, Yi + d.dof0,
@@ -732,14 +737,16 @@ __device__ inline bool gemv_view(int op, const FrontDev *fr, int f, const FrontD
 // One workgroup: GVR output rows x (GVW * GVC) reduction columns; each wave
 // takes GVC of the columns and the four partial sums are combined in LDS
 // before one FP64 atomic per row.
+// Factor entries of type T are widened to FP64; sums are FP64.
+template <typename T>
 __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list, int nlist,
                                                     const FrontDev *__restrict__ fr, SolveMaps mp,
-                                                    int op, double *__restrict__ arena, double *Wr,
+                                                    int op, const T *__restrict__ arena, double *Wr,
                                                     double *Wi, double *Yr, double *Yi, double *Xr,
                                                     double *Xi) {
   const int f = list[blockIdx.y];
   const FrontDev d = fr[f];
-  GemvView v;
+  GemvView<T> v;
   if (!gemv_view(op, fr, f, d, mp, arena, Wr, Wi, Yr, Yi, Xr, Xi, v)) return;
   const int lane = threadIdx.x % GVR, wave = threadIdx.x / GVR;
   const int r0 = blockIdx.x * GVR;
@@ -765,7 +772,7 @@ __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list
   __syncthreads();
   double accr = 0.0, acci = 0.0;
   if (row < v.M && !(v.upper && c1 <= r0)) {
-    const double *ar = v.Ar + row, *ai = v.Ai + row;
+    const T *ar = v.Ar + row, *ai = v.Ai + row;
     int k = c0;
     // 16 columns per step so that 32 loads per lane are in flight
     for (; k + 16 <= c1; k += 16) {
@@ -773,8 +780,8 @@ __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
         const long long idx = static_cast<long long>(k + u) * v.lda;
-        a_r[u] = ar[idx];
-        a_i[u] = ai[idx];
+        a_r[u] = static_cast<double>(ar[idx]);
+        a_i[u] = static_cast<double>(ai[idx]);
       }
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
@@ -785,7 +792,7 @@ __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list
     }
     for (; k < c1; ++k) {
       const long long idx = static_cast<long long>(k) * v.lda;
-      const double a_r = ar[idx], a_i = ai[idx];
+      const double a_r = static_cast<double>(ar[idx]), a_i = static_cast<double>(ai[idx]);
       const double x_r = sxr[k - cw], x_i = sxi[k - cw];
       accr += a_r * x_r - a_i * x_i;
       acci += a_r * x_i + a_i * x_r;
@@ -825,16 +832,20 @@ hipError_t lu_fill(hipStream_t s, double *p, long long n, double v) {
   return hipGetLastError();
 }
 
-hipError_t lu_factor(const LuPlan &P, const LuInputs &in, double sig_re, double sig_im,
-                     double tiny, double perturb, hipStream_t s) {
+template <typename T>
+static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re, double sig_im,
+                              double tiny, double perturb, hipStream_t s) {
   hipError_t e;
-  if ((e = lu_fill(s, P.arena, P.arena_doubles, 0.0)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_init_identity, dim3(8, P.nsn), dim3(256), 0, s, P.d_front, P.nsn, P.arena);
+  T *arena = reinterpret_cast<T *>(P.arena);
+  // zero the planes in use (the arena is sized for FP64 fronts)
+  const long long used = (P.arena_doubles * static_cast<long long>(sizeof(T)) + 7) / 8;
+  if ((e = lu_fill(s, P.arena, used, 0.0)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_init_identity<T>, dim3(8, P.nsn), dim3(256), 0, s, P.d_front, P.nsn, arena);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const long long threads = in.nnzb * 25;
-  hipLaunchKernelGGL(k_assemble_fronts, dim3(static_cast<unsigned>((threads + 255) / 256)),
+  hipLaunchKernelGGL(k_assemble_fronts<T>, dim3(static_cast<unsigned>((threads + 255) / 256)),
                      dim3(256), 0, s, in.nnzb, in.per, in.avals, in.colidx, in.bvals, sig_re,
-                     sig_im, P.d_amap, P.d_front, P.arena);
+                     sig_im, P.d_amap, P.d_front, arena);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, s, reinterpret_cast<double *>(P.d_counters), 1LL, 0.0);
   const int nb = P.nb, NB = kLuNbOuter;
@@ -843,9 +854,9 @@ hipError_t lu_factor(const LuPlan &P, const LuInputs &in, double sig_re, double 
     for (int slot = 0; slot < 2; ++slot) {
       const int nt = L.child_slot_count[slot];
       if (nt == 0) continue;
-      hipLaunchKernelGGL(k_extend_add, dim3(256, nt), dim3(256), 0, s,
+      hipLaunchKernelGGL(k_extend_add<T>, dim3(256, nt), dim3(256), 0, s,
                          P.d_child_slots + L.child_slot_off[slot], nt, P.d_front, P.d_parent,
-                         P.d_rel, P.d_rel_off, P.arena);
+                         P.d_rel, P.d_rel_off, arena);
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     const int *list = P.d_level_list + L.list_off;
@@ -853,52 +864,60 @@ hipError_t lu_factor(const LuPlan &P, const LuInputs &in, double sig_re, double 
     for (int K0 = 0; K0 < L.pmax; K0 += NB) {
       const int Kend = std::min(K0 + NB, L.pmax);
       for (int k0 = K0; k0 < Kend; k0 += nb) {
-        hipLaunchKernelGGL(k_panel<double>, dim3(L.count), dim3(kPanelThreads), 0, s, list,
-                           L.count, P.d_front, k0, nb, K0, NB, tiny, perturb, P.arena, P.d_ipiv,
+        hipLaunchKernelGGL(k_panel<T>, dim3(L.count), dim3(kPanelThreads), 0, s, list,
+                           L.count, P.d_front, k0, nb, K0, NB, tiny, perturb, arena, P.d_ipiv,
                            P.d_counters);
         if (L.cmax > 0)
-          hipLaunchKernelGGL(k_l21<double>, dim3(cdivu(L.cmax, 256), L.count), dim3(256), 0, s,
-                             list, L.count, P.d_front, k0, nb, P.arena);
+          hipLaunchKernelGGL(k_l21<T>, dim3(cdivu(L.cmax, 256), L.count), dim3(256), 0, s,
+                             list, L.count, P.d_front, k0, nb, arena);
         if (k0 + nb < Kend) {
-          hipLaunchKernelGGL(k_trsm_lower<double>, dim3(1, L.count), dim3(256), 0, s, list,
-                             L.count, P.d_front, k0, nb, K0, NB, 0, 0, P.d_ipiv, P.arena);
-          hipLaunchKernelGGL(k_gemm<double>, dim3(tm_m * cdivu(NB - nb, GT), L.count), dim3(256),
+          hipLaunchKernelGGL(k_trsm_lower<T>, dim3(1, L.count), dim3(256), 0, s, list,
+                             L.count, P.d_front, k0, nb, K0, NB, 0, 0, P.d_ipiv, arena);
+          hipLaunchKernelGGL(k_gemm<T>, dim3(tm_m * cdivu(NB - nb, GT), L.count), dim3(256),
                              0, s, list, L.count, P.d_front, static_cast<int>(kGemmInner), k0, nb,
-                             K0, NB, P.arena);
+                             K0, NB, arena);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
       }
       for (int k0 = K0; k0 < Kend; k0 += nb) {
-        hipLaunchKernelGGL(k_trsm_lower<double>, dim3(cdivu(L.mmax + L.pmax, 256), L.count),
+        hipLaunchKernelGGL(k_trsm_lower<T>, dim3(cdivu(L.mmax + L.pmax, 256), L.count),
                            dim3(256), 0, s, list, L.count, P.d_front, k0, nb, K0, NB, 1,
-                           k0 == K0 ? 1 : 0, P.d_ipiv, P.arena);
+                           k0 == K0 ? 1 : 0, P.d_ipiv, arena);
         if (k0 + nb < Kend)
-          hipLaunchKernelGGL(k_gemm<double>, dim3(cdivu(NB - nb, GT) * tm_mp, L.count),
+          hipLaunchKernelGGL(k_gemm<T>, dim3(cdivu(NB - nb, GT) * tm_mp, L.count),
                              dim3(256), 0, s, list, L.count, P.d_front,
-                             static_cast<int>(kGemmBlkRows), k0, nb, K0, NB, P.arena);
+                             static_cast<int>(kGemmBlkRows), k0, nb, K0, NB, arena);
       }
-      hipLaunchKernelGGL(k_gemm<double>, dim3(tm_m * tm_mp, L.count), dim3(256), 0, s, list,
+      hipLaunchKernelGGL(k_gemm<T>, dim3(tm_m * tm_mp, L.count), dim3(256), 0, s, list,
                          L.count, P.d_front, static_cast<int>(kGemmTrail), K0, nb, K0, NB,
-                         P.arena);
+                         arena);
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     // Z = U11^-1 by backward Jordan steps on the pivot blocks
     const unsigned tp = cdivu(L.pmax, GT);
     for (int k0 = ((L.pmax - 1) / nb) * nb; k0 >= 0; k0 -= nb) {
-      hipLaunchKernelGGL(k_trsm_upper<double>, dim3(cdivu(L.pmax, 256), L.count), dim3(256), 0,
-                         s, list, L.count, P.d_front, k0, nb, P.arena);
+      hipLaunchKernelGGL(k_trsm_upper<T>, dim3(cdivu(L.pmax, 256), L.count), dim3(256), 0,
+                         s, list, L.count, P.d_front, k0, nb, arena);
       if (k0 > 0)
-        hipLaunchKernelGGL(k_gemm<double>, dim3(tp * tp, L.count), dim3(256), 0, s, list,
-                           L.count, P.d_front, static_cast<int>(kGemmZ), k0, nb, 0, NB, P.arena);
+        hipLaunchKernelGGL(k_gemm<T>, dim3(tp * tp, L.count), dim3(256), 0, s, list,
+                           L.count, P.d_front, static_cast<int>(kGemmZ), k0, nb, 0, NB, arena);
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
   }
   return hipSuccess;
 }
 
+hipError_t lu_factor(const LuPlan &P, const LuInputs &in, double sig_re, double sig_im,
+                     double tiny, double perturb, bool fp32, hipStream_t s) {
+  return fp32 ? factor_impl<float>(P, in, sig_re, sig_im, tiny, perturb, s)
+              : factor_impl<double>(P, in, sig_re, sig_im, tiny, perturb, s);
+}
+
 // x = K^-1 b for vectors in elimination order (split planes).  b may alias x.
-hipError_t lu_solve(const LuPlan &P, const double *br, const double *bi, double *xr, double *xi,
-                    hipStream_t s) {
+template <typename T>
+static hipError_t solve_impl(const LuPlan &P, const double *br, const double *bi, double *xr,
+                             double *xi, hipStream_t s) {
+  const T *arena = reinterpret_cast<const T *>(P.arena);
   double *Wr = P.W, *Wi = P.W + P.w_len;
   double *Yr = P.Y, *Yi = P.Y + P.n;
   const SolveMaps mp{P.d_parent, P.d_rel, P.d_upd_pos, P.d_rel_off, P.d_upd_off};
@@ -907,9 +926,9 @@ hipError_t lu_solve(const LuPlan &P, const double *br, const double *bi, double 
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   auto gemv = [&](const LevelPlan &L, int op, int rows, int cols) {
-    hipLaunchKernelGGL(k_gemv, dim3(cdivu(rows, GVR), L.count, cdivu(cols, GVC * GVW)),
+    hipLaunchKernelGGL(k_gemv<T>, dim3(cdivu(rows, GVR), L.count, cdivu(cols, GVC * GVW)),
                        dim3(GVR * GVW), 0, s, P.d_level_list + L.list_off, L.count, P.d_front, mp,
-                       op, P.arena, Wr, Wi, Yr, Yi, xr, xi);
+                       op, arena, Wr, Wi, Yr, Yi, xr, xi);
   };
   for (size_t lev = 0; lev < P.levels.size(); ++lev) {
     const LevelPlan &L = P.levels[lev];
@@ -924,6 +943,11 @@ hipError_t lu_solve(const LuPlan &P, const double *br, const double *bi, double 
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+hipError_t lu_solve(const LuPlan &P, const double *br, const double *bi, double *xr, double *xi,
+                    bool fp32, hipStream_t s) {
+  return fp32 ? solve_impl<float>(P, br, bi, xr, xi, s) : solve_impl<double>(P, br, bi, xr, xi, s);
 }
 
 }  // namespace tailor

@@ -72,10 +72,11 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
 void lu_plan_destroy(LuPlan &P);
 
 hipError_t lu_fill(hipStream_t s, double *p, long long n, double v);
+// Fronts in complex FP64, or complex FP32 (fp32) in the same arena.
 hipError_t lu_factor(const LuPlan &P, const LuInputs &in, double sig_re, double sig_im,
-                     double tiny, double perturb, hipStream_t s);
+                     double tiny, double perturb, bool fp32, hipStream_t s);
 // x = K^-1 b, vectors in elimination dof order, split planes (re, im)
 hipError_t lu_solve(const LuPlan &P, const double *br, const double *bi, double *xr, double *xi,
-                    hipStream_t s);
+                    bool fp32, hipStream_t s);
 
 }  // namespace tailor

@@ -14,6 +14,7 @@ namespace tailor {
 struct SolverState {
   bool ready = false;     // symbolic analysis + device plan built
   bool factored = false;
+  bool fp32 = false;      // fronts held in complex FP32 (mixed precision)
   Symbolic sym;
   LuPlan plan;
   long long n = 0;

@@ -85,3 +85,54 @@ def test_eigenvalues_match_dense_lapack(oracle_lib, shape, omega):
         assert abs(np.linalg.norm(q) - 1.0) <= 1e-13
         k = np.argmax(np.abs(q))
         assert abs(q[k].imag) <= 1e-14 and q[k].real > 0
+
+
+@pytest.mark.parametrize("shape,sigma", [((24, 12), 0.3 - 0.2j), ((40, 16), -0.02j)])
+def test_mixed_precision_solve_refines_to_1e12(oracle_lib, shape, sigma):
+    """Complex FP32 fronts + FP64 refinement: relative residual 1e-12."""
+    c = small_case(*shape)
+    A, B = operators(oracle_lib, c)
+    K = (A - sigma * B).tocsc()
+    rng = np.random.default_rng(11)
+    b = rng.standard_normal(K.shape[0]) + 1j * rng.standard_normal(K.shape[0])
+    x_ref = spla.spsolve(K, b)
+    with ta.Context(0) as ctx:
+        ctx.load_case(c)
+        ctx.assemble(diagnostics=False)
+        ctx.set_precision(ta.PRECISION_MIXED)
+        fd = ctx.factor(sigma)
+        x0, rel0, _ = ctx.linear_solve(b, refine_max=0)
+        x, rel, steps = ctx.linear_solve(b, refine_max=10, refine_tol=1e-12)
+    res0 = np.linalg.norm(K @ x0 - b) / np.linalg.norm(b)
+    res = np.linalg.norm(K @ x - b) / np.linalg.norm(b)
+    err = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+    print(f"mixed {shape} sigma {sigma}: FP32 residual {res0:.2e}, refined {res:.2e} after "
+          f"{steps} steps, error vs SuperLU {err:.2e}, arena {fd['arena_bytes']}", flush=True)
+    assert 1e-9 < res0 <= 1e-3      # an FP32 factorization, not an FP64 one
+    assert res <= 1e-12 and rel <= 1e-12
+    assert steps <= 6
+    assert err <= 1e-9
+
+
+def test_mixed_precision_eigenvalues_match_fp64(oracle_lib):
+    c = small_case(24, 12)
+    A, B = operators(oracle_lib, c)
+    omega = 0.3 + 0.05j
+    sigma = complex(omega.imag, -omega.real)
+    ref = dense_reference(A, B, sigma, 8)
+    with ta.Context(0) as ctx:
+        ctx.load_case(c)
+        ctx.assemble(diagnostics=False)
+        ctx.set_precision(ta.PRECISION_MIXED)
+        d = ctx.solve(nev=8, omega=omega, tol=1e-10, refine_max=10, refine_tol=1e-12)
+        pairs = ctx.eigenpairs(vectors=True)
+    print(f"mixed: converged {d['converged']} solves {d['solves']} refinement steps "
+          f"{d['refinement_steps']} max residual {d['max_residual']:.2e}", flush=True)
+    assert d["refinement_steps"] > 0
+    got = np.array([p["lambda"] for p in pairs[:8]])
+    for lam_ref in ref:
+        k = np.argmin(np.abs(got - lam_ref))
+        assert abs(got[k] - lam_ref) <= 1e-8 * max(1.0, abs(lam_ref)), (got[k], lam_ref)
+    for p in pairs[:8]:
+        q = p["mode"]
+        assert np.linalg.norm(A @ q - p["lambda"] * (B @ q)) / np.linalg.norm(q) <= 1e-10
