@@ -188,6 +188,9 @@ typedef struct {
   double omega_re, omega_im;  /* SearchCenterOmega; lambda_s = Im(omega) - i Re(omega) */
   int32_t refine_max;      /* iterative refinement steps per shift-invert solve */
   double refine_tol;       /* relative residual target of a refined solve */
+  int32_t warm_start;      /* 1: start from the Ritz vectors of the previous solve of this
+                              context (same structure), e.g. the previous wavenumber of a
+                              sweep; 0: the fixed pseudo-random start vector */
 } tailor_solve_opts;
 
 typedef struct {
@@ -213,6 +216,29 @@ int tailor_eigenpair_count(const tailor_ctx *ctx);
  * ||A q - lambda B q|| / ||q|| and relative error residual / |lambda|. */
 int tailor_get_eigenpair(tailor_ctx *ctx, int32_t k, double *lambda, double *q, double *residual,
                          double *relative_error);
+
+/* ------------------------------------------------------------------------
+ * Shift-proposal network (no reference counterpart).  A 3 -> 64 -> 64 -> 16
+ * perceptron with GELU activations in FP32: normalised (alpha, Re, Ma) in,
+ * normalised real/imaginary parts of the 8 leading omega out.  Forward,
+ * backward (mean squared error) and Adam run as the library's own kernels on
+ * the model's HIP stream.  X is n x 3 and targets/outputs n x 16, row-major.
+ * ---------------------------------------------------------------------- */
+typedef struct tailor_mlp tailor_mlp;
+int tailor_mlp_param_count(void);   /* 5456: W1 b1 W2 b2 W3 b3, W row-major [out][in] */
+int tailor_mlp_create(tailor_mlp **m, int device, uint64_t seed);  /* He-uniform init */
+void tailor_mlp_destroy(tailor_mlp *m);
+const char *tailor_mlp_last_error(const tailor_mlp *m);
+int tailor_mlp_get_params(tailor_mlp *m, float *params);
+int tailor_mlp_set_params(tailor_mlp *m, const float *params);  /* also resets Adam */
+int tailor_mlp_predict(tailor_mlp *m, const float *X, int32_t n, float *Y);
+/* Loss (mean over n x 16 entries) and its gradient w.r.t. every parameter. */
+int tailor_mlp_gradients(tailor_mlp *m, const float *X, const float *T, int32_t n, float *grad,
+                         float *loss);
+/* `steps` full-batch Adam steps (beta 0.9 / 0.999, eps 1e-8); loss_history
+ * (may be NULL) receives the loss before each step. */
+int tailor_mlp_train(tailor_mlp *m, const float *X, const float *T, int32_t n, int32_t steps,
+                     float lr, float *loss_history);
 
 #ifdef __cplusplus
 }

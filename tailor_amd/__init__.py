@@ -86,7 +86,8 @@ class FactorDiag(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("nev", C.c_int32), ("ncv", C.c_int32), ("maxit", C.c_int32),
                 ("tol", C.c_double), ("omega_re", C.c_double), ("omega_im", C.c_double),
-                ("refine_max", C.c_int32), ("refine_tol", C.c_double)]
+                ("refine_max", C.c_int32), ("refine_tol", C.c_double),
+                ("warm_start", C.c_int32)]
 
 
 class SolveDiag(C.Structure):
@@ -283,10 +284,12 @@ def _linear_solve(self, b: np.ndarray, refine_max: int = 0, refine_tol: float = 
 
 def _solve(self, nev: int = 8, omega: complex = 0j, tol: float = 1e-10, maxit: int = 1000,
            ncv: int = 0, refine_max: int = 2, refine_tol: float = 1e-13,
-           raise_on_failure: bool = True) -> dict:
-    """Shift-invert Krylov-Schur (EigenSolver::solve)."""
+           raise_on_failure: bool = True, warm_start: bool = False) -> dict:
+    """Shift-invert Krylov-Schur (EigenSolver::solve).  With warm_start the
+    iteration starts from the previous solve's Ritz vectors."""
     omega = complex(omega)
-    o = SolveOpts(nev, ncv, maxit, tol, omega.real, omega.imag, refine_max, refine_tol)
+    o = SolveOpts(nev, ncv, maxit, tol, omega.real, omega.imag, refine_max, refine_tol,
+                  1 if warm_start else 0)
     d = SolveDiag()
     st = self.lib.tailor_solve(self.ptr, C.byref(o), C.byref(d))
     out = _sdict(d)

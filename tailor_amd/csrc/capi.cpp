@@ -247,24 +247,14 @@ int tailor_load_case(tailor_ctx *ctx, const tailor_case *c) {
                 "At least two xi planes are required for wall/far-field boundaries");
   if (!c->grid || !c->baseflow) return fail(ctx, TAILOR_ERR_USER_INPUT, "grid/baseflow missing");
 
-  free_case(ctx);
   const int ny = c->ny, nz = c->nz, ssx = c->xi.q + 1, sse = c->eta.q + 1;
-  tailor::CaseDims &d = ctx->dims;
+  tailor::CaseDims d{};
   d.ny = ny;
   d.nz = nz;
   d.ssx = ssx;
   d.sse = sse;
   d.nodes = static_cast<long long>(ny) * nz;
   d.nnzb = d.nodes * ssx * sse;
-  ctx->period = c->spanwise_period;
-  ctx->recipe[0] = c->reynolds;
-  ctx->recipe[1] = c->mach;
-  ctx->recipe[2] = c->prandtl;
-  ctx->recipe[3] = c->gamma;
-  ctx->recipe[4] = c->reference_temperature;
-  ctx->recipe[5] = c->sutherland_constant;
-  ctx->alpha_re = c->alpha_re;
-  ctx->alpha_im = c->alpha_im;
 
   // structure: sorted block columns of the tensor-product stencil
   std::vector<int> xoff(static_cast<size_t>(ny) * ssx), eoff(static_cast<size_t>(nz) * sse);
@@ -283,38 +273,63 @@ int tailor_load_case(tailor_ctx *ctx, const tailor_case *c) {
       return fail(ctx, TAILOR_ERR_ARG_OUTOFRANGE,
                   "periodic eta stencil wraps onto itself: Nz must exceed q_z");
   }
-  ctx->h_rowptr.resize(d.nodes + 1);
-  ctx->h_colidx.resize(d.nnzb);
+  std::vector<int> rowptr(d.nodes + 1), colidx(d.nnzb);
   for (int j = 0; j < nz; ++j) {
     std::vector<int> jcols;
     for (int t = 0; t < sse; ++t) jcols.push_back(static_cast<int>(c->eta.stencil_indices[j * sse + t]));
     std::sort(jcols.begin(), jcols.end());
     for (int i = 0; i < ny; ++i) {
       const long long row = static_cast<long long>(j) * ny + i;
-      ctx->h_rowptr[row] = static_cast<int>(row * ssx * sse);
-      int *dst = ctx->h_colidx.data() + row * ssx * sse;
+      rowptr[row] = static_cast<int>(row * ssx * sse);
+      int *dst = colidx.data() + row * ssx * sse;
       for (int tr = 0; tr < sse; ++tr)
         for (int s = 0; s < ssx; ++s)
           dst[tr * ssx + s] = jcols[tr] * ny + static_cast<int>(c->xi.stencil_indices[i * ssx + s]);
     }
   }
-  ctx->h_rowptr[d.nodes] = static_cast<int>(d.nnzb);
-  ctx->h_xoff = xoff;
-  ctx->h_eoff = eoff;
+  rowptr[d.nodes] = static_cast<int>(d.nnzb);
 
-  HIP_TRY(ctx, dalloc(&ctx->d_xoff, xoff.size()));
-  HIP_TRY(ctx, dalloc(&ctx->d_eoff, eoff.size()));
-  HIP_TRY(ctx, dalloc(&ctx->d_xw, 3 * xoff.size()));
-  HIP_TRY(ctx, dalloc(&ctx->d_ew, 3 * eoff.size()));
-  HIP_TRY(ctx, dalloc(&ctx->d_grid, static_cast<size_t>(d.nodes) * 6));
-  HIP_TRY(ctx, dalloc(&ctx->d_base, static_cast<size_t>(d.nodes) * 10));
-  HIP_TRY(ctx, dalloc(&ctx->d_metrics, static_cast<size_t>(d.nodes) * 10));
-  HIP_TRY(ctx, dalloc(&ctx->d_deriv, static_cast<size_t>(d.nodes) * 25));
-  HIP_TRY(ctx, dalloc(&ctx->d_rowptr, static_cast<size_t>(d.nodes) + 1));
-  HIP_TRY(ctx, dalloc(&ctx->d_colidx, static_cast<size_t>(d.nnzb)));
-  HIP_TRY(ctx, dalloc(&ctx->d_vals, static_cast<size_t>(d.nnzb) * 50));
-  HIP_TRY(ctx, dalloc(&ctx->d_mvals, static_cast<size_t>(d.nodes) * 50));
-  HIP_TRY(ctx, dalloc(&ctx->d_bcdata, static_cast<size_t>(nz) * 106));
+  // A case with the structure of the loaded one (same grid sizes and
+  // stencils; other coordinates, base flow, weights or physics) keeps the
+  // device buffers and the symbolic analysis of the factorization.
+  const tailor::CaseDims &o = ctx->dims;
+  const bool same = ctx->loaded && o.ny == d.ny && o.nz == d.nz && o.ssx == d.ssx &&
+                    o.sse == d.sse && xoff == ctx->h_xoff && eoff == ctx->h_eoff &&
+                    colidx == ctx->h_colidx;
+  if (!same) {
+    free_case(ctx);
+    ctx->dims = d;
+    ctx->h_rowptr = std::move(rowptr);
+    ctx->h_colidx = std::move(colidx);
+    ctx->h_xoff = xoff;
+    ctx->h_eoff = eoff;
+    HIP_TRY(ctx, dalloc(&ctx->d_xoff, xoff.size()));
+    HIP_TRY(ctx, dalloc(&ctx->d_eoff, eoff.size()));
+    HIP_TRY(ctx, dalloc(&ctx->d_xw, 3 * xoff.size()));
+    HIP_TRY(ctx, dalloc(&ctx->d_ew, 3 * eoff.size()));
+    HIP_TRY(ctx, dalloc(&ctx->d_grid, static_cast<size_t>(d.nodes) * 6));
+    HIP_TRY(ctx, dalloc(&ctx->d_base, static_cast<size_t>(d.nodes) * 10));
+    HIP_TRY(ctx, dalloc(&ctx->d_metrics, static_cast<size_t>(d.nodes) * 10));
+    HIP_TRY(ctx, dalloc(&ctx->d_deriv, static_cast<size_t>(d.nodes) * 25));
+    HIP_TRY(ctx, dalloc(&ctx->d_rowptr, static_cast<size_t>(d.nodes) + 1));
+    HIP_TRY(ctx, dalloc(&ctx->d_colidx, static_cast<size_t>(d.nnzb)));
+    HIP_TRY(ctx, dalloc(&ctx->d_vals, static_cast<size_t>(d.nnzb) * 50));
+    HIP_TRY(ctx, dalloc(&ctx->d_mvals, static_cast<size_t>(d.nodes) * 50));
+    HIP_TRY(ctx, dalloc(&ctx->d_bcdata, static_cast<size_t>(nz) * 106));
+  } else {
+    ctx->assembled = false;
+    ctx->vals_kind = ctx->mvals_kind = -1;
+    if (ctx->solver) ctx->solver->factored = false;
+  }
+  ctx->period = c->spanwise_period;
+  ctx->recipe[0] = c->reynolds;
+  ctx->recipe[1] = c->mach;
+  ctx->recipe[2] = c->prandtl;
+  ctx->recipe[3] = c->gamma;
+  ctx->recipe[4] = c->reference_temperature;
+  ctx->recipe[5] = c->sutherland_constant;
+  ctx->alpha_re = c->alpha_re;
+  ctx->alpha_im = c->alpha_im;
   auto up = [&](void *dst, const void *src, size_t bytes) {
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
   };
@@ -324,8 +339,10 @@ int tailor_load_case(tailor_ctx *ctx, const tailor_case *c) {
   HIP_TRY(ctx, up(ctx->d_ew, c->eta.weights, 3 * eoff.size() * sizeof(double)));
   HIP_TRY(ctx, up(ctx->d_grid, c->grid, static_cast<size_t>(d.nodes) * 6 * sizeof(double)));
   HIP_TRY(ctx, up(ctx->d_base, c->baseflow, static_cast<size_t>(d.nodes) * 10 * sizeof(double)));
-  HIP_TRY(ctx, up(ctx->d_rowptr, ctx->h_rowptr.data(), ctx->h_rowptr.size() * sizeof(int)));
-  HIP_TRY(ctx, up(ctx->d_colidx, ctx->h_colidx.data(), ctx->h_colidx.size() * sizeof(int)));
+  if (!same) {
+    HIP_TRY(ctx, up(ctx->d_rowptr, ctx->h_rowptr.data(), ctx->h_rowptr.size() * sizeof(int)));
+    HIP_TRY(ctx, up(ctx->d_colidx, ctx->h_colidx.data(), ctx->h_colidx.size() * sizeof(int)));
+  }
   HIP_TRY(ctx, up(ctx->d_recipe, ctx->recipe, 6 * sizeof(double)));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   ctx->loaded = true;

@@ -250,13 +250,15 @@ int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solv
   ncv = std::min<long long>(ncv, n - 1);
   if (ncv > 128) ncv = 128;
   if (nev <= 0 || nev >= ncv) return fail(ctx, TAILOR_ERR_USER_INPUT, "invalid nev / ncv");
+  // the previous solve's Ritz vectors survive until the start vector is built
+  const int prev_ritz = (o.warm_start && S.ritz && S.nritz > 0) ? std::min(S.nritz, nev) : 0;
+  const int prev_ld = S.ritz_ld;
   HIP_OK(grow(&S.V, S.V_cap, static_cast<size_t>(2 * n * (ncv + 1))));
   HIP_OK(grow(&S.work, S.work_cap, static_cast<size_t>(12 * n)));
   // dot partials for up to ncv + 1 columns, and the small host-computed
   // matrices (QY: ncv x ncheck, Q_l: ncv x l, both below (ncv + 1)^2)
   HIP_OK(grow(&S.partials, S.partials_cap, static_cast<size_t>(2 * kDotBlocks * (ncv + 1))));
   HIP_OK(grow(&S.small, S.small_cap, static_cast<size_t>(2 * (ncv + 1) * (ncv + 1))));
-  HIP_OK(grow(&S.ritz, S.ritz_cap, static_cast<size_t>(2 * n * ncv)));
   double *Vr = S.V, *Vi = S.V + n * (ncv + 1);
   auto col_r = [&](int j) { return Vr + static_cast<long long>(j) * n; };
   auto col_i = [&](int j) { return Vi + static_cast<long long>(j) * n; };
@@ -282,7 +284,22 @@ int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solv
     double nv;
     if (int s2 = solver_norm2(ctx, col_r(0), col_i(0), &nv)) return s2;
     HIP_OK(vec_axpby(ctx->stream, n, 0.0, 0.0, col_r(0), col_i(0), 1.0 / nv, 0.0, col_r(0), col_i(0)));
+    // warm start: v0 = sum of the previous unit Ritz vectors + 1e-2 * random
+    if (prev_ritz > 0) {
+      HIP_OK(vec_axpby(ctx->stream, n, 0.0, 0.0, col_r(0), col_i(0), 1e-2, 0.0, col_r(0), col_i(0)));
+      for (int i = 0; i < prev_ritz; ++i) {
+        const double *Rr = S.ritz + i * n, *Ri = S.ritz + n * prev_ld + i * n;
+        double nr;
+        if (int s2 = solver_norm2(ctx, Rr, Ri, &nr)) return s2;
+        if (!(nr > 0.0)) continue;
+        HIP_OK(vec_axpby(ctx->stream, n, 1.0 / nr, 0.0, Rr, Ri, 1.0, 0.0, col_r(0), col_i(0)));
+      }
+      if (int s2 = solver_norm2(ctx, col_r(0), col_i(0), &nv)) return s2;
+      HIP_OK(vec_axpby(ctx->stream, n, 0.0, 0.0, col_r(0), col_i(0), 1.0 / nv, 0.0, col_r(0), col_i(0)));
+    }
   }
+  S.nritz = 0;
+  HIP_OK(grow(&S.ritz, S.ritz_cap, static_cast<size_t>(2 * n * ncv)));
 
   CMat Sm(ncv + 1, ncv);  // projected matrix (rows 0..ncv)
   int k = 0;

@@ -1,0 +1,208 @@
+"""Wavenumber sweeps: many (grid, base flow, alpha) instances through the
+assemble -> factor -> Krylov-Schur path of one GPU.
+
+Forms of a sweep:
+
+* ``plain``   complex FP64 fronts, the fixed search centre omega_s for every
+              instance, cold start (the reference's method, instance by
+              instance);
+* ``mixed``   complex FP32 fronts with FP64 refinement of every solve to
+              1e-12, fixed omega_s;
+* ``learned`` shift centres proposed by a ShiftModel (centroid of the
+              predicted 8 leading omega), instances ordered by wavenumber so
+              each Krylov-Schur run starts from the previous one's Ritz
+              vectors, and the plain fixed-shift solve as the fallback of any
+              instance whose pairs miss the residual tolerance.
+
+One context per grid size is kept for the whole sweep, so the symbolic
+analysis and device buffers are built once per size; each instance uploads
+its grid and base flow and reassembles.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import Context, PRECISION_FP64, PRECISION_MIXED
+from .case import Physics, SplitMix64, synthetic_case
+
+
+@dataclass
+class Instance:
+    ny: int
+    nz: int
+    seed: int
+    alpha: float
+    reynolds: float = 1000.0
+    mach: float = 0.5
+
+    def build(self, omega_s: complex):
+        phys = Physics(reynolds=self.reynolds, mach=self.mach)
+        return synthetic_case(self.ny, self.nz, seed=self.seed, alpha=self.alpha,
+                              omega_shift=omega_s, physics=phys)
+
+
+@dataclass
+class Record:
+    instance: Instance
+    shift: complex
+    converged: int
+    good: int                 # pairs with true residual <= tol among the first nev
+    omegas: np.ndarray
+    residuals: np.ndarray
+    restarts: int
+    solves: int
+    refinement_steps: int
+    factorizations: int
+    fallback: bool
+    seconds: float
+
+
+@dataclass
+class SweepResult:
+    form: str
+    records: list = field(default_factory=list)
+    seconds: float = 0.0
+
+    @property
+    def good_pairs(self) -> int:
+        return sum(r.good for r in self.records)
+
+    @property
+    def eigenpairs_per_s(self) -> float:
+        return self.good_pairs / self.seconds if self.seconds > 0 else 0.0
+
+    def totals(self) -> dict:
+        rs = self.records
+        return {"instances": len(rs), "good_pairs": self.good_pairs,
+                "seconds": round(self.seconds, 4),
+                "eigenpairs_per_s": round(self.eigenpairs_per_s, 4),
+                "factorizations": sum(r.factorizations for r in rs),
+                "restarts": sum(r.restarts for r in rs),
+                "solves": sum(r.solves for r in rs),
+                "refinement_steps": sum(r.refinement_steps for r in rs),
+                "fallbacks": sum(int(r.fallback) for r in rs)}
+
+
+def sweep_instances(count: int, seed: int, sizes, alpha_range=(0.5, 3.0), reynolds=1000.0,
+                    mach=0.5, per_instance_seeds: bool = True) -> list:
+    """`count` instances, alpha ~ U(alpha_range) from a SplitMix64 stream;
+    sizes cycle over `sizes` ((ny, nz) pairs); base-flow seeds 1000 + k."""
+    rng = SplitMix64(seed)
+    out = []
+    for k in range(count):
+        a = rng.uniform(*alpha_range)
+        ny, nz = sizes[k % len(sizes)]
+        out.append(Instance(ny, nz, 1000 + k if per_instance_seeds else 0, a, reynolds, mach))
+    return out
+
+
+def training_instances(count: int, seed: int = 2, ny: int = 100, nz: int = 50) -> list:
+    """(alpha, Re, Ma) ~ U([0.5,3] x [500,3000] x [0.3,0.9]) on one grid."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        a = rng.uniform(0.5, 3.0)
+        re = rng.uniform(500.0, 3000.0)
+        ma = rng.uniform(0.3, 0.9)
+        out.append(Instance(ny, nz, 0, a, re, ma))
+    return out
+
+
+class Sweeper:
+    def __init__(self, device: int = 0, nev: int = 8, tol: float = 1e-10, ncv: int = 64,
+                 omega_s: complex = 0.02 + 0j, maxit: int = 1000):
+        self.device = device
+        self.nev, self.tol, self.ncv, self.maxit = nev, tol, ncv, maxit
+        self.omega_s = complex(omega_s)
+        self.ctx = {}
+
+    def close(self):
+        for c in self.ctx.values():
+            c.close()
+        self.ctx = {}
+
+    def context(self, ny: int, nz: int) -> Context:
+        key = (ny, nz)
+        if key not in self.ctx:
+            self.ctx[key] = Context(self.device)
+        return self.ctx[key]
+
+    def _solve(self, ctx, omega, precision, warm):
+        ctx.set_precision(precision)
+        mixed = precision == PRECISION_MIXED
+        d = ctx.solve(nev=self.nev, omega=omega, tol=self.tol, maxit=self.maxit, ncv=self.ncv,
+                      refine_max=20 if mixed else 0, refine_tol=1e-12, raise_on_failure=False,
+                      warm_start=warm)
+        pairs = ctx.eigenpairs()[:self.nev]
+        res = np.array([p["residual"] for p in pairs])
+        om = np.array([p["omega"] for p in pairs])
+        good = int(np.sum(res <= self.tol))
+        return d, om, res, good
+
+    def run(self, cases, form: str = "plain", shifts=None, precision=None, order=None,
+            on_record=None) -> SweepResult:
+        """Run the sweep over prepared (instance, case) pairs.
+
+        `shifts` (learned form) gives one omega shift centre per instance;
+        `order` the processing order (defaults to the given order)."""
+        if precision is None:
+            precision = PRECISION_MIXED if form == "mixed" else PRECISION_FP64
+        idx = list(range(len(cases))) if order is None else list(order)
+        result = SweepResult(form)
+        last_size = None
+        t_all = time.perf_counter()
+        for k in idx:
+            inst, case = cases[k]
+            ctx = self.context(inst.ny, inst.nz)
+            t0 = time.perf_counter()
+            ctx.load_case(case)
+            ctx.assemble(diagnostics=False)
+            warm = form == "learned" and last_size == (inst.ny, inst.nz)
+            omega = self.omega_s if shifts is None else complex(shifts[k])
+            d, om, res, good = self._solve(ctx, omega, precision, warm)
+            facts = 1
+            fallback = False
+            restarts, solves, refine = d["iterations"], d["solves"], d["refinement_steps"]
+            if form == "learned" and good < self.nev:
+                fallback = True
+                d, om, res, good = self._solve(ctx, self.omega_s, precision, False)
+                facts += 1
+                restarts += d["iterations"]
+                solves += d["solves"]
+                refine += d["refinement_steps"]
+                omega = self.omega_s
+            el = time.perf_counter() - t0
+            last_size = (inst.ny, inst.nz)
+            rec = Record(inst, omega, d["converged"], good, om, res, restarts, solves, refine,
+                         facts, fallback, el)
+            result.records.append(rec)
+            if on_record:
+                on_record(rec)
+        result.seconds = time.perf_counter() - t_all
+        return result
+
+
+def prepare(instances, omega_s: complex = 0.02 + 0j) -> list:
+    """Host-side case construction (preprocessing, outside any timed region)."""
+    return [(inst, inst.build(omega_s)) for inst in instances]
+
+
+def label_omegas(record: Record) -> np.ndarray:
+    """Training label of an instance: its 8 omegas by growth rate."""
+    from .shift_model import leading_order
+    return leading_order(record.omegas)
+
+
+def alpha_order(instances) -> list:
+    """Indices ordered by grid size, then wavenumber (the learned form's
+    processing order: consecutive instances are close in alpha)."""
+    return sorted(range(len(instances)),
+                  key=lambda k: (instances[k].ny * instances[k].nz, instances[k].alpha))
+
+
+def isclose_alpha(a: float, b: float) -> bool:
+    return math.isclose(a, b, rel_tol=0.0, abs_tol=1e-12)
