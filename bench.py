@@ -1,18 +1,22 @@
 """Benchmark of the BiGlobal assemble-and-solve path on one MI355X.
 
-Workload (N=1): the 200x100 synthetic case -- FD-q q_y=10 bounded by q_z=6
-periodic, 100,000 complex unknowns, 1,540,000 5x5 blocks, alpha = 2.43,
-8 eigenvalues nearest omega_s = 0.02 (lambda_s = -0.02i), tolerance 1e-10.
+Headline (`value`): eigenpairs/s of a wavenumber sweep on the 200x100 case
+(FD-q q_y=10 bounded x q_z=6 periodic, 100,000 complex unknowns, Re=1500,
+Ma=0.7, alpha ~ U[0.5, 3] from seed 1) in the learned-shift form: the shift
+network proposes each instance's shift centre, instances run in wavenumber
+order with warm-started Krylov-Schur, and the fixed-shift search is the
+fallback.  A step is one sweep instance (assembly of A_bc, B_bc, sparse LU
+of A_bc - sigma B_bc, Krylov-Schur for 8 eigenpairs); an eigenpair counts if
+its true residual ||A q - lambda B q|| / ||q|| <= 1e-10.  The same instances
+are also run in the plain form (complex FP64, fixed omega_s = 0.02, cold
+start) and the ratio is reported.
 
-A step is one full pass of the path on device-resident inputs: assembly of
-A_bc and B_bc, the sparse LU of A_bc - sigma B_bc, and the shift-invert
-Krylov-Schur solve.  `value` counts the requested eigenpairs whose true
-residual ||A q - lambda B q|| / ||q|| is <= 1e-10, per second of step time.
-
-Also reported: assembly HBM GB/s (writer kernel and whole assembly), the
-factorization's TFLOP/s, and the roofline of the kernel family that
-dominates the step (the per-level GEMV sweep of the triangular solves),
-measured with HIP events on the library's stream.
+Also reported: the plain single-instance step at alpha = 2.43 (Re=1000,
+Ma=0.5), the mixed-precision form on that operator (per-eigenvalue error
+against the FP64 result, refinement steps per solve), a small sweep over
+both grid sizes with per-instance base flows in mixed precision, assembly
+HBM GB/s, and the roofline of the frontal-update GEMM (the kernel with the
+largest share of a sweep instance) from HIP events on the library stream.
 """
 from __future__ import annotations
 
@@ -21,6 +25,8 @@ import json
 import sys
 import time
 from pathlib import Path
+
+import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
@@ -39,11 +45,10 @@ def algorithmic_assembly_bytes(ny: int, nz: int, ssx: int, sse: int) -> int:
     return 16 * 25 * (nnzb_a + nodes) + 128 * nodes + 3 * 8 * (ny * ssx + nz * sse)
 
 
-def cpu_baseline(seconds_hint: float = 30.0) -> dict:
+def cpu_baseline() -> dict:
     """The CPU oracle (C assembly + multifrontal LU on LAPACK/BLAS + numpy
     Krylov-Schur) on the 100x50 grid of the same generator (the 200x100
     case needs minutes of host time)."""
-    import numpy as np
     import oracle
     from oracle import multifrontal
     from tailor_amd import case as cs
@@ -61,24 +66,8 @@ def cpu_baseline(seconds_hint: float = 30.0) -> dict:
                       f"{res['restarts']} restarts; BLAS threads {threads}"}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--ncv", type=int, default=NCV)
-    ap.add_argument("--refine", type=int, default=0,
-                    help="iterative refinement steps per shift-invert solve")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(alpha sweeps shard by instance across GPUs; see DESIGN.md)", file=sys.stderr)
-        return 2
-
-    import tailor_amd as ta
-    from tailor_amd import case as cs
-
+def plain_step(ta, cs, steps: int, ncv: int) -> dict:
+    """The plain FP64 form on one instance (alpha 2.43, omega_s 0.02)."""
     case = cs.synthetic_case(200, 100, omega_shift=OMEGA)
     ctx = ta.Context(0)
     ctx.load_case(case)
@@ -89,62 +78,162 @@ def main() -> int:
 
     def step():
         ctx.assemble(diagnostics=False)
-        d = ctx.solve(nev=NEV, omega=OMEGA, tol=TOL, ncv=args.ncv, refine_max=args.refine,
-                      raise_on_failure=False)
+        d = ctx.solve(nev=NEV, omega=OMEGA, tol=TOL, ncv=ncv, refine_max=0, raise_on_failure=False)
         pairs = ctx.eigenpairs()[:NEV]
-        good = sum(1 for p in pairs if p["residual"] <= TOL)
-        return good, d
+        return sum(1 for p in pairs if p["residual"] <= TOL), d, pairs
 
-    for _ in range(args.warmup):
-        step()
-    good_total = 0
-    diags = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        g, d = step()
-        good_total += g
-        diags.append(d)
-    elapsed = time.perf_counter() - t0
-    ms_step = elapsed * 1e3 / args.steps
-    last = diags[-1]
+    step()
+    good, t0 = 0, time.perf_counter()
+    for _ in range(steps):
+        g, d, pairs = step()
+        good += g
+    el = time.perf_counter() - t0
+    lam64 = np.array([p["lambda"] for p in pairs])
     solve_ms, solve_bytes = ctx.time_linear_solve(20)
-    achieved = solve_bytes / (solve_ms * 1e-3) / 1e9
+    prof = ctx.profile_factor(case.lambda_shift)
+    # mixed precision on the same operator and shift
+    ctx.set_precision(ta.PRECISION_MIXED)
+    t0 = time.perf_counter()
+    dm = ctx.solve(nev=NEV, omega=OMEGA, tol=TOL, ncv=ncv, refine_max=20, refine_tol=1e-12,
+                   raise_on_failure=False)
+    mixed_s = time.perf_counter() - t0
+    pm = ctx.eigenpairs()[:NEV]
+    lam32 = np.array([p["lambda"] for p in pm])
+    errs = [float(np.min(np.abs(lam32 - l)) / abs(l)) for l in lam64]
+    ctx.close()
+    return {
+        "plain": {"eigenpairs_per_s": round(good / el, 4), "ms_per_step": round(el * 1e3 / steps, 2),
+                  "steps": steps, "factor_ms": round(d["factor_ms"], 2),
+                  "krylov_ms": round(d["solve_ms"], 2), "solves": d["solves"],
+                  "restarts": d["iterations"], "max_residual": d["max_residual"],
+                  "factor_TFLOPs_lu_flops": round(d["factor_flops"] / (d["factor_ms"] * 1e-3) / 1e12, 3)},
+        "assembly": {"GBps_writer": round(asm_bytes / (write_ms * 1e-3) / 1e9, 2),
+                     "GBps_full": round(asm_bytes / (asm_ms * 1e-3) / 1e9, 2),
+                     "frac_hbm_writer": round(asm_bytes / (write_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "writer_ms": round(write_ms, 5), "full_ms": round(asm_ms, 5),
+                     "algorithmic_bytes": asm_bytes},
+        "solve_sweep": {"kernel": "k_gemv (triangular solves)", "ms_per_solve": round(solve_ms, 4),
+                        "bytes": int(solve_bytes),
+                        "GBps": round(solve_bytes / (solve_ms * 1e-3) / 1e9, 2),
+                        "frac_hbm": round(solve_bytes / (solve_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "gemm": prof,
+        "mixed": {"seconds": round(mixed_s, 3), "factor_ms": round(dm["factor_ms"], 2),
+                  "solves": dm["solves"], "refinement_steps": dm["refinement_steps"],
+                  "refinement_steps_per_solve": round(dm["refinement_steps"] / max(1, dm["solves"]), 2),
+                  "converged": dm["converged"], "max_residual": dm["max_residual"],
+                  "eig_rel_err_vs_fp64": [float(f"{e:.3e}") for e in errs],
+                  "fp64_seconds_same_operator": round(el / steps, 3)},
+    }
+
+
+def sweeps(ta, steps: int, warmup: int, ncv: int) -> dict:
+    from tailor_amd import sweep as sw
+    from tailor_amd.shift_model import ShiftModel
+    from tailor_amd.train_shift_model import MODEL
+    if not MODEL.exists():
+        raise SystemExit(f"{MODEL} is missing: run `python -m tailor_amd.train_shift_model`")
+    model = ShiftModel(0).load(MODEL)
+    insts = sw.sweep_instances(warmup + steps, seed=1, sizes=[(200, 100)], reynolds=1500.0,
+                               mach=0.7, per_instance_seeds=False)
+    cases = sw.prepare(insts, OMEGA)
+    alphas = np.array([i.alpha for i in insts])
+    shifts = model.shift_centres(alphas, np.full(len(insts), 1500.0), np.full(len(insts), 0.7))
+    runner = sw.Sweeper(ncv=ncv, omega_s=OMEGA)
+    # warm-up instances (untimed): build the plan, and let the first one
+    # choose the factorization precision by measurement
+    warm = list(range(warmup))
+    choice = {}
+    precision = ta.PRECISION_FP64
+    if warm:
+        t = {}
+        for prec in (ta.PRECISION_FP64, ta.PRECISION_MIXED):
+            r = runner.run(cases, "learned", shifts=shifts, precision=prec, order=warm[:1])
+            t[prec] = r.seconds
+        precision = min(t, key=t.get)
+        choice = {"fp64_s": round(t[ta.PRECISION_FP64], 3), "mixed_s": round(t[ta.PRECISION_MIXED], 3),
+                  "chosen": "mixed" if precision == ta.PRECISION_MIXED else "fp64"}
+    timed = list(range(warmup, warmup + steps))
+    order = sorted(timed, key=lambda k: insts[k].alpha)
+    learned = runner.run(cases, "learned", shifts=shifts, precision=precision, order=order)
+    plain = runner.run(cases, "plain", precision=ta.PRECISION_FP64, order=timed)
+    # eigenvalues found by both forms must agree
+    lam_diff = 0.0
+    by_inst = {id(r.instance): r for r in plain.records}
+    for r in learned.records:
+        p = by_inst[id(r.instance)]
+        for w in r.omegas[r.residuals <= TOL]:
+            d = np.min(np.abs(p.omegas - w)) / max(abs(w), 1e-300)
+            if d < 1e-6:
+                lam_diff = max(lam_diff, float(d))
+    # config 3: both grid sizes, per-instance base flows, mixed precision
+    c3 = sw.sweep_instances(4, seed=1, sizes=[(100, 50), (200, 100)])
+    c3_cases = sw.prepare(c3, OMEGA)
+    mixed_sweep = runner.run(c3_cases, "mixed")
+    runner.close()
+    return {"learned": learned, "plain": plain, "choice": choice, "common_pair_rel_diff": lam_diff,
+            "mixed_sweep": mixed_sweep, "shifts": shifts}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=32, help="timed sweep instances")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed warm-up instances")
+    ap.add_argument("--ncv", type=int, default=NCV)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(alpha sweeps shard by instance across GPUs; see DESIGN.md)", file=sys.stderr)
+        return 2
+    if args.steps < 1:
+        print("bench.py: --steps must be at least 1", file=sys.stderr)
+        return 2
+
+    import tailor_amd as ta
+    from tailor_amd import case as cs
+
+    single = plain_step(ta, cs, 2, args.ncv)
+    sw = sweeps(ta, args.steps, max(1, args.warmup), args.ncv)
+    L, P = sw["learned"], sw["plain"]
+    g = single["gemm"]
+    gemm_tf = g["gemm_flops"] / (g["gemm_ms"] * 1e-3) / 1e12
     out = {
-        "metric": "eigenpairs/s at resid<=1e-10 (plain FP64 form)",
-        "value": round(good_total / elapsed, 4),
+        "metric": "eigenpairs/s at resid<=1e-10 (learned-shift sweep; plain FP64 alongside)",
+        "value": round(L.eigenpairs_per_s, 4),
         "unit": "eigenpairs/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 3),
+        "warmup": max(1, args.warmup),
+        "ms_per_step": round(L.seconds * 1e3 / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128",
+        "dtype": "c128" if sw["choice"].get("chosen", "fp64") == "fp64" else "c64 factors + c128",
         "data": "synthetic (erf boundary layer on tanh-clustered wall-bumped grid, seed 0)",
-        "config": {"workload": "BiGlobal assemble + sparse LU + shift-invert Krylov-Schur, 200x100 "
-                               "FD-q grid (q_y=10, q_z=6), 100,000 unknowns, alpha=2.43, nev=8, "
-                               "omega_s=0.02, tol=1e-10",
-                   "form": "plain complex FP64", "nev": NEV, "ncv": args.ncv,
-                   "refinement_steps_per_solve": args.refine},
-        "roofline": {"bound": "hbm", "kernel": "k_gemv (triangular-solve sweep)",
-                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch_set": int(solve_bytes), "ms_per_solve": round(solve_ms, 4)},
-        "assembly": {"GBps_writer": round(asm_bytes / (write_ms * 1e-3) / 1e9, 2),
-                     "GBps_full": round(asm_bytes / (asm_ms * 1e-3) / 1e9, 2),
-                     "writer_ms": round(write_ms, 5), "full_ms": round(asm_ms, 5),
-                     "algorithmic_bytes": asm_bytes},
-        "factor": {"ms": round(last["factor_ms"], 2), "flops": last["factor_flops"],
-                   "TFLOPs": round(last["factor_flops"] / (last["factor_ms"] * 1e-3) / 1e12, 3),
-                   "frac_fp64_peak": round(last["factor_flops"] / (last["factor_ms"] * 1e-3) / 1e12
-                                           / FP64_PEAK_TFLOPS, 4),
-                   "bytes": last["factor_bytes"]},
-        "krylov": {"restarts": last["iterations"], "solves": last["solves"],
-                   "refinement_steps": last["refinement_steps"], "ms": round(last["solve_ms"], 2),
-                   "converged": last["converged"], "max_residual": last["max_residual"]},
+        "config": {"workload": "BiGlobal wavenumber sweep, 200x100 FD-q grid (q_y=10, q_z=6), "
+                               "100,000 unknowns, Re=1500, Ma=0.7, alpha ~ U[0.5,3] seed 1, nev=8, "
+                               "tol=1e-10; learned shift centres, warm-started Krylov-Schur, "
+                               "fixed-shift fallback",
+                   "nev": NEV, "ncv": args.ncv, "instances": args.steps,
+                   "precision_choice": sw["choice"]},
+        "learned_sweep": L.totals(),
+        "plain_sweep": P.totals(),
+        "speedup_vs_plain_sweep": round(L.eigenpairs_per_s / max(P.eigenpairs_per_s, 1e-300), 3),
+        "common_pair_max_rel_diff": sw["common_pair_rel_diff"],
+        "roofline": {"bound": "mfma", "kernel": "k_gemm (frontal updates, MFMA f64 16x16x4)",
+                     "achieved": round(gemm_tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(gemm_tf / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "gemm_ms_per_factorization": round(g["gemm_ms"], 2),
+                     "gemm_flops_per_factorization": g["gemm_flops"],
+                     "gemm_launches": g["gemm_launches"],
+                     "factor_ms": round(g["factor_ms"], 2)},
+        "plain_instance": single["plain"],
+        "assembly": single["assembly"],
+        "solve_sweep": single["solve_sweep"],
+        "mixed_precision": single["mixed"],
+        "mixed_size_sweep": sw["mixed_sweep"].totals(),
     }
-    ctx.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -157,6 +157,13 @@ typedef struct {
 
 int tailor_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, tailor_factor_diag *diag);
 
+/* Factor like tailor_factor, timing every frontal-update GEMM launch with
+ * HIP events on the context stream: total factorization time, summed GEMM
+ * time, the GEMMs' algorithmic flops (8 M N K per front and launch) and the
+ * number of GEMM launches.  No reference counterpart (measurement only). */
+int tailor_profile_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, double *factor_ms,
+                          double *gemm_ms, double *gemm_flops, int32_t *gemm_launches);
+
 /* Precision of the factorization used by tailor_factor / tailor_solve.
  * FP64: complex FP64 fronts (the plain form).  MIXED: complex FP32 fronts
  * factored on FP32 MFMA, with FP64 right-hand sides, FP64 residuals and

@@ -136,6 +136,8 @@ def library() -> C.CDLL:
                                         C.c_double, P(C.c_double), P(C.c_int32)]
     lib.tailor_solve.argtypes = [C.c_void_p, P(SolveOpts), P(SolveDiag)]
     lib.tailor_set_precision.argtypes = [C.c_void_p, C.c_int32]
+    lib.tailor_profile_factor.argtypes = [C.c_void_p, C.c_double, C.c_double, P(C.c_double),
+                                          P(C.c_double), P(C.c_double), P(C.c_int32)]
     lib.tailor_time_linear_solve.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double)]
     lib.tailor_eigenpair_count.argtypes = [C.c_void_p]
     lib.tailor_get_eigenpair.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double),
@@ -263,6 +265,16 @@ def _set_precision(self, precision: int) -> None:
     self._check(self.lib.tailor_set_precision(self.ptr, int(precision)))
 
 
+def _profile_factor(self, sigma: complex) -> dict:
+    """Factor with HIP events around every frontal GEMM launch."""
+    sigma = complex(sigma)
+    f, g, fl, n = C.c_double(), C.c_double(), C.c_double(), C.c_int32()
+    self._check(self.lib.tailor_profile_factor(self.ptr, sigma.real, sigma.imag, C.byref(f),
+                                               C.byref(g), C.byref(fl), C.byref(n)))
+    return {"factor_ms": f.value, "gemm_ms": g.value, "gemm_flops": fl.value,
+            "gemm_launches": n.value}
+
+
 def _factor(self, sigma: complex) -> dict:
     d = FactorDiag()
     sigma = complex(sigma)
@@ -325,6 +337,7 @@ def _time_linear_solve(self, reps: int = 20):
 Context.time_linear_solve = _time_linear_solve
 Context.factor = _factor
 Context.set_precision = _set_precision
+Context.profile_factor = _profile_factor
 Context.linear_solve = _linear_solve
 Context.solve = _solve
 Context.eigenpairs = _eigenpairs

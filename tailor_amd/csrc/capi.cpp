@@ -583,6 +583,21 @@ int tailor_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, tailor_fact
   return TAILOR_OK;
 }
 
+int tailor_profile_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, double *factor_ms,
+                          double *gemm_ms, double *gemm_flops, int32_t *gemm_launches) {
+  if (!ctx) return TAILOR_ERR_USER_INPUT;
+  if (!std::isfinite(sigma_re) || !std::isfinite(sigma_im))
+    return fail(ctx, TAILOR_ERR_USER_INPUT, "shift must be finite");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  tailor::FactorTiming tm;
+  if (int st = tailor::solver_factor(ctx, tailor::cplx(sigma_re, sigma_im), &tm)) return st;
+  if (factor_ms) *factor_ms = ctx->solver->factor_ms;
+  if (gemm_ms) *gemm_ms = tm.gemm_ms;
+  if (gemm_flops) *gemm_flops = tm.gemm_flops;
+  if (gemm_launches) *gemm_launches = tm.gemm_launches;
+  return TAILOR_OK;
+}
+
 int tailor_set_precision(tailor_ctx *ctx, int32_t precision) {
   if (!ctx) return TAILOR_ERR_USER_INPUT;
   if (precision != TAILOR_PRECISION_FP64 && precision != TAILOR_PRECISION_MIXED)

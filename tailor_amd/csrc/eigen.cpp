@@ -91,7 +91,7 @@ int solver_prepare(tailor_ctx *ctx) {
   return TAILOR_OK;
 }
 
-int solver_factor(tailor_ctx *ctx, cplx sigma) {
+int solver_factor(tailor_ctx *ctx, cplx sigma, FactorTiming *timing) {
   if (!ctx->assembled || ctx->vals_kind != TAILOR_OP_A || ctx->mvals_kind != TAILOR_OP_B)
     return fail(ctx, TAILOR_ERR_ARG_WRONGSTATE,
                 "Boundary-constrained matrices are required before EPSSolve");
@@ -119,7 +119,7 @@ int solver_factor(tailor_ctx *ctx, cplx sigma) {
   const bool fp32 = ctx->precision == TAILOR_PRECISION_MIXED;
   const double tiny = (fp32 ? 1e-7 : 1e-14) * scale;
   const double perturb = (fp32 ? 3.4526698300124393e-04 : 1.4901161193847656e-08) * scale;
-  HIP_OK(lu_factor(S.plan, in, sigma.real(), sigma.imag(), tiny, perturb, fp32, ctx->stream));
+  HIP_OK(lu_factor(S.plan, in, sigma.real(), sigma.imag(), tiny, perturb, fp32, ctx->stream, timing));
   HIP_OK(hipEventRecord(e1, ctx->stream));
   HIP_OK(hipEventSynchronize(e1));
   float ms = 0.f;

@@ -130,11 +130,14 @@ __global__ void k_extend_add(const int *__restrict__ tasks, int ntasks,
 // factorization.
 
 constexpr int kPanelThreads = 1024;
+constexpr int kPanelChunk = 8;  // panel columns updated per batch of loads
 
 // Unblocked LU of the panel columns [k0, k0+kb) over the fully summed rows
 // [k0, p) of each front; pivot rows are chosen among those rows and the
 // exchange is applied to the columns [K0, K1) of the current block.  The
-// update rows [p, m) of the panel are finished by k_l21.
+// update rows [p, m) of the panel are finished by k_l21.  The pivot search
+// reduces within waves by shuffles; each row's update loads kPanelChunk
+// columns before storing any, so the loads overlap.
 template <typename T>
 __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__ list, int nlist,
                                                          const FrontDev *__restrict__ fr, int k0,
@@ -148,12 +151,16 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__
   const int kb = min(nb, d.p - k0);
   const int K1 = min(K0 + NB, d.p);
   const long long mm = static_cast<long long>(d.m) * d.m;
-  T *Fr = arena + d.off, *Fi = Fr + mm;
-  __shared__ double sv[kPanelThreads];
-  __shared__ int si[kPanelThreads];
+  T *__restrict__ Fr = arena + d.off;
+  T *__restrict__ Fi = Fr + mm;
+  constexpr int kWaves = kPanelThreads / 64;
+  __shared__ double sv[kWaves];
+  __shared__ int si[kWaves];
+  __shared__ int spiv;
+  __shared__ bool ssmall;
   __shared__ T prow_r[kLuNb], prow_i[kLuNb];
   __shared__ T inv_r, inv_i;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   for (int j = k0; j < k0 + kb; ++j) {
     const long long cj = static_cast<long long>(j) * d.m;
     double best = -1.0;
@@ -165,23 +172,40 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__
         bi = r;
       }
     }
-    sv[tid] = best;
-    si[tid] = bi;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(best, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      sv[wave] = best;
+      si[wave] = bi;
+    }
     __syncthreads();
-    for (int w = kPanelThreads / 2; w > 0; w >>= 1) {
-      if (tid < w) {
-        const double o = sv[tid + w];
-        const int oi = si[tid + w];
-        if (o > sv[tid] || (o == sv[tid] && oi < si[tid])) {
-          sv[tid] = o;
-          si[tid] = oi;
+    if (wave == 0) {
+      best = lane < kWaves ? sv[lane] : -1.0;
+      bi = lane < kWaves ? si[lane] : 0x7fffffff;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
         }
       }
-      __syncthreads();
+      if (lane == 0) {
+        spiv = bi;
+        ssmall = !(best > tiny);
+      }
     }
-    const int piv = si[0];
-    const bool small = !(sv[0] > tiny);
     __syncthreads();
+    const int piv = spiv;
+    const bool small = ssmall;
     if (!small && piv != j && tid < K1 - K0) {
       const long long col = static_cast<long long>(K0 + tid) * d.m;
       T t = Fr[col + j]; Fr[col + j] = Fr[col + piv]; Fr[col + piv] = t;
@@ -216,11 +240,24 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__
       const T lr = ar * ir - ai * ii, li = ar * ii + ai * ir;
       Fr[cj + r] = lr;
       Fi[cj + r] = li;
-      for (int q = 0; q < ncol; ++q) {
-        const long long cq = static_cast<long long>(j + 1 + q) * d.m + r;
-        const T ur = prow_r[q], ui = prow_i[q];
-        Fr[cq] -= lr * ur - li * ui;
-        Fi[cq] -= lr * ui + li * ur;
+      for (int q0 = 0; q0 < ncol; q0 += kPanelChunk) {
+        T xr[kPanelChunk], xi[kPanelChunk];
+#pragma unroll
+        for (int u = 0; u < kPanelChunk; ++u) {
+          const int q = q0 + u < ncol ? q0 + u : q0;  // clamped: loads stay in the panel
+          const long long cq = static_cast<long long>(j + 1 + q) * d.m + r;
+          xr[u] = Fr[cq];
+          xi[u] = Fi[cq];
+        }
+#pragma unroll
+        for (int u = 0; u < kPanelChunk; ++u) {
+          if (q0 + u < ncol) {
+            const long long cq = static_cast<long long>(j + 1 + q0 + u) * d.m + r;
+            const T ur = prow_r[q0 + u], ui = prow_i[q0 + u];
+            Fr[cq] = xr[u] - (lr * ur - li * ui);
+            Fi[cq] = xi[u] - (lr * ui + li * ur);
+          }
+        }
       }
     }
     __syncthreads();
@@ -479,6 +516,35 @@ template <typename T> struct GemmView {
   int M, M2, N, N1, K;
 };
 
+// Output rows (M for the first column segment, M2 for the second), columns
+// N and depth K of one front's update; shared by the kernel and the host's
+// flop accounting.
+struct GemmShape {
+  int M, M2, N, N1, K;
+};
+__host__ __device__ inline GemmShape gemm_shape(int region, const FrontDev &d, int k0, int nb,
+                                                int K0, int NB) {
+  GemmShape g{0, 0, 0, 0, 0};
+  if (region == kGemmTrail ? K0 >= d.p : k0 >= d.p) return g;
+  const int kb = min(nb, d.p - k0);
+  const int K1 = min(K0 + NB, d.p);
+  switch (region) {
+    case kGemmInner:
+      g = {d.m - k0 - kb, d.m - k0 - kb, K1 - k0 - kb, K1 - k0 - kb, kb};
+      break;
+    case kGemmBlkRows:
+      g = {K1 - k0 - kb, K1 - k0 - kb, d.m - K1 + d.p, d.m - K1, kb};
+      break;
+    case kGemmTrail:
+      g = {d.m - K1, d.p - K1, d.m - K1 + d.p, d.m - K1, K1 - K0};
+      break;
+    default:
+      g = {k0, k0, d.p - k0, d.p - k0, kb};
+      break;
+  }
+  return g;
+}
+
 template <typename T>
 __device__ inline bool gemm_view(int region, const FrontDev &d, int k0, int nb, int K0, int NB,
                                  T *arena, GemmView<T> &v) {
@@ -538,10 +604,14 @@ __global__ __launch_bounds__(256) void k_gemm(const int *__restrict__ list, int 
   const int tile = blockIdx.x;
   if (tile >= tiles_m * tiles_n) return;
   const int row0 = (tile % tiles_m) * GT, col0 = (tile / tiles_m) * GT;
-  __shared__ T sAr[GKC][GLD], sAi[GKC][GLD], sBr[GKC][GLD], sBi[GKC][GLD];
+  // two LDS buffers: the next K chunk is stored while the current one feeds
+  // the MFMAs, so each chunk needs one barrier
+  __shared__ T sAr[2][GKC][GLD], sAi[2][GKC][GLD], sBr[2][GKC][GLD], sBi[2][GKC][GLD];
   const int t = threadIdx.x, l = t % 64, w = t / 64;
   const int wm = w % 2, wn = w / 2;
-  // global -> register staging: A (row t%64, k t/64 + 4s), B (k t%16, col t/16 + 16s)
+  // global -> register staging: A (row t%64, k t/64 + 4s), B (k t%16, col t/16 + 16s).
+  // Loads are unconditional from clamped in-bounds addresses; out-of-range
+  // entries are replaced by zero afterwards.
   const int ar_ = t % 64, ak_ = t / 64;
   const int bk_ = t % 16, bc_ = t / 16;
   const bool arow_ok = row0 + ar_ < v.M;
@@ -551,13 +621,15 @@ __global__ __launch_bounds__(256) void k_gemm(const int *__restrict__ list, int 
   for (int s = 0; s < 4; ++s) {
     const int col = col0 + bc_ + 16 * s;
     bcol_ok[s] = col < v.N;
-    if (col < v.N1) {
+    if (!bcol_ok[s]) {
+      bcol_r[s] = v.Br;
+      bcol_i[s] = v.Bi;
+    } else if (col < v.N1) {
       bcol_r[s] = v.Br + col * v.ldb;
       bcol_i[s] = v.Bi + col * v.ldb;
     } else {
-      const long long c2 = bcol_ok[s] ? (col - v.N1) * v.ldb2 : 0;
-      bcol_r[s] = v.B2r + c2;
-      bcol_i[s] = v.B2i + c2;
+      bcol_r[s] = v.B2r + (col - v.N1) * v.ldb2;
+      bcol_i[s] = v.B2i + (col - v.N1) * v.ldb2;
     }
   }
   T ga_r[4], ga_i[4], gb_r[4], gb_i[4];
@@ -566,13 +638,25 @@ __global__ __launch_bounds__(256) void k_gemm(const int *__restrict__ list, int 
     for (int s = 0; s < 4; ++s) {
       const int ka = kc + ak_ + 4 * s;
       const bool oka = arow_ok && ka < v.K;
-      const long long ia = static_cast<long long>(ka) * v.lda + row0 + ar_;
-      ga_r[s] = oka ? v.Ar[ia] : T(0);
-      ga_i[s] = oka ? v.Ai[ia] : T(0);
+      const long long ia = oka ? static_cast<long long>(ka) * v.lda + row0 + ar_ : 0;
+      const T xr = v.Ar[ia], xi = v.Ai[ia];
+      ga_r[s] = oka ? xr : T(0);
+      ga_i[s] = oka ? xi : T(0);
       const int kbb = kc + bk_;
       const bool okb = bcol_ok[s] && kbb < v.K;
-      gb_r[s] = okb ? bcol_r[s][kbb] : T(0);
-      gb_i[s] = okb ? bcol_i[s][kbb] : T(0);
+      const int ib = okb ? kbb : 0;
+      const T yr = bcol_r[s][ib], yi = bcol_i[s][ib];
+      gb_r[s] = okb ? yr : T(0);
+      gb_i[s] = okb ? yi : T(0);
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      sAr[buf][ak_ + 4 * s][ar_] = ga_r[s];
+      sAi[buf][ak_ + 4 * s][ar_] = ga_i[s];
+      sBr[buf][bk_][bc_ + 16 * s] = gb_r[s];
+      sBi[buf][bk_][bc_ + 16 * s] = gb_i[s];
     }
   };
   using acc_t = typename Mfma<T>::acc_t;
@@ -585,26 +669,22 @@ __global__ __launch_bounds__(256) void k_gemm(const int *__restrict__ list, int 
       ci[a][b] = acc_t{0, 0, 0, 0};
     }
   fetch(0);
+  stash(0);
+  __syncthreads();
+  int buf = 0;
   for (int kc = 0; kc < v.K; kc += GKC) {
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      sAr[ak_ + 4 * s][ar_] = ga_r[s];
-      sAi[ak_ + 4 * s][ar_] = ga_i[s];
-      sBr[bk_][bc_ + 16 * s] = gb_r[s];
-      sBi[bk_][bc_ + 16 * s] = gb_i[s];
-    }
-    __syncthreads();
-    if (kc + GKC < v.K) fetch(kc + GKC);
+    const bool more = kc + GKC < v.K;
+    if (more) fetch(kc + GKC);
 #pragma unroll
     for (int kk = 0; kk < GKC; kk += 4) {
       const int kr = kk + l / 16;
       T a_r[2], a_i[2], b_r[2], b_i[2];
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
-        a_r[q] = sAr[kr][wm * 32 + q * 16 + l % 16];
-        a_i[q] = sAi[kr][wm * 32 + q * 16 + l % 16];
-        b_r[q] = sBr[kr][wn * 32 + q * 16 + l % 16];
-        b_i[q] = sBi[kr][wn * 32 + q * 16 + l % 16];
+        a_r[q] = sAr[buf][kr][wm * 32 + q * 16 + l % 16];
+        a_i[q] = sAi[buf][kr][wm * 32 + q * 16 + l % 16];
+        b_r[q] = sBr[buf][kr][wn * 32 + q * 16 + l % 16];
+        b_i[q] = sBi[buf][kr][wn * 32 + q * 16 + l % 16];
       }
 #pragma unroll
       for (int a = 0; a < 2; ++a)
@@ -616,7 +696,9 @@ __global__ __launch_bounds__(256) void k_gemm(const int *__restrict__ list, int 
           ci[a][b] = Mfma<T>::mma(b_r[b], a_i[a], ci[a][b]);
         }
     }
+    if (more) stash(buf ^ 1);
     __syncthreads();
+    buf ^= 1;
   }
 #pragma unroll
   for (int a = 0; a < 2; ++a) {
@@ -834,8 +916,9 @@ hipError_t lu_fill(hipStream_t s, double *p, long long n, double v) {
 
 template <typename T>
 static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re, double sig_im,
-                              double tiny, double perturb, hipStream_t s) {
+                              double tiny, double perturb, hipStream_t s, FactorTiming *tm) {
   hipError_t e;
+  std::vector<hipEvent_t> events;  // start/stop pairs around GEMM launches when timing
   T *arena = reinterpret_cast<T *>(P.arena);
   // zero the planes in use (the arena is sized for FP64 fronts)
   const long long used = (P.arena_doubles * static_cast<long long>(sizeof(T)) + 7) / 8;
@@ -861,6 +944,28 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
     }
     const int *list = P.d_level_list + L.list_off;
     const unsigned tm_m = cdivu(L.mmax, GT), tm_mp = cdivu(L.mmax + L.pmax, GT);
+    auto gemm = [&](unsigned tiles, int region, int k0, int K0) -> hipError_t {
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (tm) {
+        hipError_t r;
+        if ((r = hipEventCreate(&e0)) != hipSuccess) return r;
+        if ((r = hipEventCreate(&e1)) != hipSuccess) return r;
+        events.push_back(e0);
+        events.push_back(e1);
+        if ((r = hipEventRecord(e0, s)) != hipSuccess) return r;
+        for (int q = 0; q < L.count; ++q) {
+          const GemmShape g = gemm_shape(region, P.h_front[P.h_level_list[L.list_off + q]], k0, nb, K0, NB);
+          if (g.M > 0 && g.N > 0 && g.K > 0)
+            tm->gemm_flops += 8.0 * g.K * (static_cast<double>(g.M) * g.N1 +
+                                           static_cast<double>(g.M2) * (g.N - g.N1));
+        }
+        ++tm->gemm_launches;
+      }
+      hipLaunchKernelGGL(k_gemm<T>, dim3(tiles, L.count), dim3(256), 0, s, list, L.count,
+                         P.d_front, region, k0, nb, K0, NB, arena);
+      if (tm) return hipEventRecord(e1, s);
+      return hipGetLastError();
+    };
     for (int K0 = 0; K0 < L.pmax; K0 += NB) {
       const int Kend = std::min(K0 + NB, L.pmax);
       for (int k0 = K0; k0 < Kend; k0 += nb) {
@@ -873,9 +978,7 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
         if (k0 + nb < Kend) {
           hipLaunchKernelGGL(k_trsm_lower<T>, dim3(1, L.count), dim3(256), 0, s, list,
                              L.count, P.d_front, k0, nb, K0, NB, 0, 0, P.d_ipiv, arena);
-          hipLaunchKernelGGL(k_gemm<T>, dim3(tm_m * cdivu(NB - nb, GT), L.count), dim3(256),
-                             0, s, list, L.count, P.d_front, static_cast<int>(kGemmInner), k0, nb,
-                             K0, NB, arena);
+          if ((e = gemm(tm_m * cdivu(NB - nb, GT), kGemmInner, k0, K0)) != hipSuccess) return e;
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
       }
@@ -884,13 +987,9 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
                            dim3(256), 0, s, list, L.count, P.d_front, k0, nb, K0, NB, 1,
                            k0 == K0 ? 1 : 0, P.d_ipiv, arena);
         if (k0 + nb < Kend)
-          hipLaunchKernelGGL(k_gemm<T>, dim3(cdivu(NB - nb, GT) * tm_mp, L.count),
-                             dim3(256), 0, s, list, L.count, P.d_front,
-                             static_cast<int>(kGemmBlkRows), k0, nb, K0, NB, arena);
+          if ((e = gemm(cdivu(NB - nb, GT) * tm_mp, kGemmBlkRows, k0, K0)) != hipSuccess) return e;
       }
-      hipLaunchKernelGGL(k_gemm<T>, dim3(tm_m * tm_mp, L.count), dim3(256), 0, s, list,
-                         L.count, P.d_front, static_cast<int>(kGemmTrail), K0, nb, K0, NB,
-                         arena);
+      if ((e = gemm(tm_m * tm_mp, kGemmTrail, K0, K0)) != hipSuccess) return e;
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     // Z = U11^-1 by backward Jordan steps on the pivot blocks
@@ -899,18 +998,26 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
       hipLaunchKernelGGL(k_trsm_upper<T>, dim3(cdivu(L.pmax, 256), L.count), dim3(256), 0,
                          s, list, L.count, P.d_front, k0, nb, arena);
       if (k0 > 0)
-        hipLaunchKernelGGL(k_gemm<T>, dim3(tp * tp, L.count), dim3(256), 0, s, list,
-                           L.count, P.d_front, static_cast<int>(kGemmZ), k0, nb, 0, NB, arena);
+        if ((e = gemm(tp * tp, kGemmZ, k0, 0)) != hipSuccess) return e;
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
+  }
+  if (tm) {
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    for (size_t q = 0; q + 1 < events.size(); q += 2) {
+      float ms = 0.f;
+      if ((e = hipEventElapsedTime(&ms, events[q], events[q + 1])) != hipSuccess) return e;
+      tm->gemm_ms += ms;
+    }
+    for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
   }
   return hipSuccess;
 }
 
 hipError_t lu_factor(const LuPlan &P, const LuInputs &in, double sig_re, double sig_im,
-                     double tiny, double perturb, bool fp32, hipStream_t s) {
-  return fp32 ? factor_impl<float>(P, in, sig_re, sig_im, tiny, perturb, s)
-              : factor_impl<double>(P, in, sig_re, sig_im, tiny, perturb, s);
+                     double tiny, double perturb, bool fp32, hipStream_t s, FactorTiming *tm) {
+  return fp32 ? factor_impl<float>(P, in, sig_re, sig_im, tiny, perturb, s, tm)
+              : factor_impl<double>(P, in, sig_re, sig_im, tiny, perturb, s, tm);
 }
 
 // x = K^-1 b for vectors in elimination order (split planes).  b may alias x.

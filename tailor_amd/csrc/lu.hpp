@@ -35,6 +35,8 @@ struct LuPlan {
   long long arena_doubles = 0;
   long long w_len = 0;          // per-plane length of the front workspace
   std::vector<LevelPlan> levels;
+  std::vector<FrontDev> h_front;   // host copies (flop accounting)
+  std::vector<int> h_level_list;
   FrontDev *d_front = nullptr;
   int *d_level_list = nullptr;
   int *d_child_slots = nullptr;
@@ -72,9 +74,20 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
 void lu_plan_destroy(LuPlan &P);
 
 hipError_t lu_fill(hipStream_t s, double *p, long long n, double v);
-// Fronts in complex FP64, or complex FP32 (fp32) in the same arena.
+// Per-kernel timing of one factorization (HIP events around every frontal
+// GEMM launch) and the algorithmic flops of those launches (8 M N K per
+// front and launch: a complex multiply-add is 8 real flops).
+struct FactorTiming {
+  double gemm_ms = 0.0;
+  double gemm_flops = 0.0;
+  int gemm_launches = 0;
+};
+
+// Fronts in complex FP64, or complex FP32 (fp32) in the same arena.  With
+// `timing`, every GEMM launch is bracketed by events (slightly slower).
 hipError_t lu_factor(const LuPlan &P, const LuInputs &in, double sig_re, double sig_im,
-                     double tiny, double perturb, bool fp32, hipStream_t s);
+                     double tiny, double perturb, bool fp32, hipStream_t s,
+                     FactorTiming *timing = nullptr);
 // x = K^-1 b, vectors in elimination dof order, split planes (re, im)
 hipError_t lu_solve(const LuPlan &P, const double *br, const double *bi, double *xr, double *xi,
                     bool fp32, hipStream_t s);

@@ -105,6 +105,8 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
 
   hipError_t e;
   if ((e = upload(&P.d_front, fr, s)) != hipSuccess) return e;
+  P.h_front = fr;
+  P.h_level_list = level_list;
   if ((e = upload(&P.d_level_list, level_list, s)) != hipSuccess) return e;
   if ((e = upload(&P.d_child_slots, child_slots, s)) != hipSuccess) return e;
   if ((e = upload(&P.d_parent, parent, s)) != hipSuccess) return e;

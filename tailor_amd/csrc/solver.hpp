@@ -33,7 +33,7 @@ struct SolverState {
 
 void solver_release(SolverState *S);
 int solver_prepare(tailor_ctx *ctx);
-int solver_factor(tailor_ctx *ctx, cplx sigma);
+int solver_factor(tailor_ctx *ctx, cplx sigma, FactorTiming *timing = nullptr);
 int solver_linear_solve(tailor_ctx *ctx, const double *b_nat, double *x_nat, int refine_max,
                         double refine_tol, double *relres, int *steps);
 int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solve_diag &diag);
