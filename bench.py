@@ -35,6 +35,7 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 NEV, TOL, OMEGA = 8, 1e-10, 0.02 + 0j
 NCV = 64  # Krylov subspace size (the -eps_ncv of the reference's options database)
+LEARNED_NCV = 40  # subspace of the learned form: the wanted cluster converges in one pass
 
 
 def algorithmic_assembly_bytes(ny: int, nz: int, ssx: int, sse: int) -> int:
@@ -82,6 +83,7 @@ def plain_step(ta, cs, steps: int, ncv: int) -> dict:
         pairs = ctx.eigenpairs()[:NEV]
         return sum(1 for p in pairs if p["residual"] <= TOL), d, pairs
 
+    tune = ctx.autotune(case.lambda_shift)  # frontal GEMM tile order per level
     step()
     good, t0 = 0, time.perf_counter()
     for _ in range(steps):
@@ -117,6 +119,7 @@ def plain_step(ta, cs, steps: int, ncv: int) -> dict:
                         "GBps": round(solve_bytes / (solve_ms * 1e-3) / 1e9, 2),
                         "frac_hbm": round(solve_bytes / (solve_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "gemm": prof,
+        "tune": tune,
         "mixed": {"seconds": round(mixed_s, 3), "factor_ms": round(dm["factor_ms"], 2),
                   "solves": dm["solves"], "refinement_steps": dm["refinement_steps"],
                   "refinement_steps_per_solve": round(dm["refinement_steps"] / max(1, dm["solves"]), 2),
@@ -126,7 +129,7 @@ def plain_step(ta, cs, steps: int, ncv: int) -> dict:
     }
 
 
-def sweeps(ta, steps: int, warmup: int, ncv: int) -> dict:
+def sweeps(ta, steps: int, warmup: int, ncv: int, learned_ncv: int) -> dict:
     from tailor_amd import sweep as sw
     from tailor_amd.shift_model import ShiftModel
     from tailor_amd.train_shift_model import MODEL
@@ -138,7 +141,8 @@ def sweeps(ta, steps: int, warmup: int, ncv: int) -> dict:
     cases = sw.prepare(insts, OMEGA)
     alphas = np.array([i.alpha for i in insts])
     shifts = model.shift_centres(alphas, np.full(len(insts), 1500.0), np.full(len(insts), 0.7))
-    runner = sw.Sweeper(ncv=ncv, omega_s=OMEGA)
+    runner = sw.Sweeper(ncv=learned_ncv, omega_s=OMEGA)
+    plain_runner = sw.Sweeper(ncv=ncv, omega_s=OMEGA)
     # warm-up instances (untimed): build the plan, and let the first one
     # choose the factorization precision by measurement
     warm = list(range(warmup))
@@ -155,7 +159,9 @@ def sweeps(ta, steps: int, warmup: int, ncv: int) -> dict:
     timed = list(range(warmup, warmup + steps))
     order = sorted(timed, key=lambda k: insts[k].alpha)
     learned = runner.run(cases, "learned", shifts=shifts, precision=precision, order=order)
-    plain = runner.run(cases, "plain", precision=ta.PRECISION_FP64, order=timed)
+    runner.close()
+    plain_runner.run(cases, "plain", precision=ta.PRECISION_FP64, order=warm[:1])  # warm-up
+    plain = plain_runner.run(cases, "plain", precision=ta.PRECISION_FP64, order=timed)
     # eigenvalues found by both forms must agree
     lam_diff = 0.0
     by_inst = {id(r.instance): r for r in plain.records}
@@ -168,10 +174,11 @@ def sweeps(ta, steps: int, warmup: int, ncv: int) -> dict:
     # config 3: both grid sizes, per-instance base flows, mixed precision
     c3 = sw.sweep_instances(4, seed=1, sizes=[(100, 50), (200, 100)])
     c3_cases = sw.prepare(c3, OMEGA)
-    mixed_sweep = runner.run(c3_cases, "mixed")
-    runner.close()
+    mixed_sweep = plain_runner.run(c3_cases, "mixed")
+    plain_runner.close()
     return {"learned": learned, "plain": plain, "choice": choice, "common_pair_rel_diff": lam_diff,
-            "mixed_sweep": mixed_sweep, "shifts": shifts}
+            "mixed_sweep": mixed_sweep, "shifts": shifts,
+            "autotune": {f"{k[0]}x{k[1]}": v for k, v in runner.tuned.items()}}
 
 
 def main() -> int:
@@ -179,7 +186,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=32, help="timed sweep instances")
     ap.add_argument("--warmup", type=int, default=1, help="untimed warm-up instances")
-    ap.add_argument("--ncv", type=int, default=NCV)
+    ap.add_argument("--ncv", type=int, default=NCV, help="Krylov subspace of the plain form")
+    ap.add_argument("--learned-ncv", type=int, default=LEARNED_NCV)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -194,7 +202,7 @@ def main() -> int:
     from tailor_amd import case as cs
 
     single = plain_step(ta, cs, 2, args.ncv)
-    sw = sweeps(ta, args.steps, max(1, args.warmup), args.ncv)
+    sw = sweeps(ta, args.steps, max(1, args.warmup), args.ncv, args.learned_ncv)
     L, P = sw["learned"], sw["plain"]
     g = single["gemm"]
     gemm_tf = g["gemm_flops"] / (g["gemm_ms"] * 1e-3) / 1e12
@@ -215,8 +223,9 @@ def main() -> int:
                                "100,000 unknowns, Re=1500, Ma=0.7, alpha ~ U[0.5,3] seed 1, nev=8, "
                                "tol=1e-10; learned shift centres, warm-started Krylov-Schur, "
                                "fixed-shift fallback",
-                   "nev": NEV, "ncv": args.ncv, "instances": args.steps,
-                   "precision_choice": sw["choice"]},
+                   "nev": NEV, "ncv_learned": args.learned_ncv, "ncv_plain": args.ncv,
+                   "instances": args.steps,
+                   "precision_choice": sw["choice"], "gemm_autotune": sw["autotune"]},
         "learned_sweep": L.totals(),
         "plain_sweep": P.totals(),
         "speedup_vs_plain_sweep": round(L.eigenpairs_per_s / max(P.eigenpairs_per_s, 1e-300), 3),
@@ -227,6 +236,7 @@ def main() -> int:
                      "gemm_ms_per_factorization": round(g["gemm_ms"], 2),
                      "gemm_flops_per_factorization": g["gemm_flops"],
                      "gemm_launches": g["gemm_launches"],
+                     "tile_order_per_level": single["tune"]["order"],
                      "factor_ms": round(g["factor_ms"], 2)},
         "plain_instance": single["plain"],
         "assembly": single["assembly"],

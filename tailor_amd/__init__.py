@@ -90,6 +90,11 @@ class SolveOpts(C.Structure):
                 ("warm_start", C.c_int32)]
 
 
+class TuneReport(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("order", C.c_int32 * 64), ("gemm_ms", C.c_double * 4),
+                ("tuned_gemm_ms", C.c_double)]
+
+
 class SolveDiag(C.Structure):
     _fields_ = [("requested", C.c_int32), ("converged", C.c_int32), ("iterations", C.c_int32),
                 ("solves", C.c_int32), ("refinement_steps", C.c_int32),
@@ -136,6 +141,7 @@ def library() -> C.CDLL:
                                         C.c_double, P(C.c_double), P(C.c_int32)]
     lib.tailor_solve.argtypes = [C.c_void_p, P(SolveOpts), P(SolveDiag)]
     lib.tailor_set_precision.argtypes = [C.c_void_p, C.c_int32]
+    lib.tailor_autotune.argtypes = [C.c_void_p, C.c_double, C.c_double, P(TuneReport)]
     lib.tailor_profile_factor.argtypes = [C.c_void_p, C.c_double, C.c_double, P(C.c_double),
                                           P(C.c_double), P(C.c_double), P(C.c_int32)]
     lib.tailor_time_linear_solve.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double)]
@@ -275,6 +281,15 @@ def _profile_factor(self, sigma: complex) -> dict:
             "gemm_launches": n.value}
 
 
+def _autotune(self, sigma: complex) -> dict:
+    """Choose the frontal GEMM tile order per elimination level by timing."""
+    sigma = complex(sigma)
+    r = TuneReport()
+    self._check(self.lib.tailor_autotune(self.ptr, sigma.real, sigma.imag, C.byref(r)))
+    return {"levels": r.levels, "order": list(r.order)[:r.levels],
+            "gemm_ms": list(r.gemm_ms)[:2], "tuned_gemm_ms": r.tuned_gemm_ms}
+
+
 def _factor(self, sigma: complex) -> dict:
     d = FactorDiag()
     sigma = complex(sigma)
@@ -338,6 +353,7 @@ Context.time_linear_solve = _time_linear_solve
 Context.factor = _factor
 Context.set_precision = _set_precision
 Context.profile_factor = _profile_factor
+Context.autotune = _autotune
 Context.linear_solve = _linear_solve
 Context.solve = _solve
 Context.eigenpairs = _eigenpairs

@@ -713,137 +713,65 @@ __global__ __launch_bounds__(64) void k_characteristics(CaseDims d, const double
 // ---------------------------------------------------------------------------
 // The operator writer.
 
-template <int SSX, int SSE>
-struct NodeRec {
-  double phys[11 * 25];
-  dcx K0[25], Kxi[25], Keta[25];
-  double Vxx[25], Vxe[25], Vee[25];  // Vxixi, Vxieta, Vetaeta (real)
-  double w1x[SSX], mw2x[SSX], w1e[SSE], mw2e[SSE];
-  dcx P[25], Ri[25];
-  int slot_of_rank[SSE];
-  int incoming[5];
-  int t0, s0, kind, status;
-};
+// Per-node coefficient record written by k_coefficients and read by the
+// writer (doubles): K0, Kxi, Keta (25 complex each), Vxixi, Vxieta, Vetaeta
+// (25 real each) and Gamma (25 real).
+constexpr int kCoefDoubles = 250;
+constexpr int kCoefNodes = 8;  // nodes per k_coefficients workgroup
 
-// Value of entry (r, c) of the block in sorted position (tr, s) of -L or L
-// (GlobalOperator.cpp:323-365, accumulation order K0, xi, eta, mixed).
-template <int SSX, int SSE>
-__device__ inline dcx l_entry(const NodeRec<SSX, SSE> &R, int tr, int s, int e) {
-  const int t = R.slot_of_rank[tr];
-  double vr = 0.0, vi = 0.0;
-  if (t == R.t0) {
-    if (s == R.s0) {
-      vr = vr + R.K0[e].re;
-      vi = vi + R.K0[e].im;
-    }
-    const double w = R.w1x[s];
-    double ar = R.Kxi[e].re * w, ai = R.Kxi[e].im * w;
-    ar = ar + R.Vxx[e] * R.mw2x[s];
-    vr = vr + ar;
-    vi = vi + ai;
-  }
-  if (s == R.s0) {
-    const double w = R.w1e[t];
-    double ar = R.Keta[e].re * w, ai = R.Keta[e].im * w;
-    ar = ar + R.Vee[e] * R.mw2e[t];
-    vr = vr + ar;
-    vi = vi + ai;
-  }
-  const double scale = -R.w1e[t] * R.w1x[s];
-  vr = vr + R.Vxe[e] * scale;
-  return dcx{vr, vi};
-}
-
-template <int SSX, int SSE, int NPW>
-__global__ __launch_bounds__(kThreads) void k_operator(
+// Physical LNS blocks (LNSCoefficients.cpp:173-404), their finiteness, and
+// the Fourier (StreamwiseFourier.cpp:24-41) and curvilinear
+// (CurvilinearTransform.cpp:68-96) combinations of every node; status 0 on
+// success, nonzero for the reference's NonFinite/NonPositive outcomes.
+__global__ __launch_bounds__(kThreads) void k_coefficients(
     CaseDims d, const double *__restrict__ recipe, double alpha_re, double alpha_im,
-    const int *__restrict__ xoff, const double *__restrict__ xw, const int *__restrict__ eoff,
-    const double *__restrict__ ew, const double *__restrict__ base,
-    const double *__restrict__ deriv, const double *__restrict__ metrics,
-    const double *__restrict__ bcdata, int constrained, double *__restrict__ vals,
-    double *__restrict__ mvals, DeviceDiag *diag) {
-  constexpr int PER = SSX * SSE;
-  __shared__ NodeRec<SSX, SSE> rec[NPW];
-  __shared__ int bad;
+    const double *__restrict__ base, const double *__restrict__ deriv,
+    const double *__restrict__ metrics, double *__restrict__ coef, int *__restrict__ status) {
+  __shared__ double phys[kCoefNodes][11 * 25];
+  __shared__ int st[kCoefNodes];
   const int tid = threadIdx.x;
-  const long long first = static_cast<long long>(blockIdx.x) * NPW;
-  if (tid == 0) bad = 0;
-
-  // phase 1a: node metadata and the physical blocks (six threads per node)
-  if (tid < 6 * NPW) {
+  const long long first = static_cast<long long>(blockIdx.x) * kCoefNodes;
+  // six threads per node evaluate the physical blocks
+  if (tid < 6 * kCoefNodes) {
     const int n = tid / 6, part = tid - 6 * (tid / 6);
     const long long node = first + n;
-    NodeRec<SSX, SSE> &R = rec[n];
     if (node < d.nodes) {
-      const int i = static_cast<int>(node % d.ny), j = static_cast<int>(node / d.ny);
       Point p;
-      const int st = load_point(base, deriv, node, p);
+      const int s0 = load_point(base, deriv, node, p);
       if (part == 5) {
-        R.status = st;
-        R.kind = i == 0 ? 1 : (i == d.ny - 1 ? 2 : 0);
-        const double *w1x = xw + static_cast<size_t>(d.ny) * SSX + static_cast<size_t>(i) * SSX;
-        const double *w2x = w1x + static_cast<size_t>(d.ny) * SSX;
-        const double *w1e = ew + static_cast<size_t>(d.nz) * SSE + static_cast<size_t>(j) * SSE;
-        const double *w2e = w1e + static_cast<size_t>(d.nz) * SSE;
-        for (int s = 0; s < SSX; ++s) {
-          R.w1x[s] = w1x[s];
-          R.mw2x[s] = -w2x[s];
-          if (xoff[i * SSX + s] == 0) R.s0 = s;
-        }
-        for (int t = 0; t < SSE; ++t) {
-          R.w1e[t] = w1e[t];
-          R.mw2e[t] = -w2e[t];
-          const int off = eoff[j * SSE + t];
-          if (off == 0) R.t0 = t;
-          const int ct = wrap_mod(j + off, d.nz);
-          int rank = 0;
-          for (int u = 0; u < SSE; ++u) rank += wrap_mod(j + eoff[j * SSE + u], d.nz) < ct;
-          R.slot_of_rank[rank] = t;
-        }
-        if (R.kind == 2 && constrained) {
-          const double *bc = bcdata + 106 * static_cast<size_t>(j);
-          for (int k = 0; k < 25; ++k) {
-            R.P[k] = dcx{bc[2 * k], bc[2 * k + 1]};
-            R.Ri[k] = dcx{bc[50 + 2 * k], bc[50 + 2 * k + 1]};
-          }
-          for (int k = 0; k < 5; ++k) R.incoming[k] = bc[100 + k] != 0.0;
-        }
-      } else if (st == 0) {
+        st[n] = s0;
+      } else if (s0 == 0) {
         const Transport q = transport(recipe, p.T);
-        physical_part(part, p, q, R.phys);
+        physical_part(part, p, q, phys[n]);
       }
     }
   }
   __syncthreads();
-  if (tid < 5 * NPW) {
+  if (tid < 5 * kCoefNodes) {
     // finiteness of the physical blocks (NonFiniteOutput of evaluate)
     const int n = tid / 5, part = tid - 5 * (tid / 5);
-    NodeRec<SSX, SSE> &R = rec[n];
-    if (first + n < d.nodes && R.status == 0) {
+    if (first + n < d.nodes && st[n] == 0) {
       const int b0 = part == 0 ? 0 : part, b1 = part == 0 ? 1 : part + 1;
       bool ok = true;
       for (int b = b0; b < b1; ++b)
-        for (int k = 0; k < 25; ++k) ok = ok && finite(R.phys[b * 25 + k]);
+        for (int k = 0; k < 25; ++k) ok = ok && finite(phys[n][b * 25 + k]);
       if (part == 0)
         for (int b = 5; b < 11; ++b)
-          for (int k = 0; k < 25; ++k) ok = ok && finite(R.phys[b * 25 + k]);
-      if (!ok) atomicMax(&R.status, 4);
+          for (int k = 0; k < 25; ++k) ok = ok && finite(phys[n][b * 25 + k]);
+      if (!ok) atomicMax(&st[n], 4);
     }
   }
   __syncthreads();
-
-  // phase 1b: Fourier (StreamwiseFourier.cpp:24-41) and curvilinear
-  // (CurvilinearTransform.cpp:68-96) entries, 25 threads per node
-  if (tid < 25 * NPW) {
+  // 25 threads per node: Fourier and curvilinear entries
+  if (tid < 25 * kCoefNodes) {
     const int n = tid / 25, e = tid - 25 * (tid / 25);
     const long long node = first + n;
-    NodeRec<SSX, SSE> &R = rec[n];
-    if (node < d.nodes && R.status == 0) {
+    if (node < d.nodes && st[n] == 0) {
       const double ia_re = 0.0 * alpha_re - 1.0 * alpha_im;
       const double ia_im = 0.0 * alpha_im + 1.0 * alpha_re;
       const double a2_re = alpha_re * alpha_re - alpha_im * alpha_im;
       const double a2_im = alpha_re * alpha_im + alpha_im * alpha_re;
-      const double *ph = R.phys;
+      const double *ph = phys[n];
       const double A = ph[25 + e], B = ph[50 + e], C = ph[75 + e], D = ph[100 + e];
       const double Vxx = ph[125 + e], Vxy = ph[150 + e], Vxz = ph[175 + e];
       const double Vyy = ph[200 + e], Vyz = ph[225 + e], Vzz = ph[250 + e];
@@ -867,25 +795,122 @@ __global__ __launch_bounds__(kThreads) void k_operator(
       const double Veta = etaY * etaY * Vyy + etaZ * etaZ * Vzz + etaY * etaZ * Vyz;
       ok = ok && finite(Kxr) && finite(Kxi) && finite(Ker) && finite(Kei) && finite(Vxixi) &&
            finite(Vxieta) && finite(Veta);
-      R.K0[e] = dcx{K0r, K0i};
-      R.Kxi[e] = dcx{Kxr, Kxi};
-      R.Keta[e] = dcx{Ker, Kei};
-      R.Vxx[e] = Vxixi;
-      R.Vxe[e] = Vxieta;
-      R.Vee[e] = Veta;
-      if (!ok) atomicMax(&R.status, 4);
+      double *c = coef + kCoefDoubles * node;
+      c[2 * e] = K0r;
+      c[2 * e + 1] = K0i;
+      c[50 + 2 * e] = Kxr;
+      c[50 + 2 * e + 1] = Kxi;
+      c[100 + 2 * e] = Ker;
+      c[100 + 2 * e + 1] = Kei;
+      c[150 + e] = Vxixi;
+      c[175 + e] = Vxieta;
+      c[200 + e] = Veta;
+      c[225 + e] = ph[e];  // Gamma
+      if (!ok) atomicMax(&st[n], 4);
     }
   }
   __syncthreads();
-  if (tid < NPW && first + tid < d.nodes && rec[tid].status != 0) atomicOr(&bad, 1);
+  if (tid < kCoefNodes && first + tid < d.nodes) status[first + tid] = st[tid];
+}
+
+template <int SSX, int SSE>
+struct NodeRec {
+  double gamma[25];
+  dcx K0[25], Kxi[25], Keta[25];
+  double Vxx[25], Vxe[25], Vee[25];  // Vxixi, Vxieta, Vetaeta (real)
+  double w1x[SSX], mw2x[SSX], w1e[SSE], mw2e[SSE];
+  dcx P[25], Ri[25];
+  int slot_of_rank[SSE];
+  int incoming[5];
+  int t0, s0, kind, status;
+};
+
+// The operator writer: workgroup = NPW consecutive block rows, one wave per
+// row.  The node's coefficient record is staged in LDS; the row is then
+// streamed out with each lane owning one block entry.
+template <int SSX, int SSE, int NPW>
+__global__ __launch_bounds__(kThreads) void k_operator(
+    CaseDims d, const int *__restrict__ xoff, const double *__restrict__ xw,
+    const int *__restrict__ eoff, const double *__restrict__ ew, const double *__restrict__ coef,
+    const int *__restrict__ status, const double *__restrict__ bcdata, int constrained,
+    double *__restrict__ vals, double *__restrict__ mvals, DeviceDiag *diag) {
+  constexpr int PER = SSX * SSE;
+  __shared__ NodeRec<SSX, SSE> rec[NPW];
+  __shared__ int bad;
+  const int tid = threadIdx.x;
+  const long long first = static_cast<long long>(blockIdx.x) * NPW;
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  // node metadata (one thread per node)
+  if (tid < NPW) {
+    const long long node = first + tid;
+    NodeRec<SSX, SSE> &R = rec[tid];
+    if (node < d.nodes) {
+      const int i = static_cast<int>(node % d.ny), j = static_cast<int>(node / d.ny);
+      R.status = status[node];
+      if (R.status != 0) atomicOr(&bad, 1);
+      R.kind = i == 0 ? 1 : (i == d.ny - 1 ? 2 : 0);
+      const double *w1x = xw + static_cast<size_t>(d.ny) * SSX + static_cast<size_t>(i) * SSX;
+      const double *w2x = w1x + static_cast<size_t>(d.ny) * SSX;
+      const double *w1e = ew + static_cast<size_t>(d.nz) * SSE + static_cast<size_t>(j) * SSE;
+      const double *w2e = w1e + static_cast<size_t>(d.nz) * SSE;
+      for (int s = 0; s < SSX; ++s) {
+        R.w1x[s] = w1x[s];
+        R.mw2x[s] = -w2x[s];
+        if (xoff[i * SSX + s] == 0) R.s0 = s;
+      }
+      for (int t = 0; t < SSE; ++t) {
+        R.w1e[t] = w1e[t];
+        R.mw2e[t] = -w2e[t];
+        const int off = eoff[j * SSE + t];
+        if (off == 0) R.t0 = t;
+        const int ct = wrap_mod(j + off, d.nz);
+        int rank = 0;
+        for (int u = 0; u < SSE; ++u) rank += wrap_mod(j + eoff[j * SSE + u], d.nz) < ct;
+        R.slot_of_rank[rank] = t;
+      }
+      if (R.kind == 2 && constrained) {
+        const double *bc = bcdata + 106 * static_cast<size_t>(j);
+        for (int k = 0; k < 25; ++k) {
+          R.P[k] = dcx{bc[2 * k], bc[2 * k + 1]};
+          R.Ri[k] = dcx{bc[50 + 2 * k], bc[50 + 2 * k + 1]};
+        }
+        for (int k = 0; k < 5; ++k) R.incoming[k] = bc[100 + k] != 0.0;
+      }
+    }
+  }
+  // coefficient records: NPW consecutive nodes are contiguous in `coef`
+  {
+    const long long nn = min(static_cast<long long>(NPW), d.nodes - first);
+    const double *src = coef + kCoefDoubles * first;
+    for (int k = tid; k < nn * kCoefDoubles; k += kThreads) {
+      const int n = k / kCoefDoubles, q = k - kCoefDoubles * n;
+      NodeRec<SSX, SSE> &R = rec[n];
+      const double v = src[k];
+      if (q < 150) {
+        dcx *dst = q < 50 ? R.K0 : (q < 100 ? R.Kxi : R.Keta);
+        const int qq = q - 50 * (q / 50);
+        if (qq & 1) dst[qq >> 1].im = v;
+        else dst[qq >> 1].re = v;
+      } else if (q < 175) {
+        R.Vxx[q - 150] = v;
+      } else if (q < 200) {
+        R.Vxe[q - 175] = v;
+      } else if (q < 225) {
+        R.Vee[q - 200] = v;
+      } else {
+        R.gamma[q - 225] = v;
+      }
+    }
+  }
   __syncthreads();
   if (bad) {
     if (tid == 0) atomicOr(&diag->flags, kCoefficient);
     return;
   }
 
-  // phase 2: wave w streams block row `first + w` out.  Lanes 0..49 each own
-  // one entry e = (r, c) of a 5x5 block and walk the row two blocks at a time
+  // wave w streams block row `first + w` out.  Lanes 0..49 each own one
+  // entry e = (r, c) of a 5x5 block and walk the row two blocks at a time
   // (lanes 0..24 the even blocks, 25..49 the odd ones), so each store
   // instruction writes 50 x 16 B contiguous bytes and the entry's coefficient
   // values stay in registers for the whole row.
@@ -974,7 +999,7 @@ __global__ __launch_bounds__(kThreads) void k_operator(
     if (node < d.nodes) {
       const NodeRec<SSX, SSE> &R = rec[n];
       const int r = e / 5, c = e - 5 * (e / 5);
-      dcx val{R.phys[e], 0.0};
+      dcx val{R.gamma[e], 0.0};
       if (constrained) {
         if (R.kind == 1 && r >= 1) {
           val = dcx{0.0, 0.0};
@@ -986,7 +1011,7 @@ __global__ __launch_bounds__(kThreads) void k_operator(
             for (int k = 0; k < 5; ++k) {
               const dcx pk = R.P[r * 5 + k];
               if (pk.re == 0.0 && pk.im == 0.0) continue;
-              const double br = R.phys[k * 5 + c], bi = 0.0;
+              const double br = R.gamma[k * 5 + c], bi = 0.0;
               sr += pk.re * br - pk.im * bi;
               si += pk.re * bi + pk.im * br;
             }
@@ -1001,15 +1026,14 @@ __global__ __launch_bounds__(kThreads) void k_operator(
 
 // Generic-size variant dispatch table.
 template <int SSX, int SSE>
-hipError_t launch_op(tailor_ctx *ctx, bool constrained, const double *d_recipe) {
+hipError_t launch_op(tailor_ctx *ctx, bool constrained) {
   constexpr int NPW = 4;
   const CaseDims &d = ctx->dims;
   const unsigned grid = static_cast<unsigned>((d.nodes + NPW - 1) / NPW);
   hipLaunchKernelGGL((k_operator<SSX, SSE, NPW>), dim3(grid), dim3(kThreads), 0, ctx->stream, d,
-                     d_recipe, ctx->alpha_re, ctx->alpha_im, ctx->d_xoff, ctx->d_xw, ctx->d_eoff,
-                     ctx->d_ew, ctx->d_base, ctx->d_deriv, ctx->d_metrics,
-                     ctx->d_bcdata, constrained ? 1 : 0, ctx->d_vals, ctx->d_mvals,
-                     ctx->d_diag);
+                     ctx->d_xoff, ctx->d_xw, ctx->d_eoff, ctx->d_ew, ctx->d_coef,
+                     ctx->d_coef_status, ctx->d_bcdata, constrained ? 1 : 0, ctx->d_vals,
+                     ctx->d_mvals, ctx->d_diag);
   return hipGetLastError();
 }
 
@@ -1071,16 +1095,24 @@ hipError_t launch_characteristics(tailor_ctx *ctx) {
   return hipGetLastError();
 }
 
+hipError_t launch_coefficients(tailor_ctx *ctx) {
+  const CaseDims &d = ctx->dims;
+  const unsigned grid = static_cast<unsigned>((d.nodes + kCoefNodes - 1) / kCoefNodes);
+  hipLaunchKernelGGL(k_coefficients, dim3(grid), dim3(kThreads), 0, ctx->stream, d, ctx->d_recipe,
+                     ctx->alpha_re, ctx->alpha_im, ctx->d_base, ctx->d_deriv, ctx->d_metrics,
+                     ctx->d_coef, ctx->d_coef_status);
+  return hipGetLastError();
+}
+
 hipError_t launch_operator(tailor_ctx *ctx, bool constrained) {
   const CaseDims &d = ctx->dims;
-  const double *d_recipe = ctx->d_recipe;
-  if (d.ssx == 11 && d.sse == 7) return launch_op<11, 7>(ctx, constrained, d_recipe);
-  if (d.ssx == 5 && d.sse == 3) return launch_op<5, 3>(ctx, constrained, d_recipe);
-  if (d.ssx == 5 && d.sse == 5) return launch_op<5, 5>(ctx, constrained, d_recipe);
-  if (d.ssx == 7 && d.sse == 5) return launch_op<7, 5>(ctx, constrained, d_recipe);
-  if (d.ssx == 9 && d.sse == 7) return launch_op<9, 7>(ctx, constrained, d_recipe);
-  if (d.ssx == 11 && d.sse == 5) return launch_op<11, 5>(ctx, constrained, d_recipe);
-  if (d.ssx == 7 && d.sse == 7) return launch_op<7, 7>(ctx, constrained, d_recipe);
+  if (d.ssx == 11 && d.sse == 7) return launch_op<11, 7>(ctx, constrained);
+  if (d.ssx == 5 && d.sse == 3) return launch_op<5, 3>(ctx, constrained);
+  if (d.ssx == 5 && d.sse == 5) return launch_op<5, 5>(ctx, constrained);
+  if (d.ssx == 7 && d.sse == 5) return launch_op<7, 5>(ctx, constrained);
+  if (d.ssx == 9 && d.sse == 7) return launch_op<9, 7>(ctx, constrained);
+  if (d.ssx == 11 && d.sse == 5) return launch_op<11, 5>(ctx, constrained);
+  if (d.ssx == 7 && d.sse == 7) return launch_op<7, 7>(ctx, constrained);
   return hipErrorInvalidValue;
 }
 

@@ -64,6 +64,8 @@ void free_case(tailor_ctx *ctx) {
   dfree(ctx->d_vals);
   dfree(ctx->d_mvals);
   dfree(ctx->d_bcdata);
+  dfree(ctx->d_coef);
+  dfree(ctx->d_coef_status);
   ctx->loaded = false;
   ctx->assembled = false;
   ctx->vals_kind = ctx->mvals_kind = -1;
@@ -316,6 +318,8 @@ int tailor_load_case(tailor_ctx *ctx, const tailor_case *c) {
     HIP_TRY(ctx, dalloc(&ctx->d_vals, static_cast<size_t>(d.nnzb) * 50));
     HIP_TRY(ctx, dalloc(&ctx->d_mvals, static_cast<size_t>(d.nodes) * 50));
     HIP_TRY(ctx, dalloc(&ctx->d_bcdata, static_cast<size_t>(nz) * 106));
+    HIP_TRY(ctx, dalloc(&ctx->d_coef, static_cast<size_t>(d.nodes) * 250));
+    HIP_TRY(ctx, dalloc(&ctx->d_coef_status, static_cast<size_t>(d.nodes)));
   } else {
     ctx->assembled = false;
     ctx->vals_kind = ctx->mvals_kind = -1;
@@ -413,6 +417,7 @@ static int run_assembly(tailor_ctx *ctx, bool constrained, tailor::DeviceDiag *h
   HIP_TRY(ctx, tailor::launch_metrics(ctx));
   HIP_TRY(ctx, tailor::launch_derivatives(ctx));
   if (constrained) HIP_TRY(ctx, tailor::launch_characteristics(ctx));
+  HIP_TRY(ctx, tailor::launch_coefficients(ctx));
   HIP_TRY(ctx, tailor::launch_operator(ctx, constrained));
   tailor::DeviceDiag g{};
   HIP_TRY(ctx, hipMemcpyAsync(&g, ctx->d_diag, sizeof g, hipMemcpyDeviceToHost, ctx->stream));
@@ -531,20 +536,23 @@ int tailor_time_assembly(tailor_ctx *ctx, int reps, double *write_kernel_ms,
   HIP_TRY(ctx, hipEventCreate(&e0));
   HIP_TRY(ctx, hipEventCreate(&e1));
   float ms = 0.0f;
-  // full assembly: diag reset, metrics, derivatives, characteristics, operator
+  // full assembly: diag reset, metrics, derivatives, characteristics,
+  // coefficient records, operator writer
   HIP_TRY(ctx, hipEventRecord(e0, ctx->stream));
   for (int r = 0; r < reps; ++r) {
     HIP_TRY(ctx, tailor::launch_reset_diag(ctx));
     HIP_TRY(ctx, tailor::launch_metrics(ctx));
     HIP_TRY(ctx, tailor::launch_derivatives(ctx));
     HIP_TRY(ctx, tailor::launch_characteristics(ctx));
+    HIP_TRY(ctx, tailor::launch_coefficients(ctx));
     HIP_TRY(ctx, tailor::launch_operator(ctx, true));
   }
   HIP_TRY(ctx, hipEventRecord(e1, ctx->stream));
   HIP_TRY(ctx, hipEventSynchronize(e1));
   HIP_TRY(ctx, hipEventElapsedTime(&ms, e0, e1));
   if (full_assembly_ms) *full_assembly_ms = static_cast<double>(ms) / reps;
-  // operator writer alone (k_operator; its inputs are resident from above)
+  // operator writer alone (k_operator; its coefficient records are resident
+  // from above)
   HIP_TRY(ctx, hipEventRecord(e0, ctx->stream));
   for (int r = 0; r < reps; ++r) HIP_TRY(ctx, tailor::launch_operator(ctx, true));
   HIP_TRY(ctx, hipEventRecord(e1, ctx->stream));
@@ -595,6 +603,17 @@ int tailor_profile_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, dou
   if (gemm_ms) *gemm_ms = tm.gemm_ms;
   if (gemm_flops) *gemm_flops = tm.gemm_flops;
   if (gemm_launches) *gemm_launches = tm.gemm_launches;
+  return TAILOR_OK;
+}
+
+int tailor_autotune(tailor_ctx *ctx, double sigma_re, double sigma_im, tailor_tune_report *report) {
+  if (!ctx) return TAILOR_ERR_USER_INPUT;
+  if (!std::isfinite(sigma_re) || !std::isfinite(sigma_im))
+    return fail(ctx, TAILOR_ERR_USER_INPUT, "shift must be finite");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  tailor_tune_report local{};
+  if (int st = tailor::solver_autotune(ctx, tailor::cplx(sigma_re, sigma_im), &local)) return st;
+  if (report) *report = local;
   return TAILOR_OK;
 }
 

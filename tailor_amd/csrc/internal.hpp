@@ -79,6 +79,8 @@ struct tailor_ctx {
   int *d_colidx = nullptr;
   double *d_vals = nullptr;   // nnzb*50
   double *d_mvals = nullptr;  // nodes*50
+  double *d_coef = nullptr;   // nodes*250: per-node coefficient records (assembly)
+  int *d_coef_status = nullptr;  // nodes
   tailor::DeviceDiag *d_diag = nullptr;
   double *d_bcdata = nullptr;    // nz * 106: P, R^-1, incoming mask, status
   double *d_recipe = nullptr;    // Re, Ma, Pr, gamma, Tref, S
@@ -106,6 +108,7 @@ hipError_t launch_reset_diag(tailor_ctx *ctx);
 hipError_t launch_metrics(tailor_ctx *ctx);
 hipError_t launch_derivatives(tailor_ctx *ctx);
 hipError_t launch_characteristics(tailor_ctx *ctx);
+hipError_t launch_coefficients(tailor_ctx *ctx);  // per-node LNS coefficient records
 hipError_t launch_operator(tailor_ctx *ctx, bool constrained);
 hipError_t launch_sumsq(tailor_ctx *ctx, const double *data, long long rows, int ncomp,
                         long long stride, double *host_out);

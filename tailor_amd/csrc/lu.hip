@@ -589,21 +589,49 @@ __device__ inline bool gemm_view(int region, const FrontDev &d, int k0, int nb, 
 }
 
 constexpr int GT = 64;   // output tile (complex) per workgroup
+constexpr int kXcds = 8;       // accelerator complex dies of an MI355X
+constexpr int kGemmGroup = 8;  // row blocks per tile group
 constexpr int GKC = 16;  // K chunk staged in LDS
 constexpr int GLD = 80;  // LDS row stride: 64 + 16 keeps the k rows of a wave's reads in different banks
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemm(const int *__restrict__ list, int nlist,
                                               const FrontDev *__restrict__ fr, int region, int k0,
-                                              int nb, int K0, int NB, T *__restrict__ arena) {
+                                              int nb, int K0, int NB, int order,
+                                              T *__restrict__ arena) {
   const int f = list[blockIdx.y];
   const FrontDev d = fr[f];
   GemmView<T> v;
   if (!gemm_view(region, d, k0, nb, K0, NB, arena, v)) return;
   const int tiles_m = (v.M + GT - 1) / GT, tiles_n = (v.N + GT - 1) / GT;
-  const int tile = blockIdx.x;
-  if (tile >= tiles_m * tiles_n) return;
-  const int row0 = (tile % tiles_m) * GT, col0 = (tile / tiles_m) * GT;
+  int row0, col0;
+  if (order == kOrderLinear) {
+    // tile = blockIdx.x, row blocks fastest
+    const int tile = blockIdx.x;
+    if (tile >= tiles_m * tiles_n) return;
+    row0 = (tile % tiles_m) * GT;
+    col0 = (tile / tiles_m) * GT;
+  } else {
+  // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs,
+  // so XCD x (= blockIdx.x % 8, gridDim.x being a multiple of 8) receives a
+  // contiguous eighth of this front's tiles (rotated by front); within it,
+  // tiles go kGemmGroup row blocks at a time so that co-resident workgroups
+  // share A and B tiles in that XCD's L2.
+  const int ntiles = tiles_m * tiles_n;
+  const int per = (ntiles + kXcds - 1) / kXcds;
+  const int q = static_cast<int>(blockIdx.x / kXcds);
+  if (q >= per) return;
+  const int slot = (static_cast<int>(blockIdx.x % kXcds) - static_cast<int>(blockIdx.y % kXcds) +
+                    kXcds) % kXcds;
+  const int tile = slot * per + q;
+  if (tile >= ntiles) return;
+  const int in_group = kGemmGroup * tiles_n;
+  const int first_m = (tile / in_group) * kGemmGroup;
+  const int group_m = min(tiles_m - first_m, kGemmGroup);
+  const int tm = first_m + (tile % in_group) % group_m, tn = (tile % in_group) / group_m;
+  row0 = tm * GT;
+  col0 = tn * GT;
+  }
   // two LDS buffers: the next K chunk is stored while the current one feeds
   // the MFMAs, so each chunk needs one barrier
   __shared__ T sAr[2][GKC][GLD], sAi[2][GKC][GLD], sBr[2][GKC][GLD], sBi[2][GKC][GLD];
@@ -919,6 +947,7 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
                               double tiny, double perturb, hipStream_t s, FactorTiming *tm) {
   hipError_t e;
   std::vector<hipEvent_t> events;  // start/stop pairs around GEMM launches when timing
+  std::vector<int> event_level;    // level of each pair
   T *arena = reinterpret_cast<T *>(P.arena);
   // zero the planes in use (the arena is sized for FP64 fronts)
   const long long used = (P.arena_doubles * static_cast<long long>(sizeof(T)) + 7) / 8;
@@ -944,6 +973,7 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
     }
     const int *list = P.d_level_list + L.list_off;
     const unsigned tm_m = cdivu(L.mmax, GT), tm_mp = cdivu(L.mmax + L.pmax, GT);
+    const int order = lev < P.gemm_order.size() ? P.gemm_order[lev] : kOrderLinear;
     auto gemm = [&](unsigned tiles, int region, int k0, int K0) -> hipError_t {
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (tm) {
@@ -952,6 +982,7 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
         if ((r = hipEventCreate(&e1)) != hipSuccess) return r;
         events.push_back(e0);
         events.push_back(e1);
+        event_level.push_back(static_cast<int>(lev));
         if ((r = hipEventRecord(e0, s)) != hipSuccess) return r;
         for (int q = 0; q < L.count; ++q) {
           const GemmShape g = gemm_shape(region, P.h_front[P.h_level_list[L.list_off + q]], k0, nb, K0, NB);
@@ -961,8 +992,9 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
         }
         ++tm->gemm_launches;
       }
+      if (order != kOrderLinear) tiles = (tiles + kXcds - 1) / kXcds * kXcds;  // XCD order: multiple of 8
       hipLaunchKernelGGL(k_gemm<T>, dim3(tiles, L.count), dim3(256), 0, s, list, L.count,
-                         P.d_front, region, k0, nb, K0, NB, arena);
+                         P.d_front, region, k0, nb, K0, NB, order, arena);
       if (tm) return hipEventRecord(e1, s);
       return hipGetLastError();
     };
@@ -1004,10 +1036,12 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
   }
   if (tm) {
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    tm->level_gemm_ms.assign(P.levels.size(), 0.0);
     for (size_t q = 0; q + 1 < events.size(); q += 2) {
       float ms = 0.f;
       if ((e = hipEventElapsedTime(&ms, events[q], events[q + 1])) != hipSuccess) return e;
       tm->gemm_ms += ms;
+      tm->level_gemm_ms[event_level[q / 2]] += ms;
     }
     for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
   }

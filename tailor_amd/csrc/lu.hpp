@@ -35,6 +35,7 @@ struct LuPlan {
   long long arena_doubles = 0;
   long long w_len = 0;          // per-plane length of the front workspace
   std::vector<LevelPlan> levels;
+  std::vector<int> gemm_order;     // per level (kOrder*); empty = linear everywhere
   std::vector<FrontDev> h_front;   // host copies (flop accounting)
   std::vector<int> h_level_list;
   FrontDev *d_front = nullptr;
@@ -81,7 +82,11 @@ struct FactorTiming {
   double gemm_ms = 0.0;
   double gemm_flops = 0.0;
   int gemm_launches = 0;
+  std::vector<double> level_gemm_ms;  // GEMM time per elimination level
 };
+
+// Tile orders of the frontal GEMM (chosen per level by the autotuner).
+enum { kOrderLinear = 0, kOrderXcdGrouped = 1, kOrderCount = 2 };
 
 // Fronts in complex FP64, or complex FP32 (fp32) in the same arena.  With
 // `timing`, every GEMM launch is bracketed by events (slightly slower).

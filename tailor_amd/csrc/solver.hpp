@@ -34,6 +34,9 @@ struct SolverState {
 void solver_release(SolverState *S);
 int solver_prepare(tailor_ctx *ctx);
 int solver_factor(tailor_ctx *ctx, cplx sigma, FactorTiming *timing = nullptr);
+// Factor once per GEMM tile order with per-level timing and keep, for each
+// elimination level, the order with the smallest GEMM time.
+int solver_autotune(tailor_ctx *ctx, cplx sigma, tailor_tune_report *report);
 int solver_linear_solve(tailor_ctx *ctx, const double *b_nat, double *x_nat, int refine_max,
                         double refine_tol, double *relres, int *steps);
 int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solve_diag &diag);

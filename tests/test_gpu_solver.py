@@ -162,4 +162,6 @@ def test_multiblock_front_solve(oracle_lib, precision):
           f"unrefined {res0:.2e} refined {res:.2e} ({steps} steps)", flush=True)
     assert fd["perturbed_pivots"] == 0
     assert res0 <= (1e-9 if precision == 0 else 1e-2)
-    assert res <= (1e-13 if precision == 0 else 1e-12)
+    # the device stops refining at the target (1e-13 / 1e-12); the host's
+    # recomputed residual differs by rounding
+    assert res <= (3e-13 if precision == 0 else 3e-12)
