@@ -46,6 +46,16 @@ def algorithmic_assembly_bytes(ny: int, nz: int, ssx: int, sse: int) -> int:
     return 16 * 25 * (nnzb_a + nodes) + 128 * nodes + 3 * 8 * (ny * ssx + nz * sse)
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per dispatch of `kernel` from the committed counter runs
+    (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a 1 GiB copy), or None."""
+    path = ROOT / "profiles" / "hbm_traffic_200x100.json"
+    if not path.exists():
+        return None
+    k = json.loads(path.read_text()).get("kernels", {}).get(kernel)
+    return k["traffic_bytes_per_dispatch"] if k else None
+
+
 def cpu_baseline() -> dict:
     """The CPU oracle (C assembly + multifrontal LU on LAPACK/BLAS + numpy
     Krylov-Schur) on the 100x50 grid of the same generator (the 200x100
@@ -232,7 +242,9 @@ def main() -> int:
         "common_pair_max_rel_diff": sw["common_pair_rel_diff"],
         "roofline": {"bound": "mfma", "kernel": "k_gemm (frontal updates, MFMA f64 16x16x4)",
                      "achieved": round(gemm_tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(gemm_tf / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac": round(gemm_tf / FP64_PEAK_TFLOPS, 4),
+                     "traffic": measured_traffic("k_gemm<double>"),
+                     "traffic_note": "mean HBM bytes per k_gemm dispatch (counter runs in profiles/)",
                      "gemm_ms_per_factorization": round(g["gemm_ms"], 2),
                      "gemm_flops_per_factorization": g["gemm_flops"],
                      "gemm_launches": g["gemm_launches"],

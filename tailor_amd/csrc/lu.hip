@@ -501,7 +501,13 @@ template <> struct Mfma<float> {
   static __device__ inline int ocol(int l, int v) { return 4 * (l / 16) + v; }
 };
 
-enum GemmRegion { kGemmInner = 0, kGemmBlkRows = 1, kGemmTrail = 2, kGemmZ = 3 };
+// TRAIL = TRAILHEAD (the next block's fully summed columns of F) + TRAILREST
+// (the remaining columns of F and all of E); the split lets the next
+// block's panels start while the rest of the update runs.
+enum GemmRegion {
+  kGemmInner = 0, kGemmBlkRows = 1, kGemmTrail = 2, kGemmZ = 3, kGemmTrailHead = 4,
+  kGemmTrailRest = 5
+};
 
 // Operands of one front's update.  Columns [0, N1) of B and C come from the
 // first segment (rows [0, M) of C), columns [N1, N) from the second (rows
@@ -525,9 +531,11 @@ struct GemmShape {
 __host__ __device__ inline GemmShape gemm_shape(int region, const FrontDev &d, int k0, int nb,
                                                 int K0, int NB) {
   GemmShape g{0, 0, 0, 0, 0};
-  if (region == kGemmTrail ? K0 >= d.p : k0 >= d.p) return g;
+  const bool trail = region == kGemmTrail || region == kGemmTrailHead || region == kGemmTrailRest;
+  if (trail ? K0 >= d.p : k0 >= d.p) return g;
   const int kb = min(nb, d.p - k0);
   const int K1 = min(K0 + NB, d.p);
+  const int K1n = min(K1 + NB, d.p);  // end of the next block
   switch (region) {
     case kGemmInner:
       g = {d.m - k0 - kb, d.m - k0 - kb, K1 - k0 - kb, K1 - k0 - kb, kb};
@@ -537,6 +545,12 @@ __host__ __device__ inline GemmShape gemm_shape(int region, const FrontDev &d, i
       break;
     case kGemmTrail:
       g = {d.m - K1, d.p - K1, d.m - K1 + d.p, d.m - K1, K1 - K0};
+      break;
+    case kGemmTrailHead:
+      g = {d.m - K1, d.m - K1, K1n - K1, K1n - K1, K1 - K0};
+      break;
+    case kGemmTrailRest:
+      g = {d.m - K1, d.p - K1, d.m - K1n + d.p, d.m - K1n, K1 - K0};
       break;
     default:
       g = {k0, k0, d.p - k0, d.p - k0, kb};
@@ -548,14 +562,31 @@ __host__ __device__ inline GemmShape gemm_shape(int region, const FrontDev &d, i
 template <typename T>
 __device__ inline bool gemm_view(int region, const FrontDev &d, int k0, int nb, int K0, int NB,
                                  T *arena, GemmView<T> &v) {
-  if (region == kGemmTrail ? K0 >= d.p : k0 >= d.p) return false;
+  const bool trail = region == kGemmTrail || region == kGemmTrailHead || region == kGemmTrailRest;
+  if (trail ? K0 >= d.p : k0 >= d.p) return false;
   const long long m = d.m, p = d.p, mm = m * m, pp = p * p;
   T *Fr = arena + d.off, *Fi = Fr + mm;
   T *Er = Fi + mm, *Ei = Er + pp;
   T *Zr = Ei + pp, *Zi = Zr + pp;
   const int kb = min(nb, d.p - k0);
   const int K1 = min(K0 + NB, d.p);
+  const int K1n = min(K1 + NB, d.p);
   switch (region) {
+    case kGemmTrailHead: {
+      const long long a = K0 * m + K1, b = K1 * m + K0, c = K1 * m + K1;
+      v = {Fr + a, Fi + a, m, Fr + b, Fi + b, nullptr, nullptr, m, 0,
+           Fr + c, Fi + c, nullptr, nullptr, m, 0,
+           d.m - K1, d.m - K1, K1n - K1, K1n - K1, K1 - K0};
+      break;
+    }
+    case kGemmTrailRest: {
+      const long long a = K0 * m + K1, b = static_cast<long long>(K1n) * m + K0,
+                      c = static_cast<long long>(K1n) * m + K1;
+      v = {Fr + a, Fi + a, m, Fr + b, Fi + b, Er + K0, Ei + K0, m, p,
+           Fr + c, Fi + c, Er + K1, Ei + K1, m, p,
+           d.m - K1, d.p - K1, d.m - K1n + d.p, d.m - K1n, K1 - K0};
+      break;
+    }
     case kGemmInner: {
       const long long a = k0 * m + k0 + kb, b = (k0 + kb) * m + k0, c = (k0 + kb) * m + k0 + kb;
       v = {Fr + a, Fi + a, m, Fr + b, Fi + b, nullptr, nullptr, m, 0,
@@ -974,7 +1005,7 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
     const int *list = P.d_level_list + L.list_off;
     const unsigned tm_m = cdivu(L.mmax, GT), tm_mp = cdivu(L.mmax + L.pmax, GT);
     const int order = lev < P.gemm_order.size() ? P.gemm_order[lev] : kOrderLinear;
-    auto gemm = [&](unsigned tiles, int region, int k0, int K0) -> hipError_t {
+    auto gemm = [&](unsigned tiles, int region, int k0, int K0, hipStream_t st) -> hipError_t {
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (tm) {
         hipError_t r;
@@ -983,7 +1014,7 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
         events.push_back(e0);
         events.push_back(e1);
         event_level.push_back(static_cast<int>(lev));
-        if ((r = hipEventRecord(e0, s)) != hipSuccess) return r;
+        if ((r = hipEventRecord(e0, st)) != hipSuccess) return r;
         for (int q = 0; q < L.count; ++q) {
           const GemmShape g = gemm_shape(region, P.h_front[P.h_level_list[L.list_off + q]], k0, nb, K0, NB);
           if (g.M > 0 && g.N > 0 && g.K > 0)
@@ -993,35 +1024,66 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
         ++tm->gemm_launches;
       }
       if (order != kOrderLinear) tiles = (tiles + kXcds - 1) / kXcds * kXcds;  // XCD order: multiple of 8
-      hipLaunchKernelGGL(k_gemm<T>, dim3(tiles, L.count), dim3(256), 0, s, list, L.count,
+      hipLaunchKernelGGL(k_gemm<T>, dim3(tiles, L.count), dim3(256), 0, st, list, L.count,
                          P.d_front, region, k0, nb, K0, NB, order, arena);
-      if (tm) return hipEventRecord(e1, s);
+      if (tm) return hipEventRecord(e1, st);
       return hipGetLastError();
     };
-    for (int K0 = 0; K0 < L.pmax; K0 += NB) {
+    // panels of block [K0, K0 + NB): LU of each panel, its L21 rows, and the
+    // block columns right of it (touches only the block's columns and ipiv)
+    auto panel_phase = [&](int K0, hipStream_t st) -> hipError_t {
+      hipError_t r;
       const int Kend = std::min(K0 + NB, L.pmax);
       for (int k0 = K0; k0 < Kend; k0 += nb) {
-        hipLaunchKernelGGL(k_panel<T>, dim3(L.count), dim3(kPanelThreads), 0, s, list,
+        hipLaunchKernelGGL(k_panel<T>, dim3(L.count), dim3(kPanelThreads), 0, st, list,
                            L.count, P.d_front, k0, nb, K0, NB, tiny, perturb, arena, P.d_ipiv,
                            P.d_counters);
         if (L.cmax > 0)
-          hipLaunchKernelGGL(k_l21<T>, dim3(cdivu(L.cmax, 256), L.count), dim3(256), 0, s,
+          hipLaunchKernelGGL(k_l21<T>, dim3(cdivu(L.cmax, 256), L.count), dim3(256), 0, st,
                              list, L.count, P.d_front, k0, nb, arena);
         if (k0 + nb < Kend) {
-          hipLaunchKernelGGL(k_trsm_lower<T>, dim3(1, L.count), dim3(256), 0, s, list,
+          hipLaunchKernelGGL(k_trsm_lower<T>, dim3(1, L.count), dim3(256), 0, st, list,
                              L.count, P.d_front, k0, nb, K0, NB, 0, 0, P.d_ipiv, arena);
-          if ((e = gemm(tm_m * cdivu(NB - nb, GT), kGemmInner, k0, K0)) != hipSuccess) return e;
+          if ((r = gemm(tm_m * cdivu(NB - nb, GT), kGemmInner, k0, K0, st)) != hipSuccess) return r;
         }
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((r = hipGetLastError()) != hipSuccess) return r;
       }
+      return hipSuccess;
+    };
+    // Look-ahead: the panels of block i+1 run on the side stream once the
+    // TRAILHEAD update of block i has produced their columns, while the main
+    // stream applies TRAILREST (later columns and E).  The two touch
+    // disjoint columns; the main stream waits for the panels before the next
+    // block's row exchanges and U rows.  A timed factorization runs the same
+    // launches in order on one stream, so each kernel's duration is its own.
+    const bool lookahead = tm == nullptr;
+    if ((e = panel_phase(0, s)) != hipSuccess) return e;
+    for (int K0 = 0; K0 < L.pmax; K0 += NB) {
+      const int Kend = std::min(K0 + NB, L.pmax);
       for (int k0 = K0; k0 < Kend; k0 += nb) {
         hipLaunchKernelGGL(k_trsm_lower<T>, dim3(cdivu(L.mmax + L.pmax, 256), L.count),
                            dim3(256), 0, s, list, L.count, P.d_front, k0, nb, K0, NB, 1,
                            k0 == K0 ? 1 : 0, P.d_ipiv, arena);
         if (k0 + nb < Kend)
-          if ((e = gemm(cdivu(NB - nb, GT) * tm_mp, kGemmBlkRows, k0, K0)) != hipSuccess) return e;
+          if ((e = gemm(cdivu(NB - nb, GT) * tm_mp, kGemmBlkRows, k0, K0, s)) != hipSuccess) return e;
       }
-      if ((e = gemm(tm_m * tm_mp, kGemmTrail, K0, K0)) != hipSuccess) return e;
+      const bool next = K0 + NB < L.pmax;
+      if (next) {
+        if ((e = gemm(tm_m * cdivu(NB, GT), kGemmTrailHead, K0, K0, s)) != hipSuccess) return e;
+        if (lookahead) {
+          if ((e = hipEventRecord(P.ev_head, s)) != hipSuccess) return e;
+          if ((e = hipStreamWaitEvent(P.side, P.ev_head, 0)) != hipSuccess) return e;
+          if ((e = panel_phase(K0 + NB, P.side)) != hipSuccess) return e;
+          if ((e = hipEventRecord(P.ev_panel, P.side)) != hipSuccess) return e;
+          if ((e = gemm(tm_m * tm_mp, kGemmTrailRest, K0, K0, s)) != hipSuccess) return e;
+          if ((e = hipStreamWaitEvent(s, P.ev_panel, 0)) != hipSuccess) return e;
+        } else {
+          if ((e = panel_phase(K0 + NB, s)) != hipSuccess) return e;
+          if ((e = gemm(tm_m * tm_mp, kGemmTrailRest, K0, K0, s)) != hipSuccess) return e;
+        }
+      } else {
+        if ((e = gemm(tm_m * tm_mp, kGemmTrail, K0, K0, s)) != hipSuccess) return e;
+      }
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     // Z = U11^-1 by backward Jordan steps on the pivot blocks
@@ -1030,7 +1092,7 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
       hipLaunchKernelGGL(k_trsm_upper<T>, dim3(cdivu(L.pmax, 256), L.count), dim3(256), 0,
                          s, list, L.count, P.d_front, k0, nb, arena);
       if (k0 > 0)
-        if ((e = gemm(tp * tp, kGemmZ, k0, 0)) != hipSuccess) return e;
+        if ((e = gemm(tp * tp, kGemmZ, k0, 0, s)) != hipSuccess) return e;
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
   }

@@ -53,6 +53,8 @@ struct LuPlan {
   int *d_counters = nullptr;    // [0] perturbed pivots
   int *d_perm = nullptr;        // elimination dof -> natural dof (n entries)
   double *arena = nullptr;
+  hipStream_t side = nullptr;   // look-ahead panels (default priority)
+  hipEvent_t ev_head = nullptr, ev_panel = nullptr;
   double *W = nullptr;          // 2 * w_len
   double *Y = nullptr;          // 2 * n
 };

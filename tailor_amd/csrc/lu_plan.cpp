@@ -127,10 +127,19 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
     return e;
   if ((e = hipMalloc(reinterpret_cast<void **>(&P.Y), 2 * P.n * sizeof(double))) != hipSuccess)
     return e;
+  if ((e = hipStreamCreateWithFlags(&P.side, hipStreamNonBlocking)) != hipSuccess) return e;
+  if ((e = hipEventCreateWithFlags(&P.ev_head, hipEventDisableTiming)) != hipSuccess) return e;
+  if ((e = hipEventCreateWithFlags(&P.ev_panel, hipEventDisableTiming)) != hipSuccess) return e;
   return hipStreamSynchronize(s);
 }
 
 void lu_plan_destroy(LuPlan &P) {
+  if (P.side) (void)hipStreamSynchronize(P.side);
+  if (P.ev_head) (void)hipEventDestroy(P.ev_head);
+  if (P.ev_panel) (void)hipEventDestroy(P.ev_panel);
+  if (P.side) (void)hipStreamDestroy(P.side);
+  P.side = nullptr;
+  P.ev_head = P.ev_panel = nullptr;
   release(P.d_front);
   release(P.d_level_list);
   release(P.d_child_slots);
