@@ -248,7 +248,8 @@ def main() -> int:
                      "gemm_ms_per_factorization": round(g["gemm_ms"], 2),
                      "gemm_flops_per_factorization": g["gemm_flops"],
                      "gemm_launches": g["gemm_launches"],
-                     "tile_order_per_level": single["tune"]["order"],
+                     "tile_config_per_level": single["tune"]["config"],
+                     "gemm_ms_per_config": single["tune"]["gemm_ms"],
                      "factor_ms": round(g["factor_ms"], 2)},
         "plain_instance": single["plain"],
         "assembly": single["assembly"],

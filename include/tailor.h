@@ -164,19 +164,23 @@ int tailor_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, tailor_fact
 int tailor_profile_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, double *factor_ms,
                           double *gemm_ms, double *gemm_flops, int32_t *gemm_launches);
 
-/* Autotuning of the frontal GEMM: factor at `sigma` once per tile order
- * (0: row blocks in launch order; 1: XCD-aware grouped order), timing the
- * GEMM launches of each elimination level with HIP events, and keep the
- * faster order per level for later factorizations of this context.  No
- * reference counterpart (performance only; results are unchanged up to
- * rounding). */
+/* Autotuning of the frontal GEMM: factor at `sigma` once per configuration
+ * c = 2 * shape + order, shape 0: 64x64 complex tile (256 threads),
+ * 1: 128x64 (512), 2: 128x128 (512); order 0: tiles in launch order,
+ * 1: XCD-aware grouped order.  The GEMM launches of each elimination level
+ * are timed with HIP events and each level keeps its fastest configuration
+ * for later factorizations of this context.  No reference counterpart
+ * (performance only; results are unchanged up to rounding). */
 typedef struct {
   int32_t levels;          /* elimination levels tuned (at most 64 reported) */
-  int32_t order[64];       /* chosen order per level */
-  double gemm_ms[4];       /* total GEMM time per order (entries 0 and 1 used) */
-  double tuned_gemm_ms;    /* sum over levels of the chosen order's time */
+  int32_t order[64];       /* chosen configuration c per level */
+  double gemm_ms[8];       /* total GEMM time per configuration (entries 0..5 used) */
+  double tuned_gemm_ms;    /* sum over levels of the chosen configurations' times */
 } tailor_tune_report;
 int tailor_autotune(tailor_ctx *ctx, double sigma_re, double sigma_im, tailor_tune_report *report);
+/* Force configuration c (0..5, as above) on every level, replacing any
+ * autotuned choice, for later factorizations of this context. */
+int tailor_set_gemm_config(tailor_ctx *ctx, int32_t config);
 
 /* Precision of the factorization used by tailor_factor / tailor_solve.
  * FP64: complex FP64 fronts (the plain form).  MIXED: complex FP32 fronts

@@ -91,7 +91,7 @@ class SolveOpts(C.Structure):
 
 
 class TuneReport(C.Structure):
-    _fields_ = [("levels", C.c_int32), ("order", C.c_int32 * 64), ("gemm_ms", C.c_double * 4),
+    _fields_ = [("levels", C.c_int32), ("order", C.c_int32 * 64), ("gemm_ms", C.c_double * 8),
                 ("tuned_gemm_ms", C.c_double)]
 
 
@@ -142,6 +142,7 @@ def library() -> C.CDLL:
     lib.tailor_solve.argtypes = [C.c_void_p, P(SolveOpts), P(SolveDiag)]
     lib.tailor_set_precision.argtypes = [C.c_void_p, C.c_int32]
     lib.tailor_autotune.argtypes = [C.c_void_p, C.c_double, C.c_double, P(TuneReport)]
+    lib.tailor_set_gemm_config.argtypes = [C.c_void_p, C.c_int32]
     lib.tailor_profile_factor.argtypes = [C.c_void_p, C.c_double, C.c_double, P(C.c_double),
                                           P(C.c_double), P(C.c_double), P(C.c_int32)]
     lib.tailor_time_linear_solve.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double)]
@@ -282,12 +283,14 @@ def _profile_factor(self, sigma: complex) -> dict:
 
 
 def _autotune(self, sigma: complex) -> dict:
-    """Choose the frontal GEMM tile order per elimination level by timing."""
+    """Choose the frontal GEMM tile shape and order per elimination level by
+    timing (configuration c = 2 * shape + order)."""
     sigma = complex(sigma)
     r = TuneReport()
     self._check(self.lib.tailor_autotune(self.ptr, sigma.real, sigma.imag, C.byref(r)))
-    return {"levels": r.levels, "order": list(r.order)[:r.levels],
-            "gemm_ms": list(r.gemm_ms)[:2], "tuned_gemm_ms": r.tuned_gemm_ms}
+    return {"levels": r.levels, "config": list(r.order)[:r.levels],
+            "gemm_ms": [round(x, 3) for x in list(r.gemm_ms)[:6]],
+            "tuned_gemm_ms": r.tuned_gemm_ms}
 
 
 def _factor(self, sigma: complex) -> dict:
@@ -354,6 +357,7 @@ Context.factor = _factor
 Context.set_precision = _set_precision
 Context.profile_factor = _profile_factor
 Context.autotune = _autotune
+Context.set_gemm_config = lambda self, c: self._check(self.lib.tailor_set_gemm_config(self.ptr, int(c)))
 Context.linear_solve = _linear_solve
 Context.solve = _solve
 Context.eigenpairs = _eigenpairs

@@ -617,6 +617,19 @@ int tailor_autotune(tailor_ctx *ctx, double sigma_re, double sigma_im, tailor_tu
   return TAILOR_OK;
 }
 
+int tailor_set_gemm_config(tailor_ctx *ctx, int32_t config) {
+  if (!ctx) return TAILOR_ERR_USER_INPUT;
+  if (config < 0 || config >= tailor::kGemmShapes * tailor::kOrderCount)
+    return fail(ctx, TAILOR_ERR_ARG_OUTOFRANGE, "unknown GEMM configuration");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (int st = tailor::solver_prepare(ctx)) return st;
+  tailor::LuPlan &P = ctx->solver->plan;
+  P.gemm_shape.assign(P.levels.size(), config / tailor::kOrderCount);
+  P.gemm_order.assign(P.levels.size(), config % tailor::kOrderCount);
+  ctx->solver->factored = false;
+  return TAILOR_OK;
+}
+
 int tailor_set_precision(tailor_ctx *ctx, int32_t precision) {
   if (!ctx) return TAILOR_ERR_USER_INPUT;
   if (precision != TAILOR_PRECISION_FP64 && precision != TAILOR_PRECISION_MIXED)

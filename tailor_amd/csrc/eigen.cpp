@@ -140,27 +140,32 @@ int solver_autotune(tailor_ctx *ctx, cplx sigma, tailor_tune_report *report) {
   if (int st = solver_prepare(ctx)) return st;
   SolverState &S = *ctx->solver;
   const size_t nl = S.plan.levels.size();
-  std::vector<std::vector<double>> t(kOrderCount);
-  std::vector<double> total(kOrderCount, 0.0);
-  for (int o = 0; o < kOrderCount; ++o) {
-    S.plan.gemm_order.assign(nl, o);
+  // every (tile shape, tile order) configuration, applied to all levels
+  const int nconf = kGemmShapes * kOrderCount;
+  std::vector<std::vector<double>> t(nconf);
+  std::vector<double> total(nconf, 0.0);
+  for (int c = 0; c < nconf; ++c) {
+    S.plan.gemm_shape.assign(nl, c / kOrderCount);
+    S.plan.gemm_order.assign(nl, c % kOrderCount);
     FactorTiming tm;
     if (int st = solver_factor(ctx, sigma, &tm)) return st;
-    t[o] = tm.level_gemm_ms;
-    total[o] = tm.gemm_ms;
+    t[c] = tm.level_gemm_ms;
+    total[c] = tm.gemm_ms;
   }
   double tuned = 0.0;
   for (size_t l = 0; l < nl; ++l) {
     int best = 0;
-    for (int o = 1; o < kOrderCount; ++o)
-      if (t[o][l] < t[best][l]) best = o;
-    S.plan.gemm_order[l] = best;
+    for (int c = 1; c < nconf; ++c)
+      if (t[c][l] < t[best][l]) best = c;
+    S.plan.gemm_shape[l] = best / kOrderCount;
+    S.plan.gemm_order[l] = best % kOrderCount;
     tuned += t[best][l];
   }
   if (report) {
     report->levels = static_cast<int32_t>(nl);
-    for (size_t l = 0; l < nl && l < 64; ++l) report->order[l] = S.plan.gemm_order[l];
-    for (int o = 0; o < kOrderCount && o < 4; ++o) report->gemm_ms[o] = total[o];
+    for (size_t l = 0; l < nl && l < 64; ++l)
+      report->order[l] = S.plan.gemm_shape[l] * kOrderCount + S.plan.gemm_order[l];
+    for (int c = 0; c < nconf && c < 8; ++c) report->gemm_ms[c] = total[c];
     report->tuned_gemm_ms = tuned;
   }
   return TAILOR_OK;

@@ -619,66 +619,75 @@ __device__ inline bool gemm_view(int region, const FrontDev &d, int k0, int nb, 
   return v.M > 0 && v.N > 0 && v.K > 0;
 }
 
-constexpr int GT = 64;   // output tile (complex) per workgroup
 constexpr int kXcds = 8;       // accelerator complex dies of an MI355X
 constexpr int kGemmGroup = 8;  // row blocks per tile group
 constexpr int GKC = 16;  // K chunk staged in LDS
-constexpr int GLD = 80;  // LDS row stride: 64 + 16 keeps the k rows of a wave's reads in different banks
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_gemm(const int *__restrict__ list, int nlist,
-                                              const FrontDev *__restrict__ fr, int region, int k0,
-                                              int nb, int K0, int NB, int order,
-                                              T *__restrict__ arena) {
+// Tile shapes (chosen per level by the autotuner): WMW x WNW waves, each
+// owning WMB x WNB MFMA 16x16 blocks, so a workgroup computes a
+// (16 WMW WMB) x (16 WNW WNB) complex tile.
+//   shape 0: 64 x 64, 256 threads; 1: 128 x 64, 512 threads;
+//   shape 2: 128 x 128, 512 threads.
+enum { kShape64x64 = 0, kShape128x64 = 1, kShape128x128 = 2, kShapeCount = 3 };
+
+template <typename T, int WMW, int WNW, int WMB, int WNB>
+__global__ __launch_bounds__(64 * WMW * WNW) void k_gemm(const int *__restrict__ list, int nlist,
+                                                         const FrontDev *__restrict__ fr,
+                                                         int region, int k0, int nb, int K0,
+                                                         int NB, int order,
+                                                         T *__restrict__ arena) {
+  constexpr int THREADS = 64 * WMW * WNW;
+  constexpr int TM = 16 * WMW * WMB, TN = 16 * WNW * WNB;
+  // LDS row stride: +16 keeps the four k rows of a wave's reads in different banks
+  constexpr int LDA_ = TM + 16, LDB_ = TN + 16;
+  constexpr int AE = TM * GKC / THREADS, BE = TN * GKC / THREADS;
+  static_assert(AE * THREADS == TM * GKC && BE * THREADS == TN * GKC, "tile / thread mismatch");
   const int f = list[blockIdx.y];
   const FrontDev d = fr[f];
   GemmView<T> v;
   if (!gemm_view(region, d, k0, nb, K0, NB, arena, v)) return;
-  const int tiles_m = (v.M + GT - 1) / GT, tiles_n = (v.N + GT - 1) / GT;
+  const int tiles_m = (v.M + TM - 1) / TM, tiles_n = (v.N + TN - 1) / TN;
   int row0, col0;
   if (order == kOrderLinear) {
     // tile = blockIdx.x, row blocks fastest
     const int tile = blockIdx.x;
     if (tile >= tiles_m * tiles_n) return;
-    row0 = (tile % tiles_m) * GT;
-    col0 = (tile / tiles_m) * GT;
+    row0 = (tile % tiles_m) * TM;
+    col0 = (tile / tiles_m) * TN;
   } else {
-  // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs,
-  // so XCD x (= blockIdx.x % 8, gridDim.x being a multiple of 8) receives a
-  // contiguous eighth of this front's tiles (rotated by front); within it,
-  // tiles go kGemmGroup row blocks at a time so that co-resident workgroups
-  // share A and B tiles in that XCD's L2.
-  const int ntiles = tiles_m * tiles_n;
-  const int per = (ntiles + kXcds - 1) / kXcds;
-  const int q = static_cast<int>(blockIdx.x / kXcds);
-  if (q >= per) return;
-  const int slot = (static_cast<int>(blockIdx.x % kXcds) - static_cast<int>(blockIdx.y % kXcds) +
-                    kXcds) % kXcds;
-  const int tile = slot * per + q;
-  if (tile >= ntiles) return;
-  const int in_group = kGemmGroup * tiles_n;
-  const int first_m = (tile / in_group) * kGemmGroup;
-  const int group_m = min(tiles_m - first_m, kGemmGroup);
-  const int tm = first_m + (tile % in_group) % group_m, tn = (tile % in_group) / group_m;
-  row0 = tm * GT;
-  col0 = tn * GT;
+    // XCD-aware order: workgroups are dispatched round-robin over the 8
+    // XCDs, so XCD x (= blockIdx.x % 8, gridDim.x being a multiple of 8)
+    // receives a contiguous eighth of this front's tiles (rotated by front);
+    // within it, tiles go kGemmGroup row blocks at a time so that
+    // co-resident workgroups share A and B tiles in that XCD's L2.
+    const int ntiles = tiles_m * tiles_n;
+    const int per = (ntiles + kXcds - 1) / kXcds;
+    const int q = static_cast<int>(blockIdx.x / kXcds);
+    if (q >= per) return;
+    const int slot = (static_cast<int>(blockIdx.x % kXcds) - static_cast<int>(blockIdx.y % kXcds) +
+                      kXcds) % kXcds;
+    const int tile = slot * per + q;
+    if (tile >= ntiles) return;
+    const int in_group = kGemmGroup * tiles_n;
+    const int first_m = (tile / in_group) * kGemmGroup;
+    const int group_m = min(tiles_m - first_m, kGemmGroup);
+    const int tm = first_m + (tile % in_group) % group_m, tn = (tile % in_group) / group_m;
+    row0 = tm * TM;
+    col0 = tn * TN;
   }
   // two LDS buffers: the next K chunk is stored while the current one feeds
   // the MFMAs, so each chunk needs one barrier
-  __shared__ T sAr[2][GKC][GLD], sAi[2][GKC][GLD], sBr[2][GKC][GLD], sBi[2][GKC][GLD];
+  __shared__ T sAr[2][GKC][LDA_], sAi[2][GKC][LDA_], sBr[2][GKC][LDB_], sBi[2][GKC][LDB_];
   const int t = threadIdx.x, l = t % 64, w = t / 64;
-  const int wm = w % 2, wn = w / 2;
-  // global -> register staging: A (row t%64, k t/64 + 4s), B (k t%16, col t/16 + 16s).
-  // Loads are unconditional from clamped in-bounds addresses; out-of-range
-  // entries are replaced by zero afterwards.
-  const int ar_ = t % 64, ak_ = t / 64;
-  const int bk_ = t % 16, bc_ = t / 16;
-  const bool arow_ok = row0 + ar_ < v.M;
-  const T *bcol_r[4], *bcol_i[4];
-  bool bcol_ok[4];
+  const int wm = w % WMW, wn = w / WMW;
+  // global -> register staging (element e = t + s * THREADS): A (row e % TM,
+  // k e / TM), B (k e % GKC, column e / GKC).  Loads are unconditional from
+  // clamped in-bounds addresses; out-of-range entries become zero.
+  const T *bcol_r[BE], *bcol_i[BE];
+  bool bcol_ok[BE];
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int col = col0 + bc_ + 16 * s;
+  for (int s = 0; s < BE; ++s) {
+    const int col = col0 + (t + s * THREADS) / GKC;
     bcol_ok[s] = col < v.N;
     if (!bcol_ok[s]) {
       bcol_r[s] = v.Br;
@@ -691,17 +700,20 @@ __global__ __launch_bounds__(256) void k_gemm(const int *__restrict__ list, int 
       bcol_i[s] = v.B2i + (col - v.N1) * v.ldb2;
     }
   }
-  T ga_r[4], ga_i[4], gb_r[4], gb_i[4];
+  T ga_r[AE], ga_i[AE], gb_r[BE], gb_i[BE];
   auto fetch = [&](int kc) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int ka = kc + ak_ + 4 * s;
-      const bool oka = arow_ok && ka < v.K;
-      const long long ia = oka ? static_cast<long long>(ka) * v.lda + row0 + ar_ : 0;
+    for (int s = 0; s < AE; ++s) {
+      const int e = t + s * THREADS, r = e % TM, k = e / TM;
+      const bool oka = row0 + r < v.M && kc + k < v.K;
+      const long long ia = oka ? static_cast<long long>(kc + k) * v.lda + row0 + r : 0;
       const T xr = v.Ar[ia], xi = v.Ai[ia];
       ga_r[s] = oka ? xr : T(0);
       ga_i[s] = oka ? xi : T(0);
-      const int kbb = kc + bk_;
+    }
+#pragma unroll
+    for (int s = 0; s < BE; ++s) {
+      const int kbb = kc + (t + s * THREADS) % GKC;
       const bool okb = bcol_ok[s] && kbb < v.K;
       const int ib = okb ? kbb : 0;
       const T yr = bcol_r[s][ib], yi = bcol_i[s][ib];
@@ -711,19 +723,24 @@ __global__ __launch_bounds__(256) void k_gemm(const int *__restrict__ list, int 
   };
   auto stash = [&](int buf) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      sAr[buf][ak_ + 4 * s][ar_] = ga_r[s];
-      sAi[buf][ak_ + 4 * s][ar_] = ga_i[s];
-      sBr[buf][bk_][bc_ + 16 * s] = gb_r[s];
-      sBi[buf][bk_][bc_ + 16 * s] = gb_i[s];
+    for (int s = 0; s < AE; ++s) {
+      const int e = t + s * THREADS;
+      sAr[buf][e / TM][e % TM] = ga_r[s];
+      sAi[buf][e / TM][e % TM] = ga_i[s];
+    }
+#pragma unroll
+    for (int s = 0; s < BE; ++s) {
+      const int e = t + s * THREADS;
+      sBr[buf][e % GKC][e / GKC] = gb_r[s];
+      sBi[buf][e % GKC][e / GKC] = gb_i[s];
     }
   };
   using acc_t = typename Mfma<T>::acc_t;
-  acc_t cr[2][2], ci[2][2];
+  acc_t cr[WMB][WNB], ci[WMB][WNB];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < WMB; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < WNB; ++b) {
       cr[a][b] = acc_t{0, 0, 0, 0};
       ci[a][b] = acc_t{0, 0, 0, 0};
     }
@@ -737,18 +754,21 @@ __global__ __launch_bounds__(256) void k_gemm(const int *__restrict__ list, int 
 #pragma unroll
     for (int kk = 0; kk < GKC; kk += 4) {
       const int kr = kk + l / 16;
-      T a_r[2], a_i[2], b_r[2], b_i[2];
+      T a_r[WMB], a_i[WMB], b_r[WNB], b_i[WNB];
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        a_r[q] = sAr[buf][kr][wm * 32 + q * 16 + l % 16];
-        a_i[q] = sAi[buf][kr][wm * 32 + q * 16 + l % 16];
-        b_r[q] = sBr[buf][kr][wn * 32 + q * 16 + l % 16];
-        b_i[q] = sBi[buf][kr][wn * 32 + q * 16 + l % 16];
+      for (int q = 0; q < WMB; ++q) {
+        a_r[q] = sAr[buf][kr][wm * 16 * WMB + q * 16 + l % 16];
+        a_i[q] = sAi[buf][kr][wm * 16 * WMB + q * 16 + l % 16];
       }
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int q = 0; q < WNB; ++q) {
+        b_r[q] = sBr[buf][kr][wn * 16 * WNB + q * 16 + l % 16];
+        b_i[q] = sBi[buf][kr][wn * 16 * WNB + q * 16 + l % 16];
+      }
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+      for (int a = 0; a < WMB; ++a)
+#pragma unroll
+        for (int b = 0; b < WNB; ++b) {
           cr[a][b] = Mfma<T>::mma(b_r[b], a_r[a], cr[a][b]);
           cr[a][b] = Mfma<T>::mma(b_i[b], -a_i[a], cr[a][b]);
           ci[a][b] = Mfma<T>::mma(b_i[b], a_r[a], ci[a][b]);
@@ -760,13 +780,13 @@ __global__ __launch_bounds__(256) void k_gemm(const int *__restrict__ list, int 
     buf ^= 1;
   }
 #pragma unroll
-  for (int a = 0; a < 2; ++a) {
-    const int row = row0 + wm * 32 + a * 16 + l % 16;
+  for (int a = 0; a < WMB; ++a) {
+    const int row = row0 + wm * 16 * WMB + a * 16 + l % 16;
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < WNB; ++b) {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int col = col0 + wn * 32 + b * 16 + Mfma<T>::ocol(l, e);
+        const int col = col0 + wn * 16 * WNB + b * 16 + Mfma<T>::ocol(l, e);
         if (col >= v.N) continue;
         if (col < v.N1) {
           if (row >= v.M) continue;
@@ -782,6 +802,41 @@ __global__ __launch_bounds__(256) void k_gemm(const int *__restrict__ list, int 
       }
     }
   }
+}
+
+// Launch one batched GEMM with the given tile shape; rows/cols are upper
+// bounds of M and N over the level's fronts.
+template <typename T>
+hipError_t launch_gemm(int shape, int order, long long rows, long long cols, int count,
+                       hipStream_t st, const int *list, const FrontDev *fr, int region, int k0,
+                       int nb, int K0, int NB, T *arena) {
+  int tm = 64, tn = 64, threads = 256;
+  if (shape == kShape128x64) {
+    tm = 128;
+    threads = 512;
+  } else if (shape == kShape128x128) {
+    tm = tn = 128;
+    threads = 512;
+  }
+  const long long tr = rows <= 0 ? 1 : (rows + tm - 1) / tm, tc = cols <= 0 ? 1 : (cols + tn - 1) / tn;
+  unsigned tiles = static_cast<unsigned>(tr * tc);
+  if (order != kOrderLinear) tiles = (tiles + kXcds - 1) / kXcds * kXcds;  // XCD order: multiple of 8
+  const dim3 grid(tiles, count), block(threads);
+  switch (shape) {
+    case kShape128x64:
+      hipLaunchKernelGGL((k_gemm<T, 4, 2, 2, 2>), grid, block, 0, st, list, count, fr, region, k0,
+                         nb, K0, NB, order, arena);
+      break;
+    case kShape128x128:
+      hipLaunchKernelGGL((k_gemm<T, 2, 4, 4, 2>), grid, block, 0, st, list, count, fr, region, k0,
+                         nb, K0, NB, order, arena);
+      break;
+    default:
+      hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 2>), grid, block, 0, st, list, count, fr, region, k0,
+                         nb, K0, NB, order, arena);
+      break;
+  }
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ solves
@@ -1003,9 +1058,10 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     const int *list = P.d_level_list + L.list_off;
-    const unsigned tm_m = cdivu(L.mmax, GT), tm_mp = cdivu(L.mmax + L.pmax, GT);
     const int order = lev < P.gemm_order.size() ? P.gemm_order[lev] : kOrderLinear;
-    auto gemm = [&](unsigned tiles, int region, int k0, int K0, hipStream_t st) -> hipError_t {
+    const int shape = lev < P.gemm_shape.size() ? P.gemm_shape[lev] : kShape64x64;
+    auto gemm = [&](long long rows, long long cols, int region, int k0, int K0,
+                    hipStream_t st) -> hipError_t {
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (tm) {
         hipError_t r;
@@ -1023,11 +1079,11 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
         }
         ++tm->gemm_launches;
       }
-      if (order != kOrderLinear) tiles = (tiles + kXcds - 1) / kXcds * kXcds;  // XCD order: multiple of 8
-      hipLaunchKernelGGL(k_gemm<T>, dim3(tiles, L.count), dim3(256), 0, st, list, L.count,
-                         P.d_front, region, k0, nb, K0, NB, order, arena);
+      hipError_t r = launch_gemm<T>(shape, order, rows, cols, L.count, st, list, P.d_front,
+                                    region, k0, nb, K0, NB, arena);
+      if (r != hipSuccess) return r;
       if (tm) return hipEventRecord(e1, st);
-      return hipGetLastError();
+      return hipSuccess;
     };
     // panels of block [K0, K0 + NB): LU of each panel, its L21 rows, and the
     // block columns right of it (touches only the block's columns and ipiv)
@@ -1044,7 +1100,7 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
         if (k0 + nb < Kend) {
           hipLaunchKernelGGL(k_trsm_lower<T>, dim3(1, L.count), dim3(256), 0, st, list,
                              L.count, P.d_front, k0, nb, K0, NB, 0, 0, P.d_ipiv, arena);
-          if ((r = gemm(tm_m * cdivu(NB - nb, GT), kGemmInner, k0, K0, st)) != hipSuccess) return r;
+          if ((r = gemm(L.mmax, NB - nb, kGemmInner, k0, K0, st)) != hipSuccess) return r;
         }
         if ((r = hipGetLastError()) != hipSuccess) return r;
       }
@@ -1065,34 +1121,33 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
                            dim3(256), 0, s, list, L.count, P.d_front, k0, nb, K0, NB, 1,
                            k0 == K0 ? 1 : 0, P.d_ipiv, arena);
         if (k0 + nb < Kend)
-          if ((e = gemm(cdivu(NB - nb, GT) * tm_mp, kGemmBlkRows, k0, K0, s)) != hipSuccess) return e;
+          if ((e = gemm(NB - nb, L.mmax + L.pmax, kGemmBlkRows, k0, K0, s)) != hipSuccess) return e;
       }
       const bool next = K0 + NB < L.pmax;
       if (next) {
-        if ((e = gemm(tm_m * cdivu(NB, GT), kGemmTrailHead, K0, K0, s)) != hipSuccess) return e;
+        if ((e = gemm(L.mmax, NB, kGemmTrailHead, K0, K0, s)) != hipSuccess) return e;
         if (lookahead) {
           if ((e = hipEventRecord(P.ev_head, s)) != hipSuccess) return e;
           if ((e = hipStreamWaitEvent(P.side, P.ev_head, 0)) != hipSuccess) return e;
           if ((e = panel_phase(K0 + NB, P.side)) != hipSuccess) return e;
           if ((e = hipEventRecord(P.ev_panel, P.side)) != hipSuccess) return e;
-          if ((e = gemm(tm_m * tm_mp, kGemmTrailRest, K0, K0, s)) != hipSuccess) return e;
+          if ((e = gemm(L.mmax, L.mmax + L.pmax, kGemmTrailRest, K0, K0, s)) != hipSuccess) return e;
           if ((e = hipStreamWaitEvent(s, P.ev_panel, 0)) != hipSuccess) return e;
         } else {
           if ((e = panel_phase(K0 + NB, s)) != hipSuccess) return e;
-          if ((e = gemm(tm_m * tm_mp, kGemmTrailRest, K0, K0, s)) != hipSuccess) return e;
+          if ((e = gemm(L.mmax, L.mmax + L.pmax, kGemmTrailRest, K0, K0, s)) != hipSuccess) return e;
         }
       } else {
-        if ((e = gemm(tm_m * tm_mp, kGemmTrail, K0, K0, s)) != hipSuccess) return e;
+        if ((e = gemm(L.mmax, L.mmax + L.pmax, kGemmTrail, K0, K0, s)) != hipSuccess) return e;
       }
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     // Z = U11^-1 by backward Jordan steps on the pivot blocks
-    const unsigned tp = cdivu(L.pmax, GT);
     for (int k0 = ((L.pmax - 1) / nb) * nb; k0 >= 0; k0 -= nb) {
       hipLaunchKernelGGL(k_trsm_upper<T>, dim3(cdivu(L.pmax, 256), L.count), dim3(256), 0,
                          s, list, L.count, P.d_front, k0, nb, arena);
       if (k0 > 0)
-        if ((e = gemm(tp * tp, kGemmZ, k0, 0, s)) != hipSuccess) return e;
+        if ((e = gemm(L.pmax, L.pmax, kGemmZ, k0, 0, s)) != hipSuccess) return e;
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
   }

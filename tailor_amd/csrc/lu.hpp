@@ -36,6 +36,7 @@ struct LuPlan {
   long long w_len = 0;          // per-plane length of the front workspace
   std::vector<LevelPlan> levels;
   std::vector<int> gemm_order;     // per level (kOrder*); empty = linear everywhere
+  std::vector<int> gemm_shape;     // per level tile shape (0: 64x64, 1: 128x64, 2: 128x128)
   std::vector<FrontDev> h_front;   // host copies (flop accounting)
   std::vector<int> h_level_list;
   FrontDev *d_front = nullptr;
@@ -87,8 +88,10 @@ struct FactorTiming {
   std::vector<double> level_gemm_ms;  // GEMM time per elimination level
 };
 
-// Tile orders of the frontal GEMM (chosen per level by the autotuner).
+// Tile orders and shapes of the frontal GEMM (chosen per level by the
+// autotuner).
 enum { kOrderLinear = 0, kOrderXcdGrouped = 1, kOrderCount = 2 };
+constexpr int kGemmShapes = 3;
 
 // Fronts in complex FP64, or complex FP32 (fp32) in the same arena.  With
 // `timing`, every GEMM launch is bracketed by events (slightly slower).

@@ -165,3 +165,24 @@ def test_multiblock_front_solve(oracle_lib, precision):
     # the device stops refining at the target (1e-13 / 1e-12); the host's
     # recomputed residual differs by rounding
     assert res <= (3e-13 if precision == 0 else 3e-12)
+
+
+@pytest.mark.parametrize("config", range(6))
+def test_every_gemm_configuration(oracle_lib, config):
+    """Each frontal-GEMM tile shape and order (the autotuner's choices) gives
+    a factorization that meets the solve bars (64 x 32: fronts up to 2020)."""
+    c = small_case(64, 32)
+    A, B = operators(oracle_lib, c)
+    sigma = 0.1 - 0.05j
+    K = (A - sigma * B).tocsr()
+    rng = np.random.default_rng(config)
+    b = rng.standard_normal(K.shape[0]) + 1j * rng.standard_normal(K.shape[0])
+    with ta.Context(0) as ctx:
+        ctx.load_case(c)
+        ctx.assemble(diagnostics=False)
+        ctx.set_gemm_config(config)
+        ctx.factor(sigma)
+        x0, _, _ = ctx.linear_solve(b, refine_max=0)
+    res0 = np.linalg.norm(K @ x0 - b) / np.linalg.norm(b)
+    print(f"config {config}: unrefined residual {res0:.2e}", flush=True)
+    assert res0 <= 1e-9
