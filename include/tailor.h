@@ -182,6 +182,15 @@ int tailor_autotune(tailor_ctx *ctx, double sigma_re, double sigma_im, tailor_tu
  * autotuned choice, for later factorizations of this context. */
 int tailor_set_gemm_config(tailor_ctx *ctx, int32_t config);
 
+/* Panel factorization kernel.  AUTO: the register-resident left-looking
+ * panel where a level's fronts have at most 4096 fully summed rows, the
+ * streaming right-looking panel above that; STREAMING: the streaming panel
+ * on every level.  Both give the same pivots and factors up to rounding.
+ * No reference counterpart (performance only). */
+#define TAILOR_PANEL_AUTO 0
+#define TAILOR_PANEL_STREAMING 1
+int tailor_set_panel_kernel(tailor_ctx *ctx, int32_t kernel);
+
 /* Precision of the factorization used by tailor_factor / tailor_solve.
  * FP64: complex FP64 fronts (the plain form).  MIXED: complex FP32 fronts
  * factored on FP32 MFMA, with FP64 right-hand sides, FP64 residuals and

@@ -143,6 +143,7 @@ def library() -> C.CDLL:
     lib.tailor_set_precision.argtypes = [C.c_void_p, C.c_int32]
     lib.tailor_autotune.argtypes = [C.c_void_p, C.c_double, C.c_double, P(TuneReport)]
     lib.tailor_set_gemm_config.argtypes = [C.c_void_p, C.c_int32]
+    lib.tailor_set_panel_kernel.argtypes = [C.c_void_p, C.c_int32]
     lib.tailor_profile_factor.argtypes = [C.c_void_p, C.c_double, C.c_double, P(C.c_double),
                                           P(C.c_double), P(C.c_double), P(C.c_int32)]
     lib.tailor_time_linear_solve.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double)]
@@ -264,6 +265,7 @@ class Context:
 
 
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+PANEL_AUTO, PANEL_STREAMING = 0, 1
 
 
 def _set_precision(self, precision: int) -> None:
@@ -358,6 +360,7 @@ Context.set_precision = _set_precision
 Context.profile_factor = _profile_factor
 Context.autotune = _autotune
 Context.set_gemm_config = lambda self, c: self._check(self.lib.tailor_set_gemm_config(self.ptr, int(c)))
+Context.set_panel_kernel = lambda self, k: self._check(self.lib.tailor_set_panel_kernel(self.ptr, int(k)))
 Context.linear_solve = _linear_solve
 Context.solve = _solve
 Context.eigenpairs = _eigenpairs

@@ -630,6 +630,17 @@ int tailor_set_gemm_config(tailor_ctx *ctx, int32_t config) {
   return TAILOR_OK;
 }
 
+int tailor_set_panel_kernel(tailor_ctx *ctx, int32_t kernel) {
+  if (!ctx) return TAILOR_ERR_USER_INPUT;
+  if (kernel != TAILOR_PANEL_AUTO && kernel != TAILOR_PANEL_STREAMING)
+    return fail(ctx, TAILOR_ERR_ARG_OUTOFRANGE, "unknown panel kernel");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (int st = tailor::solver_prepare(ctx)) return st;
+  ctx->solver->plan.panel_streaming = kernel == TAILOR_PANEL_STREAMING;
+  ctx->solver->factored = false;
+  return TAILOR_OK;
+}
+
 int tailor_set_precision(tailor_ctx *ctx, int32_t precision) {
   if (!ctx) return TAILOR_ERR_USER_INPUT;
   if (precision != TAILOR_PRECISION_FP64 && precision != TAILOR_PRECISION_MIXED)

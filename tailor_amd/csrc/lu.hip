@@ -264,6 +264,337 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__
   }
 }
 
+// Register-resident panel LU for fronts whose fully summed rows [k0, p) fit
+// kPanelThreads * R threads' rows.  Same result as k_panel (partial pivoting
+// over rows [j, p), exchanges across the block's columns [K0, K1)), organised
+// so that one workgroup does not stream the panel through its CU once per
+// column:
+//
+//   * the panel is taken in sub-panels of kSubW columns, held in registers
+//     (thread t owns the logical rows k0 + t + s kPanelThreads, s < R);
+//   * a sub-panel is brought up to date left-looking: its rows [k0, c0) by
+//     forward substitution with the unit lower L[k0:c0, k0:c0] (one wave per
+//     column, broadcasts by readlane), the rows below by
+//     X[r, :] -= L[r, k0:c0] U[k0:c0, :], one pass over the finished columns;
+//   * its columns are factored in registers: pivot search by shuffles and
+//     LDS, the pivot row broadcast through LDS, rank-1 updates in place;
+//   * row exchanges are recorded as a logical -> physical row map in LDS, so
+//     global memory is read and written through the map and never swapped
+//     per pivot; at the end of the panel one gather moves the exchanged rows
+//     of every block column [K0, K1) into their final (logical) positions.
+constexpr int kSubW = 4;          // sub-panel width (columns held in registers)
+constexpr int kPanelMapRows = 4096;  // rows of the LDS row map (R <= 4)
+static_assert(kPanelMapRows == 4 * kPanelThreads, "row map covers R <= 4");
+
+template <typename T> __device__ inline T readlane_t(T v, int l);
+template <> __device__ inline double readlane_t<double>(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+template <> __device__ inline float readlane_t<float>(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(kPanelThreads) void k_panel_regs(
+    const int *__restrict__ list, int nlist, const FrontDev *__restrict__ fr, int k0, int nb,
+    int K0, int NB, double tiny, double perturb, T *__restrict__ arena, int *__restrict__ ipiv,
+    int *__restrict__ nperturbed) {
+  const int f = list[blockIdx.x];
+  const FrontDev d = fr[f];
+  if (k0 >= d.p) return;
+  const int kb = min(nb, d.p - k0);
+  const int K1 = min(K0 + NB, d.p);
+  const int p = d.p;
+  const long long m = d.m;
+  T *__restrict__ Fr = arena + d.off;
+  T *__restrict__ Fi = Fr + m * m;
+  constexpr int kWaves = kPanelThreads / 64;
+  // finished columns per load batch, fewer where the registers are full
+  constexpr int kBatch = R * sizeof(T) >= 24 ? 1 : (R * sizeof(T) > 8 ? 2 : kSubW);
+  static_assert(kSubW % kBatch == 0, "batches tile the sub-panel");
+  __shared__ int phys[kPanelMapRows];                          // logical row - k0 -> physical
+  __shared__ T l11r[kLuNb][kLuNb + 1], l11i[kLuNb][kLuNb + 1];  // L[k0+i, k0+q]
+  __shared__ T ur[kLuNb][kSubW], ui[kLuNb][kSubW];              // U[k0+i, c0+u]
+  __shared__ T prow_r[kSubW], prow_i[kSubW], qrow_r[kSubW], qrow_i[kSubW];
+  __shared__ double sv[kWaves];
+  __shared__ int si[kWaves];
+  __shared__ int spiv, smoved;
+  __shared__ bool ssmall;
+  __shared__ int moved[2 * kLuNb];
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  for (int r = tid; r < p - k0; r += kPanelThreads) phys[r] = k0 + r;
+  if (tid == 0) smoved = 0;
+  __syncthreads();
+
+  T ar[R][kSubW], ai[R][kSubW];
+  for (int c0 = k0; c0 < k0 + kb; c0 += kSubW) {
+    const int w = min(kSubW, k0 + kb - c0);
+    const int c = c0 - k0;  // finished columns of the panel
+    // (a) rows [k0, c0) of the sub-panel: U = L11^-1 X
+    for (int e = tid; e < c * c; e += kPanelThreads) {
+      const int i = e % c, q = e / c;
+      const long long idx = static_cast<long long>(k0 + q) * m + phys[i];
+      l11r[i][q] = Fr[idx];
+      l11i[i][q] = Fi[idx];
+    }
+    __syncthreads();
+    if (wave < w && c > 0) {
+      const long long col = static_cast<long long>(c0 + wave) * m;
+      T xr = T(0), xi = T(0);
+      if (lane < c) {
+        xr = Fr[col + phys[lane]];
+        xi = Fi[col + phys[lane]];
+      }
+      for (int q = 0; q + 1 < c; ++q) {
+        const T br = readlane_t(xr, q), bi = readlane_t(xi, q);
+        if (lane > q && lane < c) {
+          const T lr = l11r[lane][q], li = l11i[lane][q];
+          xr -= lr * br - li * bi;
+          xi -= lr * bi + li * br;
+        }
+      }
+      if (lane < c) {
+        Fr[col + phys[lane]] = xr;
+        Fi[col + phys[lane]] = xi;
+        ur[lane][wave] = xr;
+        ui[lane][wave] = xi;
+      }
+    }
+    __syncthreads();
+    // (b) rows [c0, p): X -= L[:, k0:c0] U[k0:c0, :], into registers; the
+    // finished columns are read kBatch at a time for all of a thread's rows
+    // (c is a multiple of kSubW, hence of kBatch), so the loads of a batch
+    // are in flight together
+    long long prow_phys[R];
+    bool live[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      const int r = k0 + tid + s * kPanelThreads;
+      live[s] = r >= c0 && r < p;
+      prow_phys[s] = live[s] ? phys[r - k0] : k0;
+#pragma unroll
+      for (int u = 0; u < kSubW; ++u) {
+        ar[s][u] = T(0);
+        ai[s][u] = T(0);
+        if (live[s] && u < w) {
+          ar[s][u] = Fr[static_cast<long long>(c0 + u) * m + prow_phys[s]];
+          ai[s][u] = Fi[static_cast<long long>(c0 + u) * m + prow_phys[s]];
+        }
+      }
+    }
+    for (int t0 = 0; t0 < c; t0 += kBatch) {
+      T lr[R][kBatch], li[R][kBatch];
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+#pragma unroll
+        for (int tt = 0; tt < kBatch; ++tt) {
+          lr[s][tt] = T(0);
+          li[s][tt] = T(0);
+          if (live[s]) {
+            const long long idx = static_cast<long long>(k0 + t0 + tt) * m + prow_phys[s];
+            lr[s][tt] = Fr[idx];
+            li[s][tt] = Fi[idx];
+          }
+        }
+      }
+#pragma unroll
+      for (int tt = 0; tt < kBatch; ++tt) {
+#pragma unroll
+        for (int u = 0; u < kSubW; ++u) {
+          const T xr = ur[t0 + tt][u], xi = ui[t0 + tt][u];
+#pragma unroll
+          for (int s = 0; s < R; ++s) {
+            ar[s][u] -= lr[s][tt] * xr - li[s][tt] * xi;
+            ai[s][u] -= lr[s][tt] * xi + li[s][tt] * xr;
+          }
+        }
+      }
+    }
+    // (c) the sub-panel's columns in registers
+#pragma unroll
+    for (int u = 0; u < kSubW; ++u) {
+      if (u < w) {
+        const int j = c0 + u;
+        double best = -1.0;
+        int bi = j;
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          const int r = k0 + tid + s * kPanelThreads;
+          if (r >= j && r < p) {
+            const double a = cabs1(static_cast<double>(ar[s][u]), static_cast<double>(ai[s][u]));
+            if (a > best) {
+              best = a;
+              bi = r;
+            }
+          }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          const double ob = __shfl_xor(best, off);
+          const int oi = __shfl_xor(bi, off);
+          if (ob > best || (ob == best && oi < bi)) {
+            best = ob;
+            bi = oi;
+          }
+        }
+        if (lane == 0) {
+          sv[wave] = best;
+          si[wave] = bi;
+        }
+        __syncthreads();
+        if (wave == 0) {
+          best = lane < kWaves ? sv[lane] : -1.0;
+          bi = lane < kWaves ? si[lane] : 0x7fffffff;
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1) {
+            const double ob = __shfl_xor(best, off);
+            const int oi = __shfl_xor(bi, off);
+            if (ob > best || (ob == best && oi < bi)) {
+              best = ob;
+              bi = oi;
+            }
+          }
+          if (lane == 0) {
+            const bool small = !(best > tiny);
+            ssmall = small;
+            spiv = small ? j : bi;
+          }
+        }
+        __syncthreads();
+        const int piv = spiv;
+        const bool small = ssmall;
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          const int r = k0 + tid + s * kPanelThreads;
+          if (r == piv) {
+#pragma unroll
+            for (int q = 0; q < kSubW; ++q) {
+              prow_r[q] = ar[s][q];
+              prow_i[q] = ai[s][q];
+            }
+          }
+          if (r == j) {
+#pragma unroll
+            for (int q = 0; q < kSubW; ++q) {
+              qrow_r[q] = ar[s][q];
+              qrow_i[q] = ai[s][q];
+            }
+          }
+        }
+        if (tid == 0) {
+          ipiv[d.piv_off + j] = piv;
+          if (piv != j) {
+            const int t = phys[j - k0];
+            phys[j - k0] = phys[piv - k0];
+            phys[piv - k0] = t;
+          }
+          if (small) atomicAdd(nperturbed, 1);
+        }
+        __syncthreads();
+        T pr = prow_r[u], pi = prow_i[u];
+        if (small) {
+          // static pivot perturbation: refinement recovers the accuracy
+          pr = pr >= T(0) ? static_cast<T>(perturb) : static_cast<T>(-perturb);
+          pi = T(0);
+        }
+        const double den = static_cast<double>(pr) * pr + static_cast<double>(pi) * pi;
+        const T ir = static_cast<T>(pr / den), ii = static_cast<T>(-pi / den);
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          const int r = k0 + tid + s * kPanelThreads;
+          if (r == j) {
+#pragma unroll
+            for (int q = 0; q < kSubW; ++q) {
+              ar[s][q] = prow_r[q];
+              ai[s][q] = prow_i[q];
+            }
+            ar[s][u] = pr;
+            ai[s][u] = pi;
+          } else if (r == piv) {
+#pragma unroll
+            for (int q = 0; q < kSubW; ++q) {
+              ar[s][q] = qrow_r[q];
+              ai[s][q] = qrow_i[q];
+            }
+          }
+          if (r > j && r < p) {
+            const T xr = ar[s][u], xi = ai[s][u];
+            const T lr = xr * ir - xi * ii, li = xr * ii + xi * ir;
+            ar[s][u] = lr;
+            ai[s][u] = li;
+#pragma unroll
+            for (int q = u + 1; q < kSubW; ++q) {
+              const T yr = prow_r[q], yi = prow_i[q];
+              ar[s][q] -= lr * yr - li * yi;
+              ai[s][q] -= lr * yi + li * yr;
+            }
+          }
+        }
+      }
+    }
+    // (d) the sub-panel back to global memory through the row map
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      const int r = k0 + tid + s * kPanelThreads;
+      if (r >= c0 && r < p) {
+        const long long pr = phys[r - k0];
+#pragma unroll
+        for (int u = 0; u < kSubW; ++u) {
+          if (u < w) {
+            Fr[static_cast<long long>(c0 + u) * m + pr] = ar[s][u];
+            Fi[static_cast<long long>(c0 + u) * m + pr] = ai[s][u];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // rows whose data moved: a subset of [k0, k0+kb) and the pivot rows
+  if (tid < 2 * kb) {
+    const int cand = tid < kb ? k0 + tid : ipiv[d.piv_off + k0 + tid - kb];
+    bool take = phys[cand - k0] != cand;
+    if (tid >= kb) {
+      if (cand < k0 + kb) take = false;  // listed by the first half
+      for (int q = 0; q < tid - kb; ++q)
+        if (ipiv[d.piv_off + k0 + q] == cand) take = false;
+    }
+    if (take) moved[atomicAdd(&smoved, 1)] = cand;
+  }
+  __syncthreads();
+  // gather: column entries of the moved rows in every block column, in two
+  // halves of the columns; within a half all loads precede the stores
+  const int nm = smoved;
+  constexpr int kHalf = kLuNbOuter / 2;
+  constexpr int kGather = 2 * kLuNb * kHalf / kPanelThreads;
+  for (int h0 = K0; h0 < K1; h0 += kHalf) {
+    const int pairs = nm * min(kHalf, K1 - h0);
+    T gr[kGather], gi[kGather];
+#pragma unroll
+    for (int g = 0; g < kGather; ++g) {
+      const int e = tid + g * kPanelThreads;
+      if (e < pairs) {
+        const int i = moved[e % nm];
+        const long long idx = static_cast<long long>(h0 + e / nm) * m + phys[i - k0];
+        gr[g] = Fr[idx];
+        gi[g] = Fi[idx];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < kGather; ++g) {
+      const int e = tid + g * kPanelThreads;
+      if (e < pairs) {
+        const long long idx = static_cast<long long>(h0 + e / nm) * m + moved[e % nm];
+        Fr[idx] = gr[g];
+        Fi[idx] = gi[g];
+      }
+    }
+  }
+}
+
 // Update rows [p, m) of the panel columns: X <- X U_D^-1 with U_D the upper
 // kb x kb pivot block (row-wise forward substitution, thread per row).
 template <typename T>
@@ -839,6 +1170,41 @@ hipError_t launch_gemm(int shape, int order, long long rows, long long cols, int
   return hipGetLastError();
 }
 
+// Panel LU of one level: the register-resident kernel when every front's
+// fully summed rows fit kPanelThreads * 4 rows (pmax bounds p - k0), else,
+// or when `streaming` is set, the streaming kernel.
+template <typename T>
+hipError_t launch_panel(int pmax, bool streaming, int count, hipStream_t st, const int *list,
+                        const FrontDev *fr, int k0, int nb, int K0, int NB, double tiny,
+                        double perturb, T *arena, int *ipiv, int *nperturbed) {
+  const dim3 grid(count), block(kPanelThreads);
+  const int R = streaming ? 0x7fff : (pmax + kPanelThreads - 1) / kPanelThreads;
+  switch (R) {
+    case 0:
+    case 1:
+      hipLaunchKernelGGL((k_panel_regs<T, 1>), grid, block, 0, st, list, count, fr, k0, nb, K0,
+                         NB, tiny, perturb, arena, ipiv, nperturbed);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_panel_regs<T, 2>), grid, block, 0, st, list, count, fr, k0, nb, K0,
+                         NB, tiny, perturb, arena, ipiv, nperturbed);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_panel_regs<T, 3>), grid, block, 0, st, list, count, fr, k0, nb, K0,
+                         NB, tiny, perturb, arena, ipiv, nperturbed);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_panel_regs<T, 4>), grid, block, 0, st, list, count, fr, k0, nb, K0,
+                         NB, tiny, perturb, arena, ipiv, nperturbed);
+      break;
+    default:
+      hipLaunchKernelGGL(k_panel<T>, grid, block, 0, st, list, count, fr, k0, nb, K0, NB, tiny,
+                         perturb, arena, ipiv, nperturbed);
+      break;
+  }
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ solves
 //
 // Vectors are split complex planes in elimination (dof) order: position
@@ -1091,9 +1457,10 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
       hipError_t r;
       const int Kend = std::min(K0 + NB, L.pmax);
       for (int k0 = K0; k0 < Kend; k0 += nb) {
-        hipLaunchKernelGGL(k_panel<T>, dim3(L.count), dim3(kPanelThreads), 0, st, list,
-                           L.count, P.d_front, k0, nb, K0, NB, tiny, perturb, arena, P.d_ipiv,
-                           P.d_counters);
+        if ((r = launch_panel<T>(L.pmax, P.panel_streaming, L.count, st, list, P.d_front, k0,
+                                 nb, K0, NB, tiny, perturb, arena, P.d_ipiv,
+                                 P.d_counters)) != hipSuccess)
+          return r;
         if (L.cmax > 0)
           hipLaunchKernelGGL(k_l21<T>, dim3(cdivu(L.cmax, 256), L.count), dim3(256), 0, st,
                              list, L.count, P.d_front, k0, nb, arena);

@@ -167,6 +167,42 @@ def test_multiblock_front_solve(oracle_lib, precision):
     assert res <= (3e-13 if precision == 0 else 3e-12)
 
 
+@pytest.mark.parametrize("shape,precision", [((64, 32), 0), ((64, 32), 1), ((120, 60), 0),
+                                              ((200, 100), 0), ((260, 130), 0), ((260, 130), 1)])
+def test_panel_kernels_agree(oracle_lib, shape, precision):
+    """The register-resident panel (row map, sub-panels, final gather) and
+    the streaming panel choose the same pivots, so their factorizations give
+    the same solutions up to rounding, and both meet the unrefined solve bar.
+    The sizes reach every register variant: the largest separators have
+    800, 1500, 2500 and 3250 fully summed rows (1 to 4 rows per thread).
+    Uniform xi nodes: the equal-error node sets are packaged for 100 and 200
+    nodes only and solving for others takes minutes."""
+    c = cs.synthetic_case(*shape, omega_shift=0.02, nodes=np.linspace(-1.0, 1.0, shape[0]))
+    A, B = operators(oracle_lib, c)
+    sigma = 0.1 - 0.05j
+    K = (A - sigma * B).tocsr()
+    rng = np.random.default_rng(11)
+    b = rng.standard_normal(K.shape[0]) + 1j * rng.standard_normal(K.shape[0])
+    xs = []
+    with ta.Context(0) as ctx:
+        ctx.load_case(c)
+        ctx.assemble(diagnostics=False)
+        ctx.set_precision(precision)
+        for kernel in (ta.PANEL_AUTO, ta.PANEL_STREAMING):
+            ctx.set_panel_kernel(kernel)
+            ctx.factor(sigma)
+            x, _, _ = ctx.linear_solve(b, refine_max=0)
+            xs.append(x)
+    res = [np.linalg.norm(K @ x - b) / np.linalg.norm(b) for x in xs]
+    diff = np.linalg.norm(xs[0] - xs[1]) / np.linalg.norm(xs[1])
+    print(f"{shape} precision {precision}: residual auto {res[0]:.2e} streaming {res[1]:.2e} "
+          f"rel diff {diff:.2e}", flush=True)
+    # unrefined FP32 fronts: the residual grows with the size (1.4e-2 for
+    # both kernels at 260 x 130); the two kernels must agree far closer
+    assert max(res) <= (1e-9 if precision == 0 else 5e-2)
+    assert diff <= (1e-8 if precision == 0 else 1e-2)
+
+
 @pytest.mark.parametrize("config", range(6))
 def test_every_gemm_configuration(oracle_lib, config):
     """Each frontal-GEMM tile shape and order (the autotuner's choices) gives
