@@ -265,13 +265,13 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__
 }
 
 // Register-resident panel LU for fronts whose fully summed rows [k0, p) fit
-// kPanelThreads * R threads' rows.  Same result as k_panel (partial pivoting
+// the R rows of each of NT threads.  Same result as k_panel (partial pivoting
 // over rows [j, p), exchanges across the block's columns [K0, K1)), organised
 // so that one workgroup does not stream the panel through its CU once per
 // column:
 //
 //   * the panel is taken in sub-panels of kSubW columns, held in registers
-//     (thread t owns the logical rows k0 + t + s kPanelThreads, s < R);
+//     (thread t owns the logical rows k0 + t + s NT, s < R);
 //   * a sub-panel is brought up to date left-looking: its rows [k0, c0) by
 //     forward substitution with the unit lower L[k0:c0, k0:c0] (one wave per
 //     column, broadcasts by readlane), the rows below by
@@ -283,8 +283,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__
 //     per pivot; at the end of the panel one gather moves the exchanged rows
 //     of every block column [K0, K1) into their final (logical) positions.
 constexpr int kSubW = 4;          // sub-panel width (columns held in registers)
-constexpr int kPanelMapRows = 4096;  // rows of the LDS row map (R <= 4)
-static_assert(kPanelMapRows == 4 * kPanelThreads, "row map covers R <= 4");
+constexpr int kPanelMapRows = 4096;  // rows of the LDS row map
 
 template <typename T> __device__ inline T readlane_t(T v, int l);
 template <> __device__ inline double readlane_t<double>(double v, int l) {
@@ -297,8 +296,42 @@ template <> __device__ inline float readlane_t<float>(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-template <typename T, int R>
-__global__ __launch_bounds__(kPanelThreads) void k_panel_regs(
+// Wave-wide max / min by DPP: exchanges within quads (xor 1, xor 2), half-row
+// and row mirrors, then row broadcasts 15 and 31, after which lane 63 holds
+// the result; it is returned to every lane.  Lanes a step does not write keep
+// their own value, which max / min leave unchanged.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, ROW_MASK, 0xf, false);
+}
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_i<CTRL, ROW_MASK>(static_cast<int>(b));
+  const int hi = dpp_i<CTRL, ROW_MASK>(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+__device__ inline double wave_max(double v) {
+  v = fmax(v, dpp_d<0xB1>(v));        // quad_perm [1,0,3,2]
+  v = fmax(v, dpp_d<0x4E>(v));        // quad_perm [2,3,0,1]
+  v = fmax(v, dpp_d<0x141>(v));       // row_half_mirror
+  v = fmax(v, dpp_d<0x140>(v));       // row_mirror
+  v = fmax(v, dpp_d<0x142, 0xA>(v));  // row_bcast:15
+  v = fmax(v, dpp_d<0x143, 0xC>(v));  // row_bcast:31
+  return readlane_t(v, 63);
+}
+__device__ inline int wave_min(int v) {
+  v = min(v, dpp_i<0xB1>(v));
+  v = min(v, dpp_i<0x4E>(v));
+  v = min(v, dpp_i<0x141>(v));
+  v = min(v, dpp_i<0x140>(v));
+  v = min(v, dpp_i<0x142, 0xA>(v));
+  v = min(v, dpp_i<0x143, 0xC>(v));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+template <typename T, int R, int NT>
+__global__ __launch_bounds__(NT) void k_panel_regs(
     const int *__restrict__ list, int nlist, const FrontDev *__restrict__ fr, int k0, int nb,
     int K0, int NB, double tiny, double perturb, T *__restrict__ arena, int *__restrict__ ipiv,
     int *__restrict__ nperturbed) {
@@ -311,21 +344,26 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_regs(
   const long long m = d.m;
   T *__restrict__ Fr = arena + d.off;
   T *__restrict__ Fi = Fr + m * m;
-  constexpr int kWaves = kPanelThreads / 64;
+  constexpr int kWaves = NT / 64;
   // finished columns per load batch, fewer where the registers are full
-  constexpr int kBatch = R * sizeof(T) >= 24 ? 1 : (R * sizeof(T) > 8 ? 2 : kSubW);
+  constexpr int kBatch = NT >= 1024 ? (R * sizeof(T) >= 24 ? 1 : (R * sizeof(T) > 8 ? 2 : kSubW))
+                                    : (R * sizeof(T) > 48 ? 1 : (R * sizeof(T) > 16 ? 2 : kSubW));
+  static_assert(R * NT <= kPanelMapRows, "rows beyond the row map");
   static_assert(kSubW % kBatch == 0, "batches tile the sub-panel");
   __shared__ int phys[kPanelMapRows];                          // logical row - k0 -> physical
   __shared__ T l11r[kLuNb][kLuNb + 1], l11i[kLuNb][kLuNb + 1];  // L[k0+i, k0+q]
   __shared__ T ur[kLuNb][kSubW], ui[kLuNb][kSubW];              // U[k0+i, c0+u]
-  __shared__ T prow_r[kSubW], prow_i[kSubW], qrow_r[kSubW], qrow_i[kSubW];
-  __shared__ double sv[kWaves];
-  __shared__ int si[kWaves];
-  __shared__ int spiv, smoved;
-  __shared__ bool ssmall;
+  // per column, double-buffered by the column's parity: each wave's best
+  // candidate (magnitude, row, the row's sub-panel values) and row j
+  __shared__ double sv[2][kWaves];
+  __shared__ int si[2][kWaves];
+  __shared__ T cand_r[2][kWaves][kSubW], cand_i[2][kWaves][kSubW];
+  __shared__ T jrow_r[2][kSubW], jrow_i[2][kSubW];
+  __shared__ int spiv[kLuNb];  // pivots of the panel, stored to ipiv at the end
+  __shared__ int smoved;
   __shared__ int moved[2 * kLuNb];
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-  for (int r = tid; r < p - k0; r += kPanelThreads) phys[r] = k0 + r;
+  for (int r = tid; r < p - k0; r += NT) phys[r] = k0 + r;
   if (tid == 0) smoved = 0;
   __syncthreads();
 
@@ -334,7 +372,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_regs(
     const int w = min(kSubW, k0 + kb - c0);
     const int c = c0 - k0;  // finished columns of the panel
     // (a) rows [k0, c0) of the sub-panel: U = L11^-1 X
-    for (int e = tid; e < c * c; e += kPanelThreads) {
+    for (int e = tid; e < c * c; e += NT) {
       const int i = e % c, q = e / c;
       const long long idx = static_cast<long long>(k0 + q) * m + phys[i];
       l11r[i][q] = Fr[idx];
@@ -364,42 +402,48 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_regs(
       }
     }
     __syncthreads();
-    // (b) rows [c0, p): X -= L[:, k0:c0] U[k0:c0, :], into registers; the
+    // (b) rows [c0, p): X -= L[:, k0:c0] U[k0:c0, :], into registers.  The
     // finished columns are read kBatch at a time for all of a thread's rows
-    // (c is a multiple of kSubW, hence of kBatch), so the loads of a batch
-    // are in flight together
-    long long prow_phys[R];
-    bool live[R];
+    // (c is a multiple of kSubW, hence of kBatch), double-buffered: the next
+    // batch's loads are in flight during this batch's FMAs.  Rows outside
+    // [c0, p) load a clamped in-front row and their results are never used.
+    int prow[R];
 #pragma unroll
     for (int s = 0; s < R; ++s) {
-      const int r = k0 + tid + s * kPanelThreads;
-      live[s] = r >= c0 && r < p;
-      prow_phys[s] = live[s] ? phys[r - k0] : k0;
+      const int r = k0 + tid + s * NT;
+      prow[s] = r >= c0 && r < p ? phys[r - k0] : k0;
 #pragma unroll
       for (int u = 0; u < kSubW; ++u) {
-        ar[s][u] = T(0);
-        ai[s][u] = T(0);
-        if (live[s] && u < w) {
-          ar[s][u] = Fr[static_cast<long long>(c0 + u) * m + prow_phys[s]];
-          ai[s][u] = Fi[static_cast<long long>(c0 + u) * m + prow_phys[s]];
-        }
+        const long long cu = static_cast<long long>(c0 + min(u, w - 1)) * m;
+        ar[s][u] = Fr[cu + prow[s]];
+        ai[s][u] = Fi[cu + prow[s]];
       }
     }
+    T nr[R][kBatch], ni[R][kBatch];
+    auto load_batch = [&](int t0) {
+#pragma unroll
+      for (int tt = 0; tt < kBatch; ++tt) {
+        const T *__restrict__ colr = Fr + static_cast<long long>(k0 + t0 + tt) * m;
+        const T *__restrict__ coli = Fi + static_cast<long long>(k0 + t0 + tt) * m;
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          nr[s][tt] = colr[prow[s]];
+          ni[s][tt] = coli[prow[s]];
+        }
+      }
+    };
+    if (c > 0) load_batch(0);
     for (int t0 = 0; t0 < c; t0 += kBatch) {
       T lr[R][kBatch], li[R][kBatch];
 #pragma unroll
       for (int s = 0; s < R; ++s) {
 #pragma unroll
         for (int tt = 0; tt < kBatch; ++tt) {
-          lr[s][tt] = T(0);
-          li[s][tt] = T(0);
-          if (live[s]) {
-            const long long idx = static_cast<long long>(k0 + t0 + tt) * m + prow_phys[s];
-            lr[s][tt] = Fr[idx];
-            li[s][tt] = Fi[idx];
-          }
+          lr[s][tt] = nr[s][tt];
+          li[s][tt] = ni[s][tt];
         }
       }
+      if (t0 + kBatch < c) load_batch(t0 + kBatch);
 #pragma unroll
       for (int tt = 0; tt < kBatch; ++tt) {
 #pragma unroll
@@ -413,16 +457,22 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_regs(
         }
       }
     }
-    // (c) the sub-panel's columns in registers
+    // (c) the sub-panel's columns in registers, one barrier per column:
+    // each wave reduces its pivot candidate by DPP and publishes it with its
+    // row values, the owner of row j publishes row j; after the barrier
+    // every wave reduces the candidates itself and takes the pivot row from
+    // LDS.  The parity double buffer keeps a column's slots intact until all
+    // waves have passed the next column's barrier.
 #pragma unroll
     for (int u = 0; u < kSubW; ++u) {
       if (u < w) {
         const int j = c0 + u;
+        const int b = j & 1;
         double best = -1.0;
-        int bi = j;
+        int bi = 0x7fffffff;
 #pragma unroll
         for (int s = 0; s < R; ++s) {
-          const int r = k0 + tid + s * kPanelThreads;
+          const int r = k0 + tid + s * NT;
           if (r >= j && r < p) {
             const double a = cabs1(static_cast<double>(ar[s][u]), static_cast<double>(ai[s][u]));
             if (a > best) {
@@ -430,62 +480,49 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_regs(
               bi = r;
             }
           }
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          const double ob = __shfl_xor(best, off);
-          const int oi = __shfl_xor(bi, off);
-          if (ob > best || (ob == best && oi < bi)) {
-            best = ob;
-            bi = oi;
-          }
-        }
-        if (lane == 0) {
-          sv[wave] = best;
-          si[wave] = bi;
-        }
-        __syncthreads();
-        if (wave == 0) {
-          best = lane < kWaves ? sv[lane] : -1.0;
-          bi = lane < kWaves ? si[lane] : 0x7fffffff;
-#pragma unroll
-          for (int off = 32; off > 0; off >>= 1) {
-            const double ob = __shfl_xor(best, off);
-            const int oi = __shfl_xor(bi, off);
-            if (ob > best || (ob == best && oi < bi)) {
-              best = ob;
-              bi = oi;
-            }
-          }
-          if (lane == 0) {
-            const bool small = !(best > tiny);
-            ssmall = small;
-            spiv = small ? j : bi;
-          }
-        }
-        __syncthreads();
-        const int piv = spiv;
-        const bool small = ssmall;
-#pragma unroll
-        for (int s = 0; s < R; ++s) {
-          const int r = k0 + tid + s * kPanelThreads;
-          if (r == piv) {
-#pragma unroll
-            for (int q = 0; q < kSubW; ++q) {
-              prow_r[q] = ar[s][q];
-              prow_i[q] = ai[s][q];
-            }
-          }
           if (r == j) {
 #pragma unroll
             for (int q = 0; q < kSubW; ++q) {
-              qrow_r[q] = ar[s][q];
-              qrow_i[q] = ai[s][q];
+              jrow_r[b][q] = ar[s][q];
+              jrow_i[b][q] = ai[s][q];
             }
           }
         }
+        const double wbest = wave_max(best);
+        const int wbi = wave_min(best == wbest ? bi : 0x7fffffff);
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          const int r = k0 + tid + s * NT;
+          if (r == wbi) {
+#pragma unroll
+            for (int q = 0; q < kSubW; ++q) {
+              cand_r[b][wave][q] = ar[s][q];
+              cand_i[b][wave][q] = ai[s][q];
+            }
+          }
+        }
+        if (lane == 0) {
+          sv[b][wave] = wbest;
+          si[b][wave] = wbi;
+        }
+        __syncthreads();
+        const double gv = lane < kWaves ? sv[b][lane] : -1.0;
+        const int gi = lane < kWaves ? si[b][lane] : 0x7fffffff;
+        const double gbest = wave_max(gv);
+        const int gpiv = wave_min(gv == gbest ? gi : 0x7fffffff);
+        const bool small = !(gbest > tiny);
+        const int piv = small ? j : gpiv;
+        // the winning wave: the one lane whose candidate is the pivot row
+        const unsigned long long hit = __ballot(lane < kWaves && gi == gpiv && gv == gbest);
+        const int wwin = hit ? __ffsll(static_cast<long long>(hit)) - 1 : 0;
+        T pr_[kSubW], pi_[kSubW];
+#pragma unroll
+        for (int q = 0; q < kSubW; ++q) {
+          pr_[q] = small ? jrow_r[b][q] : cand_r[b][wwin][q];
+          pi_[q] = small ? jrow_i[b][q] : cand_i[b][wwin][q];
+        }
         if (tid == 0) {
-          ipiv[d.piv_off + j] = piv;
+          spiv[j - k0] = piv;
           if (piv != j) {
             const int t = phys[j - k0];
             phys[j - k0] = phys[piv - k0];
@@ -493,8 +530,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_regs(
           }
           if (small) atomicAdd(nperturbed, 1);
         }
-        __syncthreads();
-        T pr = prow_r[u], pi = prow_i[u];
+        T pr = pr_[u], pi = pi_[u];
         if (small) {
           // static pivot perturbation: refinement recovers the accuracy
           pr = pr >= T(0) ? static_cast<T>(perturb) : static_cast<T>(-perturb);
@@ -504,20 +540,20 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_regs(
         const T ir = static_cast<T>(pr / den), ii = static_cast<T>(-pi / den);
 #pragma unroll
         for (int s = 0; s < R; ++s) {
-          const int r = k0 + tid + s * kPanelThreads;
+          const int r = k0 + tid + s * NT;
           if (r == j) {
 #pragma unroll
             for (int q = 0; q < kSubW; ++q) {
-              ar[s][q] = prow_r[q];
-              ai[s][q] = prow_i[q];
+              ar[s][q] = pr_[q];
+              ai[s][q] = pi_[q];
             }
             ar[s][u] = pr;
             ai[s][u] = pi;
           } else if (r == piv) {
 #pragma unroll
             for (int q = 0; q < kSubW; ++q) {
-              ar[s][q] = qrow_r[q];
-              ai[s][q] = qrow_i[q];
+              ar[s][q] = jrow_r[b][q];
+              ai[s][q] = jrow_i[b][q];
             }
           }
           if (r > j && r < p) {
@@ -527,7 +563,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_regs(
             ai[s][u] = li;
 #pragma unroll
             for (int q = u + 1; q < kSubW; ++q) {
-              const T yr = prow_r[q], yi = prow_i[q];
+              const T yr = pr_[q], yi = pi_[q];
               ar[s][q] -= lr * yr - li * yi;
               ai[s][q] -= lr * yi + li * yr;
             }
@@ -535,10 +571,11 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_regs(
         }
       }
     }
+    __syncthreads();  // the row map of this sub-panel's exchanges
     // (d) the sub-panel back to global memory through the row map
 #pragma unroll
     for (int s = 0; s < R; ++s) {
-      const int r = k0 + tid + s * kPanelThreads;
+      const int r = k0 + tid + s * NT;
       if (r >= c0 && r < p) {
         const long long pr = phys[r - k0];
 #pragma unroll
@@ -553,13 +590,14 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_regs(
     __syncthreads();
   }
   // rows whose data moved: a subset of [k0, k0+kb) and the pivot rows
+  if (tid < kb) ipiv[d.piv_off + k0 + tid] = spiv[tid];
   if (tid < 2 * kb) {
-    const int cand = tid < kb ? k0 + tid : ipiv[d.piv_off + k0 + tid - kb];
+    const int cand = tid < kb ? k0 + tid : spiv[tid - kb];
     bool take = phys[cand - k0] != cand;
     if (tid >= kb) {
       if (cand < k0 + kb) take = false;  // listed by the first half
       for (int q = 0; q < tid - kb; ++q)
-        if (ipiv[d.piv_off + k0 + q] == cand) take = false;
+        if (spiv[q] == cand) take = false;
     }
     if (take) moved[atomicAdd(&smoved, 1)] = cand;
   }
@@ -568,13 +606,13 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_regs(
   // halves of the columns; within a half all loads precede the stores
   const int nm = smoved;
   constexpr int kHalf = kLuNbOuter / 2;
-  constexpr int kGather = 2 * kLuNb * kHalf / kPanelThreads;
+  constexpr int kGather = 2 * kLuNb * kHalf / NT;
   for (int h0 = K0; h0 < K1; h0 += kHalf) {
     const int pairs = nm * min(kHalf, K1 - h0);
     T gr[kGather], gi[kGather];
 #pragma unroll
     for (int g = 0; g < kGather; ++g) {
-      const int e = tid + g * kPanelThreads;
+      const int e = tid + g * NT;
       if (e < pairs) {
         const int i = moved[e % nm];
         const long long idx = static_cast<long long>(h0 + e / nm) * m + phys[i - k0];
@@ -585,7 +623,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_regs(
     __syncthreads();
 #pragma unroll
     for (int g = 0; g < kGather; ++g) {
-      const int e = tid + g * kPanelThreads;
+      const int e = tid + g * NT;
       if (e < pairs) {
         const long long idx = static_cast<long long>(h0 + e / nm) * m + moved[e % nm];
         Fr[idx] = gr[g];
@@ -1170,38 +1208,39 @@ hipError_t launch_gemm(int shape, int order, long long rows, long long cols, int
   return hipGetLastError();
 }
 
-// Panel LU of one level: the register-resident kernel when every front's
-// fully summed rows fit kPanelThreads * 4 rows (pmax bounds p - k0), else,
-// or when `streaming` is set, the streaming kernel.
+// Panel LU of one level: the register-resident kernel (kRegsThreads threads,
+// the fewest rows per thread that cover the level's largest front: pmax
+// bounds p - k0) when the fronts have at most kPanelMapRows fully summed
+// rows, else, or when `streaming` is set, the streaming kernel.
+constexpr int kRegsThreads = 512;
+
+template <typename T, int R>
+hipError_t launch_panel_regs(int rows, dim3 grid, hipStream_t st, const int *list, int count,
+                             const FrontDev *fr, int k0, int nb, int K0, int NB, double tiny,
+                             double perturb, T *arena, int *ipiv, int *nperturbed) {
+  if constexpr (R * kRegsThreads > kPanelMapRows) {
+    return hipErrorInvalidValue;  // excluded by the caller
+  } else {
+    if (rows > R)
+      return launch_panel_regs<T, R + 1>(rows, grid, st, list, count, fr, k0, nb, K0, NB, tiny,
+                                         perturb, arena, ipiv, nperturbed);
+    hipLaunchKernelGGL((k_panel_regs<T, R, kRegsThreads>), grid, dim3(kRegsThreads), 0, st, list,
+                       count, fr, k0, nb, K0, NB, tiny, perturb, arena, ipiv, nperturbed);
+    return hipGetLastError();
+  }
+}
+
 template <typename T>
 hipError_t launch_panel(int pmax, bool streaming, int count, hipStream_t st, const int *list,
                         const FrontDev *fr, int k0, int nb, int K0, int NB, double tiny,
                         double perturb, T *arena, int *ipiv, int *nperturbed) {
-  const dim3 grid(count), block(kPanelThreads);
-  const int R = streaming ? 0x7fff : (pmax + kPanelThreads - 1) / kPanelThreads;
-  switch (R) {
-    case 0:
-    case 1:
-      hipLaunchKernelGGL((k_panel_regs<T, 1>), grid, block, 0, st, list, count, fr, k0, nb, K0,
-                         NB, tiny, perturb, arena, ipiv, nperturbed);
-      break;
-    case 2:
-      hipLaunchKernelGGL((k_panel_regs<T, 2>), grid, block, 0, st, list, count, fr, k0, nb, K0,
-                         NB, tiny, perturb, arena, ipiv, nperturbed);
-      break;
-    case 3:
-      hipLaunchKernelGGL((k_panel_regs<T, 3>), grid, block, 0, st, list, count, fr, k0, nb, K0,
-                         NB, tiny, perturb, arena, ipiv, nperturbed);
-      break;
-    case 4:
-      hipLaunchKernelGGL((k_panel_regs<T, 4>), grid, block, 0, st, list, count, fr, k0, nb, K0,
-                         NB, tiny, perturb, arena, ipiv, nperturbed);
-      break;
-    default:
-      hipLaunchKernelGGL(k_panel<T>, grid, block, 0, st, list, count, fr, k0, nb, K0, NB, tiny,
-                         perturb, arena, ipiv, nperturbed);
-      break;
-  }
+  const dim3 grid(count);
+  if (!streaming && pmax <= kPanelMapRows)
+    return launch_panel_regs<T, 1>((pmax + kRegsThreads - 1) / kRegsThreads, grid, st, list,
+                                   count, fr, k0, nb, K0, NB, tiny, perturb, arena, ipiv,
+                                   nperturbed);
+  hipLaunchKernelGGL(k_panel<T>, grid, dim3(kPanelThreads), 0, st, list, count, fr, k0, nb, K0,
+                     NB, tiny, perturb, arena, ipiv, nperturbed);
   return hipGetLastError();
 }
 
