@@ -112,16 +112,22 @@ __global__ void k_extend_add(const int *__restrict__ tasks, int ntasks,
 //
 //   for each block [K0, K1):
 //     for each panel [k0, k0+kb) of the block:
-//       k_panel       LU of the panel over the fully summed rows [k0, p),
-//                     row exchanges applied across the block's columns
-//       k_l21         rows [p, m) of the panel:  X <- X U_D^-1
-//       k_trsm_lower  (mode 0) panel rows of the block columns right of it
-//       k_gemm INNER  F[k0+kb:m, k0+kb:K1] -= L_panel U_panel
-//     k_trsm_lower    (mode 1) all row exchanges of the block, then the
-//                     panel rows of the trailing F columns and of E
-//     k_gemm BLKROWS  remaining block rows of those columns, panel by panel
-//     k_gemm TRAIL    F[K1:m, K1:m] -= L21_blk U12_blk and
-//                     E[K1:p, :]    -= L21_blk E[K0:K1, :]   (K = K1 - K0)
+//       k_panel[_regs] LU of the panel over the fully summed rows [k0, p),
+//                      row exchanges applied across the block's columns
+//       k_diag_inv     L_D^-1 and U_D^-1 of the panel's pivot block
+//       k_right_apply  rows [p, m) of the panel:  X <- X U_D^-1
+//       k_left_apply   (mode 0) panel rows of the block columns right of it:
+//                      X <- L_D^-1 X
+//       k_gemm INNER   F[k0+kb:m, k0+kb:K1] -= L_panel U_panel
+//     k_block_perm,    the block's row exchanges on the trailing F columns
+//     k_apply_perm     and on E, as one permutation
+//     per panel:
+//       k_left_apply   (mode 1) panel rows of the trailing F columns and E
+//       k_gemm BLKROWS remaining block rows of those columns
+//     k_gemm TRAIL     F[K1:m, K1:m] -= L21_blk U12_blk and
+//                      E[K1:p, :]    -= L21_blk E[K0:K1, :]   (K = K1 - K0)
+//   then Z = U11^-1 panel by panel from the last: k_left_apply (mode 2,
+//   X <- U_D^-1 X on the panel rows of Z) and k_gemm Z.
 //
 // Row exchanges of a block are applied to the block's own columns at once
 // but never to earlier blocks: L11 is stored block by block in the row order
@@ -135,9 +141,9 @@ constexpr int kPanelChunk = 8;  // panel columns updated per batch of loads
 // Unblocked LU of the panel columns [k0, k0+kb) over the fully summed rows
 // [k0, p) of each front; pivot rows are chosen among those rows and the
 // exchange is applied to the columns [K0, K1) of the current block.  The
-// update rows [p, m) of the panel are finished by k_l21.  The pivot search
-// reduces within waves by shuffles; each row's update loads kPanelChunk
-// columns before storing any, so the loads overlap.
+// update rows [p, m) of the panel are finished by k_right_apply.  The pivot
+// search reduces within waves by shuffles; each row's update loads
+// kPanelChunk columns before storing any, so the loads overlap.
 template <typename T>
 __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__ list, int nlist,
                                                          const FrontDev *__restrict__ fr, int k0,
@@ -633,212 +639,6 @@ __global__ __launch_bounds__(NT) void k_panel_regs(
   }
 }
 
-// Update rows [p, m) of the panel columns: X <- X U_D^-1 with U_D the upper
-// kb x kb pivot block (row-wise forward substitution, thread per row).
-template <typename T>
-__global__ __launch_bounds__(256) void k_l21(const int *__restrict__ list, int nlist,
-                                             const FrontDev *__restrict__ fr, int k0, int nb,
-                                             T *__restrict__ arena) {
-  const int f = list[blockIdx.y];
-  const FrontDev d = fr[f];
-  if (k0 >= d.p || d.c == 0) return;
-  const int kb = min(nb, d.p - k0);
-  const long long mm = static_cast<long long>(d.m) * d.m;
-  T *Fr = arena + d.off, *Fi = Fr + mm;
-  __shared__ T Ur[kLuNb * kLuNb], Ui[kLuNb * kLuNb], Dr[kLuNb], Di[kLuNb];
-  for (int k = threadIdx.x; k < kb * kb; k += blockDim.x) {
-    const int r = k % kb, s = k / kb;
-    const long long idx = static_cast<long long>(k0 + s) * d.m + k0 + r;
-    Ur[s * kLuNb + r] = Fr[idx];
-    Ui[s * kLuNb + r] = Fi[idx];
-  }
-  __syncthreads();
-  if (threadIdx.x < kb) {
-    const double pr = Ur[threadIdx.x * kLuNb + threadIdx.x], pi = Ui[threadIdx.x * kLuNb + threadIdx.x];
-    const double den = pr * pr + pi * pi;
-    Dr[threadIdx.x] = static_cast<T>(pr / den);
-    Di[threadIdx.x] = static_cast<T>(-pi / den);
-  }
-  __syncthreads();
-  const int row = d.p + blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= d.m) return;
-  T xr[kLuNb], xi[kLuNb];
-#pragma unroll
-  for (int j = 0; j < kLuNb; ++j) {
-    if (j < kb) {
-      const long long idx = static_cast<long long>(k0 + j) * d.m + row;
-      T ar = Fr[idx], ai = Fi[idx];
-#pragma unroll
-      for (int i = 0; i < kLuNb; ++i) {
-        if (i < j) {
-          const T ur = Ur[j * kLuNb + i], ui = Ui[j * kLuNb + i];
-          ar -= xr[i] * ur - xi[i] * ui;
-          ai -= xr[i] * ui + xi[i] * ur;
-        }
-      }
-      xr[j] = ar * Dr[j] - ai * Di[j];
-      xi[j] = ar * Di[j] + ai * Dr[j];
-      Fr[idx] = xr[j];
-      Fi[idx] = xi[j];
-    }
-  }
-}
-
-// Panel rows [k0, k0+kb) of a set of columns: X <- L_D^-1 X with L_D the
-// unit lower kb x kb pivot block.  Thread per column.
-//   mode 0: F columns [k0+kb, K1) of the current block (exchanges already
-//           applied by k_panel);
-//   mode 1: F columns [K1, m) and all p columns of E; with `swaps`, the
-//           row exchanges of rows [K0, K1) are applied first, in order.
-template <typename T>
-__global__ __launch_bounds__(256) void k_trsm_lower(const int *__restrict__ list, int nlist,
-                                                    const FrontDev *__restrict__ fr, int k0, int nb,
-                                                    int K0, int NB, int mode, int swaps,
-                                                    const int *__restrict__ ipiv,
-                                                    T *__restrict__ arena) {
-  const int f = list[blockIdx.y];
-  const FrontDev d = fr[f];
-  if (k0 >= d.p) return;
-  const int kb = min(nb, d.p - k0);
-  const int K1 = min(K0 + NB, d.p);
-  const long long mm = static_cast<long long>(d.m) * d.m;
-  T *Fr = arena + d.off, *Fi = Fr + mm;
-  T *Er = Fi + mm, *Ei = Er + static_cast<long long>(d.p) * d.p;
-  __shared__ T Lr[kLuNb * kLuNb], Li[kLuNb * kLuNb];
-  __shared__ int pv[kLuNbOuter];
-  for (int k = threadIdx.x; k < kb * kb; k += blockDim.x) {
-    const int r = k % kb, s = k / kb;
-    const long long idx = static_cast<long long>(k0 + s) * d.m + k0 + r;
-    Lr[s * kLuNb + r] = Fr[idx];
-    Li[s * kLuNb + r] = Fi[idx];
-  }
-  if (swaps && threadIdx.x < K1 - K0) pv[threadIdx.x] = ipiv[d.piv_off + K0 + threadIdx.x];
-  __syncthreads();
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  T *Xr, *Xi;
-  if (mode == 0) {
-    if (col >= K1 - (k0 + kb)) return;
-    const long long base = static_cast<long long>(k0 + kb + col) * d.m;
-    Xr = Fr + base;
-    Xi = Fi + base;
-  } else {
-    const int nF = d.m - K1;
-    if (col >= nF + d.p) return;
-    if (col < nF) {
-      const long long base = static_cast<long long>(K1 + col) * d.m;
-      Xr = Fr + base;
-      Xi = Fi + base;
-    } else {
-      const long long base = static_cast<long long>(col - nF) * d.p;
-      Xr = Er + base;
-      Xi = Ei + base;
-    }
-    if (swaps) {
-      for (int r = K0; r < K1; ++r) {
-        const int pr = pv[r - K0];
-        if (pr != r) {
-          T t = Xr[r]; Xr[r] = Xr[pr]; Xr[pr] = t;
-          t = Xi[r]; Xi[r] = Xi[pr]; Xi[pr] = t;
-        }
-      }
-    }
-  }
-  Xr += k0;
-  Xi += k0;
-  T xr[kLuNb], xi[kLuNb];
-#pragma unroll
-  for (int r = 0; r < kLuNb; ++r) {
-    if (r < kb) {
-      xr[r] = Xr[r];
-      xi[r] = Xi[r];
-    }
-  }
-#pragma unroll
-  for (int r = 1; r < kLuNb; ++r) {
-    if (r < kb) {
-      T ar = xr[r], ai = xi[r];
-#pragma unroll
-      for (int s = 0; s < kLuNb; ++s) {
-        if (s < r) {
-          const T lr = Lr[s * kLuNb + r], li = Li[s * kLuNb + r];
-          ar -= lr * xr[s] - li * xi[s];
-          ai -= lr * xi[s] + li * xr[s];
-        }
-      }
-      xr[r] = ar;
-      xi[r] = ai;
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < kLuNb; ++r) {
-    if (r < kb) {
-      Xr[r] = xr[r];
-      Xi[r] = xi[r];
-    }
-  }
-}
-
-// Rows [k0, k0+kb) of Z columns [k0, p): X <- U_D^-1 X with U_D the upper
-// (non-unit) kb x kb pivot block.
-template <typename T>
-__global__ __launch_bounds__(256) void k_trsm_upper(const int *__restrict__ list, int nlist,
-                                                    const FrontDev *__restrict__ fr, int k0, int nb,
-                                                    T *__restrict__ arena) {
-  const int f = list[blockIdx.y];
-  const FrontDev d = fr[f];
-  if (k0 >= d.p) return;
-  const int kb = min(nb, d.p - k0);
-  const long long mm = static_cast<long long>(d.m) * d.m;
-  const T *Fr = arena + d.off, *Fi = Fr + mm;
-  T *Zr = arena + d.off + 2 * mm + 2LL * d.p * d.p;
-  T *Zi = Zr + static_cast<long long>(d.p) * d.p;
-  __shared__ T Ur[kLuNb * kLuNb], Ui[kLuNb * kLuNb];
-  for (int k = threadIdx.x; k < kb * kb; k += blockDim.x) {
-    const int r = k % kb, s = k / kb;
-    const long long idx = static_cast<long long>(k0 + s) * d.m + k0 + r;
-    Ur[s * kLuNb + r] = Fr[idx];
-    Ui[s * kLuNb + r] = Fi[idx];
-  }
-  __syncthreads();
-  const int col = k0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= d.p) return;
-  T *Xr = Zr + static_cast<long long>(col) * d.p + k0;
-  T *Xi = Zi + static_cast<long long>(col) * d.p + k0;
-  T xr[kLuNb], xi[kLuNb];
-#pragma unroll
-  for (int r = 0; r < kLuNb; ++r) {
-    if (r < kb) {
-      xr[r] = Xr[r];
-      xi[r] = Xi[r];
-    }
-  }
-#pragma unroll
-  for (int r = kLuNb - 1; r >= 0; --r) {
-    if (r < kb) {
-      T ar = xr[r], ai = xi[r];
-#pragma unroll
-      for (int s = 0; s < kLuNb; ++s) {
-        if (s > r && s < kb) {
-          const T ur = Ur[s * kLuNb + r], ui = Ui[s * kLuNb + r];
-          ar -= ur * xr[s] - ui * xi[s];
-          ai -= ur * xi[s] + ui * xr[s];
-        }
-      }
-      const T dr = Ur[r * kLuNb + r], di = Ui[r * kLuNb + r];
-      const T den = dr * dr + di * di;
-      xr[r] = (ar * dr + ai * di) / den;
-      xi[r] = (ai * dr - ar * di) / den;
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < kLuNb; ++r) {
-    if (r < kb) {
-      Xr[r] = xr[r];
-      Xi[r] = xi[r];
-    }
-  }
-}
-
 // ------------------------------------------------------- MFMA complex GEMM
 //
 // C -= A B on split (re, im) column-major planes, batched over the fronts
@@ -869,6 +669,480 @@ template <> struct Mfma<float> {
   }
   static __device__ inline int ocol(int l, int v) { return 4 * (l / 16) + v; }
 };
+
+// ------------------------------------------- inverted pivot blocks (MFMA)
+//
+// The triangular solves with a panel's kb x kb pivot block are applied as
+// multiplications by its inverse.  After the panel, k_diag_inv writes
+// L_D^-1 (unit lower) and U_D^-1 (upper), zero-padded to kLuNb x kLuNb, to
+// the panel's slot of the level's inverse workspace (slot dinv_off +
+// k0 / kLuNb of the front: planes Lr, Li, Ur, Ui, column-major); the apply
+// kernels below multiply by them on MFMA, in place.
+constexpr int kInvSlot = kLuInvSlot;  // elements per panel slot
+
+template <typename T>
+__device__ inline T *inv_slot(T *dinv, const FrontDev &d, int k0) {
+  return dinv + static_cast<long long>(d.dinv_off + k0 / kLuNb) * kInvSlot;
+}
+
+// One front per workgroup: wave 0 computes L_D^-1 (lane l: column l by
+// forward substitution), wave 1 U_D^-1 (lane l: column l by back
+// substitution); lanes 32..63 only help load the block.
+template <typename T>
+__global__ __launch_bounds__(128) void k_diag_inv(const int *__restrict__ list, int nlist,
+                                                  const FrontDev *__restrict__ fr, int k0, int nb,
+                                                  const T *__restrict__ arena,
+                                                  T *__restrict__ dinv) {
+  const FrontDev d = fr[list[blockIdx.x]];
+  if (k0 >= d.p) return;
+  const int kb = min(nb, d.p - k0);
+  const long long m = d.m;
+  const T *Fr = arena + d.off, *Fi = Fr + m * m;
+  __shared__ T Dr[kLuNb][kLuNb + 1], Di[kLuNb][kLuNb + 1];  // D[i][q] = F[k0+i, k0+q]
+  for (int e = threadIdx.x; e < kLuNb * kLuNb; e += blockDim.x) {
+    const int i = e % kLuNb, q = e / kLuNb;
+    T vr = T(0), vi = T(0);
+    if (i < kb && q < kb) {
+      const long long idx = static_cast<long long>(k0 + q) * m + k0 + i;
+      vr = Fr[idx];
+      vi = Fi[idx];
+    }
+    Dr[i][q] = vr;
+    Di[i][q] = vi;
+  }
+  __syncthreads();
+  const int l = threadIdx.x % 64, wave = threadIdx.x / 64;
+  if (l >= kLuNb) return;  // no barrier below
+  T *slot = inv_slot(dinv, d, k0);
+  T xr[kLuNb], xi[kLuNb];
+  if (wave == 0) {
+    // L x = e_l with unit diagonal: x_i = [i == l] - sum_{q < i} L[i][q] x_q
+#pragma unroll
+    for (int i = 0; i < kLuNb; ++i) {
+      T ar = i == l ? T(1) : T(0), ai = T(0);
+#pragma unroll
+      for (int q = 0; q < i; ++q) {
+        const T lr = Dr[i][q], li = Di[i][q];
+        ar -= lr * xr[q] - li * xi[q];
+        ai -= lr * xi[q] + li * xr[q];
+      }
+      const bool in = i < kb && l < kb;
+      xr[i] = in ? ar : T(0);
+      xi[i] = in ? ai : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < kLuNb; ++i) {
+      slot[l * kLuNb + i] = xr[i];
+      slot[kLuNb * kLuNb + l * kLuNb + i] = xi[i];
+    }
+  } else {
+    // U x = e_l: x_i = ([i == l] - sum_{q > i} U[i][q] x_q) / U[i][i]
+#pragma unroll
+    for (int i = kLuNb - 1; i >= 0; --i) {
+      T ar = i == l ? T(1) : T(0), ai = T(0);
+#pragma unroll
+      for (int q = i + 1; q < kLuNb; ++q) {
+        const T ur = Dr[i][q], ui = Di[i][q];
+        ar -= ur * xr[q] - ui * xi[q];
+        ai -= ur * xi[q] + ui * xr[q];
+      }
+      xr[i] = T(0);
+      xi[i] = T(0);
+      if (i < kb && l < kb) {
+        const double dr = Dr[i][i], di = Di[i][i];
+        const double den = dr * dr + di * di;
+        xr[i] = static_cast<T>((ar * dr + ai * di) / den);
+        xi[i] = static_cast<T>((ai * dr - ar * di) / den);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kLuNb; ++i) {
+      slot[2 * kLuNb * kLuNb + l * kLuNb + i] = xr[i];
+      slot[3 * kLuNb * kLuNb + l * kLuNb + i] = xi[i];
+    }
+  }
+}
+
+// X[k0:k0+kb, cols] <- M X (M kb x kb from the panel's slot):
+//   mode 0: F columns [k0+kb, K1) of the current block, M = L_D^-1;
+//   mode 1: F columns [K1, m) and all p columns of E, M = L_D^-1;
+//   mode 2: Z columns [k0, p), M = U_D^-1.
+// Four waves per workgroup, 16 columns per wave.  The MFMAs compute
+// (M X)^T = X^T M^T: lane l supplies X[4s + l/16][l%16] and
+// M[16 ib + l%16][4s + l/16], and holds the results of rows 16 ib + l%16.
+// A wave reads all kb rows of its columns before storing any (the MFMA
+// chain needs every k), so the update is in place.
+template <typename T>
+__global__ __launch_bounds__(256) void k_left_apply(const int *__restrict__ list, int nlist,
+                                                    const FrontDev *__restrict__ fr, int k0, int nb,
+                                                    int K0, int NB, int mode,
+                                                    const T *__restrict__ dinv,
+                                                    T *__restrict__ arena) {
+  using acc_t = typename Mfma<T>::acc_t;
+  const FrontDev d = fr[list[blockIdx.y]];
+  if (k0 >= d.p) return;
+  const int kb = min(nb, d.p - k0);
+  const int K1 = min(K0 + NB, d.p);
+  const long long m = d.m, p = d.p, mm = m * m, pp = p * p;
+  T *Fr = arena + d.off, *Fi = Fr + mm;
+  T *Er = Fi + mm, *Ei = Er + pp;
+  T *Zr = Ei + pp, *Zi = Zr + pp;
+  const int nF = d.m - K1;
+  const int ncols = mode == 0 ? K1 - (k0 + kb) : mode == 1 ? nF + d.p : d.p - k0;
+  const int col0 = blockIdx.x * 64;
+  if (col0 >= ncols) return;
+  __shared__ T Mr[kLuNb][kLuNb + 1], Mi[kLuNb][kLuNb + 1];
+  const T *slot = inv_slot(dinv, d, k0) + (mode == 2 ? 2 * kLuNb * kLuNb : 0);
+  for (int e = threadIdx.x; e < kLuNb * kLuNb; e += blockDim.x) {
+    Mr[e % kLuNb][e / kLuNb] = slot[e];
+    Mi[e % kLuNb][e / kLuNb] = slot[kLuNb * kLuNb + e];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  // row k0 of column j of the region, as (re, im) plane offsets
+  auto column = [&](int j, T *&xr, T *&xi) {
+    long long base;
+    if (mode == 0) {
+      base = static_cast<long long>(k0 + kb + j) * m + k0;
+      xr = Fr + base;
+      xi = Fi + base;
+    } else if (mode == 1 && j < nF) {
+      base = static_cast<long long>(K1 + j) * m + k0;
+      xr = Fr + base;
+      xi = Fi + base;
+    } else if (mode == 1) {
+      base = static_cast<long long>(j - nF) * p + k0;
+      xr = Er + base;
+      xi = Ei + base;
+    } else {
+      base = static_cast<long long>(k0 + j) * p + k0;
+      xr = Zr + base;
+      xi = Zi + base;
+    }
+  };
+  const int jl = col0 + wave * 16 + lane % 16;  // operand column of this lane
+  T xr[kLuNb / 4], xi[kLuNb / 4];
+  {
+    T *cr, *ci;
+    column(min(jl, ncols - 1), cr, ci);
+#pragma unroll
+    for (int s = 0; s < kLuNb / 4; ++s) {
+      const int k = 4 * s + lane / 16;
+      const bool ok = jl < ncols && k < kb;
+      const int kk = min(k, kb - 1);  // clamped: addresses stay in the column's kb rows
+      xr[s] = ok ? cr[kk] : T(0);
+      xi[s] = ok ? ci[kk] : T(0);
+    }
+  }
+  acc_t cr_[2], ci_[2];
+#pragma unroll
+  for (int ib = 0; ib < 2; ++ib) {
+    cr_[ib] = acc_t{0, 0, 0, 0};
+    ci_[ib] = acc_t{0, 0, 0, 0};
+  }
+#pragma unroll
+  for (int s = 0; s < kLuNb / 4; ++s) {
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib) {
+      const T mr = Mr[16 * ib + lane % 16][4 * s + lane / 16];
+      const T mi = Mi[16 * ib + lane % 16][4 * s + lane / 16];
+      cr_[ib] = Mfma<T>::mma(xr[s], mr, cr_[ib]);
+      cr_[ib] = Mfma<T>::mma(-xi[s], mi, cr_[ib]);
+      ci_[ib] = Mfma<T>::mma(xi[s], mr, ci_[ib]);
+      ci_[ib] = Mfma<T>::mma(xr[s], mi, ci_[ib]);
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int j = col0 + wave * 16 + Mfma<T>::ocol(lane, v);
+    if (j >= ncols) continue;
+    T *cr, *ci;
+    column(j, cr, ci);
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib) {
+      const int i = 16 * ib + lane % 16;
+      if (i < kb) {
+        cr[i] = cr_[ib][v];
+        ci[i] = ci_[ib][v];
+      }
+    }
+  }
+}
+
+// Rows [p, m) of the panel columns: X <- X U_D^-1.  Four waves per
+// workgroup, 32 rows x 32 columns per wave; the MFMAs compute
+// (X U)^T = U^T X^T: lane l supplies U[4s + l/16][16 jb + l%16] and
+// X[16 rb + l%16][4s + l/16], and holds the results of rows 16 rb + l%16.
+template <typename T>
+__global__ __launch_bounds__(256) void k_right_apply(const int *__restrict__ list, int nlist,
+                                                     const FrontDev *__restrict__ fr, int k0,
+                                                     int nb, const T *__restrict__ dinv,
+                                                     T *__restrict__ arena) {
+  using acc_t = typename Mfma<T>::acc_t;
+  const FrontDev d = fr[list[blockIdx.y]];
+  if (k0 >= d.p || d.c == 0) return;
+  const int kb = min(nb, d.p - k0);
+  const int row0 = d.p + blockIdx.x * 128;
+  if (row0 >= d.m) return;
+  const long long m = d.m;
+  T *Fr = arena + d.off, *Fi = Fr + m * m;
+  __shared__ T Ur[kLuNb][kLuNb + 1], Ui[kLuNb][kLuNb + 1];  // U[k][j]
+  const T *slot = inv_slot(dinv, d, k0) + 2 * kLuNb * kLuNb;
+  for (int e = threadIdx.x; e < kLuNb * kLuNb; e += blockDim.x) {
+    Ur[e % kLuNb][e / kLuNb] = slot[e];
+    Ui[e % kLuNb][e / kLuNb] = slot[kLuNb * kLuNb + e];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int rw = row0 + wave * 32;
+  T xr[2][kLuNb / 4], xi[2][kLuNb / 4];
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb) {
+    const int r = rw + 16 * rb + lane % 16;
+    const long long rr = min(r, d.m - 1);
+#pragma unroll
+    for (int s = 0; s < kLuNb / 4; ++s) {
+      const int k = 4 * s + lane / 16;
+      const long long idx = static_cast<long long>(k0 + min(k, kb - 1)) * m + rr;
+      const bool ok = r < d.m && k < kb;
+      xr[rb][s] = ok ? Fr[idx] : T(0);
+      xi[rb][s] = ok ? Fi[idx] : T(0);
+    }
+  }
+  acc_t cr_[2][2], ci_[2][2];
+#pragma unroll
+  for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+      cr_[jb][rb] = acc_t{0, 0, 0, 0};
+      ci_[jb][rb] = acc_t{0, 0, 0, 0};
+    }
+#pragma unroll
+  for (int s = 0; s < kLuNb / 4; ++s) {
+#pragma unroll
+    for (int jb = 0; jb < 2; ++jb) {
+      const T ur = Ur[4 * s + lane / 16][16 * jb + lane % 16];
+      const T ui = Ui[4 * s + lane / 16][16 * jb + lane % 16];
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) {
+        cr_[jb][rb] = Mfma<T>::mma(ur, xr[rb][s], cr_[jb][rb]);
+        cr_[jb][rb] = Mfma<T>::mma(-ui, xi[rb][s], cr_[jb][rb]);
+        ci_[jb][rb] = Mfma<T>::mma(ui, xr[rb][s], ci_[jb][rb]);
+        ci_[jb][rb] = Mfma<T>::mma(ur, xi[rb][s], ci_[jb][rb]);
+      }
+    }
+  }
+#pragma unroll
+  for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+      const int r = rw + 16 * rb + lane % 16;
+      if (r >= d.m) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int j = 16 * jb + Mfma<T>::ocol(lane, v);
+        if (j < kb) {
+          const long long idx = static_cast<long long>(k0 + j) * m + r;
+          Fr[idx] = cr_[jb][rb][v];
+          Fi[idx] = ci_[jb][rb][v];
+        }
+      }
+    }
+}
+
+// Rows [k0, k0+kb) of Z columns [k0, p): X <- U_D^-1 X by back
+// substitution (Z = U11^-1 is used by the solves as an explicit inverse, so
+// its blocks are built by substitution rather than by multiplying with the
+// inverted pivot block).  Two columns per wave: lanes 32 h + i hold row i of
+// column h; step r finalises x_r with the reciprocal of U[r][r] (the
+// diagonal of U_D^-1 in the panel's slot), broadcasts it by readlane, and the
+// rows above subtract U[i][r] x_r.
+template <typename T>
+__global__ __launch_bounds__(256) void k_upper_solve(const int *__restrict__ list, int nlist,
+                                                     const FrontDev *__restrict__ fr, int k0,
+                                                     int nb, const T *__restrict__ dinv,
+                                                     T *__restrict__ arena) {
+  const FrontDev d = fr[list[blockIdx.y]];
+  if (k0 >= d.p) return;
+  const int kb = min(nb, d.p - k0);
+  const long long m = d.m, p = d.p;
+  const T *Fr = arena + d.off, *Fi = Fr + m * m;
+  T *Zr = arena + d.off + 2 * m * m + 2 * p * p, *Zi = Zr + p * p;
+  const int col0 = k0 + blockIdx.x * 8;  // 8 columns per workgroup
+  if (col0 >= d.p) return;
+  __shared__ T Ur[kLuNb][kLuNb + 1], Ui[kLuNb][kLuNb + 1];  // U[i][q] = F[k0+i, k0+q]
+  __shared__ T Dr[kLuNb], Di[kLuNb];                         // 1 / U[i][i]
+  for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
+    const int i = e % kb, q = e / kb;
+    const long long idx = static_cast<long long>(k0 + q) * m + k0 + i;
+    Ur[i][q] = Fr[idx];
+    Ui[i][q] = Fi[idx];
+  }
+  const T *slot = inv_slot(dinv, d, k0);
+  if (threadIdx.x < kb) {
+    Dr[threadIdx.x] = slot[2 * kLuNb * kLuNb + threadIdx.x * (kLuNb + 1)];
+    Di[threadIdx.x] = slot[3 * kLuNb * kLuNb + threadIdx.x * (kLuNb + 1)];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x % 64, h = lane / 32, i = lane % 32;
+  const int col = col0 + 2 * (threadIdx.x / 64) + h;
+  const bool live = col < d.p && i < kb;
+  T *xr_p = Zr + static_cast<long long>(min(col, d.p - 1)) * p + k0;
+  T *xi_p = Zi + static_cast<long long>(min(col, d.p - 1)) * p + k0;
+  T xr = T(0), xi = T(0);
+  if (live) {
+    xr = xr_p[i];
+    xi = xi_p[i];
+  }
+  for (int r = kb - 1; r >= 0; --r) {
+    if (i == r) {
+      const T dr = Dr[r], di = Di[r];
+      const T yr = xr * dr - xi * di, yi = xr * di + xi * dr;
+      xr = yr;
+      xi = yi;
+    }
+    const T b0r = readlane_t(xr, r), b0i = readlane_t(xi, r);
+    const T b1r = readlane_t(xr, 32 + r), b1i = readlane_t(xi, 32 + r);
+    const T br = h ? b1r : b0r, bi = h ? b1i : b0i;
+    if (i < r) {
+      const T ur = Ur[i][r], ui = Ui[i][r];
+      xr -= ur * br - ui * bi;
+      xi -= ur * bi + ui * br;
+    }
+  }
+  if (live) {
+    xr_p[i] = xr;
+    xi_p[i] = xi;
+  }
+}
+
+// ---------------------------------------------- row exchanges of a block
+//
+// The exchanges of block [K0, K1) (ipiv, in order) act on the rows [K0, p)
+// of the trailing F columns and of E as one permutation.  k_block_perm
+// composes them in an LDS row map and lists the rows that move (at most
+// 2 (K1 - K0)) with the row each takes its values from; k_apply_perm moves
+// them, a wave per column.  Fronts with more than kPanelMapRows rows below
+// K0 take the exchanges one by one in k_block_swaps instead.
+constexpr int kPermStride = kLuPermStride;  // count, destinations, sources
+
+__global__ __launch_bounds__(256) void k_block_perm(const int *__restrict__ list, int nlist,
+                                                    const FrontDev *__restrict__ fr, int K0, int NB,
+                                                    const int *__restrict__ ipiv,
+                                                    int *__restrict__ bperm) {
+  const FrontDev d = fr[list[blockIdx.x]];
+  int *out = bperm + static_cast<long long>(blockIdx.x) * kPermStride;
+  const int n = d.p - K0;
+  if (n <= 0 || n > kPanelMapRows) {
+    if (threadIdx.x == 0) out[0] = 0;
+    return;
+  }
+  const int K1 = min(K0 + NB, d.p);
+  __shared__ int map[kPanelMapRows];  // row K0 + i holds the values of row K0 + map[i]
+  __shared__ int pv[kLuNbOuter];
+  __shared__ int cnt;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) map[i] = i;
+  if (threadIdx.x < K1 - K0) pv[threadIdx.x] = ipiv[d.piv_off + K0 + threadIdx.x] - K0;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int r = 0; r < K1 - K0; ++r) {
+      const int q = pv[r];
+      const int t = map[r];
+      map[r] = map[q];
+      map[q] = t;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    if (map[i] != i) {
+      const int k = atomicAdd(&cnt, 1);
+      out[1 + k] = K0 + i;
+      out[1 + 2 * kLuNbOuter + k] = K0 + map[i];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = cnt;
+}
+
+template <typename T>
+__device__ inline void trailing_column(const FrontDev &d, int K1, int col, T *arena, T *&xr,
+                                       T *&xi) {
+  const long long m = d.m, p = d.p, mm = m * m;
+  const int nF = d.m - K1;
+  if (col < nF) {
+    xr = arena + d.off + static_cast<long long>(K1 + col) * m;
+    xi = xr + mm;
+  } else {
+    xr = arena + d.off + 2 * mm + static_cast<long long>(col - nF) * p;
+    xi = xr + p * p;
+  }
+}
+
+// The listed row moves on F columns [K1, m) and E: a wave per column, lanes
+// over the moved rows; all of a column's loads complete before its stores.
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_perm(const int *__restrict__ list, int nlist,
+                                                    const FrontDev *__restrict__ fr, int K0, int NB,
+                                                    const int *__restrict__ bperm,
+                                                    T *__restrict__ arena) {
+  const FrontDev d = fr[list[blockIdx.y]];
+  if (K0 >= d.p) return;
+  const int K1 = min(K0 + NB, d.p);
+  const int *pl = bperm + static_cast<long long>(blockIdx.y) * kPermStride;
+  const int n = pl[0];
+  const int col = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (n == 0 || col >= d.m - K1 + d.p) return;
+  T *Xr, *Xi;
+  trailing_column(d, K1, col, arena, Xr, Xi);
+  constexpr int kPer = 2 * kLuNbOuter / 64;
+  T vr[kPer], vi[kPer];
+#pragma unroll
+  for (int g = 0; g < kPer; ++g) {
+    const int e = lane + 64 * g;
+    if (e < n) {
+      const int src = pl[1 + 2 * kLuNbOuter + e];
+      vr[g] = Xr[src];
+      vi[g] = Xi[src];
+    }
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // every source read before any write
+#pragma unroll
+  for (int g = 0; g < kPer; ++g) {
+    const int e = lane + 64 * g;
+    if (e < n) {
+      const int dst = pl[1 + e];
+      Xr[dst] = vr[g];
+      Xi[dst] = vi[g];
+    }
+  }
+}
+
+// Fallback for fronts whose rows [K0, p) exceed the row map: the exchanges
+// of the block applied in order, a thread per trailing column.
+template <typename T>
+__global__ __launch_bounds__(256) void k_block_swaps(const int *__restrict__ list, int nlist,
+                                                     const FrontDev *__restrict__ fr, int K0, int NB,
+                                                     const int *__restrict__ ipiv,
+                                                     T *__restrict__ arena) {
+  const FrontDev d = fr[list[blockIdx.y]];
+  if (K0 >= d.p || d.p - K0 <= kPanelMapRows) return;
+  const int K1 = min(K0 + NB, d.p);
+  __shared__ int pv[kLuNbOuter];
+  if (threadIdx.x < K1 - K0) pv[threadIdx.x] = ipiv[d.piv_off + K0 + threadIdx.x];
+  __syncthreads();
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= d.m - K1 + d.p) return;
+  T *Xr, *Xi;
+  trailing_column(d, K1, col, arena, Xr, Xi);
+  for (int r = K0; r < K1; ++r) {
+    const int q = pv[r - K0];
+    if (q != r) {
+      T t = Xr[r]; Xr[r] = Xr[q]; Xr[q] = t;
+      t = Xi[r]; Xi[r] = Xi[q]; Xi[q] = t;
+    }
+  }
+}
 
 // TRAIL = TRAILHEAD (the next block's fully summed columns of F) + TRAILREST
 // (the remaining columns of F and all of E); the split lets the next
@@ -1452,6 +1726,7 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, s, reinterpret_cast<double *>(P.d_counters), 1LL, 0.0);
   const int nb = P.nb, NB = kLuNbOuter;
+  T *dinv = reinterpret_cast<T *>(P.dinv);
   for (size_t lev = 0; lev < P.levels.size(); ++lev) {
     const LevelPlan &L = P.levels[lev];
     for (int slot = 0; slot < 2; ++slot) {
@@ -1500,12 +1775,14 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
                                  nb, K0, NB, tiny, perturb, arena, P.d_ipiv,
                                  P.d_counters)) != hipSuccess)
           return r;
+        hipLaunchKernelGGL(k_diag_inv<T>, dim3(L.count), dim3(128), 0, st, list, L.count,
+                           P.d_front, k0, nb, arena, dinv);
         if (L.cmax > 0)
-          hipLaunchKernelGGL(k_l21<T>, dim3(cdivu(L.cmax, 256), L.count), dim3(256), 0, st,
-                             list, L.count, P.d_front, k0, nb, arena);
+          hipLaunchKernelGGL(k_right_apply<T>, dim3(cdivu(L.cmax, 128), L.count), dim3(256), 0,
+                             st, list, L.count, P.d_front, k0, nb, dinv, arena);
         if (k0 + nb < Kend) {
-          hipLaunchKernelGGL(k_trsm_lower<T>, dim3(1, L.count), dim3(256), 0, st, list,
-                             L.count, P.d_front, k0, nb, K0, NB, 0, 0, P.d_ipiv, arena);
+          hipLaunchKernelGGL(k_left_apply<T>, dim3(cdivu(NB - nb, 64), L.count), dim3(256), 0,
+                             st, list, L.count, P.d_front, k0, nb, K0, NB, 0, dinv, arena);
           if ((r = gemm(L.mmax, NB - nb, kGemmInner, k0, K0, st)) != hipSuccess) return r;
         }
         if ((r = hipGetLastError()) != hipSuccess) return r;
@@ -1523,9 +1800,20 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
     for (int K0 = 0; K0 < L.pmax; K0 += NB) {
       const int Kend = std::min(K0 + NB, L.pmax);
       for (int k0 = K0; k0 < Kend; k0 += nb) {
-        hipLaunchKernelGGL(k_trsm_lower<T>, dim3(cdivu(L.mmax + L.pmax, 256), L.count),
-                           dim3(256), 0, s, list, L.count, P.d_front, k0, nb, K0, NB, 1,
-                           k0 == K0 ? 1 : 0, P.d_ipiv, arena);
+        if (k0 == K0) {
+          // the block's row exchanges on the trailing F columns and E
+          hipLaunchKernelGGL(k_block_perm, dim3(L.count), dim3(256), 0, s, list, L.count,
+                             P.d_front, K0, NB, P.d_ipiv, P.d_bperm);
+          hipLaunchKernelGGL(k_apply_perm<T>, dim3(cdivu(L.mmax + L.pmax, 4), L.count),
+                             dim3(256), 0, s, list, L.count, P.d_front, K0, NB, P.d_bperm, arena);
+          if (L.pmax > kPanelMapRows)
+            hipLaunchKernelGGL(k_block_swaps<T>, dim3(cdivu(L.mmax + L.pmax, 256), L.count),
+                               dim3(256), 0, s, list, L.count, P.d_front, K0, NB, P.d_ipiv,
+                               arena);
+        }
+        hipLaunchKernelGGL(k_left_apply<T>, dim3(cdivu(L.mmax + L.pmax, 64), L.count),
+                           dim3(256), 0, s, list, L.count, P.d_front, k0, nb, K0, NB, 1, dinv,
+                           arena);
         if (k0 + nb < Kend)
           if ((e = gemm(NB - nb, L.mmax + L.pmax, kGemmBlkRows, k0, K0, s)) != hipSuccess) return e;
       }
@@ -1550,8 +1838,8 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
     }
     // Z = U11^-1 by backward Jordan steps on the pivot blocks
     for (int k0 = ((L.pmax - 1) / nb) * nb; k0 >= 0; k0 -= nb) {
-      hipLaunchKernelGGL(k_trsm_upper<T>, dim3(cdivu(L.pmax, 256), L.count), dim3(256), 0,
-                         s, list, L.count, P.d_front, k0, nb, arena);
+      hipLaunchKernelGGL(k_upper_solve<T>, dim3(cdivu(L.pmax, 8), L.count), dim3(256), 0, s,
+                         list, L.count, P.d_front, k0, nb, dinv, arena);
       if (k0 > 0)
         if ((e = gemm(L.pmax, L.pmax, kGemmZ, k0, 0, s)) != hipSuccess) return e;
       if ((e = hipGetLastError()) != hipSuccess) return e;

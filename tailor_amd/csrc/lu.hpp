@@ -17,7 +17,7 @@ struct FrontDev {
   int m, p, c;
   int piv_off;     // offset into the pivot array
   int dof0;        // first elimination-order dof of the front's pivots
-  int pad;
+  int dinv_off;    // first panel slot of the front in its level's inverse workspace
 };
 
 struct LevelPlan {
@@ -57,6 +57,10 @@ struct LuPlan {
   double *arena = nullptr;
   hipStream_t side = nullptr;   // look-ahead panels (default priority)
   hipEvent_t ev_head = nullptr, ev_panel = nullptr;
+  double *dinv = nullptr;       // inverted pivot blocks: dinv_slots panel slots
+  long long dinv_slots = 0;     // most panels of any level
+  int *d_bperm = nullptr;       // block row permutations: one list per front of a level
+  int count_max = 0;            // most fronts of any level
   double *W = nullptr;          // 2 * w_len
   double *Y = nullptr;          // 2 * n
 };
@@ -71,6 +75,8 @@ struct LuInputs {
 
 constexpr int kLuNb = 32;        // panel width
 constexpr int kLuNbOuter = 128;  // outer block width (K of the trailing update)
+constexpr int kLuInvSlot = 4 * kLuNb * kLuNb;      // inverted pivot blocks of one panel
+constexpr int kLuPermStride = 1 + 4 * kLuNbOuter;  // one front's block row permutation
 
 // Build the device plan from the symbolic analysis and the BSR structure
 // (host arrays).  Allocates device memory; returns hipSuccess or an error.

@@ -44,7 +44,7 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
     d.woff = woff;
     d.piv_off = poff;
     d.dof0 = 5 * S.sn_start[f];
-    d.pad = 0;
+    d.dinv_off = 0;
     off += 2LL * d.m * d.m + 4LL * d.p * d.p;
     woff += d.m;
     poff += d.p;
@@ -71,6 +71,14 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
         if (static_cast<int>(S.children[f].size()) > slot) child_slots.push_back(S.children[f][slot]);
       L.child_slot_count[slot] = static_cast<int>(child_slots.size()) - L.child_slot_off[slot];
     }
+    // inverse workspace: the level's fronts take consecutive panel slots
+    long long slots = 0;
+    for (int f : S.levels[l]) {
+      fr[f].dinv_off = static_cast<int>(slots);
+      slots += (S.p[f] + kLuNb - 1) / kLuNb;
+    }
+    P.dinv_slots = std::max(P.dinv_slots, slots);
+    P.count_max = std::max(P.count_max, L.count);
   }
   std::vector<int> parent = S.parent, rel, child_ptr(S.nsn + 1, 0), child_list, upd_pos;
   std::vector<long long> rel_off(S.nsn, 0), upd_off(S.nsn, 0);
@@ -125,6 +133,14 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
     return e;
   if ((e = hipMalloc(reinterpret_cast<void **>(&P.W), 2 * P.w_len * sizeof(double))) != hipSuccess)
     return e;
+  if ((e = hipMalloc(reinterpret_cast<void **>(&P.dinv), std::max<long long>(1, P.dinv_slots) *
+                                                             kLuInvSlot * sizeof(double))) !=
+      hipSuccess)
+    return e;
+  if ((e = hipMalloc(reinterpret_cast<void **>(&P.d_bperm),
+                     std::max(1, P.count_max) * static_cast<size_t>(kLuPermStride) * sizeof(int))) !=
+      hipSuccess)
+    return e;
   if ((e = hipMalloc(reinterpret_cast<void **>(&P.Y), 2 * P.n * sizeof(double))) != hipSuccess)
     return e;
   if ((e = hipStreamCreateWithFlags(&P.side, hipStreamNonBlocking)) != hipSuccess) return e;
@@ -156,8 +172,12 @@ void lu_plan_destroy(LuPlan &P) {
   release(P.d_perm);
   release(P.arena);
   release(P.W);
+  release(P.dinv);
+  release(P.d_bperm);
   release(P.Y);
   P.levels.clear();
+  P.dinv_slots = 0;
+  P.count_max = 0;
 }
 
 }  // namespace tailor
