@@ -1281,8 +1281,11 @@ __global__ __launch_bounds__(64 * WMW * WNW) void k_gemm(const int *__restrict__
                                                          T *__restrict__ arena) {
   constexpr int THREADS = 64 * WMW * WNW;
   constexpr int TM = 16 * WMW * WMB, TN = 16 * WNW * WNB;
-  // LDS row stride: +16 keeps the four k rows of a wave's reads in different banks
-  constexpr int LDA_ = TM + 16, LDB_ = TN + 16;
+  // LDS layouts: A as [k][row] with row stride TM + 16 (the four k rows of a
+  // wave's reads fall in different banks); B as [column][k] with stride
+  // GKC + 1, so the stash writes (consecutive k of a column) are contiguous
+  // and the operand reads (16 columns at one k) hit distinct banks
+  constexpr int LDA_ = TM + 16, LDB_ = GKC + 1;
   constexpr int AE = TM * GKC / THREADS, BE = TN * GKC / THREADS;
   static_assert(AE * THREADS == TM * GKC && BE * THREADS == TN * GKC, "tile / thread mismatch");
   const int f = list[blockIdx.y];
@@ -1320,7 +1323,7 @@ __global__ __launch_bounds__(64 * WMW * WNW) void k_gemm(const int *__restrict__
   }
   // two LDS buffers: the next K chunk is stored while the current one feeds
   // the MFMAs, so each chunk needs one barrier
-  __shared__ T sAr[2][GKC][LDA_], sAi[2][GKC][LDA_], sBr[2][GKC][LDB_], sBi[2][GKC][LDB_];
+  __shared__ T sAr[2][GKC][LDA_], sAi[2][GKC][LDA_], sBr[2][TN][LDB_], sBi[2][TN][LDB_];
   const int t = threadIdx.x, l = t % 64, w = t / 64;
   const int wm = w % WMW, wn = w / WMW;
   // global -> register staging (element e = t + s * THREADS): A (row e % TM,
@@ -1374,8 +1377,8 @@ __global__ __launch_bounds__(64 * WMW * WNW) void k_gemm(const int *__restrict__
 #pragma unroll
     for (int s = 0; s < BE; ++s) {
       const int e = t + s * THREADS;
-      sBr[buf][e % GKC][e / GKC] = gb_r[s];
-      sBi[buf][e % GKC][e / GKC] = gb_i[s];
+      sBr[buf][e / GKC][e % GKC] = gb_r[s];
+      sBi[buf][e / GKC][e % GKC] = gb_i[s];
     }
   };
   using acc_t = typename Mfma<T>::acc_t;
@@ -1405,8 +1408,8 @@ __global__ __launch_bounds__(64 * WMW * WNW) void k_gemm(const int *__restrict__
       }
 #pragma unroll
       for (int q = 0; q < WNB; ++q) {
-        b_r[q] = sBr[buf][kr][wn * 16 * WNB + q * 16 + l % 16];
-        b_i[q] = sBi[buf][kr][wn * 16 * WNB + q * 16 + l % 16];
+        b_r[q] = sBr[buf][wn * 16 * WNB + q * 16 + l % 16][kr];
+        b_i[q] = sBi[buf][wn * 16 * WNB + q * 16 + l % 16][kr];
       }
 #pragma unroll
       for (int a = 0; a < WMB; ++a)
