@@ -1377,19 +1377,40 @@ __global__ __launch_bounds__(64 * WMW * WNW) void k_gemm(const int *__restrict__
 #pragma unroll
     for (int s = 0; s < BE; ++s) {
       const int e = t + s * THREADS;
-      sBr[buf][e / GKC][e % GKC] = gb_r[s];
-      sBi[buf][e / GKC][e % GKC] = gb_i[s];
+      sBr[buf][e / GKC][e % GKC] = -gb_r[s];
+      sBi[buf][e / GKC][e % GKC] = -gb_i[s];
     }
   };
+  // The accumulators start from C and the MFMAs add (-B)^T A^T (B is
+  // staged negated), so the tile ends with plain stores; the C loads are
+  // issued with the first chunk's loads and share their latency.
   using acc_t = typename Mfma<T>::acc_t;
   acc_t cr[WMB][WNB], ci[WMB][WNB];
 #pragma unroll
-  for (int a = 0; a < WMB; ++a)
+  for (int a = 0; a < WMB; ++a) {
+    const int row = row0 + wm * 16 * WMB + a * 16 + l % 16;
 #pragma unroll
     for (int b = 0; b < WNB; ++b) {
-      cr[a][b] = acc_t{0, 0, 0, 0};
-      ci[a][b] = acc_t{0, 0, 0, 0};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int col = col0 + wn * 16 * WNB + b * 16 + Mfma<T>::ocol(l, e);
+        T xr = T(0), xi = T(0);
+        if (col < v.N1) {
+          if (row < v.M) {
+            const long long ic = static_cast<long long>(col) * v.ldc + row;
+            xr = v.Cr[ic];
+            xi = v.Ci[ic];
+          }
+        } else if (col < v.N && row < v.M2) {
+          const long long ic = static_cast<long long>(col - v.N1) * v.ldc2 + row;
+          xr = v.C2r[ic];
+          xi = v.C2i[ic];
+        }
+        cr[a][b][e] = xr;
+        ci[a][b][e] = xi;
+      }
     }
+  }
   fetch(0);
   stash(0);
   __syncthreads();
@@ -1437,13 +1458,13 @@ __global__ __launch_bounds__(64 * WMW * WNW) void k_gemm(const int *__restrict__
         if (col < v.N1) {
           if (row >= v.M) continue;
           const long long ic = static_cast<long long>(col) * v.ldc + row;
-          v.Cr[ic] -= cr[a][b][e];
-          v.Ci[ic] -= ci[a][b][e];
+          v.Cr[ic] = cr[a][b][e];
+          v.Ci[ic] = ci[a][b][e];
         } else {
           if (row >= v.M2) continue;
           const long long ic = static_cast<long long>(col - v.N1) * v.ldc2 + row;
-          v.C2r[ic] -= cr[a][b][e];
-          v.C2i[ic] -= ci[a][b][e];
+          v.C2r[ic] = cr[a][b][e];
+          v.C2i[ic] = ci[a][b][e];
         }
       }
     }
