@@ -166,7 +166,8 @@ int tailor_profile_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, dou
 
 /* Autotuning of the frontal GEMM: factor at `sigma` once per configuration
  * c = 2 * shape + order, shape 0: 64x64 complex tile (256 threads),
- * 1: 128x64 (512), 2: 128x128 (512); order 0: tiles in launch order,
+ * 1: 128x64 (512), 2: 128x128 (512), 3: 64x64 with a K chunk of 8 and
+ * three waves per SIMD (256); order 0: tiles in launch order,
  * 1: XCD-aware grouped order.  The GEMM launches of each elimination level
  * are timed with HIP events and each level keeps its fastest configuration
  * for later factorizations of this context.  No reference counterpart
@@ -174,11 +175,11 @@ int tailor_profile_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, dou
 typedef struct {
   int32_t levels;          /* elimination levels tuned (at most 64 reported) */
   int32_t order[64];       /* chosen configuration c per level */
-  double gemm_ms[8];       /* total GEMM time per configuration (entries 0..5 used) */
+  double gemm_ms[8];       /* total GEMM time per configuration c = 0..7 */
   double tuned_gemm_ms;    /* sum over levels of the chosen configurations' times */
 } tailor_tune_report;
 int tailor_autotune(tailor_ctx *ctx, double sigma_re, double sigma_im, tailor_tune_report *report);
-/* Force configuration c (0..5, as above) on every level, replacing any
+/* Force configuration c (0..7, as above) on every level, replacing any
  * autotuned choice, for later factorizations of this context. */
 int tailor_set_gemm_config(tailor_ctx *ctx, int32_t config);
 

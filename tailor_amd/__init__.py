@@ -291,7 +291,7 @@ def _autotune(self, sigma: complex) -> dict:
     r = TuneReport()
     self._check(self.lib.tailor_autotune(self.ptr, sigma.real, sigma.imag, C.byref(r)))
     return {"levels": r.levels, "config": list(r.order)[:r.levels],
-            "gemm_ms": [round(x, 3) for x in list(r.gemm_ms)[:6]],
+            "gemm_ms": [round(x, 3) for x in list(r.gemm_ms)],
             "tuned_gemm_ms": r.tuned_gemm_ms}
 
 

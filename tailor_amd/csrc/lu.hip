@@ -1270,15 +1270,20 @@ constexpr int GKC = 16;  // K chunk staged in LDS
 // owning WMB x WNB MFMA 16x16 blocks, so a workgroup computes a
 // (16 WMW WMB) x (16 WNW WNB) complex tile.
 //   shape 0: 64 x 64, 256 threads; 1: 128 x 64, 512 threads;
-//   shape 2: 128 x 128, 512 threads.
-enum { kShape64x64 = 0, kShape128x64 = 1, kShape128x128 = 2, kShapeCount = 3 };
+//   shape 2: 128 x 128, 512 threads; 3: 64 x 64, 256 threads, K chunk of
+//   8 and registers held to three waves per SIMD (half the LDS of shape 0,
+//   so three workgroups share a CU).
+enum { kShape64x64 = 0, kShape128x64 = 1, kShape128x128 = 2, kShape64x64k8 = 3, kShapeCount = 4 };
+static_assert(kShapeCount == kGemmShapes, "shape count");
 
-template <typename T, int WMW, int WNW, int WMB, int WNB>
-__global__ __launch_bounds__(64 * WMW * WNW) void k_gemm(const int *__restrict__ list, int nlist,
-                                                         const FrontDev *__restrict__ fr,
-                                                         int region, int k0, int nb, int K0,
-                                                         int NB, int order,
-                                                         T *__restrict__ arena) {
+template <typename T, int WMW, int WNW, int WMB, int WNB, int KC = GKC, int MINW = 1>
+__global__ __launch_bounds__(64 * WMW * WNW, MINW) void k_gemm(const int *__restrict__ list,
+                                                               int nlist,
+                                                               const FrontDev *__restrict__ fr,
+                                                               int region, int k0, int nb,
+                                                               int K0, int NB, int order,
+                                                               T *__restrict__ arena) {
+  constexpr int GKC = KC;  // K chunk of this instance
   constexpr int THREADS = 64 * WMW * WNW;
   constexpr int TM = 16 * WMW * WMB, TN = 16 * WNW * WNB;
   // LDS layouts: A as [k][row] with row stride TM + 16 (the four k rows of a
@@ -1497,6 +1502,10 @@ hipError_t launch_gemm(int shape, int order, long long rows, long long cols, int
     case kShape128x128:
       hipLaunchKernelGGL((k_gemm<T, 2, 4, 4, 2>), grid, block, 0, st, list, count, fr, region, k0,
                          nb, K0, NB, order, arena);
+      break;
+    case kShape64x64k8:
+      hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 2, 8, 3>), grid, block, 0, st, list, count, fr,
+                         region, k0, nb, K0, NB, order, arena);
       break;
     default:
       hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 2>), grid, block, 0, st, list, count, fr, region, k0,

@@ -98,7 +98,7 @@ struct FactorTiming {
 // Tile orders and shapes of the frontal GEMM (chosen per level by the
 // autotuner).
 enum { kOrderLinear = 0, kOrderXcdGrouped = 1, kOrderCount = 2 };
-constexpr int kGemmShapes = 3;
+constexpr int kGemmShapes = 4;
 
 // Fronts in complex FP64, or complex FP32 (fp32) in the same arena.  With
 // `timing`, every GEMM launch is bracketed by events (slightly slower).

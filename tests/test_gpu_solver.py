@@ -203,7 +203,7 @@ def test_panel_kernels_agree(oracle_lib, shape, precision):
     assert diff <= (1e-8 if precision == 0 else 1e-2)
 
 
-@pytest.mark.parametrize("config", range(6))
+@pytest.mark.parametrize("config", range(8))
 def test_every_gemm_configuration(oracle_lib, config):
     """Each frontal-GEMM tile shape and order (the autotuner's choices) gives
     a factorization that meets the solve bars (64 x 32: fronts up to 2020)."""
