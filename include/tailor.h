@@ -165,9 +165,10 @@ int tailor_profile_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, dou
                           double *gemm_ms, double *gemm_flops, int32_t *gemm_launches);
 
 /* Autotuning of the frontal GEMM: factor at `sigma` once per configuration
- * c = 2 * shape + order, shape 0: 64x64 complex tile (256 threads),
- * 1: 128x64 (512), 2: 128x128 (512), 3: 64x64 with a K chunk of 8 and
- * three waves per SIMD (256); order 0: tiles in launch order,
+ * c = 2 * shape + order, shape 0: 64x64 complex tile (256 threads, K
+ * chunk 16), 1: 64x64 (K chunk 4, four waves per SIMD), 2: 128x128 (512
+ * threads), 3: 64x64 (K chunk 8, three waves per SIMD); order 0: tiles in
+ * launch order,
  * 1: XCD-aware grouped order.  The GEMM launches of each elimination level
  * are timed with HIP events and each level keeps its fastest configuration
  * for later factorizations of this context.  No reference counterpart

@@ -1269,11 +1269,14 @@ constexpr int GKC = 16;  // K chunk staged in LDS
 // Tile shapes (chosen per level by the autotuner): WMW x WNW waves, each
 // owning WMB x WNB MFMA 16x16 blocks, so a workgroup computes a
 // (16 WMW WMB) x (16 WNW WNB) complex tile.
-//   shape 0: 64 x 64, 256 threads; 1: 128 x 64, 512 threads;
-//   shape 2: 128 x 128, 512 threads; 3: 64 x 64, 256 threads, K chunk of
-//   8 and registers held to three waves per SIMD (half the LDS of shape 0,
-//   so three workgroups share a CU).
-enum { kShape64x64 = 0, kShape128x64 = 1, kShape128x128 = 2, kShape64x64k8 = 3, kShapeCount = 4 };
+//   shape 0: 64 x 64, 256 threads, K chunk 16;
+//   shape 1: 64 x 64, 256 threads, K chunk 4, registers held to four waves
+//            per SIMD (a quarter of shape 0's LDS);
+//   shape 2: 128 x 128, 512 threads;
+//   shape 3: 64 x 64, 256 threads, K chunk 8, three waves per SIMD.
+// The shorter K chunks trade more barriers for more resident workgroups,
+// which hide the first chunk's and C's load latency of each tile.
+enum { kShape64x64 = 0, kShape64x64k4 = 1, kShape128x128 = 2, kShape64x64k8 = 3, kShapeCount = 4 };
 static_assert(kShapeCount == kGemmShapes, "shape count");
 
 template <typename T, int WMW, int WNW, int WMB, int WNB, int KC = GKC, int MINW = 1>
@@ -1483,10 +1486,7 @@ hipError_t launch_gemm(int shape, int order, long long rows, long long cols, int
                        hipStream_t st, const int *list, const FrontDev *fr, int region, int k0,
                        int nb, int K0, int NB, T *arena) {
   int tm = 64, tn = 64, threads = 256;
-  if (shape == kShape128x64) {
-    tm = 128;
-    threads = 512;
-  } else if (shape == kShape128x128) {
+  if (shape == kShape128x128) {
     tm = tn = 128;
     threads = 512;
   }
@@ -1495,9 +1495,9 @@ hipError_t launch_gemm(int shape, int order, long long rows, long long cols, int
   if (order != kOrderLinear) tiles = (tiles + kXcds - 1) / kXcds * kXcds;  // XCD order: multiple of 8
   const dim3 grid(tiles, count), block(threads);
   switch (shape) {
-    case kShape128x64:
-      hipLaunchKernelGGL((k_gemm<T, 4, 2, 2, 2>), grid, block, 0, st, list, count, fr, region, k0,
-                         nb, K0, NB, order, arena);
+    case kShape64x64k4:
+      hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 2, 4, 4>), grid, block, 0, st, list, count, fr,
+                         region, k0, nb, K0, NB, order, arena);
       break;
     case kShape128x128:
       hipLaunchKernelGGL((k_gemm<T, 2, 4, 4, 2>), grid, block, 0, st, list, count, fr, region, k0,
