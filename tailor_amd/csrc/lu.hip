@@ -1666,6 +1666,20 @@ __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list
   const int c1 = min(v.N, c0 + GVC);
   __shared__ double sxr[GVW * GVC], sxi[GVW * GVC];
   __shared__ double pr[GVW][GVR], pi[GVW][GVR];
+  const bool active = row < v.M && !(v.upper && c1 <= r0);
+  const T *ar = v.Ar + (row < v.M ? row : 0), *ai = v.Ai + (row < v.M ? row : 0);
+  // the first 16 columns of A are requested before x is staged, so the two
+  // loads overlap
+  double a_r[16], a_i[16];
+  const bool first = active && c0 + 16 <= c1;
+  if (first) {
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const long long idx = static_cast<long long>(c0 + u) * v.lda;
+      a_r[u] = static_cast<double>(ar[idx]);
+      a_i[u] = static_cast<double>(ai[idx]);
+    }
+  }
   for (int k = threadIdx.x; k < GVW * GVC; k += GVR * GVW) {
     const int col = cw + k;
     double xr = 0.0, xi = 0.0;
@@ -1679,17 +1693,17 @@ __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list
   }
   __syncthreads();
   double accr = 0.0, acci = 0.0;
-  if (row < v.M && !(v.upper && c1 <= r0)) {
-    const T *ar = v.Ar + row, *ai = v.Ai + row;
+  if (active) {
     int k = c0;
     // 16 columns per step so that 32 loads per lane are in flight
     for (; k + 16 <= c1; k += 16) {
-      double a_r[16], a_i[16];
+      if (k != c0 || !first) {
 #pragma unroll
-      for (int u = 0; u < 16; ++u) {
-        const long long idx = static_cast<long long>(k + u) * v.lda;
-        a_r[u] = static_cast<double>(ar[idx]);
-        a_i[u] = static_cast<double>(ai[idx]);
+        for (int u = 0; u < 16; ++u) {
+          const long long idx = static_cast<long long>(k + u) * v.lda;
+          a_r[u] = static_cast<double>(ar[idx]);
+          a_i[u] = static_cast<double>(ai[idx]);
+        }
       }
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
@@ -1700,10 +1714,10 @@ __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list
     }
     for (; k < c1; ++k) {
       const long long idx = static_cast<long long>(k) * v.lda;
-      const double a_r = static_cast<double>(ar[idx]), a_i = static_cast<double>(ai[idx]);
+      const double br = static_cast<double>(ar[idx]), bi = static_cast<double>(ai[idx]);
       const double x_r = sxr[k - cw], x_i = sxi[k - cw];
-      accr += a_r * x_r - a_i * x_i;
-      acci += a_r * x_i + a_i * x_r;
+      accr += br * x_r - bi * x_i;
+      acci += br * x_i + bi * x_r;
     }
   }
   pr[wave][lane] = accr;
