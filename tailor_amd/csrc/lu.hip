@@ -80,28 +80,70 @@ __global__ void k_assemble_fronts(long long nnzb, int per, const double *__restr
   arena[d.off + static_cast<long long>(d.m) * d.m + idx] = static_cast<T>(vi);
 }
 
-// F_parent[rel, rel] += F_child[p:, p:] for the children listed in `tasks`.
+// F_parent[rel, rel] += F_child[p:, p:] for the children listed in `tasks`
+// (at most one child per parent in a launch).  Workgroups stride over the
+// child's update columns (the parent column is uniform), threads over its
+// rows, four rows per thread per step with every load before the stores;
+// the child's relative-index map (one entry per node, c / 5 of them) is
+// staged in LDS.
+constexpr int kRelLds = 2048;  // map entries staged in LDS (larger maps read global memory)
+
 template <typename T>
-__global__ void k_extend_add(const int *__restrict__ tasks, int ntasks,
-                             const FrontDev *__restrict__ fr, const int *__restrict__ par,
-                             const int *__restrict__ rel, const long long *__restrict__ rel_off,
-                             T *__restrict__ arena) {
+__global__ __launch_bounds__(256) void k_extend_add(const int *__restrict__ tasks, int ntasks,
+                                                    const FrontDev *__restrict__ fr,
+                                                    const int *__restrict__ par,
+                                                    const int *__restrict__ rel,
+                                                    const long long *__restrict__ rel_off,
+                                                    T *__restrict__ arena) {
   const int ti = blockIdx.y;
   if (ti >= ntasks) return;
   const int ch = tasks[ti];
   const FrontDev dc = fr[ch], dp = fr[par[ch]];
-  const int *rl = rel + rel_off[ch];
-  const long long cc = static_cast<long long>(dc.c) * dc.c;
-  const T *Cr = arena + dc.off, *Ci = Cr + static_cast<long long>(dc.m) * dc.m;
-  T *Pr = arena + dp.off, *Pi = Pr + static_cast<long long>(dp.m) * dp.m;
-  for (long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; idx < cc;
-       idx += static_cast<long long>(gridDim.x) * blockDim.x) {
-    const int i = static_cast<int>(idx % dc.c), j = static_cast<int>(idx / dc.c);
-    const long long src = static_cast<long long>(dc.p + j) * dc.m + dc.p + i;
-    const int pi = 5 * rl[i / 5] + i % 5, pj = 5 * rl[j / 5] + j % 5;
-    const long long dst = static_cast<long long>(pj) * dp.m + pi;
-    Pr[dst] += Cr[src];
-    Pi[dst] += Ci[src];
+  const int c = dc.c;
+  if (c == 0) return;
+  const int *__restrict__ rl = rel + rel_off[ch];
+  const int nodes = c / 5;
+  __shared__ int srel[kRelLds];
+  for (int q = threadIdx.x; q < min(nodes, kRelLds); q += blockDim.x) srel[q] = rl[q];
+  __syncthreads();
+  auto prow = [&](int i) {
+    const int n = i / 5;
+    return 5 * (n < kRelLds ? srel[n] : rl[n]) + (i - 5 * n);
+  };
+  const long long cm = dc.m, pm = dp.m;
+  const T *Cr0 = arena + dc.off, *Ci0 = Cr0 + cm * cm;
+  T *Pr0 = arena + dp.off, *Pi0 = Pr0 + pm * pm;
+  constexpr int U = 4;
+  for (int j = blockIdx.x; j < c; j += gridDim.x) {
+    const T *__restrict__ cr = Cr0 + (dc.p + j) * cm + dc.p;
+    const T *__restrict__ ci = Ci0 + (dc.p + j) * cm + dc.p;
+    const long long pc = static_cast<long long>(prow(j)) * pm;
+    T *__restrict__ pr = Pr0 + pc;
+    T *__restrict__ pi = Pi0 + pc;
+    for (int i0 = threadIdx.x; i0 < c; i0 += U * blockDim.x) {
+      T ar[U], ai[U], br[U], bi[U];
+      int dst[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = i0 + u * blockDim.x;
+        dst[u] = i < c ? prow(i) : 0;
+        ar[u] = i < c ? cr[i] : T(0);
+        ai[u] = i < c ? ci[i] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = i0 + u * blockDim.x;
+        br[u] = i < c ? pr[dst[u]] : T(0);
+        bi[u] = i < c ? pi[dst[u]] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (i0 + u * blockDim.x < c) {
+          pr[dst[u]] = br[u] + ar[u];
+          pi[dst[u]] = bi[u] + ai[u];
+        }
+      }
+    }
   }
 }
 
