@@ -168,8 +168,9 @@ __global__ __launch_bounds__(256) void k_extend_add(const int *__restrict__ task
 //       k_gemm BLKROWS remaining block rows of those columns
 //     k_gemm TRAIL     F[K1:m, K1:m] -= L21_blk U12_blk and
 //                      E[K1:p, :]    -= L21_blk E[K0:K1, :]   (K = K1 - K0)
-//   then Z = U11^-1 panel by panel from the last: k_left_apply (mode 2,
-//   X <- U_D^-1 X on the panel rows of Z) and k_gemm Z.
+//   then Z = U11^-1 block by block from the last: per panel k_upper_solve
+//   (back substitution on the panel rows of Z) and k_gemm ZIN (the block's
+//   rows above the panel), then k_gemm ZBLK (all rows above the block).
 //
 // Row exchanges of a block are applied to the block's own columns at once
 // but never to earlier blocks: L11 is stored block by block in the row order
@@ -1188,10 +1189,13 @@ __global__ __launch_bounds__(256) void k_block_swaps(const int *__restrict__ lis
 
 // TRAIL = TRAILHEAD (the next block's fully summed columns of F) + TRAILREST
 // (the remaining columns of F and all of E); the split lets the next
-// block's panels start while the rest of the update runs.
+// block's panels start while the rest of the update runs.  Z = U11^-1 is
+// built block by block from the last: within block [K0, K1) each panel's
+// rows are finished and ZIN updates the block's rows above the panel
+// (K = 32); ZBLK then updates all rows above the block at once (K = 128).
 enum GemmRegion {
   kGemmInner = 0, kGemmBlkRows = 1, kGemmTrail = 2, kGemmZ = 3, kGemmTrailHead = 4,
-  kGemmTrailRest = 5
+  kGemmTrailRest = 5, kGemmZBlk = 6
 };
 
 // Operands of one front's update.  Columns [0, N1) of B and C come from the
@@ -1216,7 +1220,8 @@ struct GemmShape {
 __host__ __device__ inline GemmShape gemm_shape(int region, const FrontDev &d, int k0, int nb,
                                                 int K0, int NB) {
   GemmShape g{0, 0, 0, 0, 0};
-  const bool trail = region == kGemmTrail || region == kGemmTrailHead || region == kGemmTrailRest;
+  const bool trail = region == kGemmTrail || region == kGemmTrailHead ||
+                     region == kGemmTrailRest || region == kGemmZBlk;
   if (trail ? K0 >= d.p : k0 >= d.p) return g;
   const int kb = min(nb, d.p - k0);
   const int K1 = min(K0 + NB, d.p);
@@ -1237,8 +1242,11 @@ __host__ __device__ inline GemmShape gemm_shape(int region, const FrontDev &d, i
     case kGemmTrailRest:
       g = {d.m - K1, d.p - K1, d.m - K1n + d.p, d.m - K1n, K1 - K0};
       break;
-    default:
-      g = {k0, k0, d.p - k0, d.p - k0, kb};
+    case kGemmZBlk:
+      g = {K0, K0, d.p - K0, d.p - K0, K1 - K0};
+      break;
+    default:  // ZIN
+      g = {k0 - K0, k0 - K0, d.p - k0, d.p - k0, kb};
       break;
   }
   return g;
@@ -1247,7 +1255,8 @@ __host__ __device__ inline GemmShape gemm_shape(int region, const FrontDev &d, i
 template <typename T>
 __device__ inline bool gemm_view(int region, const FrontDev &d, int k0, int nb, int K0, int NB,
                                  T *arena, GemmView<T> &v) {
-  const bool trail = region == kGemmTrail || region == kGemmTrailHead || region == kGemmTrailRest;
+  const bool trail = region == kGemmTrail || region == kGemmTrailHead ||
+                     region == kGemmTrailRest || region == kGemmZBlk;
   if (trail ? K0 >= d.p : k0 >= d.p) return false;
   const long long m = d.m, p = d.p, mm = m * m, pp = p * p;
   T *Fr = arena + d.off, *Fi = Fr + mm;
@@ -1293,11 +1302,18 @@ __device__ inline bool gemm_view(int region, const FrontDev &d, int k0, int nb, 
            d.m - K1, d.p - K1, d.m - K1 + d.p, d.m - K1, K1 - K0};
       break;
     }
-    default: {
-      const long long a = k0 * m, b = k0 * p + k0, c = k0 * p;
+    case kGemmZBlk: {
+      const long long a = K0 * m, b = K0 * p + K0, c = K0 * p;
       v = {Fr + a, Fi + a, m, Zr + b, Zi + b, nullptr, nullptr, p, 0,
            Zr + c, Zi + c, nullptr, nullptr, p, 0,
-           k0, k0, d.p - k0, d.p - k0, kb};
+           K0, K0, d.p - K0, d.p - K0, K1 - K0};
+      break;
+    }
+    default: {  // ZIN
+      const long long a = k0 * m + K0, b = k0 * p + k0, c = k0 * p + K0;
+      v = {Fr + a, Fi + a, m, Zr + b, Zi + b, nullptr, nullptr, p, 0,
+           Zr + c, Zi + c, nullptr, nullptr, p, 0,
+           k0 - K0, k0 - K0, d.p - k0, d.p - k0, kb};
       break;
     }
   }
@@ -1925,13 +1941,20 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
       }
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    // Z = U11^-1 by backward Jordan steps on the pivot blocks
-    for (int k0 = ((L.pmax - 1) / nb) * nb; k0 >= 0; k0 -= nb) {
-      hipLaunchKernelGGL(k_upper_solve<T>, dim3(cdivu(L.pmax, 8), L.count), dim3(256), 0, s,
-                         list, L.count, P.d_front, k0, nb, dinv, arena);
-      if (k0 > 0)
-        if ((e = gemm(L.pmax, L.pmax, kGemmZ, k0, 0, s)) != hipSuccess) return e;
-      if ((e = hipGetLastError()) != hipSuccess) return e;
+    // Z = U11^-1 by backward block-Jordan steps: within each block the
+    // panels from the last (panel rows by back substitution, ZIN for the
+    // block's rows above), then ZBLK for every row above the block
+    for (int K0 = ((L.pmax - 1) / NB) * NB; K0 >= 0; K0 -= NB) {
+      const int Kend = std::min(K0 + NB, L.pmax);
+      for (int k0 = K0 + ((Kend - 1 - K0) / nb) * nb; k0 >= K0; k0 -= nb) {
+        hipLaunchKernelGGL(k_upper_solve<T>, dim3(cdivu(L.pmax, 8), L.count), dim3(256), 0, s,
+                           list, L.count, P.d_front, k0, nb, dinv, arena);
+        if (k0 > K0)
+          if ((e = gemm(NB - nb, L.pmax, kGemmZ, k0, K0, s)) != hipSuccess) return e;
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+      }
+      if (K0 > 0)
+        if ((e = gemm(L.pmax, L.pmax, kGemmZBlk, K0, K0, s)) != hipSuccess) return e;
     }
   }
   if (tm) {
