@@ -1644,7 +1644,8 @@ __global__ void k_solve_init(const FrontDev *__restrict__ fr, const double *br, 
   }
 }
 
-constexpr int GVR = 64, GVC = 64, GVW = 4;  // rows per WG, columns per wave, waves per WG
+constexpr int GVR = 64, GVW = 4;  // rows per workgroup, waves per workgroup
+constexpr long long kGemvWideMin = 1024;  // workgroups of the wide form that fill the GPU
 
 // One batched GEMV y (+)= add + sign * A x.  xmap / ymap, when set, gather x
 // and scatter y through node maps (entry j -> 5 * map[j / 5] + j % 5).
@@ -1702,9 +1703,11 @@ __device__ inline bool gemv_view(int op, const FrontDev *fr, int f, const FrontD
 
 // One workgroup: GVR output rows x (GVW * GVC) reduction columns; each wave
 // takes GVC of the columns and the four partial sums are combined in LDS
-// before one FP64 atomic per row.
+// before one FP64 atomic per row.  GVC = 64 for levels with enough work to
+// fill the GPU, 16 (one batch of loads per wave, four times the workgroups)
+// for the small ones, whose launches are latency-bound.
 // Factor entries of type T are widened to FP64; sums are FP64.
-template <typename T>
+template <typename T, int GVC>
 __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list, int nlist,
                                                     const FrontDev *__restrict__ fr, SolveMaps mp,
                                                     int op, const T *__restrict__ arena, double *Wr,
@@ -1990,9 +1993,15 @@ static hipError_t solve_impl(const LuPlan &P, const double *br, const double *bi
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   auto gemv = [&](const LevelPlan &L, int op, int rows, int cols) {
-    hipLaunchKernelGGL(k_gemv<T>, dim3(cdivu(rows, GVR), L.count, cdivu(cols, GVC * GVW)),
-                       dim3(GVR * GVW), 0, s, P.d_level_list + L.list_off, L.count, P.d_front, mp,
-                       op, arena, Wr, Wi, Yr, Yi, xr, xi);
+    const long long wide = static_cast<long long>(cdivu(rows, GVR)) * L.count * cdivu(cols, 64 * GVW);
+    if (wide >= kGemvWideMin)
+      hipLaunchKernelGGL((k_gemv<T, 64>), dim3(cdivu(rows, GVR), L.count, cdivu(cols, 64 * GVW)),
+                         dim3(GVR * GVW), 0, s, P.d_level_list + L.list_off, L.count, P.d_front,
+                         mp, op, arena, Wr, Wi, Yr, Yi, xr, xi);
+    else
+      hipLaunchKernelGGL((k_gemv<T, 16>), dim3(cdivu(rows, GVR), L.count, cdivu(cols, 16 * GVW)),
+                         dim3(GVR * GVW), 0, s, P.d_level_list + L.list_off, L.count, P.d_front,
+                         mp, op, arena, Wr, Wi, Yr, Yi, xr, xi);
   };
   for (size_t lev = 0; lev < P.levels.size(); ++lev) {
     const LevelPlan &L = P.levels[lev];
