@@ -56,7 +56,7 @@ hipError_t grow(T **p, size_t &cap, size_t need) {
 void solver_release(SolverState *S) {
   if (!S) return;
   lu_plan_destroy(S->plan);
-  double *bufs[] = {S->V, S->work, S->ritz, S->partials, S->small};
+  double *bufs[] = {S->V, S->work, S->ritz, S->partials, S->small, S->hsteps};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   int *ib[] = {S->d_order, S->d_pos};
@@ -294,6 +294,11 @@ int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solv
   // matrices (QY: ncv x ncheck, Q_l: ncv x l, both below (ncv + 1)^2)
   HIP_OK(grow(&S.partials, S.partials_cap, static_cast<size_t>(2 * kDotBlocks * (ncv + 1))));
   HIP_OK(grow(&S.small, S.small_cap, static_cast<size_t>(2 * (ncv + 1) * (ncv + 1))));
+  // per Arnoldi step j: h (j + 1 complex), the reorthogonalisation's h2,
+  // and |w|^2, at hsteps + j * hstride (read back once per pass)
+  const size_t hstride = 4 * static_cast<size_t>(ncv + 1) + 2;
+  HIP_OK(grow(&S.hsteps, S.hsteps_cap, hstride * ncv));
+  std::vector<double> hhost(hstride * ncv);
   double *Vr = S.V, *Vi = S.V + n * (ncv + 1);
   auto col_r = [&](int j) { return Vr + static_cast<long long>(j) * n; };
   auto col_i = [&](int j) { return Vi + static_cast<long long>(j) * n; };
@@ -356,20 +361,28 @@ int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solv
         return st;
       ++solves;
       refine_total += steps;
-      // CGS with one reorthogonalisation
-      std::vector<cplx> h(j + 1), h2(j + 1);
-      if (int st = solver_dots(ctx, Vr, Vi, n, j + 1, wr, wi, h.data())) return st;
-      HIP_OK(hipMemcpyAsync(S.small, h.data(), (j + 1) * sizeof(cplx), hipMemcpyHostToDevice, ctx->stream));
-      HIP_OK(vec_update(ctx->stream, n, j + 1, Vr, Vi, n, S.small, wr, wi));
-      if (int st = solver_dots(ctx, Vr, Vi, n, j + 1, wr, wi, h2.data())) return st;
-      HIP_OK(hipMemcpyAsync(S.small, h2.data(), (j + 1) * sizeof(cplx), hipMemcpyHostToDevice, ctx->stream));
-      HIP_OK(vec_update(ctx->stream, n, j + 1, Vr, Vi, n, S.small, wr, wi));
-      double beta;
-      if (int st = solver_norm2(ctx, wr, wi, &beta)) return st;
-      for (int i = 0; i <= j; ++i) Sm(i, j) = h[i] + h2[i];
+      // CGS with one reorthogonalisation, entirely on the device: the
+      // coefficients stay in hsteps and the host reads them after the pass
+      double *hj = S.hsteps + hstride * j;
+      double *h2j = hj + 2 * (ncv + 1), *w2j = hj + 4 * (ncv + 1);
+      HIP_OK(vec_dots_device(ctx->stream, n, j + 1, Vr, Vi, n, wr, wi, S.partials, kDotBlocks, hj));
+      HIP_OK(vec_update(ctx->stream, n, j + 1, Vr, Vi, n, hj, wr, wi));
+      HIP_OK(vec_dots_device(ctx->stream, n, j + 1, Vr, Vi, n, wr, wi, S.partials, kDotBlocks, h2j));
+      HIP_OK(vec_update(ctx->stream, n, j + 1, Vr, Vi, n, h2j, wr, wi));
+      HIP_OK(vec_dots_device(ctx->stream, n, 1, wr, wi, n, wr, wi, S.partials, kDotBlocks, w2j));
+      HIP_OK(vec_scale_inv_norm(ctx->stream, n, w2j, wr, wi, col_r(j + 1), col_i(j + 1)));
+    }
+    HIP_OK(hipMemcpyAsync(hhost.data() + hstride * k, S.hsteps + hstride * k,
+                          hstride * (ncv - k) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    for (int j = k; j < ncv; ++j) {
+      const double *hj = hhost.data() + hstride * j;
+      const double *h2j = hj + 2 * (ncv + 1), *w2j = hj + 4 * (ncv + 1);
+      for (int i = 0; i <= j; ++i)
+        Sm(i, j) = cplx(hj[2 * i], hj[2 * i + 1]) + cplx(h2j[2 * i], h2j[2 * i + 1]);
+      const double beta = std::sqrt(std::max(0.0, w2j[0]));
       Sm(j + 1, j) = beta;
       if (beta == 0.0) return fail(ctx, TAILOR_ERR_NOT_CONVERGED, "Krylov subspace broke down");
-      HIP_OK(vec_axpby(ctx->stream, n, 1.0 / beta, 0.0, wr, wi, 0.0, 0.0, col_r(j + 1), col_i(j + 1)));
     }
     // Schur form of the projected matrix, sorted by |theta| descending
     T = CMat(ncv, ncv);

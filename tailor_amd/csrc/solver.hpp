@@ -24,6 +24,8 @@ struct SolverState {
   int *d_order = nullptr, *d_pos = nullptr;
   double *V = nullptr, *work = nullptr, *ritz = nullptr, *partials = nullptr, *small = nullptr;
   size_t V_cap = 0, work_cap = 0, ritz_cap = 0, partials_cap = 0, small_cap = 0;
+  double *hsteps = nullptr;  // Gram-Schmidt coefficients of one Krylov pass (device)
+  size_t hsteps_cap = 0;
   // results
   int nritz = 0;
   int ritz_ld = 0;  // columns per plane of the Ritz vector buffer

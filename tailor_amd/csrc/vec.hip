@@ -9,6 +9,7 @@ namespace tailor {
 namespace {
 
 constexpr int VT = 256;
+constexpr int kMaxCols = 130;  // Krylov columns a kernel may take (ncv <= 128, plus one)
 
 __global__ void k_nat_to_elim(long long n, const int *__restrict__ perm, const double *__restrict__ src,
                               double *__restrict__ dr, double *__restrict__ di) {
@@ -150,10 +151,10 @@ __global__ __launch_bounds__(VT) void k_dots(long long n, int ncols, const doubl
 __global__ void k_update(long long n, int ncols, const double *__restrict__ Vr,
                          const double *__restrict__ Vi, long long ldv, const double *__restrict__ h,
                          double *__restrict__ wr, double *__restrict__ wi) {
-  __shared__ double hr[64], hi[64];
-  if (threadIdx.x < ncols) {
-    hr[threadIdx.x] = h[2 * threadIdx.x];
-    hi[threadIdx.x] = h[2 * threadIdx.x + 1];
+  __shared__ double hr[kMaxCols], hi[kMaxCols];
+  for (int t = threadIdx.x; t < ncols; t += blockDim.x) {
+    hr[t] = h[2 * t];
+    hi[t] = h[2 * t + 1];
   }
   __syncthreads();
   const long long i = static_cast<long long>(blockIdx.x) * VT + threadIdx.x;
@@ -169,32 +170,77 @@ __global__ void k_update(long long n, int ncols, const double *__restrict__ Vr,
 }
 
 // Out[:, 0:k] = V[:, 0:m] C, C m x k column-major complex (interleaved).
-__global__ void k_combine(long long n, int m, int k, const double *__restrict__ Vr,
-                          const double *__restrict__ Vi, long long ldv, const double *__restrict__ C,
-                          double *__restrict__ Or, double *__restrict__ Oi, long long ldo) {
-  __shared__ double cr[64 * 64], ci[64 * 64];
-  for (int t = threadIdx.x; t < m * k; t += VT) {
-    cr[t] = C[2 * t];
-    ci[t] = C[2 * t + 1];
+// A workgroup covers VT rows and kCombineCols output columns (blockIdx.y
+// selects the column chunk); its columns of C are staged in LDS and each
+// output is summed over a = 0..m-1 in order.
+constexpr int kCombineCols = 16;
+__global__ __launch_bounds__(VT) void k_combine(long long n, int m, int k,
+                                                const double *__restrict__ Vr,
+                                                const double *__restrict__ Vi, long long ldv,
+                                                const double *__restrict__ C,
+                                                double *__restrict__ Or, double *__restrict__ Oi,
+                                                long long ldo) {
+  __shared__ double cr[kMaxCols * kCombineCols], ci[kMaxCols * kCombineCols];
+  const int b0 = blockIdx.y * kCombineCols;
+  const int kb = min(kCombineCols, k - b0);
+  for (int t = threadIdx.x; t < m * kb; t += VT) {
+    const int a = t % m, b = t / m;
+    cr[b * m + a] = C[2 * ((b0 + b) * m + a)];
+    ci[b * m + a] = C[2 * ((b0 + b) * m + a) + 1];
   }
   __syncthreads();
   const long long i = static_cast<long long>(blockIdx.x) * VT + threadIdx.x;
   if (i >= n) return;
-  double vr[64], vi[64];
+  double sr[kCombineCols], si[kCombineCols];
+#pragma unroll
+  for (int b = 0; b < kCombineCols; ++b) {
+    sr[b] = 0.0;
+    si[b] = 0.0;
+  }
   for (int a = 0; a < m; ++a) {
-    vr[a] = Vr[a * ldv + i];
-    vi[a] = Vi[a * ldv + i];
-  }
-  for (int b = 0; b < k; ++b) {
-    double sr = 0.0, si = 0.0;
-    for (int a = 0; a < m; ++a) {
-      const double c_r = cr[b * m + a], c_i = ci[b * m + a];
-      sr += vr[a] * c_r - vi[a] * c_i;
-      si += vr[a] * c_i + vi[a] * c_r;
+    const double vr = Vr[a * ldv + i], vi = Vi[a * ldv + i];
+#pragma unroll
+    for (int b = 0; b < kCombineCols; ++b) {
+      if (b < kb) {
+        const double c_r = cr[b * m + a], c_i = ci[b * m + a];
+        sr[b] += vr * c_r - vi * c_i;
+        si[b] += vr * c_i + vi * c_r;
+      }
     }
-    Or[b * ldo + i] = sr;
-    Oi[b * ldo + i] = si;
   }
+#pragma unroll
+  for (int b = 0; b < kCombineCols; ++b) {
+    if (b < kb) {
+      Or[(b0 + b) * ldo + i] = sr[b];
+      Oi[(b0 + b) * ldo + i] = si[b];
+    }
+  }
+}
+
+// out[j] = sum over blocks b (in order) of partials[b][j], complex.
+__global__ void k_reduce_partials(const double *__restrict__ partials, int nblocks, int ncols,
+                                  double *__restrict__ out) {
+  for (int j = threadIdx.x; j < ncols; j += blockDim.x) {
+    double sr = 0.0, si = 0.0;
+    for (int b = 0; b < nblocks; ++b) {
+      sr += partials[2 * (static_cast<long long>(b) * ncols + j)];
+      si += partials[2 * (static_cast<long long>(b) * ncols + j) + 1];
+    }
+    out[2 * j] = sr;
+    out[2 * j + 1] = si;
+  }
+}
+
+// y = x / sqrt(max(0, Re s2[0])): normalisation by a norm computed on the
+// device (s2 = x^H x from k_dots + k_reduce_partials).
+__global__ void k_scale_inv_norm(long long n, const double *__restrict__ s2,
+                                 const double *__restrict__ xr, const double *__restrict__ xi,
+                                 double *__restrict__ yr, double *__restrict__ yi) {
+  const long long i = static_cast<long long>(blockIdx.x) * VT + threadIdx.x;
+  if (i >= n) return;
+  const double inv = 1.0 / sqrt(fmax(0.0, s2[0]));
+  yr[i] = inv * xr[i];
+  yi[i] = inv * xi[i];
 }
 
 // y = alpha x + beta y (complex scalars)
@@ -249,14 +295,30 @@ hipError_t vec_dots(hipStream_t s, long long n, int ncols, const double *Vr, con
 
 hipError_t vec_update(hipStream_t s, long long n, int ncols, const double *Vr, const double *Vi,
                       long long ldv, const double *h, double *wr, double *wi) {
+  if (ncols > kMaxCols) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_update, dim3(vblocks(n)), dim3(VT), 0, s, n, ncols, Vr, Vi, ldv, h, wr, wi);
   return hipGetLastError();
 }
 
 hipError_t vec_combine(hipStream_t s, long long n, int m, int k, const double *Vr, const double *Vi,
                        long long ldv, const double *C, double *Or, double *Oi, long long ldo) {
-  hipLaunchKernelGGL(k_combine, dim3(vblocks(n)), dim3(VT), 0, s, n, m, k, Vr, Vi, ldv, C, Or, Oi,
-                     ldo);
+  if (m > kMaxCols || k <= 0) return k <= 0 ? hipSuccess : hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_combine, dim3(vblocks(n), (k + kCombineCols - 1) / kCombineCols), dim3(VT),
+                     0, s, n, m, k, Vr, Vi, ldv, C, Or, Oi, ldo);
+  return hipGetLastError();
+}
+
+hipError_t vec_dots_device(hipStream_t s, long long n, int ncols, const double *Vr,
+                           const double *Vi, long long ldv, const double *wr, const double *wi,
+                           double *partials, int nblocks, double *out) {
+  hipLaunchKernelGGL(k_dots, dim3(nblocks), dim3(VT), 0, s, n, ncols, Vr, Vi, ldv, wr, wi, partials);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(VT), 0, s, partials, nblocks, ncols, out);
+  return hipGetLastError();
+}
+
+hipError_t vec_scale_inv_norm(hipStream_t s, long long n, const double *s2, const double *xr,
+                              const double *xi, double *yr, double *yi) {
+  hipLaunchKernelGGL(k_scale_inv_norm, dim3(vblocks(n)), dim3(VT), 0, s, n, s2, xr, xi, yr, yi);
   return hipGetLastError();
 }
 

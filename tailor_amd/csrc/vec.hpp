@@ -25,5 +25,13 @@ hipError_t vec_combine(hipStream_t s, long long n, int m, int k, const double *V
                        long long ldv, const double *C, double *Or, double *Oi, long long ldo);
 hipError_t vec_axpby(hipStream_t s, long long n, double a_r, double a_i, const double *xr,
                      const double *xi, double b_r, double b_i, double *yr, double *yi);
+// out[j] = V[:, j]^H w for j < ncols, summed on the device (partials: scratch
+// of 2 nblocks ncols doubles; out: ncols interleaved complex).
+hipError_t vec_dots_device(hipStream_t s, long long n, int ncols, const double *Vr,
+                           const double *Vi, long long ldv, const double *wr, const double *wi,
+                           double *partials, int nblocks, double *out);
+// y = x / sqrt(Re s2[0]) with s2 in device memory.
+hipError_t vec_scale_inv_norm(hipStream_t s, long long n, const double *s2, const double *xr,
+                              const double *xi, double *yr, double *yi);
 
 }  // namespace tailor

@@ -87,6 +87,30 @@ def test_eigenvalues_match_dense_lapack(oracle_lib, shape, omega):
         assert abs(q[k].imag) <= 1e-14 and q[k].real > 0
 
 
+def test_large_krylov_subspace_matches_dense_lapack(oracle_lib):
+    """ncv = 100 (above the 64 columns of the sweeps): the Gram-Schmidt
+    update and the restart combination take every basis column."""
+    c = small_case(24, 12)
+    A, B = operators(oracle_lib, c)
+    omega = 0.3 + 0.05j
+    sigma = complex(omega.imag, -omega.real)
+    ref = dense_reference(A, B, sigma, 8)
+    with ta.Context(0) as ctx:
+        ctx.load_case(c)
+        ctx.assemble(diagnostics=False)
+        d = ctx.solve(nev=8, omega=omega, tol=1e-10, ncv=100)
+        pairs = ctx.eigenpairs(vectors=True)
+    print(f"ncv 100: converged {d['converged']} restarts {d['iterations']} solves {d['solves']} "
+          f"max residual {d['max_residual']:.2e}", flush=True)
+    got = np.array([p["lambda"] for p in pairs[:8]])
+    for lam_ref in ref:
+        k = np.argmin(np.abs(got - lam_ref))
+        assert abs(got[k] - lam_ref) <= 1e-8 * max(1.0, abs(lam_ref)), (got[k], lam_ref)
+    for p in pairs[:8]:
+        q = p["mode"]
+        assert np.linalg.norm(A @ q - p["lambda"] * (B @ q)) / np.linalg.norm(q) <= 1e-10
+
+
 @pytest.mark.parametrize("shape,sigma", [((24, 12), 0.3 - 0.2j), ((40, 16), -0.02j)])
 def test_mixed_precision_solve_refines_to_1e12(oracle_lib, shape, sigma):
     """Complex FP32 fronts + FP64 refinement: relative residual 1e-12."""
