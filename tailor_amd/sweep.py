@@ -16,11 +16,18 @@ Forms of a sweep:
 
 One context per grid size is kept for the whole sweep, so the symbolic
 analysis and device buffers are built once per size; each instance uploads
-its grid and base flow and reassembles.
+its grid and base flow and reassembles.  With `contexts=2` the learned form
+runs two contexts per size as a pipeline: while one context runs the
+current instance's Krylov-Schur, a worker thread assembles and factors the
+next instance on the other (each context warm-starts from the instance it
+solved before, two steps back in wavenumber).  The two HIP streams are
+non-blocking, so the GEMV-bound Krylov phase fills the GPU during the
+factorization's latency-bound panel phases.
 """
 from __future__ import annotations
 
 import math
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -114,10 +121,13 @@ def training_instances(count: int, seed: int = 2, ny: int = 100, nz: int = 50) -
 
 class Sweeper:
     def __init__(self, device: int = 0, nev: int = 8, tol: float = 1e-10, ncv: int = 64,
-                 omega_s: complex = 0.02 + 0j, maxit: int = 1000):
+                 omega_s: complex = 0.02 + 0j, maxit: int = 1000, contexts: int = 1):
+        if contexts not in (1, 2):
+            raise ValueError("contexts must be 1 or 2")
         self.device = device
         self.nev, self.tol, self.ncv, self.maxit = nev, tol, ncv, maxit
         self.omega_s = complex(omega_s)
+        self.contexts = contexts
         self.ctx = {}
         self.tuned = {}  # per grid size: the autotuner's report
 
@@ -126,11 +136,19 @@ class Sweeper:
             c.close()
         self.ctx = {}
 
-    def context(self, ny: int, nz: int) -> Context:
-        key = (ny, nz)
+    def context(self, ny: int, nz: int, slot: int = 0) -> Context:
+        key = (ny, nz, slot)
         if key not in self.ctx:
             self.ctx[key] = Context(self.device)
         return self.ctx[key]
+
+    def _tune(self, ctx, case, precision):
+        """First use of a context: build its plan and tune its GEMM tiles
+        (not part of a sweep's time)."""
+        ctx.load_case(case)
+        ctx.assemble(diagnostics=False)
+        ctx.set_precision(precision)
+        return ctx.autotune(complex(self.omega_s.imag, -self.omega_s.real))
 
     def _solve(self, ctx, omega, precision, warm):
         ctx.set_precision(precision)
@@ -153,6 +171,8 @@ class Sweeper:
         if precision is None:
             precision = PRECISION_MIXED if form == "mixed" else PRECISION_FP64
         idx = list(range(len(cases))) if order is None else list(order)
+        if self.contexts == 2 and form == "learned" and len(idx) > 1:
+            return self._run_pipelined(cases, shifts, precision, idx, on_record)
         result = SweepResult(form)
         last_size = None
         t_all = time.perf_counter()
@@ -195,6 +215,74 @@ class Sweeper:
         result.seconds = time.perf_counter() - t_all - t_tune
         return result
 
+
+    def _run_pipelined(self, cases, shifts, precision, idx, on_record) -> SweepResult:
+        """Learned form over two contexts per grid size (see the module
+        docstring).  Instance k's record covers its Krylov stage and any
+        fallback; the sweep's time is the wall time of the whole pipeline."""
+        result = SweepResult("learned")
+        # plans and tuning of both contexts of every size first (untimed)
+        for k in idx:
+            inst, case = cases[k]
+            size = (inst.ny, inst.nz)
+            for slot in (0, 1):
+                if (inst.ny, inst.nz, slot) not in self.ctx:
+                    rep = self._tune(self.context(inst.ny, inst.nz, slot), case, precision)
+                    self.tuned.setdefault(size, rep)
+        solved = set()  # contexts that hold Ritz vectors of an earlier instance
+
+        def stage(slot, k):
+            inst, case = cases[k]
+            ctx = self.context(inst.ny, inst.nz, slot)
+            ctx.load_case(case)
+            ctx.assemble(diagnostics=False)
+            ctx.set_precision(precision)
+            om = complex(shifts[k])
+            ctx.factor(complex(om.imag, -om.real))
+
+        t_all = time.perf_counter()
+        stage(0, idx[0])
+        for i, k in enumerate(idx):
+            slot = i % 2
+            inst, _ = cases[k]
+            ctx = self.context(inst.ny, inst.nz, slot)
+            err = []
+            worker = None
+            if i + 1 < len(idx):
+                def run_stage(s=1 - slot, kk=idx[i + 1]):
+                    try:
+                        stage(s, kk)
+                    except Exception as e:  # re-raised by the main thread
+                        err.append(e)
+                worker = threading.Thread(target=run_stage)
+                worker.start()
+            t0 = time.perf_counter()
+            omega = complex(shifts[k])
+            key = (inst.ny, inst.nz, slot)
+            d, om, res, good = self._solve(ctx, omega, precision, key in solved)
+            solved.add(key)
+            facts, fallback = 1, False
+            restarts, solves, refine = d["iterations"], d["solves"], d["refinement_steps"]
+            if good < self.nev:
+                fallback = True
+                d, om, res, good = self._solve(ctx, self.omega_s, precision, False)
+                facts += 1
+                restarts += d["iterations"]
+                solves += d["solves"]
+                refine += d["refinement_steps"]
+                omega = self.omega_s
+            el = time.perf_counter() - t0
+            if worker is not None:
+                worker.join()
+            if err:
+                raise err[0]
+            rec = Record(inst, omega, d["converged"], good, om, res, restarts, solves, refine,
+                         facts, fallback, el)
+            result.records.append(rec)
+            if on_record:
+                on_record(rec)
+        result.seconds = time.perf_counter() - t_all
+        return result
 
 def prepare(instances, omega_s: complex = 0.02 + 0j) -> list:
     """Host-side case construction (preprocessing, outside any timed region)."""

@@ -37,3 +37,36 @@ def test_learned_shifts_cut_solves():
             d = np.min(np.abs(p.omegas - w)) / abs(w)
             if d < 1e-6:  # found by both forms
                 assert d < 1e-9, (w, d)
+
+
+def test_pipelined_learned_sweep_matches_sequential():
+    """Two contexts in a pipeline (the next instance's assembly and
+    factorization run while the current instance's Krylov-Schur runs) give
+    every pair at the tolerance and the eigenvalues of the one-context
+    sweep."""
+    from tailor_amd import sweep as sw
+    from tailor_amd.shift_model import ShiftModel
+    from tailor_amd.train_shift_model import MODEL
+    model = ShiftModel(0).load(MODEL)
+    insts = sw.sweep_instances(5, seed=4, sizes=[(100, 50)], reynolds=1500.0, mach=0.7,
+                               per_instance_seeds=False)
+    cases = sw.prepare(insts)
+    alpha = np.array([i.alpha for i in insts])
+    shifts = model.shift_centres(alpha, np.full(5, 1500.0), np.full(5, 0.7))
+    order = sw.alpha_order(insts)
+    seq_runner = sw.Sweeper(ncv=40)
+    seq = seq_runner.run(cases, "learned", shifts=shifts, order=order)
+    seq_runner.close()
+    pipe_runner = sw.Sweeper(ncv=40, contexts=2)
+    pipe = pipe_runner.run(cases, "learned", shifts=shifts, order=order)
+    pipe_runner.close()
+    print(f"sequential {seq.totals()}\npipelined  {pipe.totals()}", flush=True)
+    assert [id(r.instance) for r in pipe.records] == [id(r.instance) for r in seq.records]
+    assert pipe.good_pairs == 8 * len(insts)
+    for a, b in zip(seq.records, pipe.records):
+        wa = np.sort_complex(a.omegas[a.residuals <= 1e-10])
+        wb = np.sort_complex(b.omegas[b.residuals <= 1e-10])
+        for w in wb:
+            d = np.min(np.abs(wa - w)) / abs(w)
+            if d < 1e-6:
+                assert d < 1e-9, (w, d)
