@@ -5,8 +5,9 @@ Headline (`value`): eigenpairs/s of a wavenumber sweep on the 200x100 case
 Ma=0.7, alpha ~ U[0.5, 3] from seed 1) in the learned-shift form: the shift
 network proposes each instance's shift centre, instances run in wavenumber
 order with warm-started Krylov-Schur, and the fixed-shift search is the
-fallback; two contexts pipeline the sweep (the next instance's assembly and
-factorization run while the current instance's Krylov-Schur runs).  A step is one sweep instance (assembly of A_bc, B_bc, sparse LU
+fallback; three contexts pipeline the sweep (the next two instances'
+assembly and factorization run while the current instance's Krylov-Schur
+runs).  A step is one sweep instance (assembly of A_bc, B_bc, sparse LU
 of A_bc - sigma B_bc, Krylov-Schur for 8 eigenpairs); an eigenpair counts if
 its true residual ||A q - lambda B q|| / ||q|| <= 1e-10.  The same instances
 are also run in the plain form (complex FP64, fixed omega_s = 0.02, cold
@@ -37,6 +38,7 @@ FP64_PEAK_TFLOPS = 78.6
 NEV, TOL, OMEGA = 8, 1e-10, 0.02 + 0j
 NCV = 64  # Krylov subspace size (the -eps_ncv of the reference's options database)
 LEARNED_NCV = 40  # subspace of the learned form: the wanted cluster converges in one pass
+PIPELINE_CONTEXTS = 3  # learned sweep: Krylov-Schur of one instance, factorization of the next two
 
 
 def algorithmic_assembly_bytes(ny: int, nz: int, ssx: int, sse: int) -> int:
@@ -152,7 +154,7 @@ def sweeps(ta, steps: int, warmup: int, ncv: int, learned_ncv: int) -> dict:
     cases = sw.prepare(insts, OMEGA)
     alphas = np.array([i.alpha for i in insts])
     shifts = model.shift_centres(alphas, np.full(len(insts), 1500.0), np.full(len(insts), 0.7))
-    runner = sw.Sweeper(ncv=learned_ncv, omega_s=OMEGA, contexts=2)
+    runner = sw.Sweeper(ncv=learned_ncv, omega_s=OMEGA, contexts=PIPELINE_CONTEXTS)
     plain_runner = sw.Sweeper(ncv=ncv, omega_s=OMEGA)
     # warm-up instances (untimed): build the plan, and let the first one
     # choose the factorization precision by measurement
@@ -233,8 +235,9 @@ def main() -> int:
         "config": {"workload": "BiGlobal wavenumber sweep, 200x100 FD-q grid (q_y=10, q_z=6), "
                                "100,000 unknowns, Re=1500, Ma=0.7, alpha ~ U[0.5,3] seed 1, nev=8, "
                                "tol=1e-10; learned shift centres, warm-started Krylov-Schur, "
-                               "fixed-shift fallback, two-context pipeline",
+                               "fixed-shift fallback, three-context pipeline",
                    "nev": NEV, "ncv_learned": args.learned_ncv, "ncv_plain": args.ncv,
+                   "pipeline_contexts": PIPELINE_CONTEXTS,
                    "instances": args.steps,
                    "precision_choice": sw["choice"], "gemm_autotune": sw["autotune"]},
         "learned_sweep": L.totals(),

@@ -16,11 +16,11 @@ Forms of a sweep:
 
 One context per grid size is kept for the whole sweep, so the symbolic
 analysis and device buffers are built once per size; each instance uploads
-its grid and base flow and reassembles.  With `contexts=2` the learned form
-runs two contexts per size as a pipeline: while one context runs the
-current instance's Krylov-Schur, a worker thread assembles and factors the
-next instance on the other (each context warm-starts from the instance it
-solved before, two steps back in wavenumber).  The two HIP streams are
+its grid and base flow and reassembles.  With `contexts=C > 1` the learned
+form runs C contexts per size as a pipeline: while one context runs the
+current instance's Krylov-Schur, worker threads assemble and factor the
+next C - 1 instances on the others (each context warm-starts from the
+instance it solved before, C steps back in wavenumber).  The two HIP streams are
 non-blocking, so the GEMV-bound Krylov phase fills the GPU during the
 factorization's latency-bound panel phases.
 """
@@ -122,8 +122,8 @@ def training_instances(count: int, seed: int = 2, ny: int = 100, nz: int = 50) -
 class Sweeper:
     def __init__(self, device: int = 0, nev: int = 8, tol: float = 1e-10, ncv: int = 64,
                  omega_s: complex = 0.02 + 0j, maxit: int = 1000, contexts: int = 1):
-        if contexts not in (1, 2):
-            raise ValueError("contexts must be 1 or 2")
+        if not 1 <= contexts <= 4:
+            raise ValueError("contexts must be 1 to 4")
         self.device = device
         self.nev, self.tol, self.ncv, self.maxit = nev, tol, ncv, maxit
         self.omega_s = complex(omega_s)
@@ -171,7 +171,7 @@ class Sweeper:
         if precision is None:
             precision = PRECISION_MIXED if form == "mixed" else PRECISION_FP64
         idx = list(range(len(cases))) if order is None else list(order)
-        if self.contexts == 2 and form == "learned" and len(idx) > 1:
+        if self.contexts > 1 and form == "learned" and len(idx) > 1:
             return self._run_pipelined(cases, shifts, precision, idx, on_record)
         result = SweepResult(form)
         last_size = None
@@ -217,45 +217,55 @@ class Sweeper:
 
 
     def _run_pipelined(self, cases, shifts, precision, idx, on_record) -> SweepResult:
-        """Learned form over two contexts per grid size (see the module
-        docstring).  Instance k's record covers its Krylov stage and any
-        fallback; the sweep's time is the wall time of the whole pipeline."""
+        """Learned form over `self.contexts` contexts per grid size (see the
+        module docstring): instance i runs its Krylov-Schur on context
+        i mod C while worker threads assemble and factor the next C - 1
+        instances on the others.  Instance k's record covers its Krylov stage
+        and any fallback; the sweep's time is the wall time of the whole
+        pipeline."""
+        C = self.contexts
         result = SweepResult("learned")
-        # plans and tuning of both contexts of every size first (untimed)
+        # plans and tuning of every context of every size first (untimed)
         for k in idx:
             inst, case = cases[k]
             size = (inst.ny, inst.nz)
-            for slot in (0, 1):
+            for slot in range(C):
                 if (inst.ny, inst.nz, slot) not in self.ctx:
                     rep = self._tune(self.context(inst.ny, inst.nz, slot), case, precision)
                     self.tuned.setdefault(size, rep)
         solved = set()  # contexts that hold Ritz vectors of an earlier instance
+        errors = []
 
         def stage(slot, k):
-            inst, case = cases[k]
-            ctx = self.context(inst.ny, inst.nz, slot)
-            ctx.load_case(case)
-            ctx.assemble(diagnostics=False)
-            ctx.set_precision(precision)
-            om = complex(shifts[k])
-            ctx.factor(complex(om.imag, -om.real))
+            try:
+                inst, case = cases[k]
+                ctx = self.context(inst.ny, inst.nz, slot)
+                ctx.load_case(case)
+                ctx.assemble(diagnostics=False)
+                ctx.set_precision(precision)
+                om = complex(shifts[k])
+                ctx.factor(complex(om.imag, -om.real))
+            except Exception as e:  # re-raised by the main thread
+                errors.append(e)
+
+        def start(i):
+            th = threading.Thread(target=stage, args=(i % C, idx[i]))
+            th.start()
+            return th
 
         t_all = time.perf_counter()
-        stage(0, idx[0])
+        workers = {i: start(i) for i in range(min(C - 1, len(idx)))}
         for i, k in enumerate(idx):
-            slot = i % 2
+            if i + C - 1 < len(idx):
+                workers[i + C - 1] = start(i + C - 1)  # its context finished instance i - 1
+            workers.pop(i).join()
+            if errors:
+                for th in workers.values():
+                    th.join()
+                raise errors[0]
+            slot = i % C
             inst, _ = cases[k]
             ctx = self.context(inst.ny, inst.nz, slot)
-            err = []
-            worker = None
-            if i + 1 < len(idx):
-                def run_stage(s=1 - slot, kk=idx[i + 1]):
-                    try:
-                        stage(s, kk)
-                    except Exception as e:  # re-raised by the main thread
-                        err.append(e)
-                worker = threading.Thread(target=run_stage)
-                worker.start()
             t0 = time.perf_counter()
             omega = complex(shifts[k])
             key = (inst.ny, inst.nz, slot)
@@ -272,10 +282,6 @@ class Sweeper:
                 refine += d["refinement_steps"]
                 omega = self.omega_s
             el = time.perf_counter() - t0
-            if worker is not None:
-                worker.join()
-            if err:
-                raise err[0]
             rec = Record(inst, omega, d["converged"], good, om, res, restarts, solves, refine,
                          facts, fallback, el)
             result.records.append(rec)

@@ -39,11 +39,11 @@ def test_learned_shifts_cut_solves():
                 assert d < 1e-9, (w, d)
 
 
-def test_pipelined_learned_sweep_matches_sequential():
-    """Two contexts in a pipeline (the next instance's assembly and
-    factorization run while the current instance's Krylov-Schur runs) give
-    every pair at the tolerance and the eigenvalues of the one-context
-    sweep."""
+@pytest.mark.parametrize("contexts", [2, 3])
+def test_pipelined_learned_sweep_matches_sequential(contexts):
+    """Contexts in a pipeline (the next instances' assembly and factorization
+    run while the current instance's Krylov-Schur runs) give every pair at
+    the tolerance and the eigenvalues of the one-context sweep."""
     from tailor_amd import sweep as sw
     from tailor_amd.shift_model import ShiftModel
     from tailor_amd.train_shift_model import MODEL
@@ -57,7 +57,7 @@ def test_pipelined_learned_sweep_matches_sequential():
     seq_runner = sw.Sweeper(ncv=40)
     seq = seq_runner.run(cases, "learned", shifts=shifts, order=order)
     seq_runner.close()
-    pipe_runner = sw.Sweeper(ncv=40, contexts=2)
+    pipe_runner = sw.Sweeper(ncv=40, contexts=contexts)
     pipe = pipe_runner.run(cases, "learned", shifts=shifts, order=order)
     pipe_runner.close()
     print(f"sequential {seq.totals()}\npipelined  {pipe.totals()}", flush=True)
