@@ -1327,17 +1327,17 @@ constexpr int GKC = 16;  // K chunk staged in LDS
 // Tile shapes (chosen per level by the autotuner): WMW x WNW waves, each
 // owning WMB x WNB MFMA 16x16 blocks, so a workgroup computes a
 // (16 WMW WMB) x (16 WNW WNB) complex tile.
-//   shape 0: 64 x 64, 256 threads, K chunk 16;
-//   shape 1: 64 x 64, 256 threads, K chunk 4, registers held to four waves
-//            per SIMD (a quarter of shape 0's LDS);
-//   shape 2: 128 x 128, 512 threads;
-//   shape 3: 64 x 64, 256 threads, K chunk 8, three waves per SIMD.
+//   shape 0: 64 x 64, 256 threads, K chunk 16, four real products;
+//   shape 1: 64 x 64, 256 threads, K chunk 4, 3M form, registers held to
+//            three waves per SIMD (a quarter of shape 0's LDS);
+//   shape 2: 128 x 128, 512 threads, four real products;
+//   shape 3: 64 x 64, 256 threads, K chunk 8, 3M form, two waves per SIMD.
 // The shorter K chunks trade more barriers for more resident workgroups,
 // which hide the first chunk's and C's load latency of each tile.
 enum { kShape64x64 = 0, kShape64x64k4 = 1, kShape128x128 = 2, kShape64x64k8 = 3, kShapeCount = 4 };
 static_assert(kShapeCount == kGemmShapes, "shape count");
 
-template <typename T, int WMW, int WNW, int WMB, int WNB, int KC = GKC, int MINW = 1>
+template <typename T, int WMW, int WNW, int WMB, int WNB, int KC = GKC, int MINW = 1, bool M3 = false>
 __global__ __launch_bounds__(64 * WMW * WNW, MINW) void k_gemm(const int *__restrict__ list,
                                                                int nlist,
                                                                const FrontDev *__restrict__ fr,
@@ -1447,11 +1447,17 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void k_gemm(const int *__rest
       sBi[buf][e / GKC][e % GKC] = -gb_i[s];
     }
   };
-  // The accumulators start from C and the MFMAs add (-B)^T A^T (B is
-  // staged negated), so the tile ends with plain stores; the C loads are
-  // issued with the first chunk's loads and share their latency.
+  // B is staged negated (Bn = -B) and the accumulators start from C, so the
+  // tile ends with plain stores; the C loads are issued with the first
+  // chunk's loads and share their latency.  Complex products:
+  //  * four real MFMAs:  cr = Cr + sum (Ar Bnr - Ai Bni),  ci = Ci + sum (Ar Bni + Ai Bnr);
+  //  * three (M3, the 3M form):  cr = x = Cr + sum Ar Bnr,  cy = y = -sum Ai Bni,
+  //    ci = z = Cr + Ci + sum (Ar + Ai)(Bnr + Bni), and C - AB = (x + y) + i (z - x + y).
+  //    The 3M form is normwise stable (its error bound is a small multiple
+  //    of the four-product form's) and needs a third accumulator set.
   using acc_t = typename Mfma<T>::acc_t;
-  acc_t cr[WMB][WNB], ci[WMB][WNB];
+  constexpr int NY = M3 ? WMB : 1;
+  acc_t cr[WMB][WNB], ci[WMB][WNB], cy[NY][WNB];
 #pragma unroll
   for (int a = 0; a < WMB; ++a) {
     const int row = row0 + wm * 16 * WMB + a * 16 + l % 16;
@@ -1473,7 +1479,12 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void k_gemm(const int *__rest
           xi = v.C2i[ic];
         }
         cr[a][b][e] = xr;
-        ci[a][b][e] = xi;
+        if constexpr (M3) {
+          ci[a][b][e] = xr + xi;
+          cy[a][b][e] = T(0);
+        } else {
+          ci[a][b][e] = xi;
+        }
       }
     }
   }
@@ -1498,15 +1509,31 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void k_gemm(const int *__rest
         b_r[q] = sBr[buf][wn * 16 * WNB + q * 16 + l % 16][kr];
         b_i[q] = sBi[buf][wn * 16 * WNB + q * 16 + l % 16][kr];
       }
+      if constexpr (M3) {
+        T a_s[WMB], b_s[WNB];
 #pragma unroll
-      for (int a = 0; a < WMB; ++a)
+        for (int q = 0; q < WMB; ++q) a_s[q] = a_r[q] + a_i[q];
 #pragma unroll
-        for (int b = 0; b < WNB; ++b) {
-          cr[a][b] = Mfma<T>::mma(b_r[b], a_r[a], cr[a][b]);
-          cr[a][b] = Mfma<T>::mma(b_i[b], -a_i[a], cr[a][b]);
-          ci[a][b] = Mfma<T>::mma(b_i[b], a_r[a], ci[a][b]);
-          ci[a][b] = Mfma<T>::mma(b_r[b], a_i[a], ci[a][b]);
-        }
+        for (int q = 0; q < WNB; ++q) b_s[q] = b_r[q] + b_i[q];
+#pragma unroll
+        for (int a = 0; a < WMB; ++a)
+#pragma unroll
+          for (int b = 0; b < WNB; ++b) {
+            cr[a][b] = Mfma<T>::mma(b_r[b], a_r[a], cr[a][b]);
+            cy[a][b] = Mfma<T>::mma(b_i[b], -a_i[a], cy[a][b]);
+            ci[a][b] = Mfma<T>::mma(b_s[b], a_s[a], ci[a][b]);
+          }
+      } else {
+#pragma unroll
+        for (int a = 0; a < WMB; ++a)
+#pragma unroll
+          for (int b = 0; b < WNB; ++b) {
+            cr[a][b] = Mfma<T>::mma(b_r[b], a_r[a], cr[a][b]);
+            cr[a][b] = Mfma<T>::mma(b_i[b], -a_i[a], cr[a][b]);
+            ci[a][b] = Mfma<T>::mma(b_i[b], a_r[a], ci[a][b]);
+            ci[a][b] = Mfma<T>::mma(b_r[b], a_i[a], ci[a][b]);
+          }
+      }
     }
     if (more) stash(buf ^ 1);
     __syncthreads();
@@ -1521,21 +1548,37 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void k_gemm(const int *__rest
       for (int e = 0; e < 4; ++e) {
         const int col = col0 + wn * 16 * WNB + b * 16 + Mfma<T>::ocol(l, e);
         if (col >= v.N) continue;
+        T re = cr[a][b][e], im = ci[a][b][e];
+        if constexpr (M3) {
+          const T y = cy[a][b][e];
+          im = (im - re) + y;
+          re = re + y;
+        }
         if (col < v.N1) {
           if (row >= v.M) continue;
           const long long ic = static_cast<long long>(col) * v.ldc + row;
-          v.Cr[ic] = cr[a][b][e];
-          v.Ci[ic] = ci[a][b][e];
+          v.Cr[ic] = re;
+          v.Ci[ic] = im;
         } else {
           if (row >= v.M2) continue;
           const long long ic = static_cast<long long>(col - v.N1) * v.ldc2 + row;
-          v.C2r[ic] = cr[a][b][e];
-          v.C2i[ic] = ci[a][b][e];
+          v.C2r[ic] = re;
+          v.C2i[ic] = im;
         }
       }
     }
   }
 }
+
+// Product form and waves-per-SIMD bound of each tile shape.  The K-chunk-4
+// and -8 tiles use the 3M form; its third accumulator set spills at four and
+// three waves per SIMD, so their bounds are three and two (measured on the
+// 200x100 factorization: K-chunk-8 tiles 127.5 ms of GEMM with four products
+// at three waves, 125.1 ms with 3M at two; look-ahead factorization 196 ->
+// 183 ms).  The K-chunk-16 tiles keep four products (3M would leave one wave
+// per SIMD), and so do the 128x128 tiles (3M spills at 256 registers).
+constexpr int kGemmK4Waves = 3, kGemmK8Waves = 2;
+constexpr bool kGemmK4M3 = true, kGemmK8M3 = true, kGemmK16M3 = false, kGemm128M3 = false;
 
 // Launch one batched GEMM with the given tile shape; rows/cols are upper
 // bounds of M and N over the level's fronts.
@@ -1554,20 +1597,20 @@ hipError_t launch_gemm(int shape, int order, long long rows, long long cols, int
   const dim3 grid(tiles, count), block(threads);
   switch (shape) {
     case kShape64x64k4:
-      hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 2, 4, 4>), grid, block, 0, st, list, count, fr,
-                         region, k0, nb, K0, NB, order, arena);
+      hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 2, 4, kGemmK4Waves, kGemmK4M3>), grid, block, 0, st,
+                         list, count, fr, region, k0, nb, K0, NB, order, arena);
       break;
     case kShape128x128:
-      hipLaunchKernelGGL((k_gemm<T, 2, 4, 4, 2>), grid, block, 0, st, list, count, fr, region, k0,
-                         nb, K0, NB, order, arena);
+      hipLaunchKernelGGL((k_gemm<T, 2, 4, 4, 2, GKC, 1, kGemm128M3>), grid, block, 0, st, list,
+                         count, fr, region, k0, nb, K0, NB, order, arena);
       break;
     case kShape64x64k8:
-      hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 2, 8, 3>), grid, block, 0, st, list, count, fr,
-                         region, k0, nb, K0, NB, order, arena);
+      hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 2, 8, kGemmK8Waves, kGemmK8M3>), grid, block, 0, st,
+                         list, count, fr, region, k0, nb, K0, NB, order, arena);
       break;
     default:
-      hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 2>), grid, block, 0, st, list, count, fr, region, k0,
-                         nb, K0, NB, order, arena);
+      hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 2, GKC, 1, kGemmK16M3>), grid, block, 0, st, list,
+                         count, fr, region, k0, nb, K0, NB, order, arena);
       break;
   }
   return hipGetLastError();
