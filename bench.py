@@ -38,6 +38,7 @@ FP64_PEAK_TFLOPS = 78.6
 NEV, TOL, OMEGA = 8, 1e-10, 0.02 + 0j
 NCV = 64  # Krylov subspace size (the -eps_ncv of the reference's options database)
 LEARNED_NCV = 40  # subspace of the learned form: the wanted cluster converges in one pass
+LEARNED_CHECK_FROM = 28  # learned form: convergence also tested every second step from step 28
 PIPELINE_CONTEXTS = 3  # learned sweep: Krylov-Schur of one instance, factorization of the next two
 
 
@@ -154,7 +155,8 @@ def sweeps(ta, steps: int, warmup: int, ncv: int, learned_ncv: int) -> dict:
     cases = sw.prepare(insts, OMEGA)
     alphas = np.array([i.alpha for i in insts])
     shifts = model.shift_centres(alphas, np.full(len(insts), 1500.0), np.full(len(insts), 0.7))
-    runner = sw.Sweeper(ncv=learned_ncv, omega_s=OMEGA, contexts=PIPELINE_CONTEXTS)
+    runner = sw.Sweeper(ncv=learned_ncv, omega_s=OMEGA, contexts=PIPELINE_CONTEXTS,
+                        check_from=LEARNED_CHECK_FROM)
     plain_runner = sw.Sweeper(ncv=ncv, omega_s=OMEGA)
     # warm-up instances (untimed): build the plan, and let the first one
     # choose the factorization precision by measurement
@@ -238,6 +240,7 @@ def main() -> int:
                                "fixed-shift fallback, three-context pipeline",
                    "nev": NEV, "ncv_learned": args.learned_ncv, "ncv_plain": args.ncv,
                    "pipeline_contexts": PIPELINE_CONTEXTS,
+                   "learned_check_from": LEARNED_CHECK_FROM,
                    "instances": args.steps,
                    "precision_choice": sw["choice"], "gemm_autotune": sw["autotune"]},
         "learned_sweep": L.totals(),

@@ -227,6 +227,12 @@ typedef struct {
   int32_t warm_start;      /* 1: start from the Ritz vectors of the previous solve of this
                               context (same structure), e.g. the previous wavenumber of a
                               sweep; 0: the fixed pseudo-random start vector */
+  int32_t check_from;      /* 0: convergence is tested at the end of each pass of ncv
+                              steps (the reference's behaviour); m > 0: also after every
+                              second step from step m on, ending the pass there when the
+                              Ritz estimates of the nev wanted pairs meet tol (no
+                              counterpart in the reference: saves solves when a warm
+                              start converges inside the first pass) */
 } tailor_solve_opts;
 
 typedef struct {

@@ -87,7 +87,7 @@ class SolveOpts(C.Structure):
     _fields_ = [("nev", C.c_int32), ("ncv", C.c_int32), ("maxit", C.c_int32),
                 ("tol", C.c_double), ("omega_re", C.c_double), ("omega_im", C.c_double),
                 ("refine_max", C.c_int32), ("refine_tol", C.c_double),
-                ("warm_start", C.c_int32)]
+                ("warm_start", C.c_int32), ("check_from", C.c_int32)]
 
 
 class TuneReport(C.Structure):
@@ -316,12 +316,17 @@ def _linear_solve(self, b: np.ndarray, refine_max: int = 0, refine_tol: float = 
 
 def _solve(self, nev: int = 8, omega: complex = 0j, tol: float = 1e-10, maxit: int = 1000,
            ncv: int = 0, refine_max: int = 2, refine_tol: float = 1e-13,
-           raise_on_failure: bool = True, warm_start: bool = False) -> dict:
+           raise_on_failure: bool = True, warm_start: bool = False, check_from: int = 0) -> dict:
     """Shift-invert Krylov-Schur (EigenSolver::solve).  With warm_start the
-    iteration starts from the previous solve's Ritz vectors."""
+    iteration starts from the previous solve's Ritz vectors; with
+    check_from = m > 0 convergence is also tested after every second step
+    from step m on, and a pass ends as soon as the wanted pairs' Ritz
+    estimates meet tol."""
     omega = complex(omega)
+    if check_from < 0:
+        raise ValueError("check_from must be >= 0")
     o = SolveOpts(nev, ncv, maxit, tol, omega.real, omega.imag, refine_max, refine_tol,
-                  1 if warm_start else 0)
+                  1 if warm_start else 0, check_from)
     d = SolveDiag()
     st = self.lib.tailor_solve(self.ptr, C.byref(o), C.byref(d))
     out = _sdict(d)

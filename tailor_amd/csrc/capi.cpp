@@ -692,6 +692,7 @@ int tailor_solve(tailor_ctx *ctx, const tailor_solve_opts *opts, tailor_solve_di
                 "EigenSolver counts, tolerance, and iteration limit must be positive");
   if (!std::isfinite(opts->omega_re) || !std::isfinite(opts->omega_im))
     return fail(ctx, TAILOR_ERR_USER_INPUT, "EigenSolver.SearchCenterOmega must be finite");
+  if (opts->check_from < 0) return fail(ctx, TAILOR_ERR_USER_INPUT, "check_from must be >= 0");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   // lambda_s = Im(omega_s) - i Re(omega_s)  (EigenSolver.cpp:46-48)
   const tailor::cplx sigma(opts->omega_im, -opts->omega_re);

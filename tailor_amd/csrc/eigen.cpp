@@ -342,9 +342,11 @@ int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solv
   HIP_OK(grow(&S.ritz, S.ritz_cap, static_cast<size_t>(2 * n * ncv)));
 
   CMat Sm(ncv + 1, ncv);  // projected matrix (rows 0..ncv)
+  int nout_pass = ncv;     // columns of the last pass
   int k = 0;
   int solves = 0, refine_total = 0, restarts = 0;
   std::vector<cplx> theta;
+  std::vector<cplx> b;
   CMat T, Q;
   int nconv = 0;
   std::vector<double> resid(ncv, 1.0), relerr(ncv, 1.0);
@@ -352,7 +354,62 @@ int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solv
   const double tol = o.tol;
   double best_res = 1e300;
   int stagnant = 0;
+  // read back the coefficients of Arnoldi steps [from, to) into Sm
+  auto fill_steps = [&](int from, int to) -> int {
+    if (to <= from) return TAILOR_OK;
+    HIP_OK(hipMemcpyAsync(hhost.data() + hstride * from, S.hsteps + hstride * from,
+                          hstride * (to - from) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    for (int j = from; j < to; ++j) {
+      const double *hj = hhost.data() + hstride * j;
+      const double *h2j = hj + 2 * (ncv + 1), *w2j = hj + 4 * (ncv + 1);
+      for (int i = 0; i <= j; ++i)
+        Sm(i, j) = cplx(hj[2 * i], hj[2 * i + 1]) + cplx(h2j[2 * i], h2j[2 * i + 1]);
+      const double beta = std::sqrt(std::max(0.0, w2j[0]));
+      Sm(j + 1, j) = beta;
+      if (beta == 0.0) return fail(ctx, TAILOR_ERR_NOT_CONVERGED, "Krylov subspace broke down");
+    }
+    return TAILOR_OK;
+  };
+  // Schur form of the leading m x m projected matrix sorted by |theta|
+  // descending, its coupling row b = S(m, :) Q and the Ritz values
+  auto project = [&](int m) -> int {
+    T = CMat(m, m);
+    for (int jj = 0; jj < m; ++jj)
+      for (int ii = 0; ii < m; ++ii) T(ii, jj) = Sm(ii, jj);
+    if (!schur_decompose(T, Q)) return fail(ctx, TAILOR_ERR_NOT_CONVERGED, "projected Schur form failed");
+    for (int pos = 0; pos < m; ++pos) {
+      int best = pos;
+      for (int q = pos + 1; q < m; ++q)
+        if (std::abs(T(q, q)) > std::abs(T(best, best))) best = q;
+      if (best != pos) {
+        std::vector<bool> sel(m, false);
+        // move `best` to `pos`: select it together with the already placed ones
+        for (int q = 0; q < pos; ++q) sel[q] = true;
+        sel[best] = true;
+        schur_reorder(T, Q, sel);
+      }
+    }
+    b.assign(m, 0.0);
+    for (int jj = 0; jj < m; ++jj) {
+      cplx s = 0.0;
+      for (int q = 0; q < m; ++q) s += Sm(m, q) * Q(q, jj);
+      b[jj] = s;
+    }
+    theta.assign(m, 0.0);
+    for (int i = 0; i < m; ++i) theta[i] = T(i, i);
+    return TAILOR_OK;
+  };
+  // Ritz estimate |b y_i| / |theta_i| of pair i of the current projection
+  auto estimate = [&](int i, const std::vector<cplx> &y) {
+    cplx s = 0.0;
+    for (size_t q = 0; q < y.size(); ++q) s += b[q] * y[q];
+    return std::abs(s) / std::abs(theta[i]);
+  };
   for (restarts = 0; restarts < o.maxit; ++restarts) {
+    int m = ncv;       // columns of this pass
+    int filled = k;    // columns of Sm holding this pass's coefficients
+    bool projected = false;
     for (int j = k; j < ncv; ++j) {
       // w = Op v_j
       HIP_OK(vec_bmul(ctx->stream, d.nodes, S.d_order, ctx->d_mvals, col_r(j), col_i(j), tr, ti));
@@ -371,69 +428,47 @@ int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solv
       HIP_OK(vec_update(ctx->stream, n, j + 1, Vr, Vi, n, h2j, wr, wi));
       HIP_OK(vec_dots_device(ctx->stream, n, 1, wr, wi, n, wr, wi, S.partials, kDotBlocks, w2j));
       HIP_OK(vec_scale_inv_norm(ctx->stream, n, w2j, wr, wi, col_r(j + 1), col_i(j + 1)));
-    }
-    HIP_OK(hipMemcpyAsync(hhost.data() + hstride * k, S.hsteps + hstride * k,
-                          hstride * (ncv - k) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_OK(hipStreamSynchronize(ctx->stream));
-    for (int j = k; j < ncv; ++j) {
-      const double *hj = hhost.data() + hstride * j;
-      const double *h2j = hj + 2 * (ncv + 1), *w2j = hj + 4 * (ncv + 1);
-      for (int i = 0; i <= j; ++i)
-        Sm(i, j) = cplx(hj[2 * i], hj[2 * i + 1]) + cplx(h2j[2 * i], h2j[2 * i + 1]);
-      const double beta = std::sqrt(std::max(0.0, w2j[0]));
-      Sm(j + 1, j) = beta;
-      if (beta == 0.0) return fail(ctx, TAILOR_ERR_NOT_CONVERGED, "Krylov subspace broke down");
-    }
-    // Schur form of the projected matrix, sorted by |theta| descending
-    T = CMat(ncv, ncv);
-    for (int jj = 0; jj < ncv; ++jj)
-      for (int ii = 0; ii < ncv; ++ii) T(ii, jj) = Sm(ii, jj);
-    if (!schur_decompose(T, Q)) return fail(ctx, TAILOR_ERR_NOT_CONVERGED, "projected Schur form failed");
-    for (int pos = 0; pos < ncv; ++pos) {
-      int best = pos;
-      for (int q = pos + 1; q < ncv; ++q)
-        if (std::abs(T(q, q)) > std::abs(T(best, best))) best = q;
-      if (best != pos) {
-        std::vector<bool> sel(ncv, false);
-        // move `best` to `pos`: select it together with the already placed ones
-        for (int q = 0; q < pos; ++q) sel[q] = true;
-        sel[best] = true;
-        schur_reorder(T, Q, sel);
+      // early test (check_from > 0): every second step from check_from on,
+      // end the pass here when the wanted pairs' Ritz estimates meet tol
+      const int cols = j + 1;
+      if (o.check_from > 0 && cols >= o.check_from && cols > nev + 1 && cols < ncv &&
+          (cols - o.check_from) % 2 == 0) {
+        if (int st = fill_steps(filled, cols)) return st;
+        filled = cols;
+        if (int st = project(cols)) return st;
+        bool ok = true;
+        for (int i = 0; i < nev && ok; ++i) ok = estimate(i, triangular_eigenvector(T, i)) <= tol;
+        if (ok) {
+          m = cols;
+          projected = true;
+          break;
+        }
       }
     }
-    // coupling row b = S(ncv, :) Q
-    std::vector<cplx> b(ncv, 0.0);
-    for (int jj = 0; jj < ncv; ++jj) {
-      cplx s = 0.0;
-      for (int q = 0; q < ncv; ++q) s += Sm(ncv, q) * Q(q, jj);
-      b[jj] = s;
-    }
-    theta.assign(ncv, 0.0);
-    for (int i = 0; i < ncv; ++i) theta[i] = T(i, i);
+    if (int st = fill_steps(filled, m)) return st;
+    if (!projected)
+      if (int st = project(m)) return st;
     // Ritz estimates and true residuals of the leading pairs
     nconv = 0;
-    const int ncheck = std::min(ncv - 1, nev + 2);
+    const int ncheck = std::min(m - 1, nev + 2);
     std::vector<std::vector<cplx>> Y(ncheck);
     bool all_est_ok = true;
     for (int i = 0; i < ncheck; ++i) {
       Y[i] = triangular_eigenvector(T, i);
-      cplx s = 0.0;
-      for (int q = 0; q < ncv; ++q) s += b[q] * Y[i][q];
-      const double est = std::abs(s) / std::abs(theta[i]);
-      if (i < nev && est > tol) all_est_ok = false;
+      if (i < nev && estimate(i, Y[i]) > tol) all_est_ok = false;
     }
     if (all_est_ok || restarts == o.maxit - 1) {
       // Ritz vectors x_i = V Q y_i and their true residuals
-      CMat QY(ncv, ncheck);
+      CMat QY(m, ncheck);
       for (int i = 0; i < ncheck; ++i)
-        for (int q = 0; q < ncv; ++q) {
+        for (int q = 0; q < m; ++q) {
           cplx s = 0.0;
-          for (int r = 0; r < ncv; ++r) s += Q(q, r) * Y[i][r];
+          for (int r = 0; r < m; ++r) s += Q(q, r) * Y[i][r];
           QY(q, i) = s;
         }
       HIP_OK(hipMemcpyAsync(S.small, QY.a.data(), QY.a.size() * sizeof(cplx), hipMemcpyHostToDevice, ctx->stream));
       double *Rr = S.ritz, *Ri = S.ritz + n * ncv;
-      HIP_OK(vec_combine(ctx->stream, n, ncv, ncheck, Vr, Vi, n, S.small, Rr, Ri, n));
+      HIP_OK(vec_combine(ctx->stream, n, m, ncheck, Vr, Vi, n, S.small, Rr, Ri, n));
       for (int i = 0; i < ncheck; ++i) {
         const cplx lam = S.sigma + 1.0 / theta[i];
         HIP_OK(vec_kmul(ctx->stream, d.nodes, d.ssx * d.sse, ctx->d_vals, ctx->d_colidx, ctx->d_mvals,
@@ -446,7 +481,10 @@ int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solv
       }
       nconv = 0;
       while (nconv < ncheck && resid[nconv] <= tol) ++nconv;
-      if (nconv >= nev || restarts == o.maxit - 1) break;
+      if (nconv >= nev || restarts == o.maxit - 1) {
+        nout_pass = m;
+        break;
+      }
       // the Ritz estimates are below tol but the true residuals are not:
       // stop when they no longer improve (accuracy floor of the solves)
       double worst = 0.0;
@@ -455,30 +493,35 @@ int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solv
         best_res = worst;
         stagnant = 0;
       } else if (++stagnant >= 3) {
+        nout_pass = m;
         break;
       }
+    }
+    // a pass that ended early on its Ritz estimates but not on the true
+    // residuals is extended from where it stopped rather than restarted
+    if (m < ncv) {
+      k = m;
+      continue;
     }
     // restart: keep the leading l Schur vectors
     int conv_est = 0;
     for (int i = 0; i < ncheck; ++i) {
-      cplx s = 0.0;
-      for (int q = 0; q < ncv; ++q) s += b[q] * Y[i][q];
-      if (std::abs(s) / std::abs(theta[i]) <= tol) ++conv_est;
+      if (estimate(i, Y[i]) <= tol) ++conv_est;
       else break;
     }
-    int l = conv_est + std::max(1, (ncv - conv_est) / 2);
-    l = std::min(l, ncv - 1);
+    int l = conv_est + std::max(1, (m - conv_est) / 2);
+    l = std::min(l, m - 1);
     {
-      CMat Ql(ncv, l);
+      CMat Ql(m, l);
       for (int jj = 0; jj < l; ++jj)
-        for (int q = 0; q < ncv; ++q) Ql(q, jj) = Q(q, jj);
+        for (int q = 0; q < m; ++q) Ql(q, jj) = Q(q, jj);
       HIP_OK(hipMemcpyAsync(S.small, Ql.a.data(), Ql.a.size() * sizeof(cplx), hipMemcpyHostToDevice, ctx->stream));
       double *Rr = S.ritz, *Ri = S.ritz + n * ncv;
-      HIP_OK(vec_combine(ctx->stream, n, ncv, l, Vr, Vi, n, S.small, Rr, Ri, n));
+      HIP_OK(vec_combine(ctx->stream, n, m, l, Vr, Vi, n, S.small, Rr, Ri, n));
       HIP_OK(hipMemcpyAsync(Vr, Rr, l * n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
       HIP_OK(hipMemcpyAsync(Vi, Ri, l * n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-      HIP_OK(hipMemcpyAsync(col_r(l), col_r(ncv), n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-      HIP_OK(hipMemcpyAsync(col_i(l), col_i(ncv), n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      HIP_OK(hipMemcpyAsync(col_r(l), col_r(m), n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      HIP_OK(hipMemcpyAsync(col_i(l), col_i(m), n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     }
     Sm = CMat(ncv + 1, ncv);
     for (int jj = 0; jj < l; ++jj) {
@@ -486,13 +529,14 @@ int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solv
       Sm(l, jj) = b[jj];
     }
     k = l;
+    nout_pass = m;
   }
   HIP_OK(hipStreamSynchronize(ctx->stream));
   // results: leading pairs in order of |theta|
   S.lambda.clear();
   S.resid.clear();
   S.relerr.clear();
-  const int nout = std::min(ncv - 1, std::max(nconv, 0));
+  const int nout = std::min(nout_pass - 1, std::max(nconv, 0));
   for (int i = 0; i < nout; ++i) {
     S.lambda.push_back(S.sigma + 1.0 / theta[i]);
     S.resid.push_back(resid[i]);

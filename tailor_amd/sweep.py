@@ -121,9 +121,16 @@ def training_instances(count: int, seed: int = 2, ny: int = 100, nz: int = 50) -
 
 class Sweeper:
     def __init__(self, device: int = 0, nev: int = 8, tol: float = 1e-10, ncv: int = 64,
-                 omega_s: complex = 0.02 + 0j, maxit: int = 1000, contexts: int = 1):
+                 omega_s: complex = 0.02 + 0j, maxit: int = 1000, contexts: int = 1,
+                 check_from: int = 0):
+        """`check_from` > 0: the learned form's Krylov-Schur runs also test
+        convergence after every second step from that step on and end a pass
+        early (the plain form and the fallback test only at pass ends)."""
         if not 1 <= contexts <= 4:
             raise ValueError("contexts must be 1 to 4")
+        if check_from < 0:
+            raise ValueError("check_from must be >= 0")
+        self.check_from = check_from
         self.device = device
         self.nev, self.tol, self.ncv, self.maxit = nev, tol, ncv, maxit
         self.omega_s = complex(omega_s)
@@ -150,12 +157,12 @@ class Sweeper:
         ctx.set_precision(precision)
         return ctx.autotune(complex(self.omega_s.imag, -self.omega_s.real))
 
-    def _solve(self, ctx, omega, precision, warm):
+    def _solve(self, ctx, omega, precision, warm, check_from=0):
         ctx.set_precision(precision)
         mixed = precision == PRECISION_MIXED
         d = ctx.solve(nev=self.nev, omega=omega, tol=self.tol, maxit=self.maxit, ncv=self.ncv,
                       refine_max=20 if mixed else 0, refine_tol=1e-12, raise_on_failure=False,
-                      warm_start=warm)
+                      warm_start=warm, check_from=check_from)
         pairs = ctx.eigenpairs()[:self.nev]
         res = np.array([p["residual"] for p in pairs])
         om = np.array([p["omega"] for p in pairs])
@@ -193,7 +200,8 @@ class Sweeper:
                 t0 += time.perf_counter() - tt
             warm = form == "learned" and last_size == (inst.ny, inst.nz)
             omega = self.omega_s if shifts is None else complex(shifts[k])
-            d, om, res, good = self._solve(ctx, omega, precision, warm)
+            d, om, res, good = self._solve(ctx, omega, precision, warm,
+                                           self.check_from if form == "learned" else 0)
             facts = 1
             fallback = False
             restarts, solves, refine = d["iterations"], d["solves"], d["refinement_steps"]
@@ -269,7 +277,7 @@ class Sweeper:
             t0 = time.perf_counter()
             omega = complex(shifts[k])
             key = (inst.ny, inst.nz, slot)
-            d, om, res, good = self._solve(ctx, omega, precision, key in solved)
+            d, om, res, good = self._solve(ctx, omega, precision, key in solved, self.check_from)
             solved.add(key)
             facts, fallback = 1, False
             restarts, solves, refine = d["iterations"], d["solves"], d["refinement_steps"]

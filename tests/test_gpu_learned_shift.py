@@ -70,3 +70,39 @@ def test_pipelined_learned_sweep_matches_sequential(contexts):
             d = np.min(np.abs(wa - w)) / abs(w)
             if d < 1e-6:
                 assert d < 1e-9, (w, d)
+
+
+@pytest.mark.gpu
+def test_early_convergence_test_ends_passes_with_the_same_pairs():
+    """check_from: convergence tested after every second step from step 24
+    on.  Every pair still meets the tolerance, the eigenvalues are those of
+    full 40-step passes, no warm-started run needs more solves and the sweep
+    as a whole needs fewer."""
+    from tailor_amd import sweep as sw
+    from tailor_amd.shift_model import ShiftModel
+    from tailor_amd.train_shift_model import MODEL
+    model = ShiftModel(0).load(MODEL)
+    insts = sw.sweep_instances(5, seed=4, sizes=[(100, 50)], reynolds=1500.0, mach=0.7,
+                               per_instance_seeds=False)
+    cases = sw.prepare(insts)
+    alpha = np.array([i.alpha for i in insts])
+    shifts = model.shift_centres(alpha, np.full(5, 1500.0), np.full(5, 0.7))
+    order = sw.alpha_order(insts)
+    full_runner = sw.Sweeper(ncv=40)
+    full = full_runner.run(cases, "learned", shifts=shifts, order=order)
+    full_runner.close()
+    early_runner = sw.Sweeper(ncv=40, check_from=24)
+    early = early_runner.run(cases, "learned", shifts=shifts, order=order)
+    early_runner.close()
+    print(f"full  {full.totals()}\nearly {early.totals()}", flush=True)
+    print("solves per instance", [(a.solves, b.solves) for a, b in zip(full.records, early.records)])
+    assert early.good_pairs == 8 * len(insts)
+    for a, b in zip(full.records, early.records):
+        wa = np.sort_complex(a.omegas)
+        for w in b.omegas:
+            assert np.min(np.abs(wa - w)) / abs(w) < 1e-9, w
+    # warm-started instances (all but the first) need no more solves
+    assert all(b.solves <= a.solves for a, b in zip(full.records[1:], early.records[1:]))
+    assert sum(r.solves for r in early.records) < sum(r.solves for r in full.records)
+    with pytest.raises(ValueError):
+        sw.Sweeper(check_from=-1)
