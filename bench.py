@@ -143,7 +143,7 @@ def plain_step(ta, cs, steps: int, ncv: int) -> dict:
     }
 
 
-def sweeps(ta, steps: int, warmup: int, ncv: int, learned_ncv: int) -> dict:
+def sweeps(ta, steps: int, warmup: int, ncv: int, learned_ncv: int, contexts: int) -> dict:
     from tailor_amd import sweep as sw
     from tailor_amd.shift_model import ShiftModel
     from tailor_amd.train_shift_model import MODEL
@@ -155,7 +155,7 @@ def sweeps(ta, steps: int, warmup: int, ncv: int, learned_ncv: int) -> dict:
     cases = sw.prepare(insts, OMEGA)
     alphas = np.array([i.alpha for i in insts])
     shifts = model.shift_centres(alphas, np.full(len(insts), 1500.0), np.full(len(insts), 0.7))
-    runner = sw.Sweeper(ncv=learned_ncv, omega_s=OMEGA, contexts=PIPELINE_CONTEXTS,
+    runner = sw.Sweeper(ncv=learned_ncv, omega_s=OMEGA, contexts=contexts,
                         check_from=LEARNED_CHECK_FROM)
     plain_runner = sw.Sweeper(ncv=ncv, omega_s=OMEGA)
     # warm-up instances (untimed): build the plan, and let the first one
@@ -204,6 +204,8 @@ def main() -> int:
     ap.add_argument("--ncv", type=int, default=NCV, help="Krylov subspace of the plain form")
     ap.add_argument("--learned-ncv", type=int, default=LEARNED_NCV)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--contexts", type=int, default=PIPELINE_CONTEXTS,
+                    help="contexts of the learned sweep's pipeline (1: no worker threads)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -217,7 +219,7 @@ def main() -> int:
     from tailor_amd import case as cs
 
     single = plain_step(ta, cs, 2, args.ncv)
-    sw = sweeps(ta, args.steps, max(1, args.warmup), args.ncv, args.learned_ncv)
+    sw = sweeps(ta, args.steps, max(1, args.warmup), args.ncv, args.learned_ncv, args.contexts)
     L, P = sw["learned"], sw["plain"]
     g = single["gemm"]
     gemm_tf = g["gemm_flops"] / (g["gemm_ms"] * 1e-3) / 1e12
@@ -237,9 +239,10 @@ def main() -> int:
         "config": {"workload": "BiGlobal wavenumber sweep, 200x100 FD-q grid (q_y=10, q_z=6), "
                                "100,000 unknowns, Re=1500, Ma=0.7, alpha ~ U[0.5,3] seed 1, nev=8, "
                                "tol=1e-10; learned shift centres, warm-started Krylov-Schur, "
-                               "fixed-shift fallback, three-context pipeline",
+                               "fixed-shift fallback, " + (f"{args.contexts}-context pipeline"
+                                                          if args.contexts > 1 else "one context"),
                    "nev": NEV, "ncv_learned": args.learned_ncv, "ncv_plain": args.ncv,
-                   "pipeline_contexts": PIPELINE_CONTEXTS,
+                   "pipeline_contexts": args.contexts,
                    "learned_check_from": LEARNED_CHECK_FROM,
                    "instances": args.steps,
                    "precision_choice": sw["choice"], "gemm_autotune": sw["autotune"]},
