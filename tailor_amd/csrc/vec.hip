@@ -113,41 +113,67 @@ __global__ __launch_bounds__(VT) void k_kmul(long long nodes, int per, const dou
   }
 }
 
-// partial h_j = sum_i conj(V_ij) w_i over this block's rows; ncols <= 64.
+// Sum over the 64 lanes of a wave (fixed butterfly: every lane gets the
+// same, run-to-run reproducible value).
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// partial h_j = sum_i conj(V_ij) w_i over this block's rows, for the
+// kDotCols columns j of chunk blockIdx.y: each thread accumulates its rows
+// of all the chunk's columns (w read once), then wave sums and one LDS step
+// across the waves.  partials[b][j] complex, b = blockIdx.x.
+constexpr int kDotCols = 8;
 __global__ __launch_bounds__(VT) void k_dots(long long n, int ncols, const double *__restrict__ Vr,
                                              const double *__restrict__ Vi, long long ldv,
                                              const double *__restrict__ wr,
                                              const double *__restrict__ wi,
                                              double *__restrict__ partials) {
-  __shared__ double red_r[VT], red_i[VT];
-  for (int j = 0; j < ncols; ++j) {
-    double sr = 0.0, si = 0.0;
-    for (long long i = static_cast<long long>(blockIdx.x) * VT + threadIdx.x; i < n;
-         i += static_cast<long long>(gridDim.x) * VT) {
-      const double vr = Vr[j * ldv + i], vi = Vi[j * ldv + i];
-      const double x_r = wr[i], x_i = wi[i];
-      sr += vr * x_r + vi * x_i;
-      si += vr * x_i - vi * x_r;
-    }
-    red_r[threadIdx.x] = sr;
-    red_i[threadIdx.x] = si;
-    __syncthreads();
-    for (int w = VT / 2; w > 0; w >>= 1) {
-      if (threadIdx.x < w) {
-        red_r[threadIdx.x] += red_r[threadIdx.x + w];
-        red_i[threadIdx.x] += red_i[threadIdx.x + w];
+  constexpr int NW = VT / 64;
+  __shared__ double red[NW][2 * kDotCols];
+  const int j0 = blockIdx.y * kDotCols;
+  const int nc = min(kDotCols, ncols - j0);
+  double sr[kDotCols], si[kDotCols];
+#pragma unroll
+  for (int c = 0; c < kDotCols; ++c) {
+    sr[c] = 0.0;
+    si[c] = 0.0;
+  }
+  for (long long i = static_cast<long long>(blockIdx.x) * VT + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * VT) {
+    const double x_r = wr[i], x_i = wi[i];
+#pragma unroll
+    for (int c = 0; c < kDotCols; ++c) {
+      if (c < nc) {
+        const long long o = static_cast<long long>(j0 + c) * ldv + i;
+        const double vr = Vr[o], vi = Vi[o];
+        sr[c] += vr * x_r + vi * x_i;
+        si[c] += vr * x_i - vi * x_r;
       }
-      __syncthreads();
     }
-    if (threadIdx.x == 0) {
-      partials[2 * (static_cast<long long>(blockIdx.x) * ncols + j)] = red_r[0];
-      partials[2 * (static_cast<long long>(blockIdx.x) * ncols + j) + 1] = red_i[0];
+  }
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+#pragma unroll
+  for (int c = 0; c < kDotCols; ++c) {
+    const double r = wave_sum(sr[c]), q = wave_sum(si[c]);
+    if (l == 0) {
+      red[w][2 * c] = r;
+      red[w][2 * c + 1] = q;
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * nc) {
+    double t = 0.0;
+#pragma unroll
+    for (int v = 0; v < NW; ++v) t += red[v][threadIdx.x];
+    const int c = threadIdx.x / 2;
+    partials[2 * (static_cast<long long>(blockIdx.x) * ncols + j0 + c) + threadIdx.x % 2] = t;
   }
 }
 
-// w -= V h (h complex, ncols <= 64, interleaved in device memory)
+// w -= V h (h complex, ncols <= kMaxCols, interleaved in device memory)
 __global__ void k_update(long long n, int ncols, const double *__restrict__ Vr,
                          const double *__restrict__ Vi, long long ldv, const double *__restrict__ h,
                          double *__restrict__ wr, double *__restrict__ wi) {
@@ -217,15 +243,21 @@ __global__ __launch_bounds__(VT) void k_combine(long long n, int m, int k,
   }
 }
 
-// out[j] = sum over blocks b (in order) of partials[b][j], complex.
-__global__ void k_reduce_partials(const double *__restrict__ partials, int nblocks, int ncols,
-                                  double *__restrict__ out) {
-  for (int j = threadIdx.x; j < ncols; j += blockDim.x) {
-    double sr = 0.0, si = 0.0;
-    for (int b = 0; b < nblocks; ++b) {
-      sr += partials[2 * (static_cast<long long>(b) * ncols + j)];
-      si += partials[2 * (static_cast<long long>(b) * ncols + j) + 1];
-    }
+// out[j] = sum over blocks b of partials[b][j], complex: one wave per
+// column j = blockIdx.x; lane l sums blocks l, l + 64, ... in order, then
+// the wave's fixed butterfly.
+__global__ __launch_bounds__(64) void k_reduce_partials(const double *__restrict__ partials,
+                                                        int nblocks, int ncols,
+                                                        double *__restrict__ out) {
+  const int j = blockIdx.x, l = threadIdx.x;
+  double sr = 0.0, si = 0.0;
+  for (int b = l; b < nblocks; b += 64) {
+    sr += partials[2 * (static_cast<long long>(b) * ncols + j)];
+    si += partials[2 * (static_cast<long long>(b) * ncols + j) + 1];
+  }
+  sr = wave_sum(sr);
+  si = wave_sum(si);
+  if (l == 0) {
     out[2 * j] = sr;
     out[2 * j + 1] = si;
   }
@@ -289,7 +321,9 @@ hipError_t vec_kmul(hipStream_t s, long long nodes, int per, const double *avals
 hipError_t vec_dots(hipStream_t s, long long n, int ncols, const double *Vr, const double *Vi,
                     long long ldv, const double *wr, const double *wi, double *partials,
                     int nblocks) {
-  hipLaunchKernelGGL(k_dots, dim3(nblocks), dim3(VT), 0, s, n, ncols, Vr, Vi, ldv, wr, wi, partials);
+  if (ncols <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dots, dim3(nblocks, (ncols + kDotCols - 1) / kDotCols), dim3(VT), 0, s, n,
+                     ncols, Vr, Vi, ldv, wr, wi, partials);
   return hipGetLastError();
 }
 
@@ -311,8 +345,10 @@ hipError_t vec_combine(hipStream_t s, long long n, int m, int k, const double *V
 hipError_t vec_dots_device(hipStream_t s, long long n, int ncols, const double *Vr,
                            const double *Vi, long long ldv, const double *wr, const double *wi,
                            double *partials, int nblocks, double *out) {
-  hipLaunchKernelGGL(k_dots, dim3(nblocks), dim3(VT), 0, s, n, ncols, Vr, Vi, ldv, wr, wi, partials);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(VT), 0, s, partials, nblocks, ncols, out);
+  if (ncols <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dots, dim3(nblocks, (ncols + kDotCols - 1) / kDotCols), dim3(VT), 0, s, n,
+                     ncols, Vr, Vi, ldv, wr, wi, partials);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(ncols), dim3(64), 0, s, partials, nblocks, ncols, out);
   return hipGetLastError();
 }
 
