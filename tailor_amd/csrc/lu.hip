@@ -1412,6 +1412,8 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void k_gemm(const int *__rest
       bcol_i[s] = v.B2i + (col - v.N1) * v.ldb2;
     }
   }
+  // staged global loads: chunk c + 1 is requested while chunk c feeds the
+  // MFMAs (a second register set, requesting chunk c + 2, measured slower)
   T ga_r[AE], ga_i[AE], gb_r[BE], gb_i[BE];
   auto fetch = [&](int kc) {
 #pragma unroll
@@ -1488,13 +1490,7 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void k_gemm(const int *__rest
       }
     }
   }
-  fetch(0);
-  stash(0);
-  __syncthreads();
-  int buf = 0;
-  for (int kc = 0; kc < v.K; kc += GKC) {
-    const bool more = kc + GKC < v.K;
-    if (more) fetch(kc + GKC);
+  auto compute = [&](int buf) {
 #pragma unroll
     for (int kk = 0; kk < GKC; kk += 4) {
       const int kr = kk + l / 16;
@@ -1535,6 +1531,16 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void k_gemm(const int *__rest
           }
       }
     }
+  };
+  const int nch = (v.K + GKC - 1) / GKC;
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  int buf = 0;
+  for (int c = 0; c < nch; ++c) {
+    const bool more = c + 1 < nch;
+    if (more) fetch((c + 1) * GKC);
+    compute(buf);
     if (more) stash(buf ^ 1);
     __syncthreads();
     buf ^= 1;
