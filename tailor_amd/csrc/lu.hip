@@ -1413,40 +1413,44 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void k_gemm(const int *__rest
     }
   }
   // staged global loads: chunk c + 1 is requested while chunk c feeds the
-  // MFMAs (a second register set, requesting chunk c + 2, measured slower)
+  // MFMAs.  The loads come from clamped in-bounds addresses and stay raw
+  // until the stash, which zeroes the out-of-range entries: a select right
+  // after the loads would make the wave wait for them before the chunk's
+  // MFMAs.  (A second register set, requesting chunk c + 2, measured slower.)
   T ga_r[AE], ga_i[AE], gb_r[BE], gb_i[BE];
+  bool ga_ok[AE], gb_ok[BE];
   auto fetch = [&](int kc) {
 #pragma unroll
     for (int s = 0; s < AE; ++s) {
       const int e = t + s * THREADS, r = e % TM, k = e / TM;
       const bool oka = row0 + r < v.M && kc + k < v.K;
       const long long ia = oka ? static_cast<long long>(kc + k) * v.lda + row0 + r : 0;
-      const T xr = v.Ar[ia], xi = v.Ai[ia];
-      ga_r[s] = oka ? xr : T(0);
-      ga_i[s] = oka ? xi : T(0);
+      ga_r[s] = v.Ar[ia];
+      ga_i[s] = v.Ai[ia];
+      ga_ok[s] = oka;
     }
 #pragma unroll
     for (int s = 0; s < BE; ++s) {
       const int kbb = kc + (t + s * THREADS) % GKC;
       const bool okb = bcol_ok[s] && kbb < v.K;
       const int ib = okb ? kbb : 0;
-      const T yr = bcol_r[s][ib], yi = bcol_i[s][ib];
-      gb_r[s] = okb ? yr : T(0);
-      gb_i[s] = okb ? yi : T(0);
+      gb_r[s] = bcol_r[s][ib];
+      gb_i[s] = bcol_i[s][ib];
+      gb_ok[s] = okb;
     }
   };
   auto stash = [&](int buf) {
 #pragma unroll
     for (int s = 0; s < AE; ++s) {
       const int e = t + s * THREADS;
-      sAr[buf][e / TM][e % TM] = ga_r[s];
-      sAi[buf][e / TM][e % TM] = ga_i[s];
+      sAr[buf][e / TM][e % TM] = ga_ok[s] ? ga_r[s] : T(0);
+      sAi[buf][e / TM][e % TM] = ga_ok[s] ? ga_i[s] : T(0);
     }
 #pragma unroll
     for (int s = 0; s < BE; ++s) {
       const int e = t + s * THREADS;
-      sBr[buf][e / GKC][e % GKC] = -gb_r[s];
-      sBi[buf][e / GKC][e % GKC] = -gb_i[s];
+      sBr[buf][e / GKC][e % GKC] = gb_ok[s] ? -gb_r[s] : T(0);
+      sBi[buf][e / GKC][e % GKC] = gb_ok[s] ? -gb_i[s] : T(0);
     }
   };
   // B is staged negated (Bn = -B) and the accumulators start from C, so the
