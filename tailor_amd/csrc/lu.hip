@@ -1327,7 +1327,7 @@ constexpr int GKC = 16;  // K chunk staged in LDS
 // Tile shapes (chosen per level by the autotuner): WMW x WNW waves, each
 // owning WMB x WNB MFMA 16x16 blocks, so a workgroup computes a
 // (16 WMW WMB) x (16 WNW WNB) complex tile.
-//   shape 0: 64 x 64, 256 threads, K chunk 16, four real products;
+//   shape 0: 64 x 64, 256 threads, K chunk 16, 3M form, two waves per SIMD;
 //   shape 1: 64 x 64, 256 threads, K chunk 4, 3M form, registers held to
 //            three waves per SIMD (a quarter of shape 0's LDS);
 //   shape 2: 128 x 128, 512 threads, four real products;
@@ -1580,15 +1580,17 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void k_gemm(const int *__rest
   }
 }
 
-// Product form and waves-per-SIMD bound of each tile shape.  The K-chunk-4
-// and -8 tiles use the 3M form; its third accumulator set spills at four and
-// three waves per SIMD, so their bounds are three and two (measured on the
-// 200x100 factorization: K-chunk-8 tiles 127.5 ms of GEMM with four products
-// at three waves, 125.1 ms with 3M at two; look-ahead factorization 196 ->
-// 183 ms).  The K-chunk-16 tiles keep four products (3M would leave one wave
-// per SIMD), and so do the 128x128 tiles (3M spills at 256 registers).
-constexpr int kGemmK4Waves = 3, kGemmK8Waves = 2;
-constexpr bool kGemmK4M3 = true, kGemmK8M3 = true, kGemmK16M3 = false, kGemm128M3 = false;
+// Product form and waves-per-SIMD bound of each tile shape.  The 64x64
+// tiles use the 3M form; its third accumulator set spills at four waves per
+// SIMD (K chunk 4) and three (K chunk 8), so their bounds are three and two,
+// and two for K chunk 16.  Measured on the 200x100 factorization (ms of GEMM
+// per factorization, XCD-grouped order): K chunk 4: 3M at three waves 106.8,
+// four products at four waves 116.1; K chunk 8: 3M at two 110.5, four
+// products at three 118.6; K chunk 16: 3M at two 110.5, four products at
+// one 126.8.  The 128x128 tiles keep four products (3M spills at 256
+// registers).
+constexpr int kGemmK4Waves = 3, kGemmK8Waves = 2, kGemmK16Waves = 2;
+constexpr bool kGemmK4M3 = true, kGemmK8M3 = true, kGemmK16M3 = true, kGemm128M3 = false;
 
 // Launch one batched GEMM with the given tile shape; rows/cols are upper
 // bounds of M and N over the level's fronts.
@@ -1619,7 +1621,7 @@ hipError_t launch_gemm(int shape, int order, long long rows, long long cols, int
                          list, count, fr, region, k0, nb, K0, NB, order, arena);
       break;
     default:
-      hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 2, GKC, 1, kGemmK16M3>), grid, block, 0, st, list,
+      hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 2, GKC, kGemmK16Waves, kGemmK16M3>), grid, block, 0, st, list,
                          count, fr, region, k0, nb, K0, NB, order, arena);
       break;
   }
