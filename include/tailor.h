@@ -208,6 +208,14 @@ int tailor_set_precision(tailor_ctx *ctx, int32_t precision);
 int tailor_linear_solve(tailor_ctx *ctx, const double *b, double *x, int32_t refine_max,
                         double refine_tol, double *relres, int32_t *steps);
 
+/* x1 = K^-1 b1 and x2 = K^-1 b2 (natural order, as tailor_linear_solve) in
+ * one pass over the factors, without refinement: each factor entry is read
+ * once for both right-hand sides.  No reference counterpart (one KSPSolve
+ * per right-hand side, EigenSolver.cpp:61-67); for callers with two
+ * independent right-hand sides at a time. */
+int tailor_linear_solve_pair(tailor_ctx *ctx, const double *b1, const double *b2, double *x1,
+                             double *x2);
+
 /* Time `reps` unrefined solves x = K^-1 b (device-resident vectors) with
  * HIP events on the context stream: mean ms per solve and the bytes of
  * factor data one solve must read: E, L21 and U12 of every front and the

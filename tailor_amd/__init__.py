@@ -147,6 +147,8 @@ def library() -> C.CDLL:
     lib.tailor_profile_factor.argtypes = [C.c_void_p, C.c_double, C.c_double, P(C.c_double),
                                           P(C.c_double), P(C.c_double), P(C.c_int32)]
     lib.tailor_time_linear_solve.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double)]
+    lib.tailor_linear_solve_pair.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double),
+                                             P(C.c_double), P(C.c_double)]
     lib.tailor_eigenpair_count.argtypes = [C.c_void_p]
     lib.tailor_get_eigenpair.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double),
                                          P(C.c_double), P(C.c_double)]
@@ -351,6 +353,22 @@ def _eigenpairs(self, vectors: bool = False) -> list:
                     "residual": res.value, "relative_error": rel.value,
                     "mode": q.view(np.complex128) if vectors else None})
     return out
+
+
+def _linear_solve_pair(self, b1: np.ndarray, b2: np.ndarray):
+    """x1, x2 = K^-1 b1, K^-1 b2 in one pass over the factors (no refinement)."""
+    b1 = np.ascontiguousarray(b1, dtype=np.complex128)
+    b2 = np.ascontiguousarray(b2, dtype=np.complex128)
+    if b1.shape != b2.shape:
+        raise ValueError("right-hand sides differ in shape")
+    x1, x2 = np.zeros_like(b1), np.zeros_like(b2)
+    self._check(self.lib.tailor_linear_solve_pair(
+        self.ptr, _dptr(b1.view(np.float64)), _dptr(b2.view(np.float64)),
+        _dptr(x1.view(np.float64)), _dptr(x2.view(np.float64))))
+    return x1, x2
+
+
+Context.linear_solve_pair = _linear_solve_pair
 
 
 def _time_linear_solve(self, reps: int = 20):

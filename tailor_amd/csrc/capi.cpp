@@ -663,6 +663,15 @@ int tailor_linear_solve(tailor_ctx *ctx, const double *b, double *x, int32_t ref
   return st;
 }
 
+int tailor_linear_solve_pair(tailor_ctx *ctx, const double *b1, const double *b2, double *x1,
+                             double *x2) {
+  if (!ctx || !b1 || !b2 || !x1 || !x2) return TAILOR_ERR_USER_INPUT;
+  if (!ctx->solver || !ctx->solver->factored)
+    return fail(ctx, TAILOR_ERR_ARG_WRONGSTATE, "tailor_factor must precede tailor_linear_solve_pair");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  return tailor::solver_linear_solve_pair(ctx, b1, b2, x1, x2);
+}
+
 int tailor_time_linear_solve(tailor_ctx *ctx, int32_t reps, double *ms_per_solve,
                              double *bytes_per_solve) {
   if (!ctx || reps <= 0) return TAILOR_ERR_USER_INPUT;

@@ -253,6 +253,28 @@ int solver_linear_solve(tailor_ctx *ctx, const double *b_nat, double *x_nat, int
   return TAILOR_OK;
 }
 
+int solver_linear_solve_pair(tailor_ctx *ctx, const double *b1_nat, const double *b2_nat,
+                             double *x1_nat, double *x2_nat) {
+  SolverState &S = *ctx->solver;
+  const long long n = S.n;
+  HIP_OK(grow(&S.work, S.work_cap, static_cast<size_t>(12 * n)));
+  double *nat = S.work + 10 * n;  // interleaved natural vector (2n)
+  double *b1r = S.work, *b1i = b1r + n, *b2r = S.work + 2 * n, *b2i = b2r + n;
+  double *x1r = S.work + 4 * n, *x1i = x1r + n, *x2r = S.work + 6 * n, *x2i = x2r + n;
+  HIP_OK(hipMemcpyAsync(nat, b1_nat, 2 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  HIP_OK(vec_nat_to_elim(ctx->stream, n, S.plan.d_perm, nat, b1r, b1i));
+  HIP_OK(hipMemcpyAsync(nat, b2_nat, 2 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  HIP_OK(vec_nat_to_elim(ctx->stream, n, S.plan.d_perm, nat, b2r, b2i));
+  HIP_OK(lu_solve2(S.plan, b1r, b1i, b2r, b2i, x1r, x1i, x2r, x2i, S.fp32, ctx->stream));
+  HIP_OK(vec_elim_to_nat(ctx->stream, n, S.plan.d_perm, x1r, x1i, nat));
+  HIP_OK(hipMemcpyAsync(x1_nat, nat, 2 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  HIP_OK(vec_elim_to_nat(ctx->stream, n, S.plan.d_perm, x2r, x2i, nat));
+  HIP_OK(hipMemcpyAsync(x2_nat, nat, 2 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  return TAILOR_OK;
+}
+
 int solver_time_solves(tailor_ctx *ctx, int reps, double *ms) {
   SolverState &S = *ctx->solver;
   const long long n = S.n;

@@ -1756,34 +1756,48 @@ __device__ inline bool gemv_view(int op, const FrontDev *fr, int f, const FrontD
   }
 }
 
+// One right-hand side of a solve: the front workspace W, the accumulated
+// update Y and the solution x (split planes).
+struct RhsSet {
+  double *Wr, *Wi, *Yr, *Yi, *Xr, *Xi;
+};
+
 // One workgroup: GVR output rows x (GVW * GVC) reduction columns; each wave
 // takes GVC of the columns and the four partial sums are combined in LDS
 // before one FP64 atomic per row.  GVC = 64 for levels with enough work to
 // fill the GPU, 16 (one batch of loads per wave, four times the workgroups)
-// for the small ones, whose launches are latency-bound.
-// Factor entries of type T are widened to FP64; sums are FP64.
-template <typename T, int GVC>
+// for the small ones, whose launches are latency-bound.  NR right-hand
+// sides (1 or 2) share every load of A; each one's sums are formed in the
+// same order as with one.  Factor entries of type T are widened to FP64;
+// sums are FP64.
+template <typename T, int GVC, int NR>
 __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list, int nlist,
                                                     const FrontDev *__restrict__ fr, SolveMaps mp,
-                                                    int op, const T *__restrict__ arena, double *Wr,
-                                                    double *Wi, double *Yr, double *Yi, double *Xr,
-                                                    double *Xi) {
+                                                    int op, const T *__restrict__ arena,
+                                                    RhsSet rhs0, RhsSet rhs1) {
+  static_assert(NR == 1 || NR == 2, "one or two right-hand sides");
   const int f = list[blockIdx.y];
   const FrontDev d = fr[f];
-  GemvView<T> v;
-  if (!gemv_view(op, fr, f, d, mp, arena, Wr, Wi, Yr, Yi, Xr, Xi, v)) return;
+  const RhsSet rs[2] = {rhs0, rhs1};
+  GemvView<T> v[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+    if (!gemv_view(op, fr, f, d, mp, arena, rs[r].Wr, rs[r].Wi, rs[r].Yr, rs[r].Yi, rs[r].Xr,
+                   rs[r].Xi, v[r]))
+      return;
+  const GemvView<T> &V = v[0];  // A, its shape and the maps are shared
   const int lane = threadIdx.x % GVR, wave = threadIdx.x / GVR;
   const int r0 = blockIdx.x * GVR;
   const int row = r0 + lane;
   const int cw = blockIdx.z * GVW * GVC;  // first column of this workgroup
-  if (r0 >= v.M || cw >= v.N) return;
-  if (v.upper && cw + GVW * GVC <= r0) return;  // every entry of the tile is below the diagonal
+  if (r0 >= V.M || cw >= V.N) return;
+  if (V.upper && cw + GVW * GVC <= r0) return;  // every entry of the tile is below the diagonal
   const int c0 = cw + wave * GVC;
-  const int c1 = min(v.N, c0 + GVC);
-  __shared__ double sxr[GVW * GVC], sxi[GVW * GVC];
-  __shared__ double pr[GVW][GVR], pi[GVW][GVR];
-  const bool active = row < v.M && !(v.upper && c1 <= r0);
-  const T *ar = v.Ar + (row < v.M ? row : 0), *ai = v.Ai + (row < v.M ? row : 0);
+  const int c1 = min(V.N, c0 + GVC);
+  __shared__ double sxr[NR][GVW * GVC], sxi[NR][GVW * GVC];
+  __shared__ double pr[NR][GVW][GVR], pi[NR][GVW][GVR];
+  const bool active = row < V.M && !(V.upper && c1 <= r0);
+  const T *ar = V.Ar + (row < V.M ? row : 0), *ai = V.Ai + (row < V.M ? row : 0);
   // the first 16 columns of A are requested before x is staged, so the two
   // loads overlap
   double a_r[16], a_i[16];
@@ -1791,24 +1805,32 @@ __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list
   if (first) {
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
-      const long long idx = static_cast<long long>(c0 + u) * v.lda;
+      const long long idx = static_cast<long long>(c0 + u) * V.lda;
       a_r[u] = static_cast<double>(ar[idx]);
       a_i[u] = static_cast<double>(ai[idx]);
     }
   }
   for (int k = threadIdx.x; k < GVW * GVC; k += GVR * GVW) {
     const int col = cw + k;
-    double xr = 0.0, xi = 0.0;
-    if (col < v.N) {
-      const int g = v.xmap ? 5 * v.xmap[col / 5] + col % 5 : col;
-      xr = v.xr[g];
-      xi = v.xi[g];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      double xr = 0.0, xi = 0.0;
+      if (col < V.N) {
+        const int g = V.xmap ? 5 * V.xmap[col / 5] + col % 5 : col;
+        xr = v[r].xr[g];
+        xi = v[r].xi[g];
+      }
+      sxr[r][k] = xr;
+      sxi[r][k] = xi;
     }
-    sxr[k] = xr;
-    sxi[k] = xi;
   }
   __syncthreads();
-  double accr = 0.0, acci = 0.0;
+  double accr[NR], acci[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    accr[r] = 0.0;
+    acci[r] = 0.0;
+  }
   if (active) {
     int k = c0;
     // 16 columns per step so that 32 loads per lane are in flight
@@ -1816,44 +1838,56 @@ __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list
       if (k != c0 || !first) {
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
-          const long long idx = static_cast<long long>(k + u) * v.lda;
+          const long long idx = static_cast<long long>(k + u) * V.lda;
           a_r[u] = static_cast<double>(ar[idx]);
           a_i[u] = static_cast<double>(ai[idx]);
         }
       }
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
-        const double x_r = sxr[k + u - cw], x_i = sxi[k + u - cw];
-        accr += a_r[u] * x_r - a_i[u] * x_i;
-        acci += a_r[u] * x_i + a_i[u] * x_r;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const double x_r = sxr[r][k + u - cw], x_i = sxi[r][k + u - cw];
+          accr[r] += a_r[u] * x_r - a_i[u] * x_i;
+          acci[r] += a_r[u] * x_i + a_i[u] * x_r;
+        }
       }
     }
     for (; k < c1; ++k) {
-      const long long idx = static_cast<long long>(k) * v.lda;
+      const long long idx = static_cast<long long>(k) * V.lda;
       const double br = static_cast<double>(ar[idx]), bi = static_cast<double>(ai[idx]);
-      const double x_r = sxr[k - cw], x_i = sxi[k - cw];
-      accr += br * x_r - bi * x_i;
-      acci += br * x_i + bi * x_r;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const double x_r = sxr[r][k - cw], x_i = sxi[r][k - cw];
+        accr[r] += br * x_r - bi * x_i;
+        acci[r] += br * x_i + bi * x_r;
+      }
     }
   }
-  pr[wave][lane] = accr;
-  pi[wave][lane] = acci;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    pr[r][wave][lane] = accr[r];
+    pi[r][wave][lane] = acci[r];
+  }
   __syncthreads();
-  if (wave == 0 && row < v.M) {
-    double sr = 0.0, si = 0.0;
-    for (int w = 0; w < GVW; ++w) {
-      sr += pr[w][lane];
-      si += pi[w][lane];
+  if (wave == 0 && row < V.M) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      double sr = 0.0, si = 0.0;
+      for (int w = 0; w < GVW; ++w) {
+        sr += pr[r][w][lane];
+        si += pi[r][w][lane];
+      }
+      sr *= v[r].sign;
+      si *= v[r].sign;
+      if (v[r].addr && blockIdx.z == 0) {
+        sr += v[r].addr[row];
+        si += v[r].addi[row];
+      }
+      const int o = v[r].ymap ? 5 * v[r].ymap[row / 5] + row % 5 : row;
+      atomicAdd(v[r].yr + o, sr);
+      atomicAdd(v[r].yi + o, si);
     }
-    sr *= v.sign;
-    si *= v.sign;
-    if (v.addr && blockIdx.z == 0) {
-      sr += v.addr[row];
-      si += v.addi[row];
-    }
-    const int o = v.ymap ? 5 * v.ymap[row / 5] + row % 5 : row;
-    atomicAdd(v.yr + o, sr);
-    atomicAdd(v.yi + o, si);
   }
 }
 
@@ -2036,27 +2070,33 @@ hipError_t lu_factor(const LuPlan &P, const LuInputs &in, double sig_re, double 
 }
 
 // x = K^-1 b for vectors in elimination order (split planes).  b may alias x.
-template <typename T>
-static hipError_t solve_impl(const LuPlan &P, const double *br, const double *bi, double *xr,
-                             double *xi, hipStream_t s) {
+// x_r = K^-1 b_r for NR right-hand sides in one pass over the factors
+// (b[2 r], b[2 r + 1]: real and imaginary planes of right-hand side r).
+template <typename T, int NR>
+static hipError_t solve_impl(const LuPlan &P, const double *const *b, double *const *x,
+                             hipStream_t s) {
   const T *arena = reinterpret_cast<const T *>(P.arena);
-  double *Wr = P.W, *Wi = P.W + P.w_len;
-  double *Yr = P.Y, *Yi = P.Y + P.n;
-  const SolveMaps mp{P.d_parent, P.d_rel, P.d_upd_pos, P.d_rel_off, P.d_upd_off};
-  hipLaunchKernelGGL(k_solve_init, dim3(P.nsn), dim3(256), 0, s, P.d_front, br, bi, Wr, Wi, Yr,
-                     Yi, xr, xi);
+  RhsSet rs[2];
+  for (int r = 0; r < NR; ++r) {
+    double *W = r == 0 ? P.W : P.W2, *Y = r == 0 ? P.Y : P.Y2;
+    rs[r] = {W, W + P.w_len, Y, Y + P.n, x[2 * r], x[2 * r + 1]};
+    hipLaunchKernelGGL(k_solve_init, dim3(P.nsn), dim3(256), 0, s, P.d_front, b[2 * r],
+                       b[2 * r + 1], rs[r].Wr, rs[r].Wi, rs[r].Yr, rs[r].Yi, rs[r].Xr, rs[r].Xi);
+  }
+  if (NR == 1) rs[1] = rs[0];
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  const SolveMaps mp{P.d_parent, P.d_rel, P.d_upd_pos, P.d_rel_off, P.d_upd_off};
   auto gemv = [&](const LevelPlan &L, int op, int rows, int cols) {
     const long long wide = static_cast<long long>(cdivu(rows, GVR)) * L.count * cdivu(cols, 64 * GVW);
     if (wide >= kGemvWideMin)
-      hipLaunchKernelGGL((k_gemv<T, 64>), dim3(cdivu(rows, GVR), L.count, cdivu(cols, 64 * GVW)),
+      hipLaunchKernelGGL((k_gemv<T, 64, NR>), dim3(cdivu(rows, GVR), L.count, cdivu(cols, 64 * GVW)),
                          dim3(GVR * GVW), 0, s, P.d_level_list + L.list_off, L.count, P.d_front,
-                         mp, op, arena, Wr, Wi, Yr, Yi, xr, xi);
+                         mp, op, arena, rs[0], rs[1]);
     else
-      hipLaunchKernelGGL((k_gemv<T, 16>), dim3(cdivu(rows, GVR), L.count, cdivu(cols, 16 * GVW)),
+      hipLaunchKernelGGL((k_gemv<T, 16, NR>), dim3(cdivu(rows, GVR), L.count, cdivu(cols, 16 * GVW)),
                          dim3(GVR * GVW), 0, s, P.d_level_list + L.list_off, L.count, P.d_front,
-                         mp, op, arena, Wr, Wi, Yr, Yi, xr, xi);
+                         mp, op, arena, rs[0], rs[1]);
   };
   for (size_t lev = 0; lev < P.levels.size(); ++lev) {
     const LevelPlan &L = P.levels[lev];
@@ -2075,7 +2115,17 @@ static hipError_t solve_impl(const LuPlan &P, const double *br, const double *bi
 
 hipError_t lu_solve(const LuPlan &P, const double *br, const double *bi, double *xr, double *xi,
                     bool fp32, hipStream_t s) {
-  return fp32 ? solve_impl<float>(P, br, bi, xr, xi, s) : solve_impl<double>(P, br, bi, xr, xi, s);
+  const double *b[2] = {br, bi};
+  double *x[2] = {xr, xi};
+  return fp32 ? solve_impl<float, 1>(P, b, x, s) : solve_impl<double, 1>(P, b, x, s);
+}
+
+hipError_t lu_solve2(const LuPlan &P, const double *b1r, const double *b1i, const double *b2r,
+                     const double *b2i, double *x1r, double *x1i, double *x2r, double *x2i,
+                     bool fp32, hipStream_t s) {
+  const double *b[4] = {b1r, b1i, b2r, b2i};
+  double *x[4] = {x1r, x1i, x2r, x2i};
+  return fp32 ? solve_impl<float, 2>(P, b, x, s) : solve_impl<double, 2>(P, b, x, s);
 }
 
 }  // namespace tailor

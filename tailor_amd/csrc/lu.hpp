@@ -63,6 +63,8 @@ struct LuPlan {
   int count_max = 0;            // most fronts of any level
   double *W = nullptr;          // 2 * w_len
   double *Y = nullptr;          // 2 * n
+  double *W2 = nullptr;         // the second right-hand side's W and Y (lu_solve2)
+  double *Y2 = nullptr;
 };
 
 struct LuInputs {
@@ -108,5 +110,10 @@ hipError_t lu_factor(const LuPlan &P, const LuInputs &in, double sig_re, double 
 // x = K^-1 b, vectors in elimination dof order, split planes (re, im)
 hipError_t lu_solve(const LuPlan &P, const double *br, const double *bi, double *xr, double *xi,
                     bool fp32, hipStream_t s);
+// x1 = K^-1 b1 and x2 = K^-1 b2 in one pass over the factors (each factor
+// entry is read once for both); the vectors as for lu_solve, all distinct
+hipError_t lu_solve2(const LuPlan &P, const double *b1r, const double *b1i, const double *b2r,
+                     const double *b2i, double *x1r, double *x1i, double *x2r, double *x2i,
+                     bool fp32, hipStream_t s);
 
 }  // namespace tailor

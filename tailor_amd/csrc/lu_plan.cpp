@@ -143,6 +143,10 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
     return e;
   if ((e = hipMalloc(reinterpret_cast<void **>(&P.Y), 2 * P.n * sizeof(double))) != hipSuccess)
     return e;
+  if ((e = hipMalloc(reinterpret_cast<void **>(&P.W2), 2 * P.w_len * sizeof(double))) != hipSuccess)
+    return e;
+  if ((e = hipMalloc(reinterpret_cast<void **>(&P.Y2), 2 * P.n * sizeof(double))) != hipSuccess)
+    return e;
   if ((e = hipStreamCreateWithFlags(&P.side, hipStreamNonBlocking)) != hipSuccess) return e;
   if ((e = hipEventCreateWithFlags(&P.ev_head, hipEventDisableTiming)) != hipSuccess) return e;
   if ((e = hipEventCreateWithFlags(&P.ev_panel, hipEventDisableTiming)) != hipSuccess) return e;
@@ -175,6 +179,8 @@ void lu_plan_destroy(LuPlan &P) {
   release(P.dinv);
   release(P.d_bperm);
   release(P.Y);
+  release(P.W2);
+  release(P.Y2);
   P.levels.clear();
   P.dinv_slots = 0;
   P.count_max = 0;

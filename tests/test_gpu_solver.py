@@ -246,3 +246,34 @@ def test_every_gemm_configuration(oracle_lib, config):
     res0 = np.linalg.norm(K @ x0 - b) / np.linalg.norm(b)
     print(f"config {config}: unrefined residual {res0:.2e}", flush=True)
     assert res0 <= 1e-9
+
+
+@pytest.mark.parametrize("shape,precision", [((24, 12), ta.PRECISION_FP64), ((40, 16), ta.PRECISION_FP64),
+                                             ((40, 16), ta.PRECISION_MIXED)])
+def test_solve_pair_matches_single_solves(oracle_lib, shape, precision):
+    """Two right-hand sides in one pass over the factors give the two
+    single-right-hand-side solves (the sums differ only in the order of the
+    atomic accumulation across column chunks)."""
+    c = small_case(*shape)
+    sigma = 0.1 - 0.05j
+    A, B = operators(oracle_lib, c)
+    K = (A - sigma * B).tocsc()
+    rng = np.random.default_rng(11)
+    b1 = rng.standard_normal(K.shape[0]) + 1j * rng.standard_normal(K.shape[0])
+    b2 = rng.standard_normal(K.shape[0]) + 1j * rng.standard_normal(K.shape[0])
+    with ta.Context(0) as ctx:
+        ctx.load_case(c)
+        ctx.assemble(diagnostics=False)
+        ctx.set_precision(precision)
+        ctx.factor(sigma)
+        y1, _, _ = ctx.linear_solve(b1, refine_max=0)
+        y2, _, _ = ctx.linear_solve(b2, refine_max=0)
+        x1, x2 = ctx.linear_solve_pair(b1, b2)
+    d1 = np.linalg.norm(x1 - y1) / np.linalg.norm(y1)
+    d2 = np.linalg.norm(x2 - y2) / np.linalg.norm(y2)
+    r1 = np.linalg.norm(K @ x1 - b1) / np.linalg.norm(b1)
+    r2 = np.linalg.norm(K @ x2 - b2) / np.linalg.norm(b2)
+    print(f"case {shape} precision {precision}: pair vs single {d1:.2e} {d2:.2e}; residuals {r1:.2e} {r2:.2e}")
+    fp64 = precision == ta.PRECISION_FP64
+    assert d1 <= (1e-12 if fp64 else 1e-6) and d2 <= (1e-12 if fp64 else 1e-6)
+    assert r1 <= (1e-9 if fp64 else 1e-2) and r2 <= (1e-9 if fp64 else 1e-2)
