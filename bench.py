@@ -5,7 +5,7 @@ Headline (`value`): eigenpairs/s of a wavenumber sweep on the 200x100 case
 Ma=0.7, alpha ~ U[0.5, 3] from seed 1) in the learned-shift form: the shift
 network proposes each instance's shift centre, instances run in wavenumber
 order with warm-started Krylov-Schur, and the fixed-shift search is the
-fallback; three contexts pipeline the sweep (the next two instances'
+fallback; eight contexts pipeline the sweep (the next seven instances'
 assembly and factorization run while the current instance's Krylov-Schur
 runs).  A step is one sweep instance (assembly of A_bc, B_bc, sparse LU
 of A_bc - sigma B_bc, Krylov-Schur for 8 eigenpairs); an eigenpair counts if
@@ -24,9 +24,21 @@ from __future__ import annotations
 
 import argparse
 import json
+import os
 import sys
 import time
 from pathlib import Path
+
+# The learned sweep drives two HIP streams per context (the factorization's
+# main and look-ahead streams).  With HIP's default of four hardware queues
+# per process, unrelated streams share queues and wait for each other:
+# measured at 200x100 with three contexts, 199 ms per instance at 4 queues
+# against 185 ms at 8 or 16.  24 covers the eight learned contexts' streams
+# and the plain form's.  Read by HIP when it initialises, so set here,
+# before anything touches the GPU (also where the environment sets the
+# default of 4).
+os.environ["GPU_MAX_HW_QUEUES"] = "24"
+
 
 import numpy as np
 
@@ -39,7 +51,7 @@ NEV, TOL, OMEGA = 8, 1e-10, 0.02 + 0j
 NCV = 64  # Krylov subspace size (the -eps_ncv of the reference's options database)
 LEARNED_NCV = 40  # subspace of the learned form: the wanted cluster converges in one pass
 LEARNED_CHECK_FROM = 28  # learned form: convergence also tested every second step from step 28
-PIPELINE_CONTEXTS = 3  # learned sweep: Krylov-Schur of one instance, factorization of the next two
+PIPELINE_CONTEXTS = 8  # learned sweep: Krylov-Schur of one instance, factorization of the next seven
 
 
 def algorithmic_assembly_bytes(ny: int, nz: int, ssx: int, sse: int) -> int:

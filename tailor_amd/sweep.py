@@ -20,9 +20,12 @@ its grid and base flow and reassembles.  With `contexts=C > 1` the learned
 form runs C contexts per size as a pipeline: while one context runs the
 current instance's Krylov-Schur, worker threads assemble and factor the
 next C - 1 instances on the others (each context warm-starts from the
-instance it solved before, C steps back in wavenumber).  The two HIP streams are
-non-blocking, so the GEMV-bound Krylov phase fills the GPU during the
-factorization's latency-bound panel phases.
+instance it solved before, C steps back in wavenumber).  The HIP streams
+are non-blocking, so the GEMV-bound Krylov phase fills the GPU during the
+factorization's latency-bound panel phases.  Each context drives two
+streams, so the process needs at least 2 C hardware queues
+(GPU_MAX_HW_QUEUES, read when HIP initialises; HIP's default is 4), or
+unrelated streams wait for each other.
 """
 from __future__ import annotations
 
@@ -126,8 +129,8 @@ class Sweeper:
         """`check_from` > 0: the learned form's Krylov-Schur runs also test
         convergence after every second step from that step on and end a pass
         early (the plain form and the fallback test only at pass ends)."""
-        if not 1 <= contexts <= 4:
-            raise ValueError("contexts must be 1 to 4")
+        if not 1 <= contexts <= 8:
+            raise ValueError("contexts must be 1 to 8")
         if check_from < 0:
             raise ValueError("check_from must be >= 0")
         self.check_from = check_from
