@@ -14,7 +14,8 @@
   diagnostic lines of Process.cpp:62-176 and writes the schema-v1 eigenmode
   file of EigenOutput.cpp:85-304 (temporary file, validation, rename).
 * :func:`main` is ``Tailor.py`` (main.py:35-332): same arguments, the FD-q
-  cache check as the preprocessing step, then the solver as a child process;
+  preprocessing step (reuse a valid cache, else build it from the source
+  flow with :mod:`tailor_amd.prepare`), then the solver as a child process;
   a preprocessing failure's status is propagated and a launch error is 2.
 
 Everything runs the HIP library; there is no CPU path.
@@ -302,7 +303,7 @@ def load_cache(recipe: Recipe) -> Case:
 
 
 def write_cache(case: Case, path: str | os.PathLike, source_file: str = "synthetic",
-                source_sha256: str | None = None) -> Path:
+                source_sha256: str | None = None, extra_attrs: dict | None = None) -> Path:
     """Write ``case`` as a schema-v2 FD-q cache (the layout Prepare.cpp:443-668
     reads and interpolate.py:399-436 writes): root attributes, ``/grid`` and
     ``/baseflow`` in natural ``k_j`` order, ``/discretization/{y,z}``."""
@@ -313,6 +314,7 @@ def write_cache(case: Case, path: str | os.PathLike, source_file: str = "synthet
         "source_file": source_file})
     if source_sha256 is not None:
         root.attrs["source_sha256"] = source_sha256
+    root.attrs.update(extra_attrs or {})
     root.members["grid"] = h5.complex_dataset("grid", case.grid)
     root.members["baseflow"] = h5.complex_dataset("baseflow", case.baseflow)
     disc = h5.Group("discretization")
@@ -557,12 +559,14 @@ def _sha256(path: Path) -> str:
 
 
 def prepare(config: Path) -> tuple[int, Recipe | None]:
-    """The preprocessing stage: resolve the cache and check that it is current.
+    """The preprocessing stage (main.py:258-285 + interpolate.py:501-605).
 
-    A cache written for another source file (its ``source_sha256`` attribute
-    differs from the source's SHA-256) or for other degrees is reported as
-    stale.  Building a cache from a raw source flow (B-spline interpolation
-    onto FD-q nodes, interpolate.py) is not part of this build."""
+    An existing cache is reused when it loads cleanly and, if the source
+    file is present, was built from it (``source_sha256``); otherwise the
+    cache is rebuilt from the source flow by :mod:`tailor_amd.prepare`.
+    Without a source file an existing valid cache is used as is."""
+    from . import prepare as prep
+
     try:
         recipe = parse_config(config, require_cache=False)
     except InputError as error:
@@ -571,23 +575,30 @@ def prepare(config: Path) -> tuple[int, Recipe | None]:
     print("Tailor FD-q preprocessing\n"
           f"  config: {recipe.config_path}\n  source: {recipe.source_file}\n"
           f"  output: {recipe.input_file}\n  q_y: {recipe.q_y}\n  q_z: {recipe.q_z}", flush=True)
-    if not recipe.input_file.is_file():
-        print(f"Tailor FD-q preprocessing error: no FD-q cache at {recipe.input_file}; "
-              "interpolating a raw source flow onto FD-q nodes is not part of this build",
-              file=sys.stderr)
-        return 1, None
-    try:
-        case = load_cache(recipe)
-        attrs = h5.read(recipe.input_file).attrs
-    except InputError as error:
-        print(f"Tailor FD-q preprocessing error: {error}", file=sys.stderr)
-        return 1, None
-    want = attrs.get("source_sha256")
-    if want is not None and recipe.source_file.is_file() and _sha256(recipe.source_file) != want:
-        print(f"Tailor FD-q preprocessing error: {recipe.input_file} was built from another "
-              f"version of {recipe.source_file} (source_sha256 differs)", file=sys.stderr)
-        return 1, None
-    del case
+    reason = None
+    if recipe.input_file.is_file():
+        try:
+            load_cache(recipe)
+            want = h5.read(recipe.input_file).attrs.get("source_sha256")
+            if (recipe.source_file.is_file() and want is not None
+                    and _sha256(recipe.source_file) != want):
+                reason = "source_sha256 differs from the source file"
+        except InputError as error:
+            reason = str(error)
+        if reason is None:
+            print(f"FD-q cache hit: {recipe.input_file}", flush=True)
+    else:
+        reason = "no cache"
+    if reason is not None:
+        if recipe.input_file.is_file():
+            print(f"FD-q cache invalid, rebuilding: {reason}", file=sys.stderr)
+        try:
+            recipe.input_file.parent.mkdir(parents=True, exist_ok=True)
+            prep.prepare_case(recipe.source_file, recipe.input_file, recipe.q_y, recipe.q_z)
+            load_cache(recipe)
+        except (OSError, ValueError, InputError, h5.H5Error) as error:
+            print(f"Tailor FD-q preprocessing error: {error}", file=sys.stderr)
+            return 1, None
     print(f"Tailor FD-q input ready: {recipe.input_file}", flush=True)
     return 0, recipe
 

@@ -287,3 +287,106 @@ def test_solver_reports_a_missing_cache(tmp_path, capsys):
     os.remove(cli.parse_config(cfg).input_file)
     assert cli.solver_main(["-c", str(cfg), "-tailor_assemble_only"]) != 0
     assert "FD-q input file does not exist" in capsys.readouterr().err
+
+
+# ---------------------------------------------------------------- preprocessing
+
+REFERENCE_SOURCE = Path("/root/reference/case/data/sample.h5")
+
+
+def _write_source(path: Path, ny: int, nz: int, period: float):
+    """A source flow the quintic tensor spline must reproduce exactly: every
+    field is a polynomial of degree <= 5 in xi, constant in eta, except the
+    grid's z = period (eta + 1) / 2, whose periodic part is constant."""
+    xi = np.linspace(-1.0, 1.0, ny)
+    eta = -1.0 + 2.0 * np.arange(nz) / nz
+    X, E = np.meshgrid(xi, eta)                       # (nz, ny)
+    y = 20.0 * (1.0 + X) + 0.3 * X ** 5 - 0.1 * X ** 3
+    z = period * (E + 1.0) / 2.0
+    grid = np.stack([np.zeros_like(X), y, z], axis=-1).astype(np.complex128)
+    u = 0.5 * (1.0 + X) - 0.05 * X ** 4
+    t = 1.5 - 0.2 * X ** 2 + 0.01 * X ** 5
+    base = np.stack([1.0 / 1.5 + 0.1 * X ** 2, u, 0 * X, 0 * X, t], axis=-1).astype(np.complex128)
+    base[..., 2] += 1e-3j * X ** 3                    # an imaginary part travels too
+    root = h5.Group("/", attrs={"Ny": ny, "Nz": nz, "spanwise_periodic": 1,
+                                "spanwise_period": period})
+    root.members["grid"] = h5.complex_dataset("grid", grid)
+    root.members["baseflow"] = h5.complex_dataset("baseflow", base)
+    h5.write(path, root)
+    return lambda xn, en: (np.meshgrid(xn, en))
+
+
+def test_prepare_reproduces_quintic_fields_exactly(tmp_path):
+    from tailor_amd import fdq, prepare as prep
+    ny, nz, period = 24, 12, 5.0
+    src = tmp_path / "data" / "flow.h5"
+    src.parent.mkdir(parents=True)
+    _write_source(src, ny, nz, period)
+    nodes = -np.cos(np.pi * np.arange(ny) / (ny - 1))
+    nodes[0], nodes[-1] = -1.0, 1.0
+    with mock.patch.object(fdq, "load_bounded_nodes", lambda n, q: nodes):
+        out = prep.prepare_case(src, tmp_path / "data" / "FD-q" / "fdq_flow_qy10_qz6.h5", 10, 6,
+                                log=lambda *_: None)
+    root = h5.read(out)
+    X, E = np.meshgrid(nodes, -1.0 + 2.0 * np.arange(nz) / nz)
+    grid = h5.as_complex(root["grid"]).reshape(nz, ny, 3)
+    base = h5.as_complex(root["baseflow"]).reshape(nz, ny, 5)
+    np.testing.assert_allclose(grid[..., 1].real, 20.0 * (1.0 + X) + 0.3 * X ** 5 - 0.1 * X ** 3,
+                               rtol=0, atol=1e-12)
+    np.testing.assert_allclose(grid[..., 2].real, period * (E + 1.0) / 2.0, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(base[..., 4].real, 1.5 - 0.2 * X ** 2 + 0.01 * X ** 5, atol=1e-13)
+    np.testing.assert_allclose(base[..., 2].imag, 1e-3 * X ** 3, atol=1e-15)
+    assert root.attrs["source_sha256"] == prep.sha256_file(src)
+    assert root.attrs["interpolation_method"] == "bounded_y_periodic_z_quintic_bspline"
+    assert root["discretization/y"].attrs["N"] == ny - 1
+
+
+def test_prepare_rejects_bad_sources(tmp_path):
+    from tailor_amd import prepare as prep
+    src = tmp_path / "flow.h5"
+    _write_source(src, 24, 12, 5.0)
+    root = h5.read(src)
+    root["baseflow"].data[0, 0, 4, 0] = -1.0
+    h5.write(src, root)
+    with pytest.raises(prep.PrepareError, match="sample temperature must be strictly positive"):
+        prep.prepare_case(src, tmp_path / "out.h5", 10, 6)
+    _write_source(src, 24, 12, 5.0)
+    with pytest.raises(prep.PrepareError, match="periodic q_z must be even"):
+        prep.prepare_case(src, tmp_path / "out.h5", 10, 5)
+    with pytest.raises(FileNotFoundError, match="structured sample file does not exist"):
+        prep.prepare_case(tmp_path / "missing.h5", tmp_path / "out.h5", 10, 6)
+
+
+def test_launcher_builds_a_missing_cache_then_reuses_it(tmp_path, capsys):
+    from tailor_amd import fdq
+    c = small_case()
+    cfg = cli.write_config(tmp_path / "case" / "config.yaml", c)
+    src = tmp_path / "case" / "data" / "case.h5"
+    src.parent.mkdir(parents=True)
+    _write_source(src, 24, 12, 5.0)
+    nodes = c.xi_rule.nodes
+    with mock.patch.object(fdq, "load_bounded_nodes", lambda n, q: nodes):
+        assert cli.main(["-c", str(cfg), "--prepare-only"]) == 0
+        assert "FD-q cache generated" in capsys.readouterr().out
+        assert cli.main(["-c", str(cfg), "--prepare-only"]) == 0
+        assert "FD-q cache hit" in capsys.readouterr().out
+
+
+@pytest.mark.skipif(not (REFERENCE_SOURCE.is_file() and REFERENCE_CACHE.is_file()),
+                    reason="reference case not present")
+def test_prepare_matches_the_shipped_cache(tmp_path):
+    """The reference's own sample.h5 -> its shipped FD-q cache (the output of
+    interpolate.py), with this package's N = 200 equal-error nodes."""
+    from tailor_amd import prepare as prep
+    out = prep.prepare_case(REFERENCE_SOURCE, tmp_path / "fdq.h5", 10, 6, log=lambda *_: None)
+    mine, ref = h5.read(out), h5.read(REFERENCE_CACHE)
+    for name in ("grid", "baseflow"):
+        a, b = mine[name].data, ref[name].data
+        assert a.shape == b.shape
+        assert np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b))) <= 1e-12, name
+    for key in ("schema_version", "source_file", "source_sha256", "ordering", "Ny", "Nz",
+                "q_y", "q_z", "spanwise_periodic", "spanwise_period", "interpolation_method",
+                "interpolation_version"):
+        assert mine.attrs[key] == ref.attrs[key], key
+    np.testing.assert_allclose(mine["discretization/y/nodes"].data,
+                               ref["discretization/y/nodes"].data, rtol=0, atol=1e-14)
