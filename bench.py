@@ -233,7 +233,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(alpha sweeps shard by instance across GPUs; see DESIGN.md)", file=sys.stderr)
+              "(a multi-GPU sweep would shard whole alpha instances across GPUs and "
+              "gather only eigenvalues and residuals)", file=sys.stderr)
         return 2
     if args.steps < 1:
         print("bench.py: --steps must be at least 1", file=sys.stderr)

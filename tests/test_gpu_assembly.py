@@ -1,7 +1,6 @@
 """GPU assembly parity: libtailor (HIP, gfx950) vs the CPU oracle.
 
-The bar: assembled block entries within 1e-12
-relative.  The kernels follow the oracle's operation order without FP
+The bar: assembled block entries within 1e-12 relative.  The kernels follow the oracle's operation order without FP
 contraction, so most entries are expected to be bit-identical; the test
 reports the bit-exact fraction and enforces the 1e-12 bar.
 """
