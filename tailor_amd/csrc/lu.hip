@@ -1853,14 +1853,29 @@ __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list
         }
       }
     }
-    for (; k < c1; ++k) {
-      const long long idx = static_cast<long long>(k) * V.lda;
-      const double br = static_cast<double>(ar[idx]), bi = static_cast<double>(ai[idx]);
+    // the last c1 - k < 16 columns (front orders are multiples of 5, so
+    // most waves end with some): every load is issued before the first use,
+    // and the sums run over the columns in the same order as above
+    const int rem = c1 - k;
+    if (rem > 0) {
 #pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        const double x_r = sxr[r][k - cw], x_i = sxi[r][k - cw];
-        accr[r] += br * x_r - bi * x_i;
-        acci[r] += br * x_i + bi * x_r;
+      for (int u = 0; u < 16; ++u) {
+        if (u < rem) {
+          const long long idx = static_cast<long long>(k + u) * V.lda;
+          a_r[u] = static_cast<double>(ar[idx]);
+          a_i[u] = static_cast<double>(ai[idx]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        if (u < rem) {
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+            const double x_r = sxr[r][k + u - cw], x_i = sxi[r][k + u - cw];
+            accr[r] += a_r[u] * x_r - a_i[u] * x_i;
+            acci[r] += a_r[u] * x_i + a_i[u] * x_r;
+          }
+        }
       }
     }
   }
