@@ -228,7 +228,9 @@ typedef struct {
   int32_t nev;             /* NumberOfEigenvalues */
   int32_t ncv;             /* subspace size (at most 128); 0 selects max(2 nev, nev + 15) */
   int32_t maxit;           /* MaximumIterations (restarts) */
-  double tol;              /* Tolerance: accepted when ||A q - lambda B q|| / ||q|| <= tol */
+  double tol;              /* Tolerance: a pair is accepted when ||A q - lambda B q|| / ||q||
+                              and ||A q - lambda B q|| / (|lambda| ||q||) (SLEPc's relative
+                              error, EigenSolver.cpp:77-82) are both <= tol */
   double omega_re, omega_im;  /* SearchCenterOmega; lambda_s = Im(omega) - i Re(omega) */
   int32_t refine_max;      /* iterative refinement steps per shift-invert solve */
   double refine_tol;       /* relative residual target of a refined solve */

@@ -43,8 +43,10 @@ def krylov_schur(A, B, sigma: complex, nev: int = 8, tol: float = 1e-10, maxit: 
     leading pairs (EPS_CONV_REL on the transformed problem).  `solver`, if
     given, maps b to (A - sigma B)^-1 b; otherwise SuperLU factors it.
     With `true_residual`, the nev leading pairs must also meet
-    ||A q - lambda B q|| / ||q|| <= tol (the acceptance rule of the GPU path);
-    iteration stops early if those residuals stop improving.
+    ||A q - lambda B q|| / ||q|| <= tol and ||A q - lambda B q|| /
+    (|lambda| ||q||) <= tol (the acceptance rule of the GPU path; the second
+    is SLEPc's relative error, EigenSolver.cpp:77-82); iteration stops early
+    if those residuals stop improving.
     """
     A = sp.csc_matrix(A)
     B = sp.csc_matrix(B)
@@ -105,7 +107,8 @@ def krylov_schur(A, B, sigma: complex, nev: int = 8, tol: float = 1e-10, maxit: 
                 y = _tri_eigvec(T, i)
                 x = V[:, :ncv] @ (Q @ y)
                 lam = sigma + 1.0 / theta[i]
-                worst = max(worst, np.linalg.norm(A @ x - lam * (B @ x)) / np.linalg.norm(x))
+                r = np.linalg.norm(A @ x - lam * (B @ x)) / np.linalg.norm(x)
+                worst = max(worst, r, r / max(abs(lam), 1e-300))
             if worst > tol:
                 hist = getattr(krylov_schur, "_hist", [])
                 done = False

@@ -181,6 +181,7 @@ class Context:
         if st:
             raise TailorError(st, "tailor_create failed")
         self.case: Case | None = None
+        self._precision = PRECISION_FP64
 
     def close(self):
         if self.ptr:
@@ -274,6 +275,7 @@ def _set_precision(self, precision: int) -> None:
     """PRECISION_FP64 (plain form) or PRECISION_MIXED (complex FP32 fronts,
     FP64 refinement of every solve)."""
     self._check(self.lib.tailor_set_precision(self.ptr, int(precision)))
+    self._precision = int(precision)
 
 
 def _profile_factor(self, sigma: complex) -> dict:
@@ -317,16 +319,21 @@ def _linear_solve(self, b: np.ndarray, refine_max: int = 0, refine_tol: float = 
 
 
 def _solve(self, nev: int = 8, omega: complex = 0j, tol: float = 1e-10, maxit: int = 1000,
-           ncv: int = 0, refine_max: int = 2, refine_tol: float = 1e-13,
+           ncv: int = 0, refine_max: int | None = None, refine_tol: float = 1e-12,
            raise_on_failure: bool = True, warm_start: bool = False, check_from: int = 0) -> dict:
     """Shift-invert Krylov-Schur (EigenSolver::solve).  With warm_start the
     iteration starts from the previous solve's Ritz vectors; with
     check_from = m > 0 convergence is also tested after every second step
     from step m on, and a pass ends as soon as the wanted pairs' Ritz
-    estimates meet tol."""
+    estimates meet tol.  refine_max defaults to 0 on FP64 factors (the plain
+    form: one pass over the factors per solve, as PCLU + KSPPREONLY) and to
+    20 on complex FP32 factors, where every solve is refined in FP64 to a
+    relative residual of refine_tol."""
     omega = complex(omega)
     if check_from < 0:
         raise ValueError("check_from must be >= 0")
+    if refine_max is None:
+        refine_max = 20 if getattr(self, "_precision", PRECISION_FP64) == PRECISION_MIXED else 0
     o = SolveOpts(nev, ncv, maxit, tol, omega.real, omega.imag, refine_max, refine_tol,
                   1 if warm_start else 0, check_from)
     d = SolveDiag()

@@ -505,8 +505,10 @@ def solver_main(argv: Sequence[str] | None = None) -> int:
             if "-tailor_assemble_only" in opts:
                 print("  assemble-only: boundary matrices validated; EPSSolve skipped", flush=True)
                 return 0
+            # a solve that ends with fewer than nev pairs raises here, before
+            # the diagnostics and before any output (EigenSolver.cpp:84-88)
             sd = ctx.solve(nev=nev, omega=recipe.search_center_omega, tol=tol, maxit=maxit,
-                           ncv=ncv, raise_on_failure=False)
+                           ncv=ncv)
             lam_s = recipe.search_center_lambda
             print(f"  eigensolver: requested={sd['requested']}, converged={sd['converged']}, "
                   f"iterations={sd['iterations']}\n"

@@ -9,8 +9,11 @@
 // (G. W. Stewart, SIAM J. Matrix Anal. Appl. 23 (2001) 601-614): Arnoldi with
 // classical Gram-Schmidt and one reorthogonalisation, complex Schur form of
 // the projected matrix, sorting by |theta|, restart keeping the converged
-// pairs plus half of the rest (SLEPc's default restart ratio 0.5).  Pairs
-// are accepted by their true residual ||A q - lambda B q|| / ||q||.
+// pairs plus half of the rest (SLEPc's default restart ratio 0.5).  A pair
+// is accepted when both its true residual ||A q - lambda B q|| / ||q|| and
+// its relative error ||A q - lambda B q|| / (|lambda| ||q||) meet tol: the
+// second is SLEPc's EPS_CONV_REL / EPS_ERROR_RELATIVE measure, which the
+// reference stores as /spectrum/residual (EigenSolver.cpp:77-82).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -376,6 +379,7 @@ int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solv
   const double tol = o.tol;
   double best_res = 1e300;
   int stagnant = 0;
+  int reason = -1;  // SLEPc's EPS_DIVERGED_ITS unless the residuals stagnate
   // read back the coefficients of Arnoldi steps [from, to) into Sm
   auto fill_steps = [&](int from, int to) -> int {
     if (to <= from) return TAILOR_OK;
@@ -502,7 +506,7 @@ int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solv
         relerr[i] = resid[i] / std::max(std::abs(lam), 1e-300);
       }
       nconv = 0;
-      while (nconv < ncheck && resid[nconv] <= tol) ++nconv;
+      while (nconv < ncheck && resid[nconv] <= tol && relerr[nconv] <= tol) ++nconv;
       if (nconv >= nev || restarts == o.maxit - 1) {
         nout_pass = m;
         break;
@@ -510,12 +514,13 @@ int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solv
       // the Ritz estimates are below tol but the true residuals are not:
       // stop when they no longer improve (accuracy floor of the solves)
       double worst = 0.0;
-      for (int i = 0; i < nev; ++i) worst = std::max(worst, resid[i]);
+      for (int i = 0; i < nev; ++i) worst = std::max(worst, std::max(resid[i], relerr[i]));
       if (worst < 0.5 * best_res) {
         best_res = worst;
         stagnant = 0;
       } else if (++stagnant >= 3) {
         nout_pass = m;
+        reason = -2;  // EPS_DIVERGED_BREAKDOWN: the accuracy floor of the solves
         break;
       }
     }
@@ -586,10 +591,12 @@ int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solv
   diag.fronts = S.sym.nsn;
   diag.factor_bytes = static_cast<int64_t>(S.plan.arena_doubles) * (S.fp32 ? 4 : 8);
   diag.factor_flops = static_cast<double>(S.sym.flops);
+  // the reference's message (EigenSolver.cpp:84-88)
   if (nconv < nev)
     return fail(ctx, TAILOR_ERR_NOT_CONVERGED,
-                "Krylov-Schur failed to converge (" + std::to_string(nconv) + " eigenpairs after " +
-                    std::to_string(restarts + 1) + " iterations)");
+                "SLEPc failed to converge (reason " + std::to_string(reason) + ", " +
+                    std::to_string(nconv) + " eigenpairs after " + std::to_string(restarts + 1) +
+                    " iterations)");
   return TAILOR_OK;
 }
 

@@ -62,7 +62,7 @@ class Record:
     converged: int
     good: int                 # pairs with true residual <= tol among the first nev
     omegas: np.ndarray
-    residuals: np.ndarray
+    residuals: np.ndarray     # max(residual, relative error) per pair
     restarts: int
     solves: int
     refinement_steps: int
@@ -167,7 +167,8 @@ class Sweeper:
                       refine_max=20 if mixed else 0, refine_tol=1e-12, raise_on_failure=False,
                       warm_start=warm, check_from=check_from)
         pairs = ctx.eigenpairs()[:self.nev]
-        res = np.array([p["residual"] for p in pairs])
+        # a pair counts when both its residual and its relative error meet tol
+        res = np.array([max(p["residual"], p["relative_error"]) for p in pairs])
         om = np.array([p["omega"] for p in pairs])
         good = int(np.sum(res <= self.tol))
         return d, om, res, good

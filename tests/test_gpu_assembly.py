@@ -123,3 +123,21 @@ def test_physics_validation_message():
         with pytest.raises(ta.TailorError) as e:
             ctx.load_case(c)
     assert "outside their valid ranges" in str(e.value)
+
+
+def test_derivatives_closed_form_on_skewed_grid():
+    """The GPU's metrics and base-flow derivatives against the closed form of
+    a skewed affine grid with base-flow fields P(y) H(eta)
+    (conftest.skewed_product_case), with the bounds the oracle meets."""
+    from conftest import skewed_product_case
+    c, exact, metrics = skewed_product_case()
+    with ta.Context(0) as ctx:
+        ctx.load_case(c)
+        ctx.assemble(diagnostics=False)
+        m, d = ctx.export_fields()
+    assert np.max(np.abs(m[:, :4] - metrics[:4])) <= 1e-13
+    assert np.max(np.abs(m[:, 4:])) <= 1e-10
+    scale = np.max(np.abs(exact), axis=0)
+    err = np.max(np.abs(d - exact), axis=0) / np.maximum(scale, 1e-3)
+    print(f"closed-form derivative error {np.max(err):.3e}", flush=True)
+    assert np.max(err) <= 1e-9

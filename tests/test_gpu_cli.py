@@ -105,3 +105,34 @@ def test_rule_corruptions_fail_in_the_loader(tmp_path, capsys, name, edit, messa
     print(f"tailor -c <corrupted cache: {name}> -tailor_assemble_only", flush=True)
     assert cli.solver_main(["-c", str(cfg), "-tailor_assemble_only"]) != 0
     assert message in capsys.readouterr().err
+
+
+def test_stored_relative_errors_meet_the_tolerance_when_lambda_is_small(tmp_path, capsys):
+    """/spectrum/residual holds SLEPc's relative error ||Aq - lambda Bq|| /
+    (|lambda| ||q||) (EigenSolver.cpp:77-82); with |lambda| << 1 it is the
+    stricter of the two measures, and every stored value must meet tol."""
+    c = small_case(alpha=0.3 + 0j)
+    cfg = _case_dir(tmp_path, c)
+    print("tailor -c <24x12 synthetic case, alpha 0.3>", flush=True)
+    assert cli.solver_main(["-c", str(cfg)]) == 0
+    f = h5.read(tmp_path / "case" / "results" / "eigenmodes.h5")
+    lam = h5.as_complex(f["spectrum/lambda"])
+    rel = h5.as_complex(f["spectrum/residual"]).real
+    assert np.min(np.abs(lam)) < 0.05  # the relative error is > 20x the residual here
+    assert np.max(rel) <= 1e-10, rel
+    out = capsys.readouterr().out
+    max_rel = float(out.split("max relative error: ")[1].split()[0])
+    assert max_rel <= 1e-10
+
+
+def test_unconverged_solve_fails_before_any_output(tmp_path, capsys):
+    """Fewer than nev pairs after maxit restarts: the reference's
+    'SLEPc failed to converge' error (EigenSolver.cpp:84-88), a nonzero exit
+    and no output file."""
+    cfg = _case_dir(tmp_path, small_case())
+    print("tailor -c <24x12 synthetic case> -eps_max_it 1 -eps_ncv 10", flush=True)
+    rc = cli.solver_main(["-c", str(cfg), "-eps_max_it", "1", "-eps_ncv", "10"])
+    err = capsys.readouterr().err
+    assert rc != 0
+    assert "SLEPc failed to converge (reason -1, " in err and "after 1 iterations" in err
+    assert not (tmp_path / "case" / "results").exists()

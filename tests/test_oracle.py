@@ -332,3 +332,38 @@ def test_uniform_flow_characteristic_counts(oracle_lib):
     a = oracle_lib.assemble(c)
     wall, far, inc, neu, mn, mx = a.bc_diag[:6]
     assert (wall, far, inc, neu, mn, mx) == (8, 8, 8, 24, 1, 1)
+
+
+def test_derivatives_closed_form_on_skewed_grid(oracle_lib):
+    """A2-A4 against a closed form that is not the chain-rule formula:
+    a skewed affine map (eta_y != 0) and base-flow fields P(y) H(eta)
+    (conftest.skewed_product_case).  The metrics are exact constants
+    (xi_y = 1/a, xi_z = 0, eta_y = -2c/(a Lz), eta_z = 2/Lz, second metrics
+    zero) and the derivatives follow from the product rule on the physical
+    function, so a dropped, mis-signed or misplaced term of
+    BaseFlowDerivatives.cpp:188-198 changes the result."""
+    from conftest import skewed_product_case
+    c, exact, metrics = skewed_product_case()
+    a = oracle_lib.assemble(c, with_bc=False)
+    assert np.max(np.abs(a.metrics[:, :4] - metrics[:4])) <= 1e-13
+    # second metrics are FD derivatives of constants: rounding noise only
+    assert np.max(np.abs(a.metrics[:, 4:])) <= 1e-10
+    scale = np.max(np.abs(exact), axis=0)
+    err = np.max(np.abs(a.deriv - exact), axis=0) / np.maximum(scale, 1e-3)
+    assert np.max(err) <= 1e-9, err.reshape(5, 5)
+    # every chain-rule term is exercised: none of the 25 fields vanishes
+    assert np.all(scale > 1e-4)
+
+
+def test_closed_form_check_catches_a_dropped_mixed_term(oracle_lib):
+    """The closed form above is sensitive to the mixed xi-eta term: without
+    2 xi_y eta_y f_xi_eta in f_yy the result moves far outside the bound."""
+    from conftest import skewed_product_case, SKEW_A
+    c, exact, metrics = skewed_product_case()
+    a = oracle_lib.assemble(c, with_bc=False)
+    # rebuild f_xi_eta from the oracle's own f_yz: f_yz = xi_y eta_z f_xi_eta + eta_y eta_z f_eta_eta
+    xy, ey, ez = metrics[0], metrics[2], metrics[3]
+    f_ee = a.deriv[:, 15:20] / ez ** 2
+    f_xe = (a.deriv[:, 20:25] - ey * ez * f_ee) / (xy * ez)
+    wrong = a.deriv[:, 10:15] - 2 * xy * ey * f_xe
+    assert np.max(np.abs(wrong - exact[:, 10:15])) > 1e-3 * np.max(np.abs(exact[:, 10:15]))
