@@ -216,6 +216,19 @@ int tailor_linear_solve(tailor_ctx *ctx, const double *b, double *x, int32_t ref
 int tailor_linear_solve_pair(tailor_ctx *ctx, const double *b1, const double *b2, double *x1,
                              double *x2);
 
+/* x = K^-1 b like tailor_linear_solve, on DEVICE buffers b and x (natural
+ * order, interleaved complex, 5*N each), with exactly refine_steps (0..64)
+ * steps of FP64 iterative refinement, and without allocating device memory:
+ * the call's work vectors live in a static 16 MiB device array of the
+ * library, one per GPU, so it serves at most 262,144 unknowns (larger cases
+ * get TAILOR_ERR_ARG_OUTOFRANGE).  Blocks until the result is in x.  Every
+ * call on a GPU uses that GPU's one array: calls that overlap on one GPU
+ * race.  No counterpart in the reference: for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). */
+int tailor_linear_solve_noalloc(tailor_ctx *ctx, const double *b, double *x,
+                                int32_t refine_steps);
+
 /* Time `reps` unrefined solves x = K^-1 b (device-resident vectors) with
  * HIP events on the context stream: mean ms per solve and the bytes of
  * factor data one solve must read: E, L21 and U12 of every front and the

@@ -149,6 +149,8 @@ def library() -> C.CDLL:
     lib.tailor_time_linear_solve.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double)]
     lib.tailor_linear_solve_pair.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double),
                                              P(C.c_double), P(C.c_double)]
+    # device pointers (caller-owned device memory), no device allocation
+    lib.tailor_linear_solve_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
     lib.tailor_eigenpair_count.argtypes = [C.c_void_p]
     lib.tailor_get_eigenpair.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double),
                                          P(C.c_double), P(C.c_double)]

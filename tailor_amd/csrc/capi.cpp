@@ -182,6 +182,7 @@ int tailor_create(tailor_ctx **out, int device) {
   auto *ctx = new tailor_ctx();
   ctx->device = device;
   hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = tailor::noalloc_setup(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = dalloc(&ctx->d_diag, 1);
   if (e == hipSuccess) e = dalloc(&ctx->d_recipe, 8);

@@ -113,4 +113,8 @@ hipError_t launch_operator(tailor_ctx *ctx, bool constrained);
 hipError_t launch_sumsq(tailor_ctx *ctx, const double *data, long long rows, int ncomp,
                         long long stride, double *host_out);
 
+// device address of the no-allocation solve's scratch array on `device`,
+// looked up once per device (noalloc.hip); `device` must be current
+hipError_t noalloc_setup(int device);
+
 }  // namespace tailor
