@@ -1769,7 +1769,12 @@ struct RhsSet {
 // for the small ones, whose launches are latency-bound.  NR right-hand
 // sides (1 or 2) share every load of A; each one's sums are formed in the
 // same order as with one.  Factor entries of type T are widened to FP64;
-// sums are FP64.
+// sums are FP64.  A wave steps through its columns 32 at a time with FP32
+// entries and 16 at a time with FP64 ones, so both keep 256 B of factor
+// loads in flight per lane: with 16 FP32 columns per step the solve was
+// bound by the bytes in flight, at the FP64 solve's time for half the bytes
+// (200x100: 1.43 ms per FP32 solve before, 0.91 ms after; FP64 1.43 ms,
+// and 1.47 ms with 32 FP64 columns per step at half the occupancy).
 template <typename T, int GVC, int NR>
 __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list, int nlist,
                                                     const FrontDev *__restrict__ fr, SolveMaps mp,
@@ -1798,16 +1803,23 @@ __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list
   __shared__ double pr[NR][GVW][GVR], pi[NR][GVW][GVR];
   const bool active = row < V.M && !(V.upper && c1 <= r0);
   const T *ar = V.Ar + (row < V.M ? row : 0), *ai = V.Ai + (row < V.M ? row : 0);
-  // the first 16 columns of A are requested before x is staged, so the two
+  // KS columns per step: every lane keeps 2 * KS factor loads in flight,
+  // 256 B with FP64 entries (KS = 16) and, with KS = 32, also with FP32
+  // entries, so both precisions hold the same bytes in flight per wave.
+  // Entries stay in their stored type in registers and are widened to FP64
+  // where they are used; sums are FP64.
+  constexpr int KS = (sizeof(T) == sizeof(float) && GVC % 32 == 0) ? 32 : 16;
+  static_assert(GVC % KS == 0, "a wave's columns are whole steps");
+  // the first KS columns of A are requested before x is staged, so the two
   // loads overlap
-  double a_r[16], a_i[16];
-  const bool first = active && c0 + 16 <= c1;
+  T a_r[KS], a_i[KS];
+  const bool first = active && c0 + KS <= c1;
   if (first) {
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
+    for (int u = 0; u < KS; ++u) {
       const long long idx = static_cast<long long>(c0 + u) * V.lda;
-      a_r[u] = static_cast<double>(ar[idx]);
-      a_i[u] = static_cast<double>(ai[idx]);
+      a_r[u] = ar[idx];
+      a_i[u] = ai[idx];
     }
   }
   for (int k = threadIdx.x; k < GVW * GVC; k += GVR * GVW) {
@@ -1833,61 +1845,48 @@ __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list
   }
   if (active) {
     int k = c0;
-    // 16 columns per step so that 32 loads per lane are in flight
-    for (; k + 16 <= c1; k += 16) {
+    for (; k + KS <= c1; k += KS) {
       if (k != c0 || !first) {
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
+        for (int u = 0; u < KS; ++u) {
           const long long idx = static_cast<long long>(k + u) * V.lda;
-          a_r[u] = static_cast<double>(ar[idx]);
-          a_i[u] = static_cast<double>(ai[idx]);
+          a_r[u] = ar[idx];
+          a_i[u] = ai[idx];
         }
       }
 #pragma unroll
-      for (int u = 0; u < 16; ++u) {
+      for (int u = 0; u < KS; ++u) {
+        const double br = static_cast<double>(a_r[u]), bi = static_cast<double>(a_i[u]);
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
           const double x_r = sxr[r][k + u - cw], x_i = sxi[r][k + u - cw];
-          accr[r] += a_r[u] * x_r - a_i[u] * x_i;
-          acci[r] += a_r[u] * x_i + a_i[u] * x_r;
-        }
-      }
-    }
-    // the last c1 - k < 16 columns (front orders are multiples of 5, so
-    // most waves end with some).  FP64 factors: every load is issued before
-    // the first use, and the sums run over the columns in the same order as
-    // above (1.517 -> 1.427 ms per solve at 200x100).  FP32 factors keep the
-    // column-by-column loop, which measured faster for them (59.8 against
-    // 79.9 us mean per wide launch).
-    const int rem = c1 - k;
-    if constexpr (sizeof(T) != sizeof(double)) {
-      for (; k < c1; ++k) {
-        const long long idx = static_cast<long long>(k) * V.lda;
-        const double br = static_cast<double>(ar[idx]), bi = static_cast<double>(ai[idx]);
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-          const double x_r = sxr[r][k - cw], x_i = sxi[r][k - cw];
           accr[r] += br * x_r - bi * x_i;
           acci[r] += br * x_i + bi * x_r;
         }
       }
-    } else if (rem > 0) {
+    }
+    // the last c1 - k < KS columns (front orders are multiples of 5, so
+    // most waves end with some): every load is issued before the first use,
+    // and the sums run over the columns in the same order as above
+    const int rem = c1 - k;
+    if (rem > 0) {
 #pragma unroll
-      for (int u = 0; u < 16; ++u) {
+      for (int u = 0; u < KS; ++u) {
         if (u < rem) {
           const long long idx = static_cast<long long>(k + u) * V.lda;
-          a_r[u] = static_cast<double>(ar[idx]);
-          a_i[u] = static_cast<double>(ai[idx]);
+          a_r[u] = ar[idx];
+          a_i[u] = ai[idx];
         }
       }
 #pragma unroll
-      for (int u = 0; u < 16; ++u) {
+      for (int u = 0; u < KS; ++u) {
         if (u < rem) {
+          const double br = static_cast<double>(a_r[u]), bi = static_cast<double>(a_i[u]);
 #pragma unroll
           for (int r = 0; r < NR; ++r) {
             const double x_r = sxr[r][k + u - cw], x_i = sxi[r][k + u - cw];
-            accr[r] += a_r[u] * x_r - a_i[u] * x_i;
-            acci[r] += a_r[u] * x_i + a_i[u] * x_r;
+            accr[r] += br * x_r - bi * x_i;
+            acci[r] += br * x_i + bi * x_r;
           }
         }
       }
