@@ -2,23 +2,36 @@
 
 Headline (`value`): eigenpairs/s of a wavenumber sweep on the 200x100 case
 (FD-q q_y=10 bounded x q_z=6 periodic, 100,000 complex unknowns, Re=1500,
-Ma=0.7, alpha ~ U[0.5, 3] from seed 1) in the learned-shift form: the shift
-network proposes each instance's shift centre, instances run in wavenumber
-order with warm-started Krylov-Schur, and the fixed-shift search is the
-fallback; eight contexts pipeline the sweep (the next seven instances'
-assembly and factorization run while the current instance's Krylov-Schur
-runs).  A step is one sweep instance (assembly of A_bc, B_bc, sparse LU
-of A_bc - sigma B_bc, Krylov-Schur for 8 eigenpairs); an eigenpair counts if
-its true residual ||A q - lambda B q|| / ||q|| <= 1e-10.  The same instances
-are also run in the plain form (complex FP64, fixed omega_s = 0.02, cold
-start) and the ratio is reported.
+Ma=0.7, alpha ~ U[0.5, 3] from seed 1, a base flow per instance from seeds
+1000 + k) in the learned-shift form: the shift network proposes each
+instance's shift centre, instances run in wavenumber order with warm-started
+Krylov-Schur, passes end as soon as the wanted pairs' Ritz estimates meet
+tol, the fixed-shift search is the fallback, and eight contexts pipeline the
+sweep (the next seven instances' assembly and factorization run while the
+current instance's Krylov-Schur runs).  A step is one sweep instance
+(assembly of A_bc, B_bc, sparse LU of A_bc - sigma B_bc, Krylov-Schur for 8
+eigenpairs); an eigenpair counts if its residual ||A q - lambda B q|| / ||q||
+and its relative error (the same over |lambda|) are <= 1e-10.
 
-Also reported: the plain single-instance step at alpha = 2.43 (Re=1000,
-Ma=0.5), the mixed-precision form on that operator (per-eigenvalue error
-against the FP64 result, refinement steps per solve), a small sweep over
-both grid sizes with per-instance base flows in mixed precision, assembly
-HBM GB/s, and the roofline of the frontal-update GEMM (the kernel with the
-largest share of a sweep instance) from HIP events on the library stream.
+The same instances also run
+* in the plain form as the reference runs it (complex FP64, fixed
+  omega_s = 0.02, cold start, ncv = max(2 nev, nev + 15) = 23, one instance
+  after another);
+* in the plain form at its fastest subspace size through the same
+  eight-context pipeline (the strongest plain form; the headline ratio is
+  taken against it);
+* with the network switched off (fixed shift, same warm starts, early pass
+  ends and pipeline): the learned form's rate over this one is what the
+  network itself gains.
+
+Also reported: the plain single instance at alpha = 2.43 (Re=1000, Ma=0.5)
+at several subspace sizes, the mixed-precision form on that operator
+(per-eigenvalue error against the FP64 result, refinement steps per solve),
+a sweep over both grid sizes with per-instance base flows in mixed
+precision, assembly, triangular-solve and SpMV HBM GB/s, the roofline of the
+frontal-update GEMM (the kernel with the largest share of a sweep instance)
+from HIP events on the library stream, and the CPU oracle on the 100x50 case
+beside the GPU on the same case.
 """
 from __future__ import annotations
 
@@ -29,13 +42,13 @@ import sys
 import time
 from pathlib import Path
 
-# The learned sweep drives two HIP streams per context (the factorization's
-# main and look-ahead streams).  With HIP's default of four hardware queues
-# per process, unrelated streams share queues and wait for each other:
-# measured at 200x100 with three contexts, 199 ms per instance at 4 queues
-# against 185 ms at 8 or 16.  24 covers the eight learned contexts' streams
-# and the plain form's.  Read by HIP when it initialises, so set here,
-# before anything touches the GPU (also where the environment sets the
+# The sweeps drive two HIP streams per context (the factorization's main and
+# look-ahead streams).  With HIP's default of four hardware queues per
+# process, unrelated streams share queues and wait for each other: measured
+# at 200x100 with three contexts, 199 ms per instance at 4 queues against
+# 185 ms at 8 or 16.  24 covers the eight pipelined contexts' streams and
+# the one-context plain form's.  Read by HIP when it initialises, so set
+# here, before anything touches the GPU (also where the environment sets the
 # default of 4).
 os.environ["GPU_MAX_HW_QUEUES"] = "24"
 
@@ -48,10 +61,21 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 NEV, TOL, OMEGA = 8, 1e-10, 0.02 + 0j
-NCV = 64  # Krylov subspace size (the -eps_ncv of the reference's options database)
+REFERENCE_NCV = max(2 * NEV, NEV + 15)  # SLEPc's default subspace (EigenSolver.cpp:55-56)
+PLAIN_NCV_CANDIDATES = (REFERENCE_NCV, 32, 48, 64, 96)
 LEARNED_NCV = 40  # subspace of the learned form: the wanted cluster converges in one pass
 LEARNED_CHECK_FROM = 28  # learned form: convergence also tested every second step from step 28
-PIPELINE_CONTEXTS = 8  # learned sweep: Krylov-Schur of one instance, factorization of the next seven
+PIPELINE_CONTEXTS = 8  # Krylov-Schur of one instance, factorization of the next seven
+MIXED_SIZE_INSTANCES = 4  # sweep over both grid sizes in mixed precision
+
+
+_T0 = time.perf_counter()
+
+
+def log(msg: str) -> None:
+    """A progress line on stderr as each phase ends (stdout carries only the
+    JSON result)."""
+    print(f"bench [{time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def algorithmic_assembly_bytes(ny: int, nz: int, ssx: int, sse: int) -> int:
@@ -72,29 +96,87 @@ def measured_traffic(kernel: str):
     return k["traffic_bytes_per_dispatch"] if k else None
 
 
-def cpu_baseline() -> dict:
+def host_facts() -> dict:
+    """The host the CPU baseline runs on."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": usable, "cpu_model": model, "mpi_ranks": 1}
+
+
+def cpu_baseline(ncv: int) -> dict:
     """The CPU oracle (C assembly + multifrontal LU on LAPACK/BLAS + numpy
-    Krylov-Schur) on the 100x50 grid of the same generator (the 200x100
-    case needs minutes of host time)."""
+    Krylov-Schur, the same algorithm as the GPU path) on the 100x50 case of
+    the same generator (omega_s = 0, the reference's settings: nev 8, tol
+    1e-10, ncv 23), with the assembly, factorization and Krylov-Schur times
+    apart.  PETSc, SLEPc and MUMPS are not installed on the GPU host, so the
+    reference's own CPU path cannot run beside it."""
     import oracle
-    from oracle import multifrontal
+    from oracle import eigen, multifrontal
     from tailor_amd import case as cs
     oracle.build()
     c = cs.synthetic_case(100, 50, omega_shift=0j)
+    n = c.nodes
+    sigma = eigen.target_lambda(c.omega_shift)
     t0 = time.perf_counter()
-    res = multifrontal.solve_case(c, nev=NEV, tol=TOL, ncv=NCV)
-    el = time.perf_counter() - t0
-    good = int(np.sum(res["residuals"][:NEV] <= TOL))
+    a = oracle.assemble(c)
+    t1 = time.perf_counter()
+    per = len(a.colidx) // n
+    kvals = a.A.copy()
+    kvals[a.colidx == np.repeat(np.arange(n), per)] -= sigma * a.B
+    lu = multifrontal.MultifrontalLU(c, a.rowptr, a.colidx, kvals)
+    t2 = time.perf_counter()
+    A = oracle.to_scipy(a.rowptr, a.colidx, a.A, n)
+    B = oracle.block_diag_to_scipy(a.B)
+    lams, _, res, restarts = eigen.krylov_schur(A, B, sigma, nev=NEV, tol=TOL, ncv=ncv,
+                                                solver=lu.solve, true_residual=True)
+    t3 = time.perf_counter()
+    rel = res[:NEV] / np.maximum(np.abs(lams[:NEV]), 1e-300)
+    good = int(np.sum((res[:NEV] <= TOL) & (rel <= TOL)))
     threads = multifrontal.blas_threads()
-    return {"value": round(good / el, 4), "unit": "eigenpairs/s", "cores": threads,
+    facts = host_facts()
+    return {"value": round(good / (t3 - t0), 4), "unit": "eigenpairs/s", "cores": threads,
             "kind": "port",
-            "sample": f"one full assemble+factor+Krylov-Schur of the 100x50 case (25,000 unknowns, "
-                      f"omega_s=0), {el:.1f} s, {good} of {NEV} pairs at residual <= 1e-10, "
-                      f"{res['restarts']} restarts; BLAS threads {threads}"}
+            "sample": f"one assemble + factor + Krylov-Schur of the 100x50 case (25,000 unknowns, "
+                      f"omega_s = 0, nev 8, tol 1e-10, ncv {ncv}): {t3 - t0:.1f} s, {good} of "
+                      f"{NEV} pairs at tol, {restarts} restarts",
+            "assembly_s": round(t1 - t0, 3), "factor_s": round(t2 - t1, 3),
+            "krylov_s": round(t3 - t2, 3), "blas_threads": threads, **facts,
+            "lambdas": [[float(l.real), float(l.imag)] for l in lams[:NEV]]}
 
 
-def plain_step(ta, cs, steps: int, ncv: int) -> dict:
-    """The plain FP64 form on one instance (alpha 2.43, omega_s 0.02)."""
+def gpu_small_case(ta, cs, ncv: int) -> dict:
+    """The plain FP64 GPU form on the CPU baseline's 100x50 case."""
+    c = cs.synthetic_case(100, 50, omega_shift=0j)
+    with ta.Context(0) as ctx:
+        ctx.load_case(c)
+        ctx.assemble(diagnostics=False)
+        ctx.solve(nev=NEV, omega=c.omega_shift, tol=TOL, ncv=ncv)  # plan + warm-up
+        t0 = time.perf_counter()
+        ctx.assemble(diagnostics=False)
+        d = ctx.solve(nev=NEV, omega=c.omega_shift, tol=TOL, ncv=ncv)
+        el = time.perf_counter() - t0
+        pairs = ctx.eigenpairs()[:NEV]
+    good = sum(1 for p in pairs if p["residual"] <= TOL and p["relative_error"] <= TOL)
+    return {"eigenpairs_per_s": round(good / el, 4), "seconds": round(el, 4), "ncv": ncv,
+            "factor_ms": round(d["factor_ms"], 2), "krylov_ms": round(d["solve_ms"], 2),
+            "restarts": d["iterations"], "solves": d["solves"],
+            "lambdas": [[p["lambda"].real, p["lambda"].imag] for p in pairs]}
+
+
+def plain_step(ta, cs) -> dict:
+    """The plain FP64 form on one instance (alpha 2.43, omega_s 0.02) at
+    several subspace sizes, the kernels' rooflines on its operator, and the
+    mixed-precision form on the same operator."""
     case = cs.synthetic_case(200, 100, omega_shift=OMEGA)
     ctx = ta.Context(0)
     ctx.load_case(case)
@@ -103,47 +185,69 @@ def plain_step(ta, cs, steps: int, ncv: int) -> dict:
     write_ms, asm_ms = ctx.time_assembly(20)
     asm_bytes = algorithmic_assembly_bytes(200, 100, 11, 7)
 
-    def step():
+    def step(ncv):
+        t0 = time.perf_counter()
         ctx.assemble(diagnostics=False)
-        d = ctx.solve(nev=NEV, omega=OMEGA, tol=TOL, ncv=ncv, refine_max=0, raise_on_failure=False)
+        d = ctx.solve(nev=NEV, omega=OMEGA, tol=TOL, ncv=ncv, raise_on_failure=False)
+        el = time.perf_counter() - t0
         pairs = ctx.eigenpairs()[:NEV]
-        return sum(1 for p in pairs if p["residual"] <= TOL), d, pairs
+        good = sum(1 for p in pairs if p["residual"] <= TOL and p["relative_error"] <= TOL)
+        return el, good, d, pairs
 
-    tune = ctx.autotune(case.lambda_shift)  # frontal GEMM tile order per level
-    step()
-    good, t0 = 0, time.perf_counter()
-    for _ in range(steps):
-        g, d, pairs = step()
-        good += g
-    el = time.perf_counter() - t0
+    tune = ctx.autotune(case.lambda_shift)  # frontal GEMM tile shape and order per level
+    log("plain instance: autotuned")
+    step(REFERENCE_NCV)  # warm-up
+    by_ncv = {}
+    for ncv in PLAIN_NCV_CANDIDATES:
+        el, good, d, pairs = step(ncv)
+        by_ncv[ncv] = {"seconds": round(el, 4), "eigenpairs_per_s": round(good / el, 4),
+                       "restarts": d["iterations"], "solves": d["solves"],
+                       "factor_ms": round(d["factor_ms"], 2), "krylov_ms": round(d["solve_ms"], 2)}
+    best_ncv = max(by_ncv, key=lambda k: by_ncv[k]["eigenpairs_per_s"])
+    log(f"plain instance: subspace sizes {list(by_ncv)} timed, best {best_ncv}")
+    el, good, d, pairs = step(REFERENCE_NCV)
     lam64 = np.array([p["lambda"] for p in pairs])
     solve_ms, solve_bytes = ctx.time_linear_solve(20)
+    spmv_ms, spmv_bytes = ctx.time_operator(50)
     prof = ctx.profile_factor(case.lambda_shift)
+    log("plain instance: solve, SpMV and frontal GEMM timed")
     # mixed precision on the same operator and shift
     ctx.set_precision(ta.PRECISION_MIXED)
     t0 = time.perf_counter()
-    dm = ctx.solve(nev=NEV, omega=OMEGA, tol=TOL, ncv=ncv, refine_max=20, refine_tol=1e-12,
-                   raise_on_failure=False)
+    dm = ctx.solve(nev=NEV, omega=OMEGA, tol=TOL, ncv=REFERENCE_NCV, raise_on_failure=False)
     mixed_s = time.perf_counter() - t0
     pm = ctx.eigenpairs()[:NEV]
     lam32 = np.array([p["lambda"] for p in pm])
     errs = [float(np.min(np.abs(lam32 - l)) / abs(l)) for l in lam64]
+    solve32_ms, solve32_bytes = ctx.time_linear_solve(20)
     ctx.close()
+    log("plain instance: mixed precision done")
+    gbs = lambda b, ms: b / (ms * 1e-3) / 1e9
     return {
-        "plain": {"eigenpairs_per_s": round(good / el, 4), "ms_per_step": round(el * 1e3 / steps, 2),
-                  "steps": steps, "factor_ms": round(d["factor_ms"], 2),
+        "plain": {"eigenpairs_per_s": round(good / el, 4), "seconds": round(el, 4),
+                  "ncv": REFERENCE_NCV, "factor_ms": round(d["factor_ms"], 2),
                   "krylov_ms": round(d["solve_ms"], 2), "solves": d["solves"],
                   "restarts": d["iterations"], "max_residual": d["max_residual"],
-                  "factor_TFLOPs_lu_flops": round(d["factor_flops"] / (d["factor_ms"] * 1e-3) / 1e12, 3)},
-        "assembly": {"GBps_writer": round(asm_bytes / (write_ms * 1e-3) / 1e9, 2),
-                     "GBps_full": round(asm_bytes / (asm_ms * 1e-3) / 1e9, 2),
-                     "frac_hbm_writer": round(asm_bytes / (write_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                  "max_relative_error": d["max_relative_error"],
+                  "factor_TFLOPs_lu_flops": round(d["factor_flops"] / (d["factor_ms"] * 1e-3) / 1e12, 3),
+                  "by_ncv": by_ncv, "best_ncv": best_ncv},
+        "assembly": {"GBps_writer": round(gbs(asm_bytes, write_ms), 2),
+                     "GBps_full": round(gbs(asm_bytes, asm_ms), 2),
+                     "frac_hbm_writer": round(gbs(asm_bytes, write_ms) / HBM_PEAK_GBS, 4),
+                     "frac_hbm_full": round(gbs(asm_bytes, asm_ms) / HBM_PEAK_GBS, 4),
                      "writer_ms": round(write_ms, 5), "full_ms": round(asm_ms, 5),
                      "algorithmic_bytes": asm_bytes},
-        "solve_sweep": {"kernel": "k_gemv (triangular solves)", "ms_per_solve": round(solve_ms, 4),
-                        "bytes": int(solve_bytes),
-                        "GBps": round(solve_bytes / (solve_ms * 1e-3) / 1e9, 2),
-                        "frac_hbm": round(solve_bytes / (solve_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "solve_sweep": {"kernel": "k_gemv (triangular solves), FP64 factors",
+                        "ms_per_solve": round(solve_ms, 4), "bytes": int(solve_bytes),
+                        "GBps": round(gbs(solve_bytes, solve_ms), 2),
+                        "frac_hbm": round(gbs(solve_bytes, solve_ms) / HBM_PEAK_GBS, 4),
+                        "fp32_factors": {"ms_per_solve": round(solve32_ms, 4),
+                                         "bytes": int(solve32_bytes),
+                                         "GBps": round(gbs(solve32_bytes, solve32_ms), 2),
+                                         "frac_hbm": round(gbs(solve32_bytes, solve32_ms) / HBM_PEAK_GBS, 4)}},
+        "spmv": {"kernel": "k_kmul (y = (A_bc - sigma B_bc) x)", "ms": round(spmv_ms, 5),
+                 "bytes": int(spmv_bytes), "GBps": round(gbs(spmv_bytes, spmv_ms), 2),
+                 "frac_hbm": round(gbs(spmv_bytes, spmv_ms) / HBM_PEAK_GBS, 4)},
         "gemm": prof,
         "tune": tune,
         "mixed": {"seconds": round(mixed_s, 3), "factor_ms": round(dm["factor_ms"], 2),
@@ -151,7 +255,7 @@ def plain_step(ta, cs, steps: int, ncv: int) -> dict:
                   "refinement_steps_per_solve": round(dm["refinement_steps"] / max(1, dm["solves"]), 2),
                   "converged": dm["converged"], "max_residual": dm["max_residual"],
                   "eig_rel_err_vs_fp64": [float(f"{e:.3e}") for e in errs],
-                  "fp64_seconds_same_operator": round(el / steps, 3)},
+                  "fp64_seconds_same_operator": round(el, 3), "ncv": REFERENCE_NCV},
     }
 
 
@@ -166,7 +270,28 @@ def without_assembly(totals: dict, assembly_ms: float) -> dict:
     return out
 
 
-def sweeps(ta, steps: int, warmup: int, ncv: int, learned_ncv: int, contexts: int) -> dict:
+def pair_overlap(a, b, rtol=1e-8) -> dict:
+    """Per instance: how many of sweep a's converged pairs sweep b also found
+    (same eigenvalue within rtol), and the largest difference among them."""
+    by = {id(r.instance): r for r in b.records}
+    counts, worst = [], 0.0
+    for r in a.records:
+        p = by[id(r.instance)]
+        good_b = p.omegas[p.residuals <= TOL]
+        n = 0
+        for w in r.omegas[r.residuals <= TOL]:
+            if len(good_b) == 0:
+                continue
+            d = float(np.min(np.abs(good_b - w)) / max(abs(w), 1e-300))
+            if d <= rtol:
+                n += 1
+                worst = max(worst, d)
+        counts.append(n)
+    return {"matched_per_instance": counts, "matched_total": int(sum(counts)),
+            "max_rel_diff": worst}
+
+
+def sweeps(ta, steps: int, warmup: int, best_ncv: int, learned_ncv: int, contexts: int) -> dict:
     from tailor_amd import sweep as sw
     from tailor_amd.shift_model import ShiftModel
     from tailor_amd.train_shift_model import MODEL
@@ -174,49 +299,66 @@ def sweeps(ta, steps: int, warmup: int, ncv: int, learned_ncv: int, contexts: in
         raise SystemExit(f"{MODEL} is missing: run `python -m tailor_amd.train_shift_model`")
     model = ShiftModel(0).load(MODEL)
     insts = sw.sweep_instances(warmup + steps, seed=1, sizes=[(200, 100)], reynolds=1500.0,
-                               mach=0.7, per_instance_seeds=False)
+                               mach=0.7, per_instance_seeds=True)
     cases = sw.prepare(insts, OMEGA)
     alphas = np.array([i.alpha for i in insts])
     shifts = model.shift_centres(alphas, np.full(len(insts), 1500.0), np.full(len(insts), 0.7))
-    runner = sw.Sweeper(ncv=learned_ncv, omega_s=OMEGA, contexts=contexts,
-                        check_from=LEARNED_CHECK_FROM)
-    plain_runner = sw.Sweeper(ncv=ncv, omega_s=OMEGA)
-    # warm-up instances (untimed): build the plan, and let the first one
-    # choose the factorization precision by measurement
+    pipe = sw.Sweeper(ncv=learned_ncv, omega_s=OMEGA, contexts=contexts,
+                      check_from=LEARNED_CHECK_FROM)
+    # warm-up instances (untimed): build the plans, tune, and let the first
+    # one choose the factorization precision of the learned form by
+    # measurement
     warm = list(range(warmup))
     choice = {}
     precision = ta.PRECISION_FP64
     if warm:
         t = {}
         for prec in (ta.PRECISION_FP64, ta.PRECISION_MIXED):
-            r = runner.run(cases, "learned", shifts=shifts, precision=prec, order=warm[:1])
+            r = pipe.run(cases, "learned", shifts=shifts, precision=prec, order=warm[:1])
             t[prec] = r.seconds
         precision = min(t, key=t.get)
         choice = {"fp64_s": round(t[ta.PRECISION_FP64], 3), "mixed_s": round(t[ta.PRECISION_MIXED], 3),
                   "chosen": "mixed" if precision == ta.PRECISION_MIXED else "fp64"}
+    log(f"sweeps: contexts planned and tuned, precision {choice.get('chosen', 'fp64')}")
     timed = list(range(warmup, warmup + steps))
     order = sorted(timed, key=lambda k: insts[k].alpha)
-    learned = runner.run(cases, "learned", shifts=shifts, precision=precision, order=order)
-    runner.close()
-    plain_runner.run(cases, "plain", precision=ta.PRECISION_FP64, order=warm[:1])  # warm-up
-    plain = plain_runner.run(cases, "plain", precision=ta.PRECISION_FP64, order=timed)
-    # eigenvalues found by both forms must agree
-    lam_diff = 0.0
-    by_inst = {id(r.instance): r for r in plain.records}
-    for r in learned.records:
-        p = by_inst[id(r.instance)]
-        for w in r.omegas[r.residuals <= TOL]:
-            d = np.min(np.abs(p.omegas - w)) / max(abs(w), 1e-300)
-            if d < 1e-6:
-                lam_diff = max(lam_diff, float(d))
-    # config 3: both grid sizes, per-instance base flows, mixed precision
-    c3 = sw.sweep_instances(4, seed=1, sizes=[(100, 50), (200, 100)])
+    learned = pipe.run(cases, "learned", shifts=shifts, precision=precision, order=order)
+    log(f"learned sweep: {learned.eigenpairs_per_s:.2f} eigenpairs/s")
+    # the network switched off: the same pipeline, warm starts and early
+    # pass ends at the fixed shift
+    fixed = pipe.run(cases, "fixed", precision=precision, order=order)
+    log(f"network-off sweep: {fixed.eigenpairs_per_s:.2f} eigenpairs/s")
+    # the plain form at its fastest subspace through the same pipeline
+    pipe.ncv, pipe.check_from = best_ncv, 0
+    plain_best = pipe.run(cases, "plain", precision=ta.PRECISION_FP64, order=timed)
+    pipe.close()
+    log(f"plain sweep (ncv {best_ncv}, pipelined): {plain_best.eigenpairs_per_s:.2f} eigenpairs/s")
+    # the plain form as the reference runs it: ncv 23, one instance after another
+    ref = sw.Sweeper(ncv=REFERENCE_NCV, omega_s=OMEGA)
+    ref.run(cases, "plain", precision=ta.PRECISION_FP64, order=warm[:1])  # warm-up
+    plain_ref = ref.run(cases, "plain", precision=ta.PRECISION_FP64, order=timed)
+    log(f"plain sweep (reference settings): {plain_ref.eigenpairs_per_s:.2f} eigenpairs/s")
+    # both grid sizes, per-instance base flows, mixed precision
+    c3 = sw.sweep_instances(MIXED_SIZE_INSTANCES, seed=1, sizes=[(100, 50), (200, 100)])
     c3_cases = sw.prepare(c3, OMEGA)
-    mixed_sweep = plain_runner.run(c3_cases, "mixed")
-    plain_runner.close()
-    return {"learned": learned, "plain": plain, "choice": choice, "common_pair_rel_diff": lam_diff,
-            "mixed_sweep": mixed_sweep, "shifts": shifts,
-            "autotune": {f"{k[0]}x{k[1]}": v for k, v in runner.tuned.items()}}
+    ref.run(c3_cases, "mixed", order=[0, 1])  # plan and tune both sizes
+    mixed_sweep = ref.run(c3_cases, "mixed")
+    ref.close()
+    log(f"mixed-size mixed-precision sweep: {mixed_sweep.eigenpairs_per_s:.2f} eigenpairs/s")
+    per_size = {}
+    for r in mixed_sweep.records:
+        key = f"{r.instance.ny}x{r.instance.nz}"
+        e = per_size.setdefault(key, {"instances": 0, "good_pairs": 0, "seconds": 0.0})
+        e["instances"] += 1
+        e["good_pairs"] += r.good
+        e["seconds"] += r.seconds
+    for e in per_size.values():
+        e["eigenpairs_per_s"] = round(e["good_pairs"] / e["seconds"], 4) if e["seconds"] else None
+        e["seconds"] = round(e["seconds"], 4)
+    return {"learned": learned, "fixed": fixed, "plain_best": plain_best, "plain_ref": plain_ref,
+            "choice": choice, "overlap": pair_overlap(learned, plain_best),
+            "mixed_sweep": mixed_sweep, "mixed_per_size": per_size,
+            "autotune": {f"{k[0]}x{k[1]}": v for k, v in pipe.tuned.items()}}
 
 
 def main() -> int:
@@ -225,11 +367,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=64,
                     help="timed sweep instances (the learned-shift sweep is specified at 64)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed warm-up instances")
-    ap.add_argument("--ncv", type=int, default=NCV, help="Krylov subspace of the plain form")
     ap.add_argument("--learned-ncv", type=int, default=LEARNED_NCV)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--contexts", type=int, default=PIPELINE_CONTEXTS,
-                    help="contexts of the learned sweep's pipeline (1: no worker threads)")
+                    help="contexts of the pipelined sweeps (1: no worker threads)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -243,11 +384,16 @@ def main() -> int:
     import tailor_amd as ta
     from tailor_amd import case as cs
 
-    single = plain_step(ta, cs, 2, args.ncv)
-    sw = sweeps(ta, args.steps, max(1, args.warmup), args.ncv, args.learned_ncv, args.contexts)
-    L, P = sw["learned"], sw["plain"]
+    single = plain_step(ta, cs)
+    small = gpu_small_case(ta, cs, REFERENCE_NCV)
+    log("100x50 plain instance done")
+    best_ncv = single["plain"]["best_ncv"]
+    sw = sweeps(ta, args.steps, max(1, args.warmup), best_ncv, args.learned_ncv, args.contexts)
+    L, F, PB, PR = sw["learned"], sw["fixed"], sw["plain_best"], sw["plain_ref"]
+    strongest = max(PB.eigenpairs_per_s, PR.eigenpairs_per_s)
     g = single["gemm"]
     gemm_tf = g["gemm_flops"] / (g["gemm_ms"] * 1e-3) / 1e12
+    asm_ms = single["assembly"]["full_ms"]
     out = {
         "metric": "eigenpairs/s at resid<=1e-10 (learned-shift sweep; plain FP64 alongside)",
         "value": round(L.eigenpairs_per_s, 4),
@@ -260,27 +406,34 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c128" if sw["choice"].get("chosen", "fp64") == "fp64" else "c64 factors + c128",
-        "data": "synthetic (erf boundary layer on tanh-clustered wall-bumped grid, seed 0)",
+        "data": "synthetic (erf boundary layer on a tanh-clustered, wall-bumped grid; base flow "
+                "per instance from seeds 1000 + k)",
         "config": {"workload": "BiGlobal wavenumber sweep, 200x100 FD-q grid (q_y=10, q_z=6), "
                                "100,000 unknowns, Re=1500, Ma=0.7, alpha ~ U[0.5,3] seed 1, nev=8, "
                                "tol=1e-10; learned shift centres, warm-started Krylov-Schur, "
                                "fixed-shift fallback, " + (f"{args.contexts}-context pipeline"
                                                           if args.contexts > 1 else "one context"),
-                   "nev": NEV, "ncv_learned": args.learned_ncv, "ncv_plain": args.ncv,
-                   "pipeline_contexts": args.contexts,
-                   "learned_check_from": LEARNED_CHECK_FROM,
-                   "instances": args.steps,
+                   "nev": NEV, "ncv_learned": args.learned_ncv, "ncv_plain_reference": REFERENCE_NCV,
+                   "ncv_plain_best": best_ncv, "pipeline_contexts": args.contexts,
+                   "learned_check_from": LEARNED_CHECK_FROM, "instances": args.steps,
                    "precision_choice": sw["choice"], "gemm_autotune": sw["autotune"]},
-        "learned_sweep": without_assembly(L.totals(), single["assembly"]["full_ms"]),
-        "plain_sweep": without_assembly(P.totals(), single["assembly"]["full_ms"]),
-        "speedup_vs_plain_sweep": round(L.eigenpairs_per_s / max(P.eigenpairs_per_s, 1e-300), 3),
-        "common_pair_max_rel_diff": sw["common_pair_rel_diff"],
+        "learned_sweep": without_assembly(L.totals(), asm_ms),
+        "network_off_sweep": without_assembly(F.totals(), asm_ms),
+        "plain_sweep_best": without_assembly(PB.totals(), asm_ms),
+        "plain_sweep_reference": without_assembly(PR.totals(), asm_ms),
+        "speedup_vs_strongest_plain": round(L.eigenpairs_per_s / max(strongest, 1e-300), 3),
+        "speedup_vs_reference_plain": round(L.eigenpairs_per_s / max(PR.eigenpairs_per_s, 1e-300), 3),
+        "network_gain": round(L.eigenpairs_per_s / max(F.eigenpairs_per_s, 1e-300), 3),
+        "learned_pairs_found_by_plain": sw["overlap"],
         "roofline": {"bound": "mfma", "kernel": "k_gemm (frontal updates, MFMA f64 16x16x4)",
                      "achieved": round(gemm_tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(gemm_tf / FP64_PEAK_TFLOPS, 4),
                      "traffic": measured_traffic("k_gemm<double>"),
                      "traffic_note": "mean HBM bytes per k_gemm dispatch (counter runs in profiles/)",
+                     "timing": "HIP events around every k_gemm launch of one serialized "
+                               "factorization on the library stream (tailor_profile_factor)",
                      "gemm_ms_per_factorization": round(g["gemm_ms"], 2),
+                     "gemm_us_per_launch": round(g["gemm_ms"] * 1e3 / max(g["gemm_launches"], 1), 2),
                      "gemm_flops_per_factorization": g["gemm_flops"],
                      "gemm_launches": g["gemm_launches"],
                      "tile_config_per_level": single["tune"]["config"],
@@ -289,11 +442,20 @@ def main() -> int:
         "plain_instance": single["plain"],
         "assembly": single["assembly"],
         "solve_sweep": single["solve_sweep"],
+        "spmv": single["spmv"],
         "mixed_precision": single["mixed"],
-        "mixed_size_sweep": sw["mixed_sweep"].totals(),
+        "mixed_size_sweep": {**sw["mixed_sweep"].totals(), "per_size": sw["mixed_per_size"]},
+        "gpu_100x50_plain": small,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline()
+        cb = cpu_baseline(REFERENCE_NCV)
+        log("CPU baseline done")
+        gl = np.array([complex(*x) for x in small["lambdas"]])
+        cl = np.array([complex(*x) for x in cb.pop("lambdas")])
+        cb["max_rel_diff_vs_gpu"] = float(max(np.min(np.abs(gl - l)) / abs(l) for l in cl)) \
+            if len(gl) and len(cl) else None
+        out["cpu_baseline"] = cb
+    out["gpu_100x50_plain"].pop("lambdas")
     print(json.dumps(out))
     return 0
 

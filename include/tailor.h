@@ -160,7 +160,10 @@ int tailor_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, tailor_fact
 /* Factor like tailor_factor, timing every frontal-update GEMM launch with
  * HIP events on the context stream: total factorization time, summed GEMM
  * time, the GEMMs' algorithmic flops (8 M N K per front and launch) and the
- * number of GEMM launches.  No reference counterpart (measurement only). */
+ * number of GEMM launches.  The factorization is bracketed by two empty
+ * kernels, k_profile_window_begin / k_profile_window_end, so that a kernel
+ * trace can select exactly its launches.  No reference counterpart
+ * (measurement only). */
 int tailor_profile_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, double *factor_ms,
                           double *gemm_ms, double *gemm_flops, int32_t *gemm_launches);
 
@@ -235,6 +238,16 @@ int tailor_linear_solve_noalloc(tailor_ctx *ctx, const double *b, double *x,
  * upper triangle of Z. */
 int tailor_time_linear_solve(tailor_ctx *ctx, int32_t reps, double *ms_per_solve,
                              double *bytes_per_solve);
+
+/* Time `reps` applications of the shift-invert residual operator
+ * y = (A_bc - sigma B_bc) x (the SpMV of every refinement step and residual
+ * check; sigma of the last factorization, else 0) with HIP events on the
+ * context stream: mean ms per application and its algorithmic bytes
+ * (A_bc values and block column indices, B_bc values, x read, y written).
+ * No reference counterpart (MatMult inside EPSSolve, EigenSolver.cpp:72,
+ * is not timed by the reference). */
+int tailor_time_operator(tailor_ctx *ctx, int32_t reps, double *ms_per_apply,
+                         double *bytes_per_apply);
 
 /* Settings of EigenSolver in config.yaml (Parser.cpp:62-83). */
 typedef struct {

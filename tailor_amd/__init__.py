@@ -147,6 +147,7 @@ def library() -> C.CDLL:
     lib.tailor_profile_factor.argtypes = [C.c_void_p, C.c_double, C.c_double, P(C.c_double),
                                           P(C.c_double), P(C.c_double), P(C.c_int32)]
     lib.tailor_time_linear_solve.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double)]
+    lib.tailor_time_operator.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double)]
     lib.tailor_linear_solve_pair.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double),
                                              P(C.c_double), P(C.c_double)]
     # device pointers (caller-owned device memory), no device allocation
@@ -387,6 +388,16 @@ def _time_linear_solve(self, reps: int = 20):
 
 
 Context.time_linear_solve = _time_linear_solve
+
+
+def _time_operator(self, reps: int = 20):
+    """Mean ms and algorithmic bytes of one y = (A_bc - sigma B_bc) x."""
+    ms, nbytes = C.c_double(), C.c_double()
+    self._check(self.lib.tailor_time_operator(self.ptr, reps, C.byref(ms), C.byref(nbytes)))
+    return ms.value, nbytes.value
+
+
+Context.time_operator = _time_operator
 Context.factor = _factor
 Context.set_precision = _set_precision
 Context.profile_factor = _profile_factor

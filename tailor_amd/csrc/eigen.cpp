@@ -300,6 +300,36 @@ int solver_time_solves(tailor_ctx *ctx, int reps, double *ms) {
   return TAILOR_OK;
 }
 
+// y = (A - sigma B) x on the assembled operator, `reps` launches timed with
+// HIP events on the context stream (x = 1, elimination order)
+int solver_time_kmul(tailor_ctx *ctx, int reps, double *ms) {
+  if (int st = solver_prepare(ctx)) return st;
+  SolverState &S = *ctx->solver;
+  const long long n = S.n;
+  const auto &d = ctx->dims;
+  HIP_OK(grow(&S.work, S.work_cap, static_cast<size_t>(12 * n)));
+  double *xr = S.work, *xi = xr + n, *yr = S.work + 2 * n, *yi = yr + n;
+  HIP_OK(lu_fill(ctx->stream, xr, 2 * n, 1.0));
+  auto launch = [&]() {
+    return vec_kmul(ctx->stream, d.nodes, d.ssx * d.sse, ctx->d_vals, ctx->d_colidx, ctx->d_mvals,
+                    S.d_pos, S.sigma.real(), S.sigma.imag(), xr, xi, yr, yi);
+  };
+  HIP_OK(launch());  // warm-up
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  HIP_OK(hipEventRecord(e0, ctx->stream));
+  for (int r = 0; r < reps; ++r) HIP_OK(launch());
+  HIP_OK(hipEventRecord(e1, ctx->stream));
+  HIP_OK(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIP_OK(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = static_cast<double>(t) / reps;
+  return TAILOR_OK;
+}
+
 // ----------------------------------------------------------- Krylov-Schur
 
 int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solve_diag &diag) {

@@ -43,6 +43,7 @@ int solver_linear_solve(tailor_ctx *ctx, const double *b_nat, double *x_nat, int
                         double refine_tol, double *relres, int *steps);
 int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solve_diag &diag);
 int solver_time_solves(tailor_ctx *ctx, int reps, double *ms);
+int solver_time_kmul(tailor_ctx *ctx, int reps, double *ms);
 // x1 = K^-1 b1, x2 = K^-1 b2 (natural order, host vectors) in one pass over
 // the factors, without refinement
 int solver_linear_solve_pair(tailor_ctx *ctx, const double *b1_nat, const double *b2_nat,

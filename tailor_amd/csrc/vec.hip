@@ -69,7 +69,9 @@ __global__ __launch_bounds__(VT) void k_kmul(long long nodes, int per, const dou
   for (int idx = lane; idx < total; idx += 64) {
     const int k = idx / 25, e = idx - 25 * (idx / 25);
     const int r = e / 5, c = e - 5 * (e / 5);
-    const double a_r = avals[2 * (base + idx)], a_i = avals[2 * (base + idx) + 1];
+    // one 16-byte load per complex entry (interleaved re, im)
+    const double2 a = reinterpret_cast<const double2 *>(avals)[base + idx];
+    const double a_r = a.x, a_i = a.y;
     const long long xc = 5LL * pos[colidx[u * per + k]] + c;
     const double x_r = xr[xc], x_i = xi[xc];
     const double pr = a_r * x_r - a_i * x_i, pi = a_r * x_i + a_i * x_r;

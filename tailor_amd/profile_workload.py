@@ -12,6 +12,15 @@ K chunk 4, XCD-grouped order, the autotuner's choice on most levels at
 WRITE_SIZE counters of the other kernels).  Prints the algorithmic bytes or
 flops of each kernel family so that counter readings can be compared with
 them.
+
+    python -m tailor_amd.profile_workload summarize <kernel_trace.csv> --command "..." [--out f.md]
+
+Summarises a `rocprofv3 --kernel-trace --output-format csv` trace as a
+markdown table of per-kernel launch counts and durations, and separately
+the launches inside the window that tailor_profile_factor brackets with
+its k_profile_window_begin / k_profile_window_end kernels (the serialized,
+event-timed factorization of bench.py's roofline), so the trace's mean
+frontal-GEMM duration can be set beside the event-timed one.
 """
 from __future__ import annotations
 
@@ -21,7 +30,76 @@ import sys
 import numpy as np
 
 
+def _short_name(name: str) -> str:
+    """`void tailor::(anonymous namespace)::k_gemm<double, 2, ...>(int const*, ...)`
+    -> `k_gemm<double, 2, ...>`."""
+    import re
+    m = re.search(r"(k_\w+(?:<[^()]*>)?)\s*\(", name)
+    if m:
+        return m.group(1)
+    return name.split("(")[0].replace("void ", "").strip()
+
+
+def _table(rows, total_ns, limit=40):
+    out = ["| kernel | calls | total ms | avg us | share |", "|---|---|---|---|---|"]
+    for name, (calls, ns) in sorted(rows.items(), key=lambda kv: -kv[1][1])[:limit]:
+        out.append(f"| {name} | {calls} | {ns / 1e6:.2f} | {ns / calls / 1e3:.2f} | "
+                   f"{100.0 * ns / max(total_ns, 1):.1f}% |")
+    return out
+
+
+def summarize(csv_path: str, command: str) -> str:
+    """Markdown summary of a rocprofv3 kernel trace (see the module docstring)."""
+    import csv
+    launches = []
+    with open(csv_path, newline="") as f:
+        for r in csv.DictReader(f):
+            launches.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]),
+                             _short_name(r["Kernel_Name"])))
+    launches.sort()
+    rows, total = {}, 0
+    for t0, t1, name in launches:
+        c, ns = rows.get(name, (0, 0))
+        rows[name] = (c + 1, ns + (t1 - t0))
+        total += t1 - t0
+    lines = [f"Command (GPU box, repo root):", "", f"    {command}", "",
+             f"{len(launches)} kernel launches, {total / 1e6:.1f} ms of summed kernel time.", ""]
+    lines += _table(rows, total)
+    begin = [t1 for t0, t1, n in launches if n == "k_profile_window_begin"]
+    end = [t0 for t0, t1, n in launches if n == "k_profile_window_end"]
+    if begin and end:
+        lo, hi = begin[-1], end[-1]
+        win, wtot = {}, 0
+        for t0, t1, name in launches:
+            if lo <= t0 and t1 <= hi:
+                c, ns = win.get(name, (0, 0))
+                win[name] = (c + 1, ns + (t1 - t0))
+                wtot += t1 - t0
+        g_calls = sum(c for n, (c, _) in win.items() if n.startswith("k_gemm<"))
+        g_ns = sum(ns for n, (_, ns) in win.items() if n.startswith("k_gemm<"))
+        lines += ["", "## The event-timed factorization (tailor_profile_factor window)", "",
+                  f"Window {(hi - lo) / 1e6:.2f} ms, {wtot / 1e6:.2f} ms of kernel time.  "
+                  f"Frontal GEMM (k_gemm, all tile shapes): {g_calls} launches, "
+                  f"{g_ns / 1e6:.2f} ms, mean {g_ns / max(g_calls, 1) / 1e3:.2f} us per launch.", ""]
+        lines += _table(win, wtot, limit=25)
+    return "\n".join(lines) + "\n"
+
+
 def main(argv=None) -> int:
+    argv = sys.argv[1:] if argv is None else list(argv)
+    if argv and argv[0] == "summarize":
+        sp = argparse.ArgumentParser(prog="profile_workload summarize")
+        sp.add_argument("csv")
+        sp.add_argument("--command", required=True, help="the profiled command, for the record")
+        sp.add_argument("--out")
+        b = sp.parse_args(argv[1:])
+        text = summarize(b.csv, b.command)
+        if b.out:
+            with open(b.out, "w") as f:
+                f.write(text)
+        else:
+            print(text)
+        return 0
     ap = argparse.ArgumentParser()
     ap.add_argument("--ny", type=int, default=200)
     ap.add_argument("--nz", type=int, default=100)

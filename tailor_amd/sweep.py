@@ -12,12 +12,16 @@ Forms of a sweep:
               predicted 8 leading omega), instances ordered by wavenumber so
               each Krylov-Schur run starts from the previous one's Ritz
               vectors, and the plain fixed-shift solve as the fallback of any
-              instance whose pairs miss the residual tolerance.
+              instance whose pairs miss the residual tolerance;
+* ``fixed``   the learned form with the network switched off: the fixed
+              search centre omega_s for every instance, with the same warm
+              starts, early pass ends and fallback (what the network's
+              shifts gain is the learned form's rate over this one's).
 
 One context per grid size is kept for the whole sweep, so the symbolic
 analysis and device buffers are built once per size; each instance uploads
-its grid and base flow and reassembles.  With `contexts=C > 1` the learned
-form runs C contexts per size as a pipeline: while one context runs the
+its grid and base flow and reassembles.  With `contexts=C > 1` every form
+runs C contexts per size as a pipeline: while one context runs the
 current instance's Krylov-Schur, worker threads assemble and factor the
 next C - 1 instances on the others (each context warm-starts from the
 instance it solved before, C steps back in wavenumber).  The HIP streams
@@ -122,6 +126,11 @@ def training_instances(count: int, seed: int = 2, ny: int = 100, nz: int = 50) -
     return out
 
 
+# forms that warm-start from the previous instance, test convergence early
+# and fall back to the plain solve
+_WARM_FORMS = ("learned", "fixed")
+
+
 class Sweeper:
     def __init__(self, device: int = 0, nev: int = 8, tol: float = 1e-10, ncv: int = 64,
                  omega_s: complex = 0.02 + 0j, maxit: int = 1000, contexts: int = 1,
@@ -178,12 +187,19 @@ class Sweeper:
         """Run the sweep over prepared (instance, case) pairs.
 
         `shifts` (learned form) gives one omega shift centre per instance;
-        `order` the processing order (defaults to the given order)."""
+        the other forms use omega_s; `order` the processing order (defaults
+        to the given order)."""
+        if form not in ("plain", "mixed", "learned", "fixed"):
+            raise ValueError(f"unknown sweep form {form!r}")
+        if form == "learned" and shifts is None:
+            raise ValueError("the learned form needs shifts")
         if precision is None:
             precision = PRECISION_MIXED if form == "mixed" else PRECISION_FP64
+        if form != "learned":
+            shifts = [self.omega_s] * len(cases)
         idx = list(range(len(cases))) if order is None else list(order)
-        if self.contexts > 1 and form == "learned" and len(idx) > 1:
-            return self._run_pipelined(cases, shifts, precision, idx, on_record)
+        if self.contexts > 1 and len(idx) > 1:
+            return self._run_pipelined(cases, form, shifts, precision, idx, on_record)
         result = SweepResult(form)
         last_size = None
         t_all = time.perf_counter()
@@ -202,14 +218,14 @@ class Sweeper:
                     complex(self.omega_s.imag, -self.omega_s.real))
                 t_tune += time.perf_counter() - tt
                 t0 += time.perf_counter() - tt
-            warm = form == "learned" and last_size == (inst.ny, inst.nz)
-            omega = self.omega_s if shifts is None else complex(shifts[k])
+            warm = form in _WARM_FORMS and last_size == (inst.ny, inst.nz)
+            omega = complex(shifts[k])
             d, om, res, good = self._solve(ctx, omega, precision, warm,
-                                           self.check_from if form == "learned" else 0)
+                                           self.check_from if form in _WARM_FORMS else 0)
             facts = 1
             fallback = False
             restarts, solves, refine = d["iterations"], d["solves"], d["refinement_steps"]
-            if form == "learned" and good < self.nev:
+            if form in _WARM_FORMS and good < self.nev:
                 fallback = True
                 d, om, res, good = self._solve(ctx, self.omega_s, precision, False)
                 facts += 1
@@ -228,15 +244,16 @@ class Sweeper:
         return result
 
 
-    def _run_pipelined(self, cases, shifts, precision, idx, on_record) -> SweepResult:
-        """Learned form over `self.contexts` contexts per grid size (see the
+    def _run_pipelined(self, cases, form, shifts, precision, idx, on_record) -> SweepResult:
+        """A form over `self.contexts` contexts per grid size (see the
         module docstring): instance i runs its Krylov-Schur on context
         i mod C while worker threads assemble and factor the next C - 1
         instances on the others.  Instance k's record covers its Krylov stage
         and any fallback; the sweep's time is the wall time of the whole
         pipeline."""
         C = self.contexts
-        result = SweepResult("learned")
+        result = SweepResult(form)
+        warm_form = form in _WARM_FORMS
         # plans and tuning of every context of every size first (untimed)
         for k in idx:
             inst, case = cases[k]
@@ -281,11 +298,12 @@ class Sweeper:
             t0 = time.perf_counter()
             omega = complex(shifts[k])
             key = (inst.ny, inst.nz, slot)
-            d, om, res, good = self._solve(ctx, omega, precision, key in solved, self.check_from)
+            d, om, res, good = self._solve(ctx, omega, precision, warm_form and key in solved,
+                                           self.check_from if warm_form else 0)
             solved.add(key)
             facts, fallback = 1, False
             restarts, solves, refine = d["iterations"], d["solves"], d["refinement_steps"]
-            if good < self.nev:
+            if warm_form and good < self.nev:
                 fallback = True
                 d, om, res, good = self._solve(ctx, self.omega_s, precision, False)
                 facts += 1
