@@ -157,6 +157,29 @@ typedef struct {
 
 int tailor_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, tailor_factor_diag *diag);
 
+/* The same factorization replayed from a HIP graph: at the first call (and
+ * whenever the front precision, the GEMM tile choice or the panel kernel has
+ * changed since) the norms of A_bc and B_bc, the pivot thresholds derived
+ * from them and the shift, and every launch of the factorization are
+ * captured from the context's streams; later calls write the shift to the
+ * device and relaunch the graph (one host launch instead of some 3,000 at
+ * 200x100).  Results are those of tailor_factor.  No reference counterpart
+ * (performance only). */
+int tailor_factor_graph(tailor_ctx *ctx, double sigma_re, double sigma_im,
+                        tailor_factor_diag *diag);
+
+/* Factor `count` distinct contexts of one device (each assembled; any
+ * sizes) at the shifts sigmas[2k], sigmas[2k+1]: every context's
+ * factorization graph (as tailor_factor_graph) is launched on that
+ * context's stream from the calling thread, after the work already queued
+ * on it, so the factorizations run concurrently; returns when all are done.
+ * *batch_ms (may be NULL): the batch's time.  Errors are reported on the
+ * context that failed (ctxs[0] for the batch itself).  No reference
+ * counterpart (wavenumber sweeps; the reference runs one case per
+ * process). */
+int tailor_factor_batch(tailor_ctx *const *ctxs, int32_t count, const double *sigmas,
+                        double *batch_ms);
+
 /* Factor like tailor_factor, timing every frontal-update GEMM launch with
  * HIP events on the context stream: total factorization time, summed GEMM
  * time, the GEMMs' algorithmic flops (8 M N K per front and launch) and the

@@ -137,6 +137,8 @@ def library() -> C.CDLL:
     lib.tailor_export_fields.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
     lib.tailor_time_assembly.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double)]
     lib.tailor_factor.argtypes = [C.c_void_p, C.c_double, C.c_double, P(FactorDiag)]
+    lib.tailor_factor_graph.argtypes = [C.c_void_p, C.c_double, C.c_double, P(FactorDiag)]
+    lib.tailor_factor_batch.argtypes = [P(C.c_void_p), C.c_int32, P(C.c_double), P(C.c_double)]
     lib.tailor_linear_solve.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), C.c_int32,
                                         C.c_double, P(C.c_double), P(C.c_int32)]
     lib.tailor_solve.argtypes = [C.c_void_p, P(SolveOpts), P(SolveDiag)]
@@ -309,6 +311,32 @@ def _factor(self, sigma: complex) -> dict:
     return _sdict(d)
 
 
+def _factor_graph(self, sigma: complex) -> dict:
+    """tailor_factor replayed from the context's captured HIP graph."""
+    d = FactorDiag()
+    sigma = complex(sigma)
+    self._check(self.lib.tailor_factor_graph(self.ptr, sigma.real, sigma.imag, C.byref(d)))
+    return _sdict(d)
+
+
+def factor_batch(contexts, sigmas) -> float:
+    """Factor several contexts (one device) at their shifts: each context's
+    factorization graph on its own stream, launched from this thread, so
+    they run concurrently; returns the batch's time in ms."""
+    contexts = list(contexts)
+    if len(contexts) != len(sigmas) or not contexts:
+        raise ValueError("one shift per context")
+    ptrs = (C.c_void_p * len(contexts))(*[c.ptr.value for c in contexts])
+    sg = np.array([[complex(s).real, complex(s).imag] for s in sigmas], dtype=np.float64).ravel()
+    ms = C.c_double()
+    lib = contexts[0].lib
+    st = lib.tailor_factor_batch(ptrs, len(contexts), _dptr(sg), C.byref(ms))
+    if st:
+        msgs = [lib.tailor_last_error(c.ptr).decode() for c in contexts]
+        raise TailorError(st, "; ".join(m for m in msgs if m) or "tailor_factor_batch failed")
+    return ms.value
+
+
 def _linear_solve(self, b: np.ndarray, refine_max: int = 0, refine_tol: float = 1e-14):
     """x = (A_bc - sigma B_bc)^-1 b for a natural-order complex vector."""
     b = np.ascontiguousarray(b, dtype=np.complex128)
@@ -399,6 +427,7 @@ def _time_operator(self, reps: int = 20):
 
 Context.time_operator = _time_operator
 Context.factor = _factor
+Context.factor_graph = _factor_graph
 Context.set_precision = _set_precision
 Context.profile_factor = _profile_factor
 Context.autotune = _autotune

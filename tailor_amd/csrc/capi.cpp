@@ -569,27 +569,66 @@ int tailor_time_assembly(tailor_ctx *ctx, int reps, double *write_kernel_ms,
   return check_assembly_flags(ctx, g);
 }
 
+}  // extern "C"
+
+namespace {
+
+void fill_factor_diag(const tailor_ctx *ctx, tailor_factor_diag *diag) {
+  const tailor::SolverState &S = *ctx->solver;
+  diag->fronts = S.sym.nsn;
+  diag->levels = static_cast<int32_t>(S.sym.levels.size());
+  int maxm = 0;
+  for (int m : S.sym.m) maxm = std::max(maxm, m);
+  diag->max_front = maxm;
+  diag->perturbed_pivots = S.perturbed;
+  diag->arena_bytes = S.plan.arena_doubles * (S.fp32 ? 4 : 8);
+  diag->factor_entries = S.sym.factor_entries;
+  diag->flops = static_cast<double>(S.sym.flops);
+  diag->symbolic_ms = S.symbolic_ms;
+  diag->factor_ms = S.factor_ms;
+}
+
+}  // namespace
+
+extern "C" {
+
 int tailor_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, tailor_factor_diag *diag) {
   if (!ctx) return TAILOR_ERR_USER_INPUT;
   if (!std::isfinite(sigma_re) || !std::isfinite(sigma_im))
     return fail(ctx, TAILOR_ERR_USER_INPUT, "shift must be finite");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   if (int st = tailor::solver_factor(ctx, tailor::cplx(sigma_re, sigma_im))) return st;
-  if (diag) {
-    const tailor::SolverState &S = *ctx->solver;
-    diag->fronts = S.sym.nsn;
-    diag->levels = static_cast<int32_t>(S.sym.levels.size());
-    int maxm = 0;
-    for (int m : S.sym.m) maxm = std::max(maxm, m);
-    diag->max_front = maxm;
-    diag->perturbed_pivots = S.perturbed;
-    diag->arena_bytes = S.plan.arena_doubles * (S.fp32 ? 4 : 8);
-    diag->factor_entries = S.sym.factor_entries;
-    diag->flops = static_cast<double>(S.sym.flops);
-    diag->symbolic_ms = S.symbolic_ms;
-    diag->factor_ms = S.factor_ms;
-  }
+  if (diag) fill_factor_diag(ctx, diag);
   return TAILOR_OK;
+}
+
+int tailor_factor_graph(tailor_ctx *ctx, double sigma_re, double sigma_im,
+                        tailor_factor_diag *diag) {
+  if (!ctx) return TAILOR_ERR_USER_INPUT;
+  if (!std::isfinite(sigma_re) || !std::isfinite(sigma_im))
+    return fail(ctx, TAILOR_ERR_USER_INPUT, "shift must be finite");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (int st = tailor::solver_factor_graph(ctx, tailor::cplx(sigma_re, sigma_im))) return st;
+  if (diag) fill_factor_diag(ctx, diag);
+  return TAILOR_OK;
+}
+
+int tailor_factor_batch(tailor_ctx *const *ctxs, int32_t count, const double *sigmas,
+                        double *batch_ms) {
+  if (!ctxs || count <= 0 || !sigmas || !ctxs[0]) return TAILOR_ERR_USER_INPUT;
+  std::vector<tailor::cplx> sig(count);
+  for (int k = 0; k < count; ++k) {
+    if (!ctxs[k]) return fail(ctxs[0], TAILOR_ERR_USER_INPUT, "null context in the batch");
+    if (ctxs[k]->device != ctxs[0]->device)
+      return fail(ctxs[0], TAILOR_ERR_ARG_OUTOFRANGE, "the batch's contexts must share a device");
+    for (int q = 0; q < k; ++q)
+      if (ctxs[q] == ctxs[k]) return fail(ctxs[0], TAILOR_ERR_USER_INPUT, "a context appears twice");
+    if (!std::isfinite(sigmas[2 * k]) || !std::isfinite(sigmas[2 * k + 1]))
+      return fail(ctxs[0], TAILOR_ERR_USER_INPUT, "shift must be finite");
+    sig[k] = tailor::cplx(sigmas[2 * k], sigmas[2 * k + 1]);
+  }
+  HIP_TRY(ctxs[0], hipSetDevice(ctxs[0]->device));
+  return tailor::solver_factor_batch(ctxs, count, sig.data(), batch_ms);
 }
 
 // Empty kernels that bracket the timed factorization of

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -56,8 +57,23 @@ hipError_t grow(T **p, size_t &cap, size_t need) {
 
 }  // namespace
 
+namespace {
+
+void release_graphs(SolverState *S) {
+  if (S->graph_exec) (void)hipGraphExecDestroy(S->graph_exec);
+  if (S->graph) (void)hipGraphDestroy(S->graph);
+  S->graph_exec = nullptr;
+  S->graph = nullptr;
+  S->graph_id = 0;
+}
+
+std::atomic<unsigned long long> g_next_graph_id{1};
+
+}  // namespace
+
 void solver_release(SolverState *S) {
   if (!S) return;
+  release_graphs(S);
   lu_plan_destroy(S->plan);
   double *bufs[] = {S->V, S->work, S->ritz, S->partials, S->small, S->hsteps};
   for (double *b : bufs)
@@ -107,22 +123,16 @@ int solver_factor(tailor_ctx *ctx, cplx sigma, FactorTiming *timing) {
   in.avals = ctx->d_vals;
   in.colidx = ctx->d_colidx;
   in.bvals = ctx->d_mvals;
-  // pivot threshold and static perturbation, scaled by the RMS row norm of K
-  double ss[2];
-  HIP_OK(launch_sumsq(ctx, ctx->d_vals, d.nnzb * 50, 1, 1, ss));
-  HIP_OK(launch_sumsq(ctx, ctx->d_mvals, d.nodes * 50, 1, 1, ss + 1));
-  const double rows = 5.0 * static_cast<double>(d.nodes);
-  const double scale = std::max(1e-300, std::sqrt(ss[0] / rows) + std::abs(sigma) * std::sqrt(ss[1] / rows));
   hipEvent_t e0, e1;
   HIP_OK(hipEventCreate(&e0));
   HIP_OK(hipEventCreate(&e1));
   HIP_OK(hipEventRecord(e0, ctx->stream));
-  // pivots below tiny are replaced by +-perturb (sqrt(eps) of the front
-  // precision times the scale); refinement recovers the accuracy
+  // sigma, then the pivot threshold and static perturbation scaled by the
+  // RMS row norm of K, all on the device
   const bool fp32 = ctx->precision == TAILOR_PRECISION_MIXED;
-  const double tiny = (fp32 ? 1e-7 : 1e-14) * scale;
-  const double perturb = (fp32 ? 3.4526698300124393e-04 : 1.4901161193847656e-08) * scale;
-  HIP_OK(lu_factor(S.plan, in, sigma.real(), sigma.imag(), tiny, perturb, fp32, ctx->stream, timing));
+  HIP_OK(lu_set_sigma(S.plan, sigma.real(), sigma.imag(), ctx->stream));
+  HIP_OK(lu_factor_params(S.plan, in, fp32, ctx->d_partials, ctx->stream));
+  HIP_OK(lu_factor(S.plan, in, fp32, ctx->stream, timing));
   HIP_OK(hipEventRecord(e1, ctx->stream));
   HIP_OK(hipEventSynchronize(e1));
   float ms = 0.f;
@@ -136,6 +146,134 @@ int solver_factor(tailor_ctx *ctx, cplx sigma, FactorTiming *timing) {
   S.sigma = sigma;
   S.fp32 = fp32;
   S.factored = true;
+  return TAILOR_OK;
+}
+
+namespace {
+
+LuInputs factor_inputs(const tailor_ctx *ctx) {
+  const auto &d = ctx->dims;
+  LuInputs in;
+  in.nnzb = d.nnzb;
+  in.per = d.ssx * d.sse;
+  in.avals = ctx->d_vals;
+  in.colidx = ctx->d_colidx;
+  in.bvals = ctx->d_mvals;
+  return in;
+}
+
+// Capture the factorization of ctx as a HIP graph unless the one held was
+// captured for the same front precision, tile shapes and orders and panel
+// kernel.  The captured work: the norms of A_bc, B_bc and the pivot
+// thresholds (from the shift in the parameter block), then every launch of
+// lu_factor on the context stream and the plan's look-ahead stream.
+int graph_ready(tailor_ctx *ctx) {
+  SolverState &S = *ctx->solver;
+  const bool fp32 = ctx->precision == TAILOR_PRECISION_MIXED;
+  if (S.graph_exec && S.graph_fp32 == fp32 && S.graph_shape == S.plan.gemm_shape &&
+      S.graph_order == S.plan.gemm_order && S.graph_streaming == S.plan.panel_streaming)
+    return TAILOR_OK;
+  release_graphs(&S);
+  const LuInputs in = factor_inputs(ctx);
+  HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = lu_factor_params(S.plan, in, fp32, ctx->d_partials, ctx->stream);
+  if (e == hipSuccess) e = lu_factor(S.plan, in, fp32, ctx->stream, nullptr);
+  hipGraph_t g = nullptr;
+  const hipError_t e_end = hipStreamEndCapture(ctx->stream, &g);
+  if (e == hipSuccess) e = e_end;
+  hipGraphExec_t x = nullptr;
+  if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return hip_fail(ctx, e, "capture of the factorization graph");
+  }
+  S.graph = g;
+  S.graph_exec = x;
+  S.graph_id = g_next_graph_id.fetch_add(1);
+  S.graph_fp32 = fp32;
+  S.graph_shape = S.plan.gemm_shape;
+  S.graph_order = S.plan.gemm_order;
+  S.graph_streaming = S.plan.panel_streaming;
+  return TAILOR_OK;
+}
+
+int factor_checks(tailor_ctx *ctx) {
+  if (!ctx->assembled || ctx->vals_kind != TAILOR_OP_A || ctx->mvals_kind != TAILOR_OP_B)
+    return fail(ctx, TAILOR_ERR_ARG_WRONGSTATE,
+                "Boundary-constrained matrices are required before EPSSolve");
+  return solver_prepare(ctx);
+}
+
+// After a factorization of ctx at sigma has completed on the device.
+int factor_done(tailor_ctx *ctx, cplx sigma, double ms) {
+  SolverState &S = *ctx->solver;
+  int counters[8];
+  HIP_OK(hipMemcpy(counters, S.plan.d_counters, sizeof counters, hipMemcpyDeviceToHost));
+  S.perturbed = counters[0];
+  S.factor_ms = ms;
+  S.sigma = sigma;
+  S.fp32 = ctx->precision == TAILOR_PRECISION_MIXED;
+  S.factored = true;
+  return TAILOR_OK;
+}
+
+}  // namespace
+
+int solver_factor_graph(tailor_ctx *ctx, cplx sigma) {
+  if (int st = factor_checks(ctx)) return st;
+  if (int st = graph_ready(ctx)) return st;
+  SolverState &S = *ctx->solver;
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  HIP_OK(hipEventRecord(e0, ctx->stream));
+  HIP_OK(lu_set_sigma(S.plan, sigma.real(), sigma.imag(), ctx->stream));
+  HIP_OK(hipGraphLaunch(S.graph_exec, ctx->stream));
+  HIP_OK(hipEventRecord(e1, ctx->stream));
+  HIP_OK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return factor_done(ctx, sigma, ms);
+}
+
+int solver_factor_batch(tailor_ctx *const *ctxs, int count, const cplx *sigmas, double *ms_out) {
+  for (int k = 0; k < count; ++k) {
+    if (int st = factor_checks(ctxs[k])) return st;
+    if (int st = graph_ready(ctxs[k])) return st;
+  }
+  // every context's graph on its own stream, all launched from this thread:
+  // the streams run concurrently (one launch of a graph holding the
+  // contexts' graphs as parallel child graphs measured no faster than the
+  // factorizations one after another); the batch's time runs from the first
+  // context's start event to the last completion
+  tailor_ctx *ctx = ctxs[0];  // HIP_OK reports on the batch's first context
+  std::vector<hipEvent_t> done(count, nullptr);
+  hipEvent_t e0;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventRecord(e0, ctx->stream));
+  for (int k = 0; k < count; ++k) {
+    tailor_ctx *c = ctxs[k];
+    SolverState &S = *c->solver;
+    if (k > 0) HIP_OK(hipStreamWaitEvent(c->stream, e0, 0));
+    HIP_OK(lu_set_sigma(S.plan, sigmas[k].real(), sigmas[k].imag(), c->stream));
+    HIP_OK(hipGraphLaunch(S.graph_exec, c->stream));
+    HIP_OK(hipEventCreate(&done[k]));
+    HIP_OK(hipEventRecord(done[k], c->stream));
+  }
+  float ms = 0.f;
+  for (int k = 0; k < count; ++k) {
+    HIP_OK(hipEventSynchronize(done[k]));
+    float t = 0.f;
+    HIP_OK(hipEventElapsedTime(&t, e0, done[k]));
+    ms = std::max(ms, t);
+  }
+  (void)hipEventDestroy(e0);
+  for (hipEvent_t ev : done) (void)hipEventDestroy(ev);
+  if (ms_out) *ms_out = ms;
+  for (int k = 0; k < count; ++k)
+    if (int st = factor_done(ctxs[k], sigmas[k], ms)) return st;
   return TAILOR_OK;
 }
 

@@ -33,6 +33,51 @@ __device__ inline double cabs1(double re, double im) { return fabs(re) + fabs(im
 
 // ---------------------------------------------------------------- set-up
 
+// The shift of the next factorization (written on the stream before it, so
+// that a captured factorization replays with any shift).
+__global__ void k_set_sigma(double *__restrict__ fp, double sig_re, double sig_im) {
+  fp[kFpSigRe] = sig_re;
+  fp[kFpSigIm] = sig_im;
+}
+
+// Partial sums of squares of n doubles: one per workgroup (a tree over its
+// 256 threads after a grid-strided loop).
+constexpr int kParamBlocks = 512;
+__global__ __launch_bounds__(256) void k_sumsq_flat(const double *__restrict__ data, long long n,
+                                                    double *__restrict__ partials) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (long long r = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; r < n;
+       r += static_cast<long long>(gridDim.x) * 256) {
+    const double x = data[r];
+    acc += x * x;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+// Pivot threshold and static perturbation of the factorization, scaled by
+// the RMS row norm of K = A - sigma B: scale = sqrt(|A|_F^2 / rows) +
+// |sigma| sqrt(|B|_F^2 / rows); pivots below tiny = c_t * scale are replaced
+// by +-perturb = c_p * scale (c_t = 1e-14, c_p = sqrt(eps) for FP64 fronts;
+// 1e-7 and sqrt(eps_32) for FP32 fronts), and iterative refinement recovers
+// the accuracy.  One thread sums the partials in order.
+__global__ void k_factor_params(const double *__restrict__ partials, double rows, int fp32,
+                                double *__restrict__ fp) {
+  double ssa = 0.0, ssb = 0.0;
+  for (int b = 0; b < kParamBlocks; ++b) ssa += partials[b];
+  for (int b = 0; b < kParamBlocks; ++b) ssb += partials[kParamBlocks + b];
+  const double sig = hypot(fp[kFpSigRe], fp[kFpSigIm]);
+  const double scale = fmax(1e-300, sqrt(ssa / rows) + sig * sqrt(ssb / rows));
+  fp[kFpTiny] = (fp32 ? 1e-7 : 1e-14) * scale;
+  fp[kFpPerturb] = (fp32 ? 3.4526698300124393e-04 : 1.4901161193847656e-08) * scale;
+}
+
 __global__ void k_fill(double *__restrict__ p, long long n, double v) {
   for (long long k = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
        k += static_cast<long long>(gridDim.x) * blockDim.x)
@@ -59,10 +104,11 @@ __global__ void k_init_identity(const FrontDev *__restrict__ fr, int nf, T *__re
 template <typename T>
 __global__ void k_assemble_fronts(long long nnzb, int per, const double *__restrict__ avals,
                                   const int *__restrict__ colidx, const double *__restrict__ bvals,
-                                  double sig_re, double sig_im, const int *__restrict__ amap,
+                                  const double *__restrict__ fp, const int *__restrict__ amap,
                                   const FrontDev *__restrict__ fr, T *__restrict__ arena) {
   const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= nnzb * 25) return;
+  const double sig_re = fp[kFpSigRe], sig_im = fp[kFpSigIm];
   const long long k = t / 25;
   const int e = static_cast<int>(t - 25 * k);
   const int f = amap[3 * k], lu = amap[3 * k + 1], lv = amap[3 * k + 2];
@@ -190,13 +236,15 @@ constexpr int kPanelChunk = 8;  // panel columns updated per batch of loads
 template <typename T>
 __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__ list, int nlist,
                                                          const FrontDev *__restrict__ fr, int k0,
-                                                         int nb, int K0, int NB, double tiny,
-                                                         double perturb, T *__restrict__ arena,
+                                                         int nb, int K0, int NB,
+                                                         const double *__restrict__ fp,
+                                                         T *__restrict__ arena,
                                                          int *__restrict__ ipiv,
                                                          int *__restrict__ nperturbed) {
   const int f = list[blockIdx.x];
   const FrontDev d = fr[f];
   if (k0 >= d.p) return;
+  const double tiny = fp[kFpTiny], perturb = fp[kFpPerturb];
   const int kb = min(nb, d.p - k0);
   const int K1 = min(K0 + NB, d.p);
   const long long mm = static_cast<long long>(d.m) * d.m;
@@ -382,11 +430,12 @@ __device__ inline int wave_min(int v) {
 template <typename T, int R, int NT>
 __global__ __launch_bounds__(NT) void k_panel_regs(
     const int *__restrict__ list, int nlist, const FrontDev *__restrict__ fr, int k0, int nb,
-    int K0, int NB, double tiny, double perturb, T *__restrict__ arena, int *__restrict__ ipiv,
+    int K0, int NB, const double *__restrict__ fp, T *__restrict__ arena, int *__restrict__ ipiv,
     int *__restrict__ nperturbed) {
   const int f = list[blockIdx.x];
   const FrontDev d = fr[f];
   if (k0 >= d.p) return;
+  const double tiny = fp[kFpTiny], perturb = fp[kFpPerturb];
   const int kb = min(nb, d.p - k0);
   const int K1 = min(K0 + NB, d.p);
   const int p = d.p;
@@ -1636,31 +1685,30 @@ constexpr int kRegsThreads = 512;
 
 template <typename T, int R>
 hipError_t launch_panel_regs(int rows, dim3 grid, hipStream_t st, const int *list, int count,
-                             const FrontDev *fr, int k0, int nb, int K0, int NB, double tiny,
-                             double perturb, T *arena, int *ipiv, int *nperturbed) {
+                             const FrontDev *fr, int k0, int nb, int K0, int NB, const double *fp,
+                             T *arena, int *ipiv, int *nperturbed) {
   if constexpr (R * kRegsThreads > kPanelMapRows) {
     return hipErrorInvalidValue;  // excluded by the caller
   } else {
     if (rows > R)
-      return launch_panel_regs<T, R + 1>(rows, grid, st, list, count, fr, k0, nb, K0, NB, tiny,
-                                         perturb, arena, ipiv, nperturbed);
+      return launch_panel_regs<T, R + 1>(rows, grid, st, list, count, fr, k0, nb, K0, NB, fp,
+                                         arena, ipiv, nperturbed);
     hipLaunchKernelGGL((k_panel_regs<T, R, kRegsThreads>), grid, dim3(kRegsThreads), 0, st, list,
-                       count, fr, k0, nb, K0, NB, tiny, perturb, arena, ipiv, nperturbed);
+                       count, fr, k0, nb, K0, NB, fp, arena, ipiv, nperturbed);
     return hipGetLastError();
   }
 }
 
 template <typename T>
 hipError_t launch_panel(int pmax, bool streaming, int count, hipStream_t st, const int *list,
-                        const FrontDev *fr, int k0, int nb, int K0, int NB, double tiny,
-                        double perturb, T *arena, int *ipiv, int *nperturbed) {
+                        const FrontDev *fr, int k0, int nb, int K0, int NB, const double *fp,
+                        T *arena, int *ipiv, int *nperturbed) {
   const dim3 grid(count);
   if (!streaming && pmax <= kPanelMapRows)
     return launch_panel_regs<T, 1>((pmax + kRegsThreads - 1) / kRegsThreads, grid, st, list,
-                                   count, fr, k0, nb, K0, NB, tiny, perturb, arena, ipiv,
-                                   nperturbed);
+                                   count, fr, k0, nb, K0, NB, fp, arena, ipiv, nperturbed);
   hipLaunchKernelGGL(k_panel<T>, grid, dim3(kPanelThreads), 0, st, list, count, fr, k0, nb, K0,
-                     NB, tiny, perturb, arena, ipiv, nperturbed);
+                     NB, fp, arena, ipiv, nperturbed);
   return hipGetLastError();
 }
 
@@ -1933,8 +1981,8 @@ hipError_t lu_fill(hipStream_t s, double *p, long long n, double v) {
 }
 
 template <typename T>
-static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re, double sig_im,
-                              double tiny, double perturb, hipStream_t s, FactorTiming *tm) {
+static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, hipStream_t s,
+                              FactorTiming *tm) {
   hipError_t e;
   std::vector<hipEvent_t> events;  // start/stop pairs around GEMM launches when timing
   std::vector<int> event_level;    // level of each pair
@@ -1946,8 +1994,8 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const long long threads = in.nnzb * 25;
   hipLaunchKernelGGL(k_assemble_fronts<T>, dim3(static_cast<unsigned>((threads + 255) / 256)),
-                     dim3(256), 0, s, in.nnzb, in.per, in.avals, in.colidx, in.bvals, sig_re,
-                     sig_im, P.d_amap, P.d_front, arena);
+                     dim3(256), 0, s, in.nnzb, in.per, in.avals, in.colidx, in.bvals,
+                     P.d_fparams, P.d_amap, P.d_front, arena);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, s, reinterpret_cast<double *>(P.d_counters), 1LL, 0.0);
   const int nb = P.nb, NB = kLuNbOuter;
@@ -1997,7 +2045,7 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
       const int Kend = std::min(K0 + NB, L.pmax);
       for (int k0 = K0; k0 < Kend; k0 += nb) {
         if ((r = launch_panel<T>(L.pmax, P.panel_streaming, L.count, st, list, P.d_front, k0,
-                                 nb, K0, NB, tiny, perturb, arena, P.d_ipiv,
+                                 nb, K0, NB, P.d_fparams, arena, P.d_ipiv,
                                  P.d_counters)) != hipSuccess)
           return r;
         hipLaunchKernelGGL(k_diag_inv<T>, dim3(L.count), dim3(128), 0, st, list, L.count,
@@ -2091,10 +2139,26 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, double sig_re
   return hipSuccess;
 }
 
-hipError_t lu_factor(const LuPlan &P, const LuInputs &in, double sig_re, double sig_im,
-                     double tiny, double perturb, bool fp32, hipStream_t s, FactorTiming *tm) {
-  return fp32 ? factor_impl<float>(P, in, sig_re, sig_im, tiny, perturb, s, tm)
-              : factor_impl<double>(P, in, sig_re, sig_im, tiny, perturb, s, tm);
+hipError_t lu_factor(const LuPlan &P, const LuInputs &in, bool fp32, hipStream_t s,
+                     FactorTiming *tm) {
+  return fp32 ? factor_impl<float>(P, in, s, tm) : factor_impl<double>(P, in, s, tm);
+}
+
+hipError_t lu_set_sigma(const LuPlan &P, double sig_re, double sig_im, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_sigma, dim3(1), dim3(1), 0, s, P.d_fparams, sig_re, sig_im);
+  return hipGetLastError();
+}
+
+hipError_t lu_factor_params(const LuPlan &P, const LuInputs &in, bool fp32, double *partials,
+                            hipStream_t s) {
+  const long long nodes = in.nnzb / in.per;
+  hipLaunchKernelGGL(k_sumsq_flat, dim3(kParamBlocks), dim3(256), 0, s, in.avals, in.nnzb * 50,
+                     partials);
+  hipLaunchKernelGGL(k_sumsq_flat, dim3(kParamBlocks), dim3(256), 0, s, in.bvals, nodes * 50,
+                     partials + kParamBlocks);
+  hipLaunchKernelGGL(k_factor_params, dim3(1), dim3(1), 0, s, partials, 5.0 * nodes, fp32 ? 1 : 0,
+                     P.d_fparams);
+  return hipGetLastError();
 }
 
 // x = K^-1 b for vectors in elimination order (split planes).  b may alias x.

@@ -54,6 +54,7 @@ struct LuPlan {
   int *d_ipiv = nullptr;
   int *d_counters = nullptr;    // [0] perturbed pivots
   int *d_perm = nullptr;        // elimination dof -> natural dof (n entries)
+  double *d_fparams = nullptr;  // kFpCount doubles: sigma and the pivot thresholds
   double *arena = nullptr;
   hipStream_t side = nullptr;   // look-ahead panels (default priority)
   hipEvent_t ev_head = nullptr, ev_panel = nullptr;
@@ -74,6 +75,10 @@ struct LuInputs {
   const int *colidx = nullptr;
   const double *bvals = nullptr;  // B_bc block diagonal, interleaved complex
 };
+
+// The factorization's parameter block (device): the shift and the pivot
+// threshold / static perturbation derived from it and from |A|, |B|.
+enum { kFpSigRe = 0, kFpSigIm = 1, kFpTiny = 2, kFpPerturb = 3, kFpCount = 4 };
 
 constexpr int kLuNb = 32;        // panel width
 constexpr int kLuNbOuter = 128;  // outer block width (K of the trailing update)
@@ -102,10 +107,20 @@ struct FactorTiming {
 enum { kOrderLinear = 0, kOrderXcdGrouped = 1, kOrderCount = 2 };
 constexpr int kGemmShapes = 4;
 
-// Fronts in complex FP64, or complex FP32 (fp32) in the same arena.  With
-// `timing`, every GEMM launch is bracketed by events (slightly slower).
-hipError_t lu_factor(const LuPlan &P, const LuInputs &in, double sig_re, double sig_im,
-                     double tiny, double perturb, bool fp32, hipStream_t s,
+// The shift sigma of the next factorization into the parameter block.
+hipError_t lu_set_sigma(const LuPlan &P, double sig_re, double sig_im, hipStream_t s);
+// The pivot threshold and perturbation from sigma (already in the block) and
+// the Frobenius norms of A_bc and B_bc, on the device (`partials`: 1024
+// doubles of scratch).
+hipError_t lu_factor_params(const LuPlan &P, const LuInputs &in, bool fp32, double *partials,
+                            hipStream_t s);
+// LU of K = A - sigma B with the parameter block's sigma and thresholds:
+// fronts in complex FP64, or complex FP32 (fp32) in the same arena.  With
+// `timing`, every GEMM launch is bracketed by events (slightly slower) and
+// the factorization runs on one stream; without, it is a pure launch
+// sequence on s and the plan's side stream (joined back to s), so it can be
+// captured in a HIP graph.
+hipError_t lu_factor(const LuPlan &P, const LuInputs &in, bool fp32, hipStream_t s,
                      FactorTiming *timing = nullptr);
 // x = K^-1 b, vectors in elimination dof order, split planes (re, im)
 hipError_t lu_solve(const LuPlan &P, const double *br, const double *bi, double *xr, double *xi,

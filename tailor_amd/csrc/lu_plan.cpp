@@ -129,6 +129,7 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
   std::vector<int> zeros(std::max(1, poff), 0);
   if ((e = upload(&P.d_ipiv, zeros, s)) != hipSuccess) return e;
   if ((e = upload(&P.d_counters, std::vector<int>(8, 0), s)) != hipSuccess) return e;
+  if ((e = upload(&P.d_fparams, std::vector<double>(kFpCount, 0.0), s)) != hipSuccess) return e;
   if ((e = hipMalloc(reinterpret_cast<void **>(&P.arena), P.arena_doubles * sizeof(double))) != hipSuccess)
     return e;
   if ((e = hipMalloc(reinterpret_cast<void **>(&P.W), 2 * P.w_len * sizeof(double))) != hipSuccess)
@@ -173,6 +174,7 @@ void lu_plan_destroy(LuPlan &P) {
   release(P.d_amap);
   release(P.d_ipiv);
   release(P.d_counters);
+  release(P.d_fparams);
   release(P.d_perm);
   release(P.arena);
   release(P.W);

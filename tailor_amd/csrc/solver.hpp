@@ -26,6 +26,14 @@ struct SolverState {
   size_t V_cap = 0, work_cap = 0, ritz_cap = 0, partials_cap = 0, small_cap = 0;
   double *hsteps = nullptr;  // Gram-Schmidt coefficients of one Krylov pass (device)
   size_t hsteps_cap = 0;
+  // HIP graph of the factorization (norms -> pivot thresholds -> LU),
+  // captured once per plan, front precision and tile choice, replayed with
+  // any shift (written to the plan's parameter block before each replay)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  unsigned long long graph_id = 0;  // unique over the process
+  bool graph_fp32 = false, graph_streaming = false;
+  std::vector<int> graph_shape, graph_order;
   // results
   int nritz = 0;
   int ritz_ld = 0;  // columns per plane of the Ritz vector buffer
@@ -36,6 +44,13 @@ struct SolverState {
 void solver_release(SolverState *S);
 int solver_prepare(tailor_ctx *ctx);
 int solver_factor(tailor_ctx *ctx, cplx sigma, FactorTiming *timing = nullptr);
+// The same factorization replayed from the context's HIP graph (captured at
+// the first call and whenever the precision or tile choice changed).
+int solver_factor_graph(tailor_ctx *ctx, cplx sigma);
+// The factorizations of `count` contexts of one device: each context's
+// graph launched on its own stream from this thread, so they run
+// concurrently; *ms: the batch's event time.
+int solver_factor_batch(tailor_ctx *const *ctxs, int count, const cplx *sigmas, double *ms);
 // Factor once per GEMM tile order with per-level timing and keep, for each
 // elimination level, the order with the smallest GEMM time.
 int solver_autotune(tailor_ctx *ctx, cplx sigma, tailor_tune_report *report);

@@ -134,15 +134,21 @@ _WARM_FORMS = ("learned", "fixed")
 class Sweeper:
     def __init__(self, device: int = 0, nev: int = 8, tol: float = 1e-10, ncv: int = 64,
                  omega_s: complex = 0.02 + 0j, maxit: int = 1000, contexts: int = 1,
-                 check_from: int = 0):
+                 check_from: int = 0, graph: bool = False):
         """`check_from` > 0: the learned form's Krylov-Schur runs also test
         convergence after every second step from that step on and end a pass
-        early (the plain form and the fallback test only at pass ends)."""
+        early (the plain form and the fallback test only at pass ends).
+        `graph`: every factorization is replayed from the context's HIP graph
+        (tailor_factor_graph, the same results as the eager launches).  At
+        200x100 a replay takes 169 ms against 177 ms for the eager launches
+        on one context, but the eight-context pipeline measured slower with
+        replays (33.6 against 45.7 eigenpairs/s), so eager is the default."""
         if not 1 <= contexts <= 8:
             raise ValueError("contexts must be 1 to 8")
         if check_from < 0:
             raise ValueError("check_from must be >= 0")
         self.check_from = check_from
+        self.graph = graph
         self.device = device
         self.nev, self.tol, self.ncv, self.maxit = nev, tol, ncv, maxit
         self.omega_s = complex(omega_s)
@@ -169,7 +175,22 @@ class Sweeper:
         ctx.set_precision(precision)
         return ctx.autotune(complex(self.omega_s.imag, -self.omega_s.real))
 
-    def _solve(self, ctx, omega, precision, warm, check_from=0):
+    def _factor(self, ctx, omega, precision):
+        """LU of A_bc - sigma B_bc at the target of `omega`."""
+        ctx.set_precision(precision)
+        sigma = complex(omega.imag, -omega.real)
+        if self.graph:
+            ctx.factor_graph(sigma)
+        else:
+            ctx.factor(sigma)
+
+    def _solve(self, ctx, omega, precision, warm, check_from=0, factored=False):
+        """Krylov-Schur at the target of `omega`; `factored`: the context
+        already holds the factorization at that shift (a pipeline stage made
+        it), otherwise it is made here (the solve keeps a factorization at
+        its own shift)."""
+        if self.graph and not factored:
+            self._factor(ctx, omega, precision)
         ctx.set_precision(precision)
         mixed = precision == PRECISION_MIXED
         d = ctx.solve(nev=self.nev, omega=omega, tol=self.tol, maxit=self.maxit, ncv=self.ncv,
@@ -271,9 +292,7 @@ class Sweeper:
                 ctx = self.context(inst.ny, inst.nz, slot)
                 ctx.load_case(case)
                 ctx.assemble(diagnostics=False)
-                ctx.set_precision(precision)
-                om = complex(shifts[k])
-                ctx.factor(complex(om.imag, -om.real))
+                self._factor(ctx, complex(shifts[k]), precision)
             except Exception as e:  # re-raised by the main thread
                 errors.append(e)
 
@@ -299,7 +318,7 @@ class Sweeper:
             omega = complex(shifts[k])
             key = (inst.ny, inst.nz, slot)
             d, om, res, good = self._solve(ctx, omega, precision, warm_form and key in solved,
-                                           self.check_from if warm_form else 0)
+                                           self.check_from if warm_form else 0, factored=True)
             solved.add(key)
             facts, fallback = 1, False
             restarts, solves, refine = d["iterations"], d["solves"], d["refinement_steps"]
