@@ -270,28 +270,10 @@ def without_assembly(totals: dict, assembly_ms: float) -> dict:
     return out
 
 
-def pair_overlap(a, b, rtol=1e-8) -> dict:
-    """Per instance: how many of sweep a's converged pairs sweep b also found
-    (same eigenvalue within rtol), and the largest difference among them."""
-    by = {id(r.instance): r for r in b.records}
-    counts, worst = [], 0.0
-    for r in a.records:
-        p = by[id(r.instance)]
-        good_b = p.omegas[p.residuals <= TOL]
-        n = 0
-        for w in r.omegas[r.residuals <= TOL]:
-            if len(good_b) == 0:
-                continue
-            d = float(np.min(np.abs(good_b - w)) / max(abs(w), 1e-300))
-            if d <= rtol:
-                n += 1
-                worst = max(worst, d)
-        counts.append(n)
-    return {"matched_per_instance": counts, "matched_total": int(sum(counts)),
-            "max_rel_diff": worst}
-
-
-def sweeps(ta, steps: int, warmup: int, best_ncv: int, learned_ncv: int, contexts: int) -> dict:
+def headline_instances(ta, steps: int, warmup: int):
+    """The headline sweep's instances (200x100, Re 1500, Ma 0.7, alpha ~
+    U[0.5, 3] from seed 1, base-flow seeds 1000 + k), prepared on the host,
+    and the network's shift centres for them."""
     from tailor_amd import sweep as sw
     from tailor_amd.shift_model import ShiftModel
     from tailor_amd.train_shift_model import MODEL
@@ -303,12 +285,50 @@ def sweeps(ta, steps: int, warmup: int, best_ncv: int, learned_ncv: int, context
     cases = sw.prepare(insts, OMEGA)
     alphas = np.array([i.alpha for i in insts])
     shifts = model.shift_centres(alphas, np.full(len(insts), 1500.0), np.full(len(insts), 0.7))
+    warm = list(range(warmup))
+    timed = list(range(warmup, warmup + steps))
+    return insts, cases, shifts, warm, timed
+
+
+def record_dicts(result) -> list:
+    return [{"seed": r.instance.seed, "alpha": r.instance.alpha, "size": f"{r.instance.ny}x{r.instance.nz}",
+             "good": r.good, "seconds": r.seconds, "solves": r.solves,
+             "omegas": [[w.real, w.imag] for w in r.omegas],
+             "residuals": [float(x) for x in r.residuals]} for r in result.records]
+
+
+def pair_overlap(a: list, b: list, rtol=1e-8) -> dict:
+    """Per instance: how many of sweep a's converged pairs sweep b also found
+    (same eigenvalue within rtol), and the largest difference among them."""
+    by = {(r["seed"], r["alpha"]): r for r in b}
+    counts, worst = [], 0.0
+    for r in a:
+        p = by[(r["seed"], r["alpha"])]
+        good_b = np.array([complex(*w) for w, x in zip(p["omegas"], p["residuals"]) if x <= TOL])
+        n = 0
+        for w, x in zip(r["omegas"], r["residuals"]):
+            w = complex(*w)
+            if x > TOL or len(good_b) == 0:
+                continue
+            d = float(np.min(np.abs(good_b - w)) / max(abs(w), 1e-300))
+            if d <= rtol:
+                n += 1
+                worst = max(worst, d)
+        counts.append(n)
+    return {"matched_per_instance": counts, "matched_total": int(sum(counts)),
+            "max_rel_diff": worst}
+
+
+def phase_pipeline(ta, steps, warmup, best_ncv, learned_ncv, contexts) -> dict:
+    """The learned form, the network switched off and the plain form at its
+    fastest subspace, all through the same pipeline of contexts."""
+    from tailor_amd import sweep as sw
+    insts, cases, shifts, warm, timed = headline_instances(ta, steps, warmup)
     pipe = sw.Sweeper(ncv=learned_ncv, omega_s=OMEGA, contexts=contexts,
                       check_from=LEARNED_CHECK_FROM)
     # warm-up instances (untimed): build the plans, tune, and let the first
     # one choose the factorization precision of the learned form by
     # measurement
-    warm = list(range(warmup))
     choice = {}
     precision = ta.PRECISION_FP64
     if warm:
@@ -319,8 +339,7 @@ def sweeps(ta, steps: int, warmup: int, best_ncv: int, learned_ncv: int, context
         precision = min(t, key=t.get)
         choice = {"fp64_s": round(t[ta.PRECISION_FP64], 3), "mixed_s": round(t[ta.PRECISION_MIXED], 3),
                   "chosen": "mixed" if precision == ta.PRECISION_MIXED else "fp64"}
-    log(f"sweeps: contexts planned and tuned, precision {choice.get('chosen', 'fp64')}")
-    timed = list(range(warmup, warmup + steps))
+    log(f"pipeline: contexts planned and tuned, precision {choice.get('chosen', 'fp64')}")
     order = sorted(timed, key=lambda k: insts[k].alpha)
     learned = pipe.run(cases, "learned", shifts=shifts, precision=precision, order=order)
     log(f"learned sweep: {learned.eigenpairs_per_s:.2f} eigenpairs/s")
@@ -331,34 +350,88 @@ def sweeps(ta, steps: int, warmup: int, best_ncv: int, learned_ncv: int, context
     # the plain form at its fastest subspace through the same pipeline
     pipe.ncv, pipe.check_from = best_ncv, 0
     plain_best = pipe.run(cases, "plain", precision=ta.PRECISION_FP64, order=timed)
-    pipe.close()
     log(f"plain sweep (ncv {best_ncv}, pipelined): {plain_best.eigenpairs_per_s:.2f} eigenpairs/s")
-    # the plain form as the reference runs it: ncv 23, one instance after another
+    pipe.close()
+    lr, pr = record_dicts(learned), record_dicts(plain_best)
+    return {"learned": learned.totals(), "fixed": fixed.totals(), "plain_best": plain_best.totals(),
+            "choice": choice, "overlap": pair_overlap(lr, pr),
+            "autotune": {f"{k[0]}x{k[1]}": v for k, v in pipe.tuned.items()}}
+
+
+def phase_reference(ta, steps, warmup) -> dict:
+    """The plain form as the reference runs it: ncv 23, one instance after
+    another, on one context."""
+    from tailor_amd import sweep as sw
+    insts, cases, shifts, warm, timed = headline_instances(ta, steps, warmup)
     ref = sw.Sweeper(ncv=REFERENCE_NCV, omega_s=OMEGA)
-    ref.run(cases, "plain", precision=ta.PRECISION_FP64, order=warm[:1])  # warm-up
+    ref.run(cases, "plain", precision=ta.PRECISION_FP64, order=warm[:1])  # plan, tune, warm-up
     plain_ref = ref.run(cases, "plain", precision=ta.PRECISION_FP64, order=timed)
-    log(f"plain sweep (reference settings): {plain_ref.eigenpairs_per_s:.2f} eigenpairs/s")
-    # both grid sizes, per-instance base flows, mixed precision
-    c3 = sw.sweep_instances(MIXED_SIZE_INSTANCES, seed=1, sizes=[(100, 50), (200, 100)])
-    c3_cases = sw.prepare(c3, OMEGA)
-    ref.run(c3_cases, "mixed", order=[0, 1])  # plan and tune both sizes
-    mixed_sweep = ref.run(c3_cases, "mixed")
     ref.close()
-    log(f"mixed-size mixed-precision sweep: {mixed_sweep.eigenpairs_per_s:.2f} eigenpairs/s")
+    log(f"plain sweep (reference settings): {plain_ref.eigenpairs_per_s:.2f} eigenpairs/s")
+    return {"plain_ref": plain_ref.totals()}
+
+
+def phase_mixed_size(ta, count, ncv, contexts) -> dict:
+    """Wavenumber sweep over both grid sizes (half 100x50, half 200x100),
+    base flows from seeds 1000 + k, complex FP32 fronts with FP64
+    refinement of every solve to 1e-12, fixed shift, through the pipeline."""
+    from tailor_amd import sweep as sw
+    c3 = sw.sweep_instances(count + 2, seed=1, sizes=[(100, 50), (200, 100)])
+    cases = sw.prepare(c3, OMEGA)
+    runner = sw.Sweeper(ncv=ncv, omega_s=OMEGA, contexts=contexts)
+    runner.run(cases, "mixed", order=[0, 1])  # plans and tuning of both sizes
+    res = runner.run(cases, "mixed", order=list(range(2, count + 2)))
+    runner.close()
+    log(f"mixed-size mixed-precision sweep: {res.eigenpairs_per_s:.2f} eigenpairs/s")
     per_size = {}
-    for r in mixed_sweep.records:
-        key = f"{r.instance.ny}x{r.instance.nz}"
-        e = per_size.setdefault(key, {"instances": 0, "good_pairs": 0, "seconds": 0.0})
+    for r in res.records:
+        e = per_size.setdefault(f"{r.instance.ny}x{r.instance.nz}",
+                                {"instances": 0, "good_pairs": 0, "solves": 0,
+                                 "refinement_steps": 0})
         e["instances"] += 1
         e["good_pairs"] += r.good
-        e["seconds"] += r.seconds
-    for e in per_size.values():
-        e["eigenpairs_per_s"] = round(e["good_pairs"] / e["seconds"], 4) if e["seconds"] else None
-        e["seconds"] = round(e["seconds"], 4)
-    return {"learned": learned, "fixed": fixed, "plain_best": plain_best, "plain_ref": plain_ref,
-            "choice": choice, "overlap": pair_overlap(learned, plain_best),
-            "mixed_sweep": mixed_sweep, "mixed_per_size": per_size,
-            "autotune": {f"{k[0]}x{k[1]}": v for k, v in pipe.tuned.items()}}
+        e["solves"] += r.solves
+        e["refinement_steps"] += r.refinement_steps
+    return {"mixed_sweep": {**res.totals(), "ncv": ncv, "contexts": contexts,
+                            "per_size": per_size}}
+
+
+def run_phase(name: str, args, extra=()) -> dict:
+    """One measurement phase in a child process (a fresh HIP runtime: the
+    runtime keeps the hardware queues of every stream a process has created,
+    and with many queues in use each kernel launch waits longer; measured at
+    200x100, an unrelated factorization took 177 ms in a fresh process and
+    207 ms after 24 streams had been created, also once they were destroyed)."""
+    import subprocess
+    cmd = [sys.executable, str(Path(__file__).resolve()), "--phase", name,
+           "--steps", str(args.steps), "--warmup", str(max(1, args.warmup)),
+           "--learned-ncv", str(args.learned_ncv), "--contexts", str(args.contexts), *extra]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
+    if r.returncode != 0:
+        raise SystemExit(f"bench.py: phase {name} failed with status {r.returncode}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def phase_main(args) -> int:
+    """Run one phase in this process and print its result as JSON."""
+    import tailor_amd as ta
+    from tailor_amd import case as cs
+    if args.phase == "single":
+        single = plain_step(ta, cs)
+        small = gpu_small_case(ta, cs, REFERENCE_NCV)
+        log("100x50 plain instance done")
+        out = {"single": single, "small": small}
+    elif args.phase == "pipeline":
+        out = phase_pipeline(ta, args.steps, args.warmup, args.best_ncv, args.learned_ncv,
+                             args.contexts)
+    elif args.phase == "reference":
+        out = phase_reference(ta, args.steps, args.warmup)
+    elif args.phase == "mixed_size":
+        out = phase_mixed_size(ta, args.mixed_count, args.best_ncv, args.contexts)
+    else:
+        raise SystemExit(f"unknown phase {args.phase}")
+    print(json.dumps(out))
+    return 0
 
 
 def main() -> int:
@@ -371,6 +444,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--contexts", type=int, default=PIPELINE_CONTEXTS,
                     help="contexts of the pipelined sweeps (1: no worker threads)")
+    ap.add_argument("--phase", help=argparse.SUPPRESS)
+    ap.add_argument("--best-ncv", type=int, default=REFERENCE_NCV, help=argparse.SUPPRESS)
+    ap.add_argument("--mixed-count", type=int, default=MIXED_SIZE_INSTANCES, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -380,32 +456,37 @@ def main() -> int:
     if args.steps < 1:
         print("bench.py: --steps must be at least 1", file=sys.stderr)
         return 2
+    if args.phase:
+        return phase_main(args)
 
-    import tailor_amd as ta
-    from tailor_amd import case as cs
-
-    single = plain_step(ta, cs)
-    small = gpu_small_case(ta, cs, REFERENCE_NCV)
-    log("100x50 plain instance done")
+    # every GPU phase in a child process; this process never touches the GPU
+    sp = run_phase("single", args)
+    single, small = sp["single"], sp["small"]
     best_ncv = single["plain"]["best_ncv"]
-    sw = sweeps(ta, args.steps, max(1, args.warmup), best_ncv, args.learned_ncv, args.contexts)
-    L, F, PB, PR = sw["learned"], sw["fixed"], sw["plain_best"], sw["plain_ref"]
-    strongest = max(PB.eigenpairs_per_s, PR.eigenpairs_per_s)
+    pipe = run_phase("pipeline", args, ("--best-ncv", str(best_ncv)))
+    ref = run_phase("reference", args)
+    mixed_count = max(MIXED_SIZE_INSTANCES, 2 * (args.steps // 4))
+    msz = run_phase("mixed_size", args, ("--best-ncv", str(best_ncv),
+                                         "--mixed-count", str(mixed_count),
+                                         "--contexts", str(min(args.contexts, 4))))
+    L, F, PB, PR = pipe["learned"], pipe["fixed"], pipe["plain_best"], ref["plain_ref"]
+    rate = lambda t: t["eigenpairs_per_s"]
+    strongest = max(rate(PB), rate(PR))
     g = single["gemm"]
     gemm_tf = g["gemm_flops"] / (g["gemm_ms"] * 1e-3) / 1e12
     asm_ms = single["assembly"]["full_ms"]
     out = {
         "metric": "eigenpairs/s at resid<=1e-10 (learned-shift sweep; plain FP64 alongside)",
-        "value": round(L.eigenpairs_per_s, 4),
+        "value": round(rate(L), 4),
         "unit": "eigenpairs/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": max(1, args.warmup),
-        "ms_per_step": round(L.seconds * 1e3 / args.steps, 3),
+        "ms_per_step": round(L["seconds"] * 1e3 / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128" if sw["choice"].get("chosen", "fp64") == "fp64" else "c64 factors + c128",
+        "dtype": "c128" if pipe["choice"].get("chosen", "fp64") == "fp64" else "c64 factors + c128",
         "data": "synthetic (erf boundary layer on a tanh-clustered, wall-bumped grid; base flow "
                 "per instance from seeds 1000 + k)",
         "config": {"workload": "BiGlobal wavenumber sweep, 200x100 FD-q grid (q_y=10, q_z=6), "
@@ -416,15 +497,16 @@ def main() -> int:
                    "nev": NEV, "ncv_learned": args.learned_ncv, "ncv_plain_reference": REFERENCE_NCV,
                    "ncv_plain_best": best_ncv, "pipeline_contexts": args.contexts,
                    "learned_check_from": LEARNED_CHECK_FROM, "instances": args.steps,
-                   "precision_choice": sw["choice"], "gemm_autotune": sw["autotune"]},
-        "learned_sweep": without_assembly(L.totals(), asm_ms),
-        "network_off_sweep": without_assembly(F.totals(), asm_ms),
-        "plain_sweep_best": without_assembly(PB.totals(), asm_ms),
-        "plain_sweep_reference": without_assembly(PR.totals(), asm_ms),
-        "speedup_vs_strongest_plain": round(L.eigenpairs_per_s / max(strongest, 1e-300), 3),
-        "speedup_vs_reference_plain": round(L.eigenpairs_per_s / max(PR.eigenpairs_per_s, 1e-300), 3),
-        "network_gain": round(L.eigenpairs_per_s / max(F.eigenpairs_per_s, 1e-300), 3),
-        "learned_pairs_found_by_plain": sw["overlap"],
+                   "phases": "each GPU phase in its own child process",
+                   "precision_choice": pipe["choice"], "gemm_autotune": pipe["autotune"]},
+        "learned_sweep": without_assembly(L, asm_ms),
+        "network_off_sweep": without_assembly(F, asm_ms),
+        "plain_sweep_best": without_assembly(PB, asm_ms),
+        "plain_sweep_reference": without_assembly(PR, asm_ms),
+        "speedup_vs_strongest_plain": round(rate(L) / max(strongest, 1e-300), 3),
+        "speedup_vs_reference_plain": round(rate(L) / max(rate(PR), 1e-300), 3),
+        "network_gain": round(rate(L) / max(rate(F), 1e-300), 3),
+        "learned_pairs_found_by_plain": pipe["overlap"],
         "roofline": {"bound": "mfma", "kernel": "k_gemm (frontal updates, MFMA f64 16x16x4)",
                      "achieved": round(gemm_tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(gemm_tf / FP64_PEAK_TFLOPS, 4),
@@ -444,8 +526,8 @@ def main() -> int:
         "solve_sweep": single["solve_sweep"],
         "spmv": single["spmv"],
         "mixed_precision": single["mixed"],
-        "mixed_size_sweep": {**sw["mixed_sweep"].totals(), "per_size": sw["mixed_per_size"]},
-        "gpu_100x50_plain": small,
+        "mixed_size_sweep": msz["mixed_sweep"],
+        "gpu_100x50_plain": {k: v for k, v in small.items() if k != "lambdas"},
     }
     if not args.no_cpu_baseline:
         cb = cpu_baseline(REFERENCE_NCV)
@@ -455,7 +537,6 @@ def main() -> int:
         cb["max_rel_diff_vs_gpu"] = float(max(np.min(np.abs(gl - l)) / abs(l) for l in cl)) \
             if len(gl) and len(cl) else None
         out["cpu_baseline"] = cb
-    out["gpu_100x50_plain"].pop("lambdas")
     print(json.dumps(out))
     return 0
 
