@@ -96,6 +96,21 @@ def measured_traffic(kernel: str):
     return k["traffic_bytes_per_dispatch"] if k else None
 
 
+def traced_gemm_window():
+    """Mean k_gemm duration inside tailor_profile_factor's window from the
+    committed kernel-trace summary of this benchmark (rocprofv3), or None."""
+    import re
+    path = ROOT / "profiles" / "bench_sweep_200x100_kernel_stats.md"
+    if not path.exists():
+        return None
+    m = re.search(r"Frontal GEMM \(k_gemm, all tile shapes\): (\d+) launches, ([0-9.]+) ms, "
+                  r"mean ([0-9.]+) us per launch", path.read_text())
+    if not m:
+        return None
+    return {"launches": int(m.group(1)), "ms": float(m.group(2)), "us_per_launch": float(m.group(3)),
+            "source": str(path.relative_to(ROOT))}
+
+
 def host_facts() -> dict:
     """The host the CPU baseline runs on."""
     model = "unknown"
@@ -521,6 +536,7 @@ def main() -> int:
                      "tile_config_per_level": single["tune"]["config"],
                      "gemm_ms_per_config": single["tune"]["gemm_ms"],
                      "factor_ms": round(g["factor_ms"], 2)},
+        "roofline_traced": None,
         "plain_instance": single["plain"],
         "assembly": single["assembly"],
         "solve_sweep": single["solve_sweep"],
@@ -529,6 +545,13 @@ def main() -> int:
         "mixed_size_sweep": msz["mixed_sweep"],
         "gpu_100x50_plain": {k: v for k, v in small.items() if k != "lambdas"},
     }
+    tw = traced_gemm_window()
+    if tw and tw["launches"] == g["gemm_launches"]:
+        tf = g["gemm_flops"] / (tw["ms"] * 1e-3) / 1e12
+        out["roofline_traced"] = {**tw, "achieved": round(tf, 3),
+                                  "frac": round(tf / FP64_PEAK_TFLOPS, 4),
+                                  "note": "the same factorization's k_gemm launches timed by a "
+                                          "rocprofv3 kernel trace of this benchmark"}
     if not args.no_cpu_baseline:
         cb = cpu_baseline(REFERENCE_NCV)
         log("CPU baseline done")
