@@ -480,7 +480,7 @@ def main() -> int:
     best_ncv = single["plain"]["best_ncv"]
     pipe = run_phase("pipeline", args, ("--best-ncv", str(best_ncv)))
     ref = run_phase("reference", args)
-    mixed_count = max(MIXED_SIZE_INSTANCES, 2 * (args.steps // 4))
+    mixed_count = max(MIXED_SIZE_INSTANCES, 2 * (args.steps // 2))  # 64 at the default
     msz = run_phase("mixed_size", args, ("--best-ncv", str(best_ncv),
                                          "--mixed-count", str(mixed_count),
                                          "--contexts", str(min(args.contexts, 4))))

@@ -13,9 +13,10 @@ WRITE_SIZE counters of the other kernels).  Prints the algorithmic bytes or
 flops of each kernel family so that counter readings can be compared with
 them.
 
-    python -m tailor_amd.profile_workload summarize <kernel_trace.csv> --command "..." [--out f.md]
+    python -m tailor_amd.profile_workload summarize <kernel_trace.csv>... --command "..." [--out f.md]
 
-Summarises a `rocprofv3 --kernel-trace --output-format csv` trace as a
+Summarises a `rocprofv3 --kernel-trace --output-format csv` trace (one file
+per traced process, all on the same clock) as a
 markdown table of per-kernel launch counts and durations, and separately
 the launches inside the window that tailor_profile_factor brackets with
 its k_profile_window_begin / k_profile_window_end kernels (the serialized,
@@ -48,14 +49,17 @@ def _table(rows, total_ns, limit=40):
     return out
 
 
-def summarize(csv_path: str, command: str) -> str:
+def summarize(csv_paths, command: str) -> str:
     """Markdown summary of a rocprofv3 kernel trace (see the module docstring)."""
     import csv
+    if isinstance(csv_paths, str):
+        csv_paths = [csv_paths]
     launches = []
-    with open(csv_path, newline="") as f:
-        for r in csv.DictReader(f):
-            launches.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]),
-                             _short_name(r["Kernel_Name"])))
+    for path in csv_paths:
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                launches.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]),
+                                 _short_name(r["Kernel_Name"])))
     launches.sort()
     rows, total = {}, 0
     for t0, t1, name in launches:
@@ -63,7 +67,8 @@ def summarize(csv_path: str, command: str) -> str:
         rows[name] = (c + 1, ns + (t1 - t0))
         total += t1 - t0
     lines = [f"Command (GPU box, repo root):", "", f"    {command}", "",
-             f"{len(launches)} kernel launches, {total / 1e6:.1f} ms of summed kernel time.", ""]
+             f"{len(csv_paths)} traced processes, {len(launches)} kernel launches, "
+             f"{total / 1e6:.1f} ms of summed kernel time.", ""]
     lines += _table(rows, total)
     begin = [t1 for t0, t1, n in launches if n == "k_profile_window_begin"]
     end = [t0 for t0, t1, n in launches if n == "k_profile_window_end"]
@@ -89,7 +94,7 @@ def main(argv=None) -> int:
     argv = sys.argv[1:] if argv is None else list(argv)
     if argv and argv[0] == "summarize":
         sp = argparse.ArgumentParser(prog="profile_workload summarize")
-        sp.add_argument("csv")
+        sp.add_argument("csv", nargs="+")
         sp.add_argument("--command", required=True, help="the profiled command, for the record")
         sp.add_argument("--out")
         b = sp.parse_args(argv[1:])

@@ -110,3 +110,28 @@ def test_mixed_size_learned_sweep_with_per_instance_base_flows():
             d = np.min(np.abs(p.omegas - w)) / abs(w)
             if d < 1e-6:  # found by both forms
                 assert d < 1e-8, (w, d)
+
+
+def test_mixed_size_mixed_precision_sweep_matches_fp64():
+    """Config 3 at small scale: instances of both grid sizes with their own
+    base flows, complex FP32 fronts with every solve refined to 1e-12,
+    through a two-context pipeline whose factorizations are replayed from
+    HIP graphs.  Every pair meets the tolerance and equals the plain FP64
+    sweep's eigenvalue at the same shift within 1e-8."""
+    from tailor_amd import sweep as sw
+    insts = sw.sweep_instances(4, seed=1, sizes=[(100, 50), (200, 100)], per_instance_seeds=True)
+    cases = sw.prepare(insts)
+    print("mixed-precision sweep: 2 x 100x50 + 2 x 200x100, two contexts, graphs", flush=True)
+    runner = sw.Sweeper(ncv=96, contexts=2, graph=True)
+    mixed = runner.run(cases, "mixed")
+    runner.close()
+    plain_runner = sw.Sweeper(ncv=96)
+    plain = plain_runner.run(cases, "plain")
+    plain_runner.close()
+    print(f"mixed {mixed.totals()}\nplain {plain.totals()}", flush=True)
+    assert mixed.good_pairs == 8 * len(insts) and plain.good_pairs == 8 * len(insts)
+    assert mixed.totals()["refinement_steps"] >= mixed.totals()["solves"]
+    for a, b in zip(mixed.records, plain.records):
+        assert a.instance is b.instance
+        for w in b.omegas:
+            assert np.min(np.abs(a.omegas - w)) <= 1e-8 * abs(w), (a.instance, w)
