@@ -830,13 +830,16 @@ struct NodeRec {
 // The operator writer: workgroup = NPW consecutive block rows, one wave per
 // row.  The node's coefficient record is staged in LDS; the row is then
 // streamed out with each lane owning one block entry.
-template <int SSX, int SSE, int NPW>
+// SSX x SSE: the stencil sizes; with RT they are only the array bounds and
+// the case's own sizes (d.ssx <= SSX, d.sse <= SSE) are used at run time.
+template <int SSX, int SSE, int NPW, bool RT = false>
 __global__ __launch_bounds__(64 * NPW) void k_operator(
     CaseDims d, const int *__restrict__ xoff, const double *__restrict__ xw,
     const int *__restrict__ eoff, const double *__restrict__ ew, const double *__restrict__ coef,
     const int *__restrict__ status, const double *__restrict__ bcdata, int constrained,
     double *__restrict__ vals, double *__restrict__ mvals, DeviceDiag *diag) {
-  constexpr int PER = SSX * SSE;
+  const int NSX = RT ? d.ssx : SSX, NSE = RT ? d.sse : SSE;
+  const int PER = NSX * NSE;
   __shared__ NodeRec<SSX, SSE> rec[NPW];
   __shared__ int bad;
   const int tid = threadIdx.x;
@@ -849,7 +852,7 @@ __global__ __launch_bounds__(64 * NPW) void k_operator(
   // (One thread per node doing all of it measured 163 us for the writer at
   // 200x100 against 151.5 us spread like this.)
   {
-    constexpr int W = 1 + SSX + SSE + 25 + 5;
+    const int W = 1 + NSX + NSE + 25 + 5;
     for (int k = tid; k < NPW * W; k += 64 * NPW) {
       const int n = k / W, q = k - W * (k / W);
       const long long node = first + n;
@@ -861,33 +864,33 @@ __global__ __launch_bounds__(64 * NPW) void k_operator(
         R.status = status[node];
         if (R.status != 0) atomicOr(&bad, 1);
         R.kind = kind;
-      } else if (q <= SSX) {
+      } else if (q <= NSX) {
         const int sx = q - 1;
-        const double *w1x = xw + static_cast<size_t>(d.ny) * SSX + static_cast<size_t>(i) * SSX;
-        const double *w2x = w1x + static_cast<size_t>(d.ny) * SSX;
+        const double *w1x = xw + static_cast<size_t>(d.ny) * NSX + static_cast<size_t>(i) * NSX;
+        const double *w2x = w1x + static_cast<size_t>(d.ny) * NSX;
         R.w1x[sx] = w1x[sx];
         R.mw2x[sx] = -w2x[sx];
-        if (xoff[i * SSX + sx] == 0) R.s0 = sx;
-      } else if (q <= SSX + SSE) {
-        const int t = q - 1 - SSX;
-        const double *w1e = ew + static_cast<size_t>(d.nz) * SSE + static_cast<size_t>(j) * SSE;
-        const double *w2e = w1e + static_cast<size_t>(d.nz) * SSE;
+        if (xoff[i * NSX + sx] == 0) R.s0 = sx;
+      } else if (q <= NSX + NSE) {
+        const int t = q - 1 - NSX;
+        const double *w1e = ew + static_cast<size_t>(d.nz) * NSE + static_cast<size_t>(j) * NSE;
+        const double *w2e = w1e + static_cast<size_t>(d.nz) * NSE;
         R.w1e[t] = w1e[t];
         R.mw2e[t] = -w2e[t];
-        const int off = eoff[j * SSE + t];
+        const int off = eoff[j * NSE + t];
         if (off == 0) R.t0 = t;
         const int ct = wrap_mod(j + off, d.nz);
         int rank = 0;
-        for (int u = 0; u < SSE; ++u) rank += wrap_mod(j + eoff[j * SSE + u], d.nz) < ct;
+        for (int u = 0; u < NSE; ++u) rank += wrap_mod(j + eoff[j * NSE + u], d.nz) < ct;
         R.slot_of_rank[rank] = t;
       } else if (kind == 2 && constrained) {
         const double *bc = bcdata + 106 * static_cast<size_t>(j);
-        if (q <= SSX + SSE + 25) {
-          const int e = q - 1 - SSX - SSE;
+        if (q <= NSX + NSE + 25) {
+          const int e = q - 1 - NSX - NSE;
           R.P[e] = dcx{bc[2 * e], bc[2 * e + 1]};
           R.Ri[e] = dcx{bc[50 + 2 * e], bc[50 + 2 * e + 1]};
         } else {
-          const int r = q - 1 - SSX - SSE - 25;
+          const int r = q - 1 - NSX - NSE - 25;
           R.incoming[r] = bc[100 + r] != 0.0;
         }
       }
@@ -945,7 +948,7 @@ __global__ __launch_bounds__(64 * NPW) void k_operator(
         const int braw = 2 * k + half;
         const bool valid = braw < PER;
         const int b = valid ? braw : PER - 1;
-        const int tr = b / SSX, s = b - SSX * (b / SSX);
+        const int tr = b / NSX, s = b - NSX * (b / NSX);
         const int t = R.slot_of_rank[tr];
         // L entry, accumulation order K0, xi, eta, mixed (GlobalOperator.cpp:323-365)
         double vr = 0.0, vi = 0.0;
@@ -1038,13 +1041,14 @@ __global__ __launch_bounds__(64 * NPW) void k_operator(
   }
 }
 
-// Generic-size variant dispatch table.
-template <int SSX, int SSE>
+// Generic-size variant dispatch table (RT: run-time stencil sizes up to
+// SSX x SSE).
+template <int SSX, int SSE, bool RT = false>
 hipError_t launch_op(tailor_ctx *ctx, bool constrained) {
   constexpr int NPW = kOperatorRows;
   const CaseDims &d = ctx->dims;
   const unsigned grid = static_cast<unsigned>((d.nodes + NPW - 1) / NPW);
-  hipLaunchKernelGGL((k_operator<SSX, SSE, NPW>), dim3(grid), dim3(64 * NPW), 0, ctx->stream, d,
+  hipLaunchKernelGGL((k_operator<SSX, SSE, NPW, RT>), dim3(grid), dim3(64 * NPW), 0, ctx->stream, d,
                      ctx->d_xoff, ctx->d_xw, ctx->d_eoff, ctx->d_ew, ctx->d_coef,
                      ctx->d_coef_status, ctx->d_bcdata, constrained ? 1 : 0, ctx->d_vals,
                      ctx->d_mvals, ctx->d_diag);
@@ -1127,6 +1131,10 @@ hipError_t launch_operator(tailor_ctx *ctx, bool constrained) {
   if (d.ssx == 9 && d.sse == 7) return launch_op<9, 7>(ctx, constrained);
   if (d.ssx == 11 && d.sse == 5) return launch_op<11, 5>(ctx, constrained);
   if (d.ssx == 7 && d.sse == 7) return launch_op<7, 7>(ctx, constrained);
+  // any other stencil pair within the run-time instance's bounds (the loader
+  // rejects wider ones, and the derivative tile's LDS bound is tighter)
+  if (d.ssx <= kOperatorMaxSsx && d.sse <= kOperatorMaxSse)
+    return launch_op<kOperatorMaxSsx, kOperatorMaxSse, true>(ctx, constrained);
   return hipErrorInvalidValue;
 }
 

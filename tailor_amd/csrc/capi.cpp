@@ -268,6 +268,10 @@ int tailor_load_case(tailor_ctx *ctx, const tailor_case *c) {
   for (int v : eoff) d.hy = std::max(d.hy, std::abs(v));
   if (static_cast<size_t>(9 * 8) * (16 + 2 * d.hx) * (8 + 2 * d.hy) > 48 * 1024)
     return fail(ctx, TAILOR_ERR_ARG_OUTOFRANGE, "FD-q stencil halo too wide for the LDS tile");
+  if (d.ssx > tailor::kOperatorMaxSsx || d.sse > tailor::kOperatorMaxSse)
+    return fail(ctx, TAILOR_ERR_ARG_OUTOFRANGE,
+                "FD-q stencils wider than " + std::to_string(tailor::kOperatorMaxSsx) + " x " +
+                    std::to_string(tailor::kOperatorMaxSse) + " points are not supported");
   for (int j = 0; j < nz; ++j) {
     std::vector<int> cols;
     for (int t = 0; t < sse; ++t) cols.push_back(static_cast<int>(c->eta.stencil_indices[j * sse + t]));

@@ -96,6 +96,11 @@ struct tailor_ctx {
 
 namespace tailor {
 
+// Widest stencils (points per 1-D rule) the operator writer takes: its
+// run-time-sized instance holds this many per node in LDS; the derivative
+// tile's LDS bound is tighter for most pairs.
+constexpr int kOperatorMaxSsx = 27, kOperatorMaxSse = 19;
+
 constexpr int kPartialBlocks = 512;
 constexpr int kPartialComps = 32;
 

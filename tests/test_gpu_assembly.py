@@ -141,3 +141,42 @@ def test_derivatives_closed_form_on_skewed_grid():
     err = np.max(np.abs(d - exact), axis=0) / np.maximum(scale, 1e-3)
     print(f"closed-form derivative error {np.max(err):.3e}", flush=True)
     assert np.max(err) <= 1e-9
+
+
+@pytest.mark.parametrize("q_y,q_z", [(8, 4), (12, 2), (6, 8), (4, 10)])
+def test_any_stencil_pair_matches_oracle(oracle_lib, q_y, q_z):
+    """Q-Value pairs other than the writer's specialised stencil sizes (the
+    reference accepts any q >= 2, Parser.cpp:110-112) run through its
+    run-time-sized instance: A_bc, B_bc, L and M equal the oracle's."""
+    c = small_case(24, 12, q_y=q_y, q_z=q_z)
+    print(f"stencils {q_y + 1} x {q_z + 1}", flush=True)
+    ref, diag, got = assemble_both(c, oracle_lib)
+    assert np.array_equal(got["colidx"], ref.colidx)
+    for key in ("L", "M", "A", "B"):
+        assert rel_err(got[key], getattr(ref, key)) <= 1e-12, key
+    assert diag["incoming_modes"] == int(ref.bc_diag[2])
+
+
+def test_solve_with_a_generic_stencil_pair(oracle_lib):
+    """The eigen-solve on a 9 x 5 stencil case: the 8 eigenvalues nearest the
+    target equal ARPACK's (SuperLU shift-invert) on the oracle matrices."""
+    import scipy.sparse.linalg as spla
+    c = small_case(24, 12, q_y=8, q_z=4)
+    ref = oracle_lib.assemble(c)
+    n = c.ny * c.nz
+    A = oracle_lib.to_scipy(ref.rowptr, ref.colidx, ref.A, n).tocsc()
+    B = oracle_lib.block_diag_to_scipy(ref.B).tocsc()
+    lu = spla.splu(A.tocsc())
+    op = spla.LinearOperator(A.shape, matvec=lambda v: lu.solve(B @ v), dtype=np.complex128)
+    theta = spla.eigs(op, k=10, which="LM", tol=1e-14, return_eigenvectors=False)
+    lam_ref = 1.0 / theta
+    lam_ref = lam_ref[np.argsort(np.abs(lam_ref))][:8]
+    with ta.Context(0) as ctx:
+        ctx.load_case(c)
+        ctx.assemble(diagnostics=False)
+        d = ctx.solve(nev=8, omega=0j, tol=1e-10)
+        pairs = ctx.eigenpairs()[:8]
+    lam = np.array([p["lambda"] for p in pairs])
+    assert d["converged"] >= 8
+    for l in lam_ref:
+        assert np.min(np.abs(lam - l)) <= 1e-8 * abs(l), l
