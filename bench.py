@@ -418,6 +418,60 @@ def phase_mixed_size(ta, count, ncv, contexts) -> dict:
                             "ncv": ncv, "contexts": contexts, "per_size": per_size}}
 
 
+def phase_batch(ta, count: int = 4) -> dict:
+    """Factorization throughput of several 200x100 instances (headline
+    alphas, per-instance base flows, their own shifts): one after another on
+    their contexts, every context's graph on its own stream at once
+    (tailor_factor_batch), and joined into one launch sequence over their
+    joined elimination levels replayed from a HIP graph (tailor_batch); then
+    the same joined launch over two 200x100 and two 100x50 instances.  A
+    solve on each context after the joined factorization is compared with
+    the same solve after its own factorization."""
+    from tailor_amd import sweep as sw
+    insts = sw.sweep_instances(count, seed=1, sizes=[(200, 100)], reynolds=1500.0, mach=0.7)
+    mixed = sw.sweep_instances(count, seed=1, sizes=[(200, 100), (100, 50)], reynolds=1500.0,
+                               mach=0.7)
+    out = {}
+    rng = np.random.default_rng(5)
+    for name, group in (("200x100", insts), ("200x100+100x50", mixed)):
+        cases = sw.prepare(group, OMEGA)
+        # targets omega_s = 0.02 + 0.001 k: sigma = Im(omega_s) - i Re(omega_s)
+        sig = [complex(0.0, -(0.02 + 0.001 * k)) for k in range(count)]
+        ctxs = []
+        for _inst, c in cases:
+            ctx = ta.Context(0)
+            ctx.load_case(c)
+            ctx.assemble(diagnostics=False)
+            ctx.autotune(sig[0])
+            ctxs.append(ctx)
+        bs = [rng.standard_normal(5 * c.nodes) + 1j * rng.standard_normal(5 * c.nodes)
+              for _i, c in cases]
+        t = 0.0
+        for _rep in range(2):  # the second pass is timed
+            t0 = time.perf_counter()
+            for ctx, s_ in zip(ctxs, sig):
+                ctx.factor(s_)
+            t = time.perf_counter() - t0
+        own = [ctx.linear_solve(b)[0] for ctx, b in zip(ctxs, bs)]
+        conc = [ta.factor_batch(ctxs, sig) for _ in range(3)][-1]
+        batch = ta.Batch(ctxs)
+        eager = batch.factor(sig, graph=False)
+        joined = [batch.factor(sig, graph=True) for _ in range(3)][-1]
+        diffs = [float(np.linalg.norm(ctx.linear_solve(b)[0] - x) / np.linalg.norm(x))
+                 for ctx, b, x in zip(ctxs, bs, own)]
+        batch.close()
+        for ctx in ctxs:
+            ctx.close()
+        out[name] = {"instances": count, "own_ms_per_instance": round(1e3 * t / count, 2),
+                     "concurrent_graphs_ms_per_instance": round(conc / count, 2),
+                     "joined_eager_ms_per_instance": round(eager / count, 2),
+                     "joined_graph_ms_per_instance": round(joined / count, 2),
+                     "solve_rel_diff_vs_own": max(diffs)}
+        log(f"joined factorization {name}: {joined / count:.1f} ms per instance "
+            f"(own {1e3 * t / count:.1f}, concurrent graphs {conc / count:.1f})")
+    return {"batch": out}
+
+
 def run_phase(name: str, args, extra=()) -> dict:
     """One measurement phase in a child process (a fresh HIP runtime: the
     runtime keeps the hardware queues of every stream a process has created,
@@ -448,6 +502,8 @@ def phase_main(args) -> int:
                              args.contexts)
     elif args.phase == "reference":
         out = phase_reference(ta, args.steps, args.warmup)
+    elif args.phase == "batch":
+        out = phase_batch(ta)
     elif args.phase == "mixed_size":
         out = phase_mixed_size(ta, args.mixed_count, args.best_ncv, args.contexts)
     else:
@@ -491,6 +547,7 @@ def main() -> int:
     msz = run_phase("mixed_size", args, ("--best-ncv", str(best_ncv),
                                          "--mixed-count", str(mixed_count),
                                          "--contexts", str(min(args.contexts, 4))))
+    bt = run_phase("batch", args)
     L, F, PB, PR = pipe["learned"], pipe["fixed"], pipe["plain_best"], ref["plain_ref"]
     rate = lambda t: t["eigenpairs_per_s"]
     strongest = max(rate(PB), rate(PR))
@@ -550,6 +607,7 @@ def main() -> int:
         "spmv": single["spmv"],
         "mixed_precision": single["mixed"],
         "mixed_size_sweep": msz["mixed_sweep"],
+        "joined_factorization": bt["batch"],
         "gpu_100x50_plain": {k: v for k, v in small.items() if k != "lambdas"},
     }
     tw = traced_gemm_window()

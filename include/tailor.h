@@ -219,6 +219,44 @@ int tailor_set_gemm_config(tailor_ctx *ctx, int32_t config);
 #define TAILOR_PANEL_STREAMING 1
 int tailor_set_panel_kernel(tailor_ctx *ctx, int32_t kernel);
 
+/* ------------------------------------------------------------------------
+ * Batches: the factorizations of several contexts as one launch sequence.
+ * A batch joins the fronts of `count` distinct contexts of one device (each
+ * with a case loaded; any grid sizes) into one elimination forest whose
+ * level l holds level l of every context (levels counted from the leaves).
+ * Each launch of a level (extend-add, panels, pivot-block inverses, row
+ * exchanges, frontal GEMMs, Z) covers the fronts of all contexts, so the
+ * latency-bound stretches of one instance run beside the others' and a
+ * batch takes as many launches as its deepest context alone.  The factors
+ * land in each context's own front storage, as after tailor_factor, and its
+ * solves and tailor_solve use them.  The batch refers to its contexts: they
+ * must outlive it, keep the structure they had when it was made (a case of
+ * another structure needs a new batch), share one front precision at each
+ * factorization, and not be used by another thread during a batch call.
+ * Errors are reported by tailor_batch_last_error.  No reference counterpart
+ * (wavenumber sweeps; the reference runs one case per process). */
+typedef struct tailor_batch tailor_batch;
+int tailor_batch_create(tailor_batch **batch, tailor_ctx *const *ctxs, int32_t count);
+void tailor_batch_destroy(tailor_batch *batch);
+const char *tailor_batch_last_error(const tailor_batch *batch);
+/* Factor every context k at sigmas[2k] + i sigmas[2k+1] (A_bc, B_bc
+ * assembled), after the work already queued on the contexts' streams.
+ * use_graph = 1: the launch sequence (per context the norms of A_bc and
+ * B_bc, the pivot thresholds and the scatter of K into the fronts, then the
+ * joined levels) is captured as a HIP graph at the first call, and again when
+ * the front precision or the tile choice changed, and replayed; 0: the same
+ * launches directly.  Returns when the factors are complete; *batch_ms (may
+ * be NULL): the batch's time on its stream. */
+int tailor_batch_factor(tailor_batch *batch, const double *sigmas, int32_t use_graph,
+                        double *batch_ms);
+/* Autotuning of the batch's frontal GEMM tiles (as tailor_autotune, with
+ * every context at sigma), kept for its later factorizations; until then the
+ * batch uses the choice of its deepest context.  tailor_batch_set_gemm_config
+ * forces configuration c (0..7) on every level. */
+int tailor_batch_autotune(tailor_batch *batch, double sigma_re, double sigma_im,
+                          tailor_tune_report *report);
+int tailor_batch_set_gemm_config(tailor_batch *batch, int32_t config);
+
 /* Precision of the factorization used by tailor_factor / tailor_solve.
  * FP64: complex FP64 fronts (the plain form).  MIXED: complex FP32 fronts
  * factored on FP32 MFMA, with FP64 right-hand sides, FP64 residuals and

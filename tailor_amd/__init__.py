@@ -154,6 +154,14 @@ def library() -> C.CDLL:
                                              P(C.c_double), P(C.c_double)]
     # device pointers (caller-owned device memory), no device allocation
     lib.tailor_linear_solve_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
+    lib.tailor_batch_create.argtypes = [P(C.c_void_p), P(C.c_void_p), C.c_int32]
+    lib.tailor_batch_destroy.argtypes = [C.c_void_p]
+    lib.tailor_batch_destroy.restype = None
+    lib.tailor_batch_last_error.argtypes = [C.c_void_p]
+    lib.tailor_batch_last_error.restype = C.c_char_p
+    lib.tailor_batch_factor.argtypes = [C.c_void_p, P(C.c_double), C.c_int32, P(C.c_double)]
+    lib.tailor_batch_autotune.argtypes = [C.c_void_p, C.c_double, C.c_double, P(TuneReport)]
+    lib.tailor_batch_set_gemm_config.argtypes = [C.c_void_p, C.c_int32]
     lib.tailor_eigenpair_count.argtypes = [C.c_void_p]
     lib.tailor_get_eigenpair.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double),
                                          P(C.c_double), P(C.c_double)]
@@ -335,6 +343,69 @@ def factor_batch(contexts, sigmas) -> float:
         msgs = [lib.tailor_last_error(c.ptr).decode() for c in contexts]
         raise TailorError(st, "; ".join(m for m in msgs if m) or "tailor_factor_batch failed")
     return ms.value
+
+
+class Batch:
+    """The factorizations of several contexts (one device, any grid sizes) as
+    one launch sequence over their joined elimination levels
+    (tailor_batch_*); each context then holds its own factors, as after
+    Context.factor, for its solves and Krylov-Schur."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        if not self.contexts:
+            raise ValueError("a batch needs at least one context")
+        self.lib = self.contexts[0].lib
+        self.ptr = C.c_void_p()
+        ptrs = (C.c_void_p * len(self.contexts))(*[c.ptr.value for c in self.contexts])
+        st = self.lib.tailor_batch_create(C.byref(self.ptr), ptrs, len(self.contexts))
+        if st:
+            raise TailorError(st, "tailor_batch_create failed (contexts must be distinct, "
+                                  "on one device, each with a case loaded)")
+
+    def close(self):
+        if self.ptr:
+            self.lib.tailor_batch_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st: int):
+        if st:
+            raise TailorError(st, self.lib.tailor_batch_last_error(self.ptr).decode())
+
+    def factor(self, sigmas, graph: bool = True) -> float:
+        """Factor context k at sigmas[k]; returns the batch's time in ms."""
+        sigmas = list(sigmas)
+        if len(sigmas) != len(self.contexts):
+            raise ValueError("one shift per context")
+        sg = np.array([[complex(x).real, complex(x).imag] for x in sigmas],
+                      dtype=np.float64).ravel()
+        ms = C.c_double()
+        self._check(self.lib.tailor_batch_factor(self.ptr, _dptr(sg), 1 if graph else 0,
+                                                 C.byref(ms)))
+        return ms.value
+
+    def autotune(self, sigma: complex) -> dict:
+        sigma = complex(sigma)
+        r = TuneReport()
+        self._check(self.lib.tailor_batch_autotune(self.ptr, sigma.real, sigma.imag, C.byref(r)))
+        return {"levels": r.levels, "config": list(r.order)[:r.levels],
+                "gemm_ms": [round(x, 3) for x in list(r.gemm_ms)],
+                "tuned_gemm_ms": r.tuned_gemm_ms}
+
+    def set_gemm_config(self, config: int) -> None:
+        self._check(self.lib.tailor_batch_set_gemm_config(self.ptr, int(config)))
 
 
 def _linear_solve(self, b: np.ndarray, refine_max: int = 0, refine_tol: float = 1e-14):

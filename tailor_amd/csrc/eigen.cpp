@@ -68,6 +68,7 @@ void release_graphs(SolverState *S) {
 }
 
 std::atomic<unsigned long long> g_next_graph_id{1};
+std::atomic<unsigned long long> g_next_plan_id{1};
 
 }  // namespace
 
@@ -106,6 +107,7 @@ int solver_prepare(tailor_ctx *ctx) {
   HIP_OK(hipMemcpy(S.d_pos, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice));
   S.n = 5LL * d.nodes;
   S.symbolic_ms = now_ms() - t0;
+  S.plan_id = g_next_plan_id.fetch_add(1);
   S.ready = true;
   return TAILOR_OK;
 }
@@ -218,6 +220,15 @@ int factor_done(tailor_ctx *ctx, cplx sigma, double ms) {
 }
 
 }  // namespace
+
+void solver_mark_factored(tailor_ctx *ctx, cplx sigma, bool fp32, int perturbed, double ms) {
+  SolverState &S = *ctx->solver;
+  S.perturbed = perturbed;
+  S.factor_ms = ms;
+  S.sigma = sigma;
+  S.fp32 = fp32;
+  S.factored = true;
+}
 
 int solver_factor_graph(tailor_ctx *ctx, cplx sigma) {
   if (int st = factor_checks(ctx)) return st;

@@ -99,19 +99,22 @@ __global__ void k_init_identity(const FrontDev *__restrict__ fr, int nf, T *__re
 }
 
 // Scatter K = A - sigma B into the fronts.  Thread per (block, entry).
-// amap[3k..3k+2] = front, local row node, local col node of BSR block k.
+// amap[3k..3k+2] = front, local row node, local col node of BSR block k;
+// the front is fbase + amap[3k] of fr (fbase: the part's first front in a
+// batch plan, else 0) and fp the part's parameter block.
 // K is formed in FP64 and rounded once to the front precision T.
 template <typename T>
 __global__ void k_assemble_fronts(long long nnzb, int per, const double *__restrict__ avals,
                                   const int *__restrict__ colidx, const double *__restrict__ bvals,
                                   const double *__restrict__ fp, const int *__restrict__ amap,
-                                  const FrontDev *__restrict__ fr, T *__restrict__ arena) {
+                                  const FrontDev *__restrict__ fr, int fbase,
+                                  T *__restrict__ arena) {
   const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= nnzb * 25) return;
   const double sig_re = fp[kFpSigRe], sig_im = fp[kFpSigIm];
   const long long k = t / 25;
   const int e = static_cast<int>(t - 25 * k);
-  const int f = amap[3 * k], lu = amap[3 * k + 1], lv = amap[3 * k + 2];
+  const int f = fbase + amap[3 * k], lu = amap[3 * k + 1], lv = amap[3 * k + 2];
   const FrontDev d = fr[f];
   const int r = e / 5, c = e - 5 * (e / 5);
   double vr = avals[2 * (k * 25 + e)], vi = avals[2 * (k * 25 + e) + 1];
@@ -244,7 +247,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__
   const int f = list[blockIdx.x];
   const FrontDev d = fr[f];
   if (k0 >= d.p) return;
-  const double tiny = fp[kFpTiny], perturb = fp[kFpPerturb];
+  const double tiny = fp[d.inst * kFpCount + kFpTiny], perturb = fp[d.inst * kFpCount + kFpPerturb];
   const int kb = min(nb, d.p - k0);
   const int K1 = min(K0 + NB, d.p);
   const long long mm = static_cast<long long>(d.m) * d.m;
@@ -318,7 +321,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int *__restrict__
         pi = 0.0;
         Fr[cj + j] = static_cast<T>(pr);
         Fi[cj + j] = static_cast<T>(pi);
-        atomicAdd(nperturbed, 1);
+        atomicAdd(nperturbed + d.inst, 1);
       }
       const double den = pr * pr + pi * pi;
       inv_r = static_cast<T>(pr / den);
@@ -435,7 +438,7 @@ __global__ __launch_bounds__(NT) void k_panel_regs(
   const int f = list[blockIdx.x];
   const FrontDev d = fr[f];
   if (k0 >= d.p) return;
-  const double tiny = fp[kFpTiny], perturb = fp[kFpPerturb];
+  const double tiny = fp[d.inst * kFpCount + kFpTiny], perturb = fp[d.inst * kFpCount + kFpPerturb];
   const int kb = min(nb, d.p - k0);
   const int K1 = min(K0 + NB, d.p);
   const int p = d.p;
@@ -626,7 +629,7 @@ __global__ __launch_bounds__(NT) void k_panel_regs(
             phys[j - k0] = phys[piv - k0];
             phys[piv - k0] = t;
           }
-          if (small) atomicAdd(nperturbed, 1);
+          if (small) atomicAdd(nperturbed + d.inst, 1);
         }
         T pr = pr_[u], pi = pi_[u];
         if (small) {
@@ -1980,24 +1983,36 @@ hipError_t lu_fill(hipStream_t s, double *p, long long n, double v) {
   return hipGetLastError();
 }
 
+// The zeroed fronts of one plan `part` and the scatter of its K into them:
+// fr is the front table the factorization runs on (the part's own, or a
+// batch's, where the part's fronts start at fbase), fp the part's parameter
+// block and arena the table's base.
 template <typename T>
-static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, hipStream_t s,
-                              FactorTiming *tm) {
+static hipError_t factor_scatter(const LuPlan &part, const LuInputs &in, const FrontDev *fr,
+                                 int fbase, const double *fp, T *arena, hipStream_t s) {
+  hipError_t e;
+  // zero the planes in use (the arena is sized for FP64 fronts)
+  const long long used = (part.arena_doubles * static_cast<long long>(sizeof(T)) + 7) / 8;
+  if ((e = lu_fill(s, part.arena, used, 0.0)) != hipSuccess) return e;
+  const long long threads = in.nnzb * 25;
+  hipLaunchKernelGGL(k_assemble_fronts<T>, dim3(static_cast<unsigned>((threads + 255) / 256)),
+                     dim3(256), 0, s, in.nnzb, in.per, in.avals, in.colidx, in.bvals, fp,
+                     part.d_amap, fr, fbase, arena);
+  return hipGetLastError();
+}
+
+// E = I, Z = I and the pivot counters, then the blocked LU of every front of
+// P level by level (P: a context's plan or a batch plan; the fronts hold K).
+template <typename T>
+static hipError_t factor_levels(const LuPlan &P, hipStream_t s, FactorTiming *tm) {
   hipError_t e;
   std::vector<hipEvent_t> events;  // start/stop pairs around GEMM launches when timing
   std::vector<int> event_level;    // level of each pair
   T *arena = reinterpret_cast<T *>(P.arena);
-  // zero the planes in use (the arena is sized for FP64 fronts)
-  const long long used = (P.arena_doubles * static_cast<long long>(sizeof(T)) + 7) / 8;
-  if ((e = lu_fill(s, P.arena, used, 0.0)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_init_identity<T>, dim3(8, P.nsn), dim3(256), 0, s, P.d_front, P.nsn, arena);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  const long long threads = in.nnzb * 25;
-  hipLaunchKernelGGL(k_assemble_fronts<T>, dim3(static_cast<unsigned>((threads + 255) / 256)),
-                     dim3(256), 0, s, in.nnzb, in.per, in.avals, in.colidx, in.bvals,
-                     P.d_fparams, P.d_amap, P.d_front, arena);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, s, reinterpret_cast<double *>(P.d_counters), 1LL, 0.0);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, reinterpret_cast<double *>(P.d_counters),
+                     static_cast<long long>((P.instances + 1) / 2), 0.0);
   const int nb = P.nb, NB = kLuNbOuter;
   T *dinv = reinterpret_cast<T *>(P.dinv);
   for (size_t lev = 0; lev < P.levels.size(); ++lev) {
@@ -2139,26 +2154,72 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, hipStream_t s
   return hipSuccess;
 }
 
+template <typename T>
+static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, hipStream_t s,
+                              FactorTiming *tm) {
+  hipError_t e = factor_scatter<T>(P, in, P.d_front, 0, P.d_fparams,
+                                   reinterpret_cast<T *>(P.arena), s);
+  if (e != hipSuccess) return e;
+  return factor_levels<T>(P, s, tm);
+}
+
 hipError_t lu_factor(const LuPlan &P, const LuInputs &in, bool fp32, hipStream_t s,
                      FactorTiming *tm) {
   return fp32 ? factor_impl<float>(P, in, s, tm) : factor_impl<double>(P, in, s, tm);
 }
 
-hipError_t lu_set_sigma(const LuPlan &P, double sig_re, double sig_im, hipStream_t s) {
-  hipLaunchKernelGGL(k_set_sigma, dim3(1), dim3(1), 0, s, P.d_fparams, sig_re, sig_im);
+hipError_t lu_set_sigma(const LuPlan &P, double sig_re, double sig_im, hipStream_t s, int inst) {
+  if (inst < 0 || inst >= P.instances) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_set_sigma, dim3(1), dim3(1), 0, s, P.d_fparams + inst * kFpCount, sig_re,
+                     sig_im);
   return hipGetLastError();
 }
 
-hipError_t lu_factor_params(const LuPlan &P, const LuInputs &in, bool fp32, double *partials,
-                            hipStream_t s) {
+// Norms of A_bc and B_bc into `partials`, then the pivot threshold and
+// perturbation into the parameter block fp (its shift already set).
+static hipError_t factor_params_at(double *fp, const LuInputs &in, bool fp32, double *partials,
+                                   hipStream_t s) {
   const long long nodes = in.nnzb / in.per;
   hipLaunchKernelGGL(k_sumsq_flat, dim3(kParamBlocks), dim3(256), 0, s, in.avals, in.nnzb * 50,
                      partials);
   hipLaunchKernelGGL(k_sumsq_flat, dim3(kParamBlocks), dim3(256), 0, s, in.bvals, nodes * 50,
                      partials + kParamBlocks);
   hipLaunchKernelGGL(k_factor_params, dim3(1), dim3(1), 0, s, partials, 5.0 * nodes, fp32 ? 1 : 0,
-                     P.d_fparams);
+                     fp);
   return hipGetLastError();
+}
+
+hipError_t lu_factor_params(const LuPlan &P, const LuInputs &in, bool fp32, double *partials,
+                            hipStream_t s) {
+  return factor_params_at(P.d_fparams, in, fp32, partials, s);
+}
+
+template <typename T>
+static hipError_t factor_batch_impl(const LuPlan &U, const std::vector<BatchPart> &parts,
+                                    const std::vector<int> &fbase,
+                                    const std::vector<LuInputs> &in,
+                                    const std::vector<double *> &partials, bool fp32,
+                                    hipStream_t s, FactorTiming *tm) {
+  T *arena = reinterpret_cast<T *>(U.arena);
+  for (size_t b = 0; b < parts.size(); ++b) {
+    double *fp = U.d_fparams + b * kFpCount;
+    hipError_t e = factor_params_at(fp, in[b], fp32, partials[b], s);
+    if (e == hipSuccess) e = factor_scatter<T>(*parts[b].plan, in[b], U.d_front, fbase[b], fp, arena, s);
+    if (e != hipSuccess) return e;
+  }
+  return factor_levels<T>(U, s, tm);
+}
+
+hipError_t lu_factor_batch(const LuPlan &U, const std::vector<BatchPart> &parts,
+                           const std::vector<int> &fbase, const std::vector<LuInputs> &in,
+                           const std::vector<double *> &partials, bool fp32, hipStream_t s,
+                           FactorTiming *tm) {
+  const size_t nb = parts.size();
+  if (nb == 0 || static_cast<int>(nb) != U.instances || fbase.size() != nb || in.size() != nb ||
+      partials.size() != nb)
+    return hipErrorInvalidValue;
+  return fp32 ? factor_batch_impl<float>(U, parts, fbase, in, partials, true, s, tm)
+              : factor_batch_impl<double>(U, parts, fbase, in, partials, false, s, tm);
 }
 
 // x = K^-1 b for vectors in elimination order (split planes).  b may alias x.

@@ -18,6 +18,7 @@ struct FrontDev {
   int piv_off;     // offset into the pivot array
   int dof0;        // first elimination-order dof of the front's pivots
   int dinv_off;    // first panel slot of the front in its level's inverse workspace
+  int inst;        // instance of a batch (its parameter block and pivot counter); 0 alone
 };
 
 struct LevelPlan {
@@ -54,8 +55,10 @@ struct LuPlan {
   int *d_ipiv = nullptr;
   int *d_counters = nullptr;    // [0] perturbed pivots
   int *d_perm = nullptr;        // elimination dof -> natural dof (n entries)
-  double *d_fparams = nullptr;  // kFpCount doubles: sigma and the pivot thresholds
+  double *d_fparams = nullptr;  // kFpCount doubles per instance: sigma and the pivot thresholds
   double *arena = nullptr;
+  bool owns_arena = true;       // false for a batch plan, whose fronts live in its parts' arenas
+  int instances = 1;            // parameter blocks / pivot counters (one per batch part)
   hipStream_t side = nullptr;   // look-ahead panels (default priority)
   hipEvent_t ev_head = nullptr, ev_panel = nullptr;
   double *dinv = nullptr;       // inverted pivot blocks: dinv_slots panel slots
@@ -91,6 +94,21 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
                           LuPlan &P, hipStream_t s);
 void lu_plan_destroy(LuPlan &P);
 
+// A batch plan: the fronts of several plans (any structures) as one
+// elimination forest, level l holding level l of every part (levels count
+// from the leaves), so that each launch of a level covers the fronts of all
+// parts.  Fronts stay in their parts' arenas: a batch front's offset is taken
+// from the lowest part arena in units of the front type (FP32 fronts when
+// fp32), which the flat device address space makes exact.  Front f of part b
+// is batch front fbase[b] + f; its `inst` is b.  The batch owns its pivot,
+// inverse and permutation workspaces, parameter blocks and look-ahead stream.
+struct BatchPart {
+  const LuPlan *plan;
+  const Symbolic *sym;
+};
+hipError_t lu_plan_batch(const std::vector<BatchPart> &parts, bool fp32, LuPlan &U,
+                         std::vector<int> &fbase, hipStream_t s);
+
 hipError_t lu_fill(hipStream_t s, double *p, long long n, double v);
 // Per-kernel timing of one factorization (HIP events around every frontal
 // GEMM launch) and the algorithmic flops of those launches (8 M N K per
@@ -107,8 +125,10 @@ struct FactorTiming {
 enum { kOrderLinear = 0, kOrderXcdGrouped = 1, kOrderCount = 2 };
 constexpr int kGemmShapes = 4;
 
-// The shift sigma of the next factorization into the parameter block.
-hipError_t lu_set_sigma(const LuPlan &P, double sig_re, double sig_im, hipStream_t s);
+// The shift sigma of the next factorization into the parameter block
+// (instance `inst` of a batch plan).
+hipError_t lu_set_sigma(const LuPlan &P, double sig_re, double sig_im, hipStream_t s,
+                        int inst = 0);
 // The pivot threshold and perturbation from sigma (already in the block) and
 // the Frobenius norms of A_bc and B_bc, on the device (`partials`: 1024
 // doubles of scratch).
@@ -122,6 +142,16 @@ hipError_t lu_factor_params(const LuPlan &P, const LuInputs &in, bool fp32, doub
 // captured in a HIP graph.
 hipError_t lu_factor(const LuPlan &P, const LuInputs &in, bool fp32, hipStream_t s,
                      FactorTiming *timing = nullptr);
+// One factorization of every part of a batch plan U (lu_plan_batch with the
+// same fp32) at the shifts already in U's parameter blocks: per part the
+// norms of A_bc and B_bc, the pivot thresholds, the zeroed fronts and the
+// scatter of K into them (in[b], partials[b]: 1024 doubles of scratch per
+// part), then one launch sequence over the levels of U.  As lu_factor, a
+// pure launch sequence on s and U's side stream without `timing`.
+hipError_t lu_factor_batch(const LuPlan &U, const std::vector<BatchPart> &parts,
+                           const std::vector<int> &fbase, const std::vector<LuInputs> &in,
+                           const std::vector<double *> &partials, bool fp32, hipStream_t s,
+                           FactorTiming *timing = nullptr);
 // x = K^-1 b, vectors in elimination dof order, split planes (re, im)
 hipError_t lu_solve(const LuPlan &P, const double *br, const double *bi, double *xr, double *xi,
                     bool fp32, hipStream_t s);

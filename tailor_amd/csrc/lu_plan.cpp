@@ -45,6 +45,7 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
     d.piv_off = poff;
     d.dof0 = 5 * S.sn_start[f];
     d.dinv_off = 0;
+    d.inst = 0;
     off += 2LL * d.m * d.m + 4LL * d.p * d.p;
     woff += d.m;
     poff += d.p;
@@ -154,6 +155,127 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
   return hipStreamSynchronize(s);
 }
 
+hipError_t lu_plan_batch(const std::vector<BatchPart> &parts, bool fp32, LuPlan &U,
+                         std::vector<int> &fbase, hipStream_t s) {
+  lu_plan_destroy(U);
+  if (parts.empty()) return hipErrorInvalidValue;
+  const size_t unit = fp32 ? sizeof(float) : sizeof(double);
+  const char *base = nullptr;
+  size_t nlev = 0, deepest = 0;
+  for (size_t b = 0; b < parts.size(); ++b) {
+    const LuPlan &P = *parts[b].plan;
+    if (!P.arena || !parts[b].sym || parts[b].sym->nsn != P.nsn) return hipErrorInvalidValue;
+    const char *a = reinterpret_cast<const char *>(P.arena);
+    if (!base || a < base) base = a;
+    if (P.levels.size() > nlev) {
+      nlev = P.levels.size();
+      deepest = b;
+    }
+  }
+  // fronts: offsets from the lowest arena in front-type units, pivot offsets
+  // after the previous parts' pivots, the part as instance
+  std::vector<FrontDev> fr;
+  std::vector<int> parent, rel;
+  std::vector<long long> rel_off;
+  fbase.assign(parts.size(), 0);
+  int poff = 0;
+  for (size_t b = 0; b < parts.size(); ++b) {
+    const LuPlan &P = *parts[b].plan;
+    const Symbolic &S = *parts[b].sym;
+    const long long bytes = reinterpret_cast<const char *>(P.arena) - base;
+    if (bytes % static_cast<long long>(unit) != 0) return hipErrorInvalidValue;
+    const long long delta = bytes / static_cast<long long>(unit);
+    const int fb = static_cast<int>(fr.size());
+    fbase[b] = fb;
+    for (int f = 0; f < P.nsn; ++f) {
+      FrontDev d = P.h_front[f];
+      d.off += delta;
+      d.piv_off += poff;
+      d.inst = static_cast<int>(b);
+      fr.push_back(d);
+      parent.push_back(S.parent[f] < 0 ? -1 : S.parent[f] + fb);
+      rel_off.push_back(static_cast<long long>(rel.size()));
+      rel.insert(rel.end(), S.rel[f].begin(), S.rel[f].end());
+    }
+    for (int f = 0; f < P.nsn; ++f) poff += P.h_front[f].p;
+  }
+  // level l of the batch: level l of every part, part by part
+  std::vector<int> level_list, child_slots;
+  U.levels.assign(nlev, {});
+  for (size_t l = 0; l < nlev; ++l) {
+    LevelPlan &L = U.levels[l];
+    L.list_off = static_cast<int>(level_list.size());
+    for (size_t b = 0; b < parts.size(); ++b) {
+      const LuPlan &P = *parts[b].plan;
+      if (l >= P.levels.size()) continue;
+      const LevelPlan &Lb = P.levels[l];
+      for (int q = 0; q < Lb.count; ++q)
+        level_list.push_back(P.h_level_list[Lb.list_off + q] + fbase[b]);
+    }
+    L.count = static_cast<int>(level_list.size()) - L.list_off;
+    long long slots = 0;
+    for (int q = L.list_off; q < L.list_off + L.count; ++q) {
+      FrontDev &d = fr[level_list[q]];
+      L.pmax = std::max(L.pmax, d.p);
+      L.cmax = std::max(L.cmax, d.c);
+      L.mmax = std::max(L.mmax, d.m);
+      d.dinv_off = static_cast<int>(slots);
+      slots += (d.p + kLuNb - 1) / kLuNb;
+    }
+    for (int slot = 0; slot < 2; ++slot) {
+      L.child_slot_off[slot] = static_cast<int>(child_slots.size());
+      for (size_t b = 0; b < parts.size(); ++b) {
+        const LuPlan &P = *parts[b].plan;
+        const Symbolic &S = *parts[b].sym;
+        if (l >= P.levels.size()) continue;
+        for (int f : S.levels[l])
+          if (static_cast<int>(S.children[f].size()) > slot)
+            child_slots.push_back(S.children[f][slot] + fbase[b]);
+      }
+      L.child_slot_count[slot] = static_cast<int>(child_slots.size()) - L.child_slot_off[slot];
+    }
+    U.dinv_slots = std::max(U.dinv_slots, slots);
+    U.count_max = std::max(U.count_max, L.count);
+  }
+  U.nsn = static_cast<int>(fr.size());
+  U.nb = kLuNb;
+  U.instances = static_cast<int>(parts.size());
+  U.gemm_shape = parts[deepest].plan->gemm_shape;
+  U.gemm_order = parts[deepest].plan->gemm_order;
+  U.panel_streaming = parts[deepest].plan->panel_streaming;
+  U.h_front = fr;
+  U.h_level_list = level_list;
+  U.arena = reinterpret_cast<double *>(const_cast<char *>(base));
+  U.owns_arena = false;
+
+  hipError_t e;
+  if ((e = upload(&U.d_front, fr, s)) != hipSuccess) return e;
+  if ((e = upload(&U.d_level_list, level_list, s)) != hipSuccess) return e;
+  if ((e = upload(&U.d_child_slots, child_slots, s)) != hipSuccess) return e;
+  if ((e = upload(&U.d_parent, parent, s)) != hipSuccess) return e;
+  if ((e = upload(&U.d_rel, rel, s)) != hipSuccess) return e;
+  if ((e = upload(&U.d_rel_off, rel_off, s)) != hipSuccess) return e;
+  if ((e = upload(&U.d_ipiv, std::vector<int>(std::max(1, poff), 0), s)) != hipSuccess) return e;
+  if ((e = upload(&U.d_counters, std::vector<int>(std::max<size_t>(8, parts.size()), 0), s)) !=
+      hipSuccess)
+    return e;
+  if ((e = upload(&U.d_fparams, std::vector<double>(kFpCount * parts.size(), 0.0), s)) !=
+      hipSuccess)
+    return e;
+  if ((e = hipMalloc(reinterpret_cast<void **>(&U.dinv), std::max<long long>(1, U.dinv_slots) *
+                                                             kLuInvSlot * sizeof(double))) !=
+      hipSuccess)
+    return e;
+  if ((e = hipMalloc(reinterpret_cast<void **>(&U.d_bperm),
+                     std::max(1, U.count_max) * static_cast<size_t>(kLuPermStride) * sizeof(int))) !=
+      hipSuccess)
+    return e;
+  if ((e = hipStreamCreateWithFlags(&U.side, hipStreamNonBlocking)) != hipSuccess) return e;
+  if ((e = hipEventCreateWithFlags(&U.ev_head, hipEventDisableTiming)) != hipSuccess) return e;
+  if ((e = hipEventCreateWithFlags(&U.ev_panel, hipEventDisableTiming)) != hipSuccess) return e;
+  return hipStreamSynchronize(s);
+}
+
 void lu_plan_destroy(LuPlan &P) {
   if (P.side) (void)hipStreamSynchronize(P.side);
   if (P.ev_head) (void)hipEventDestroy(P.ev_head);
@@ -176,7 +298,12 @@ void lu_plan_destroy(LuPlan &P) {
   release(P.d_counters);
   release(P.d_fparams);
   release(P.d_perm);
-  release(P.arena);
+  if (P.owns_arena)
+    release(P.arena);
+  else
+    P.arena = nullptr;
+  P.owns_arena = true;
+  P.instances = 1;
   release(P.W);
   release(P.dinv);
   release(P.d_bperm);

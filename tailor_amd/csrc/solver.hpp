@@ -13,6 +13,7 @@ namespace tailor {
 
 struct SolverState {
   bool ready = false;     // symbolic analysis + device plan built
+  unsigned long long plan_id = 0;  // unique over the process, set when the plan is built
   bool factored = false;
   bool fp32 = false;      // fronts held in complex FP32 (mixed precision)
   Symbolic sym;
@@ -51,6 +52,9 @@ int solver_factor_graph(tailor_ctx *ctx, cplx sigma);
 // graph launched on its own stream from this thread, so they run
 // concurrently; *ms: the batch's event time.
 int solver_factor_batch(tailor_ctx *const *ctxs, int count, const cplx *sigmas, double *ms);
+// ctx now holds the factors of K = A_bc - sigma B_bc made by a batch
+// factorization (batch.cpp) in its front storage.
+void solver_mark_factored(tailor_ctx *ctx, cplx sigma, bool fp32, int perturbed, double ms);
 // Factor once per GEMM tile order with per-level timing and keep, for each
 // elimination level, the order with the smallest GEMM time.
 int solver_autotune(tailor_ctx *ctx, cplx sigma, tailor_tune_report *report);

@@ -30,6 +30,16 @@ factorization's latency-bound panel phases.  Each context drives two
 streams, so the process needs at least 2 C hardware queues
 (GPU_MAX_HW_QUEUES, read when HIP initialises; HIP's default is 4), or
 unrelated streams wait for each other.
+
+With `batch=B > 0` every form runs through joined factorizations instead:
+the instances go in groups of B, each group on B contexts whose fronts are
+factored together by one launch sequence over their joined elimination
+levels (a tailor_batch, replayed from its HIP graph), so every level launch
+covers the fronts of the whole group.  Two groups of contexts alternate: a
+worker thread assembles and factors group j + 1 while the main thread runs
+group j's Krylov-Schur, one instance after another.  Groups may mix grid
+sizes; each context keeps one size, so warm starts still follow a context's
+previous instance.
 """
 from __future__ import annotations
 
@@ -40,7 +50,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import Context, PRECISION_FP64, PRECISION_MIXED
+from . import Batch, Context, PRECISION_FP64, PRECISION_MIXED
 from .case import Physics, SplitMix64, synthetic_case
 
 
@@ -134,7 +144,7 @@ _WARM_FORMS = ("learned", "fixed")
 class Sweeper:
     def __init__(self, device: int = 0, nev: int = 8, tol: float = 1e-10, ncv: int = 64,
                  omega_s: complex = 0.02 + 0j, maxit: int = 1000, contexts: int = 1,
-                 check_from: int = 0, graph: bool = False):
+                 check_from: int = 0, graph: bool = False, batch: int = 0):
         """`check_from` > 0: the learned form's Krylov-Schur runs also test
         convergence after every second step from that step on and end a pass
         early (the plain form and the fallback test only at pass ends).
@@ -145,6 +155,10 @@ class Sweeper:
         replays (33.6 against 45.7 eigenpairs/s), so eager is the default."""
         if not 1 <= contexts <= 8:
             raise ValueError("contexts must be 1 to 8")
+        if not 0 <= batch <= 8:
+            raise ValueError("batch must be 0 to 8")
+        self.batch = batch
+        self.batches = {}  # joined factorizations, by the contexts they join
         if check_from < 0:
             raise ValueError("check_from must be >= 0")
         self.check_from = check_from
@@ -157,6 +171,9 @@ class Sweeper:
         self.tuned = {}  # per grid size: the autotuner's report
 
     def close(self):
+        for b in self.batches.values():
+            b.close()
+        self.batches = {}
         for c in self.ctx.values():
             c.close()
         self.ctx = {}
@@ -219,6 +236,8 @@ class Sweeper:
         if form != "learned":
             shifts = [self.omega_s] * len(cases)
         idx = list(range(len(cases))) if order is None else list(order)
+        if self.batch > 0:
+            return self._run_batched(cases, form, shifts, precision, idx, on_record)
         if self.contexts > 1 and len(idx) > 1:
             return self._run_pipelined(cases, form, shifts, precision, idx, on_record)
         result = SweepResult(form)
@@ -338,6 +357,105 @@ class Sweeper:
                 on_record(rec)
         result.seconds = time.perf_counter() - t_all
         return result
+
+    def _krylov_record(self, ctx, inst, omega, form, precision, warm, t0):
+        """Krylov-Schur of one instance on a context that holds its
+        factorization, with the fixed-shift fallback of the warm forms."""
+        warm_form = form in _WARM_FORMS
+        d, om, res, good = self._solve(ctx, omega, precision, warm,
+                                       self.check_from if warm_form else 0, factored=True)
+        facts, fallback = 1, False
+        restarts, solves, refine = d["iterations"], d["solves"], d["refinement_steps"]
+        if warm_form and good < self.nev:
+            fallback = True
+            d, om, res, good = self._solve(ctx, self.omega_s, precision, False)
+            facts += 1
+            restarts += d["iterations"]
+            solves += d["solves"]
+            refine += d["refinement_steps"]
+            omega = self.omega_s
+        return Record(inst, omega, d["converged"], good, om, res, restarts, solves, refine,
+                      facts, fallback, time.perf_counter() - t0)
+
+    def _run_batched(self, cases, form, shifts, precision, idx, on_record) -> SweepResult:
+        """A form through joined factorizations (see the module docstring).
+        Instance k's record covers its Krylov stage and any fallback; the
+        sweep's time is the wall time of the whole run."""
+        B = self.batch
+        result = SweepResult(form)
+        warm_form = form in _WARM_FORMS
+        groups = [idx[j:j + B] for j in range(0, len(idx), B)]
+
+        def key(j, pos, k):
+            inst = cases[k][0]
+            return (inst.ny, inst.nz, (j % 2) * B + pos)
+
+        def group_contexts(j):
+            return [self.ctx[key(j, pos, k)] for pos, k in enumerate(groups[j])]
+
+        # plans, tuning and the joined factorizations' graphs first (untimed)
+        for j, g in enumerate(groups):
+            for pos, k in enumerate(g):
+                kk = key(j, pos, k)
+                if kk not in self.ctx:
+                    rep = self._tune(self.context(*kk), cases[k][1], precision)
+                    self.tuned.setdefault(kk[:2], rep)
+        for j in range(len(groups)):
+            ctxs = group_contexts(j)
+            bk = tuple(id(c) for c in ctxs)
+            if bk not in self.batches:
+                self.batches[bk] = Batch(ctxs)
+                for ctx, k in zip(ctxs, groups[j]):
+                    ctx.load_case(cases[k][1])
+                    ctx.assemble(diagnostics=False)
+                    ctx.set_precision(precision)
+                self.batches[bk].factor([complex(self.omega_s.imag, -self.omega_s.real)] * len(ctxs))
+        solved = set()  # contexts that hold Ritz vectors of an earlier instance
+        errors = []
+
+        def stage(j):
+            try:
+                ctxs = group_contexts(j)
+                for ctx, k in zip(ctxs, groups[j]):
+                    ctx.load_case(cases[k][1])
+                    ctx.assemble(diagnostics=False)
+                    ctx.set_precision(precision)
+                sig = [complex(complex(shifts[k]).imag, -complex(shifts[k]).real) for k in groups[j]]
+                self.batches[tuple(id(c) for c in ctxs)].factor(sig, graph=True)
+            except Exception as e:  # re-raised by the main thread
+                errors.append(e)
+
+        def start(j):
+            th = threading.Thread(target=stage, args=(j,))
+            th.start()
+            return th
+
+        t_all = time.perf_counter()
+        worker = start(0)
+        for j, g in enumerate(groups):
+            worker.join()
+            if errors:
+                raise errors[0]
+            # group j + 1 takes the contexts group j - 1 has finished with
+            worker = start(j + 1) if j + 1 < len(groups) else None
+            try:
+                for pos, k in enumerate(g):
+                    inst = cases[k][0]
+                    kk = key(j, pos, k)
+                    t0 = time.perf_counter()
+                    rec = self._krylov_record(self.ctx[kk], inst, complex(shifts[k]), form,
+                                              precision, warm_form and kk in solved, t0)
+                    solved.add(kk)
+                    result.records.append(rec)
+                    if on_record:
+                        on_record(rec)
+            except Exception:
+                if worker:
+                    worker.join()
+                raise
+        result.seconds = time.perf_counter() - t_all
+        return result
+
 
 def prepare(instances, omega_s: complex = 0.02 + 0j) -> list:
     """Host-side case construction (preprocessing, outside any timed region)."""
