@@ -249,6 +249,14 @@ const char *tailor_batch_last_error(const tailor_batch *batch);
  * be NULL): the batch's time on its stream. */
 int tailor_batch_factor(tailor_batch *batch, const double *sigmas, int32_t use_graph,
                         double *batch_ms);
+/* As tailor_batch_factor, with every context's assembly (tailor_assemble
+ * without diagnostics: A_bc and B_bc of the case and wavenumber it holds)
+ * launched first in the same sequence, so one graph replay assembles and
+ * factors the whole batch.  The contexts need a case loaded, not assembled.
+ * A context whose assembly fails validation is reported with the
+ * reference's message (its factors are not usable). */
+int tailor_batch_assemble_factor(tailor_batch *batch, const double *sigmas, int32_t use_graph,
+                                 double *batch_ms);
 /* Autotuning of the batch's frontal GEMM tiles (as tailor_autotune, with
  * every context at sigma), kept for its later factorizations; until then the
  * batch uses the choice of its deepest context.  tailor_batch_set_gemm_config

@@ -160,6 +160,8 @@ def library() -> C.CDLL:
     lib.tailor_batch_last_error.argtypes = [C.c_void_p]
     lib.tailor_batch_last_error.restype = C.c_char_p
     lib.tailor_batch_factor.argtypes = [C.c_void_p, P(C.c_double), C.c_int32, P(C.c_double)]
+    lib.tailor_batch_assemble_factor.argtypes = [C.c_void_p, P(C.c_double), C.c_int32,
+                                                 P(C.c_double)]
     lib.tailor_batch_autotune.argtypes = [C.c_void_p, C.c_double, C.c_double, P(TuneReport)]
     lib.tailor_batch_set_gemm_config.argtypes = [C.c_void_p, C.c_int32]
     lib.tailor_eigenpair_count.argtypes = [C.c_void_p]
@@ -384,17 +386,26 @@ class Batch:
         if st:
             raise TailorError(st, self.lib.tailor_batch_last_error(self.ptr).decode())
 
-    def factor(self, sigmas, graph: bool = True) -> float:
-        """Factor context k at sigmas[k]; returns the batch's time in ms."""
+    def _run(self, fn, sigmas, graph: bool) -> float:
         sigmas = list(sigmas)
         if len(sigmas) != len(self.contexts):
             raise ValueError("one shift per context")
         sg = np.array([[complex(x).real, complex(x).imag] for x in sigmas],
                       dtype=np.float64).ravel()
         ms = C.c_double()
-        self._check(self.lib.tailor_batch_factor(self.ptr, _dptr(sg), 1 if graph else 0,
-                                                 C.byref(ms)))
+        self._check(fn(self.ptr, _dptr(sg), 1 if graph else 0, C.byref(ms)))
         return ms.value
+
+    def factor(self, sigmas, graph: bool = True) -> float:
+        """Factor context k at sigmas[k] (each assembled); returns the
+        batch's time in ms."""
+        return self._run(self.lib.tailor_batch_factor, sigmas, graph)
+
+    def assemble_factor(self, sigmas, graph: bool = True) -> float:
+        """Assemble every context's A_bc, B_bc (each with a case loaded)
+        and factor context k at sigmas[k], as one launch sequence; returns
+        the batch's time in ms."""
+        return self._run(self.lib.tailor_batch_assemble_factor, sigmas, graph)
 
     def autotune(self, sigma: complex) -> dict:
         sigma = complex(sigma)

@@ -92,6 +92,16 @@ __device__ void reduce_publish(unsigned flags, double mn, double mx, double mx2,
   }
 }
 
+// The streamwise wavenumber and spanwise period of the next assembly into
+// the case's parameter block (launched before an assembly, or before the
+// replay of a captured one).
+__global__ void k_case_scalars(double *__restrict__ recipe, double alpha_re, double alpha_im,
+                               double period) {
+  recipe[kRecipeAlphaRe] = alpha_re;
+  recipe[kRecipeAlphaIm] = alpha_im;
+  recipe[kRecipePeriod] = period;
+}
+
 __global__ void k_reset_diag(DeviceDiag *d) {
   d->flags = 0;
   d->char_status = 0;
@@ -111,11 +121,12 @@ __global__ void k_reset_diag(DeviceDiag *d) {
 // First inverse metrics, one thread per node (Metrics.cpp:91-187).
 __global__ __launch_bounds__(kThreads) void k_metrics_first(
     CaseDims d, const int *__restrict__ xoff, const double *__restrict__ xw,
-    const int *__restrict__ eoff, const double *__restrict__ ew, double period,
+    const int *__restrict__ eoff, const double *__restrict__ ew, const double *__restrict__ recipe,
     const double *__restrict__ grid, double *__restrict__ metrics, DeviceDiag *diag) {
   const long long node = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
   unsigned flags = 0;
   double jmin = DBL_MAX, jmax = -DBL_MAX, imax = 0.0;
+  const double period = recipe[kRecipePeriod];
   if (node < d.nodes) {
     const int i = static_cast<int>(node % d.ny), j = static_cast<int>(node / d.ny);
     const double *w1x = xw + static_cast<size_t>(d.ny) * d.ssx + static_cast<size_t>(i) * d.ssx;
@@ -724,9 +735,9 @@ constexpr int kCoefNodes = 8;  // nodes per k_coefficients workgroup
 // (CurvilinearTransform.cpp:68-96) combinations of every node; status 0 on
 // success, nonzero for the reference's NonFinite/NonPositive outcomes.
 __global__ __launch_bounds__(kThreads) void k_coefficients(
-    CaseDims d, const double *__restrict__ recipe, double alpha_re, double alpha_im,
-    const double *__restrict__ base, const double *__restrict__ deriv,
-    const double *__restrict__ metrics, double *__restrict__ coef, int *__restrict__ status) {
+    CaseDims d, const double *__restrict__ recipe, const double *__restrict__ base,
+    const double *__restrict__ deriv, const double *__restrict__ metrics,
+    double *__restrict__ coef, int *__restrict__ status) {
   __shared__ double phys[kCoefNodes][11 * 25];
   __shared__ int st[kCoefNodes];
   const int tid = threadIdx.x;
@@ -767,6 +778,7 @@ __global__ __launch_bounds__(kThreads) void k_coefficients(
     const int n = tid / 25, e = tid - 25 * (tid / 25);
     const long long node = first + n;
     if (node < d.nodes && st[n] == 0) {
+      const double alpha_re = recipe[kRecipeAlphaRe], alpha_im = recipe[kRecipeAlphaIm];
       const double ia_re = 0.0 * alpha_re - 1.0 * alpha_im;
       const double ia_im = 0.0 * alpha_im + 1.0 * alpha_re;
       const double a2_re = alpha_re * alpha_re - alpha_im * alpha_im;
@@ -1044,11 +1056,11 @@ __global__ __launch_bounds__(64 * NPW) void k_operator(
 // Generic-size variant dispatch table (RT: run-time stencil sizes up to
 // SSX x SSE).
 template <int SSX, int SSE, bool RT = false>
-hipError_t launch_op(tailor_ctx *ctx, bool constrained) {
+hipError_t launch_op(tailor_ctx *ctx, bool constrained, hipStream_t st) {
   constexpr int NPW = kOperatorRows;
   const CaseDims &d = ctx->dims;
   const unsigned grid = static_cast<unsigned>((d.nodes + NPW - 1) / NPW);
-  hipLaunchKernelGGL((k_operator<SSX, SSE, NPW, RT>), dim3(grid), dim3(64 * NPW), 0, ctx->stream, d,
+  hipLaunchKernelGGL((k_operator<SSX, SSE, NPW, RT>), dim3(grid), dim3(64 * NPW), 0, st, d,
                      ctx->d_xoff, ctx->d_xw, ctx->d_eoff, ctx->d_ew, ctx->d_coef,
                      ctx->d_coef_status, ctx->d_bcdata, constrained ? 1 : 0, ctx->d_vals,
                      ctx->d_mvals, ctx->d_diag);
@@ -1081,60 +1093,65 @@ __global__ __launch_bounds__(kThreads) void k_sumsq(const double *__restrict__ d
 
 }  // namespace
 
-hipError_t launch_reset_diag(tailor_ctx *ctx) {
-  hipLaunchKernelGGL(k_reset_diag, dim3(1), dim3(1), 0, ctx->stream, ctx->d_diag);
+hipError_t launch_case_scalars(tailor_ctx *ctx, hipStream_t s) {
+  hipLaunchKernelGGL(k_case_scalars, dim3(1), dim3(1), 0, s, ctx->d_recipe, ctx->alpha_re,
+                     ctx->alpha_im, ctx->period);
   return hipGetLastError();
 }
 
-hipError_t launch_metrics(tailor_ctx *ctx) {
+hipError_t launch_reset_diag(tailor_ctx *ctx, hipStream_t s) {
+  hipLaunchKernelGGL(k_reset_diag, dim3(1), dim3(1), 0, s, ctx->d_diag);
+  return hipGetLastError();
+}
+
+hipError_t launch_metrics(tailor_ctx *ctx, hipStream_t s) {
   const CaseDims &d = ctx->dims;
   const unsigned grid = static_cast<unsigned>((d.nodes + kThreads - 1) / kThreads);
-  hipLaunchKernelGGL(k_metrics_first, dim3(grid), dim3(kThreads), 0, ctx->stream, d, ctx->d_xoff,
-                     ctx->d_xw, ctx->d_eoff, ctx->d_ew, ctx->period, ctx->d_grid, ctx->d_metrics,
+  hipLaunchKernelGGL(k_metrics_first, dim3(grid), dim3(kThreads), 0, s, d, ctx->d_xoff,
+                     ctx->d_xw, ctx->d_eoff, ctx->d_ew, ctx->d_recipe, ctx->d_grid, ctx->d_metrics,
                      ctx->d_diag);
   return hipGetLastError();
 }
 
-hipError_t launch_derivatives(tailor_ctx *ctx) {
+hipError_t launch_derivatives(tailor_ctx *ctx, hipStream_t s) {
   const CaseDims &d = ctx->dims;
   const dim3 grid((d.ny + DTX - 1) / DTX, (d.nz + DTY - 1) / DTY);
   const size_t lds = sizeof(double) * 9 * static_cast<size_t>(DTX + 2 * d.hx) * (DTY + 2 * d.hy);
-  hipLaunchKernelGGL(k_metrics2_derivatives, grid, dim3(DTHREADS), lds, ctx->stream, d,
+  hipLaunchKernelGGL(k_metrics2_derivatives, grid, dim3(DTHREADS), lds, s, d,
                      ctx->d_xoff, ctx->d_xw, ctx->d_eoff, ctx->d_ew, ctx->d_base, ctx->d_metrics,
                      ctx->d_deriv, ctx->d_diag);
   return hipGetLastError();
 }
 
-hipError_t launch_characteristics(tailor_ctx *ctx) {
+hipError_t launch_characteristics(tailor_ctx *ctx, hipStream_t s) {
   const CaseDims &d = ctx->dims;
   const unsigned g = static_cast<unsigned>((d.nz + 63) / 64);
-  hipLaunchKernelGGL(k_characteristics, dim3(g), dim3(64), 0, ctx->stream, d, ctx->d_recipe,
+  hipLaunchKernelGGL(k_characteristics, dim3(g), dim3(64), 0, s, d, ctx->d_recipe,
                      ctx->d_base, ctx->d_deriv, ctx->d_metrics, ctx->d_bcdata, ctx->d_diag);
   return hipGetLastError();
 }
 
-hipError_t launch_coefficients(tailor_ctx *ctx) {
+hipError_t launch_coefficients(tailor_ctx *ctx, hipStream_t s) {
   const CaseDims &d = ctx->dims;
   const unsigned grid = static_cast<unsigned>((d.nodes + kCoefNodes - 1) / kCoefNodes);
-  hipLaunchKernelGGL(k_coefficients, dim3(grid), dim3(kThreads), 0, ctx->stream, d, ctx->d_recipe,
-                     ctx->alpha_re, ctx->alpha_im, ctx->d_base, ctx->d_deriv, ctx->d_metrics,
-                     ctx->d_coef, ctx->d_coef_status);
+  hipLaunchKernelGGL(k_coefficients, dim3(grid), dim3(kThreads), 0, s, d, ctx->d_recipe,
+                     ctx->d_base, ctx->d_deriv, ctx->d_metrics, ctx->d_coef, ctx->d_coef_status);
   return hipGetLastError();
 }
 
-hipError_t launch_operator(tailor_ctx *ctx, bool constrained) {
+hipError_t launch_operator(tailor_ctx *ctx, bool constrained, hipStream_t s) {
   const CaseDims &d = ctx->dims;
-  if (d.ssx == 11 && d.sse == 7) return launch_op<11, 7>(ctx, constrained);
-  if (d.ssx == 5 && d.sse == 3) return launch_op<5, 3>(ctx, constrained);
-  if (d.ssx == 5 && d.sse == 5) return launch_op<5, 5>(ctx, constrained);
-  if (d.ssx == 7 && d.sse == 5) return launch_op<7, 5>(ctx, constrained);
-  if (d.ssx == 9 && d.sse == 7) return launch_op<9, 7>(ctx, constrained);
-  if (d.ssx == 11 && d.sse == 5) return launch_op<11, 5>(ctx, constrained);
-  if (d.ssx == 7 && d.sse == 7) return launch_op<7, 7>(ctx, constrained);
+  if (d.ssx == 11 && d.sse == 7) return launch_op<11, 7>(ctx, constrained, s);
+  if (d.ssx == 5 && d.sse == 3) return launch_op<5, 3>(ctx, constrained, s);
+  if (d.ssx == 5 && d.sse == 5) return launch_op<5, 5>(ctx, constrained, s);
+  if (d.ssx == 7 && d.sse == 5) return launch_op<7, 5>(ctx, constrained, s);
+  if (d.ssx == 9 && d.sse == 7) return launch_op<9, 7>(ctx, constrained, s);
+  if (d.ssx == 11 && d.sse == 5) return launch_op<11, 5>(ctx, constrained, s);
+  if (d.ssx == 7 && d.sse == 7) return launch_op<7, 7>(ctx, constrained, s);
   // any other stencil pair within the run-time instance's bounds (the loader
   // rejects wider ones, and the derivative tile's LDS bound is tighter)
   if (d.ssx <= kOperatorMaxSsx && d.sse <= kOperatorMaxSse)
-    return launch_op<kOperatorMaxSsx, kOperatorMaxSse, true>(ctx, constrained);
+    return launch_op<kOperatorMaxSsx, kOperatorMaxSse, true>(ctx, constrained, s);
   return hipErrorInvalidValue;
 }
 

@@ -36,10 +36,14 @@ struct tailor_batch {
   // GEMM tile configuration chosen by tailor_batch_autotune or
   // tailor_batch_set_gemm_config (kept when the plan is rebuilt)
   std::vector<int> cfg_shape, cfg_order;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  bool graph_fp32 = false, graph_streaming = false;
-  std::vector<int> graph_shape, graph_order;
+  // captured launch sequences: [0] the joined factorization, [1] every
+  // context's assembly followed by it
+  struct Graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool fp32 = false, streaming = false;
+    std::vector<int> shape, order;
+  } graphs[2];
   hipEvent_t ev_start = nullptr, ev_done = nullptr;
   std::vector<hipEvent_t> ev_ready;    // one per context: its queued work before the batch
 };
@@ -63,10 +67,12 @@ int bhip(tailor_batch *b, hipError_t e, const char *what) {
   } while (0)
 
 void release_graph(tailor_batch *bt) {
-  if (bt->graph_exec) (void)hipGraphExecDestroy(bt->graph_exec);
-  if (bt->graph) (void)hipGraphDestroy(bt->graph);
-  bt->graph_exec = nullptr;
-  bt->graph = nullptr;
+  for (auto &g : bt->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g.exec = nullptr;
+    g.graph = nullptr;
+  }
 }
 
 std::vector<tailor::BatchPart> parts_of(const tailor_batch *bt) {
@@ -95,13 +101,19 @@ std::vector<double *> partials_of(const tailor_batch *bt) {
   return p;
 }
 
-// Every context assembled (A_bc, B_bc), at the structure the batch was made
-// with, and at one front precision; the joined plan built for it.
-int batch_ready(tailor_batch *bt, bool *fp32_out) {
+// Every context with a case loaded and, unless the batch assembles them
+// itself, assembled (A_bc, B_bc); all at the structure the batch was made
+// with and at one front precision; the joined plan built for it.
+int batch_ready(tailor_batch *bt, bool assembles, bool *fp32_out) {
   const int prec = bt->ctxs[0]->precision;
   for (size_t k = 0; k < bt->ctxs.size(); ++k) {
     tailor_ctx *c = bt->ctxs[k];
-    if (!c->assembled || c->vals_kind != TAILOR_OP_A || c->mvals_kind != TAILOR_OP_B)
+    if (!c->loaded)
+      return bfail(bt, TAILOR_ERR_ARG_WRONGSTATE,
+                   "Prepared field data are required before matrix assembly (context " +
+                       std::to_string(k) + " of the batch)");
+    if (!assembles &&
+        (!c->assembled || c->vals_kind != TAILOR_OP_A || c->mvals_kind != TAILOR_OP_B))
       return bfail(bt, TAILOR_ERR_ARG_WRONGSTATE,
                    "Boundary-constrained matrices are required before EPSSolve (context " +
                        std::to_string(k) + " of the batch)");
@@ -145,33 +157,6 @@ int set_shifts(tailor_batch *bt, const tailor::cplx *sig) {
   return TAILOR_OK;
 }
 
-int graph_ready(tailor_batch *bt, bool fp32) {
-  const tailor::LuPlan &U = bt->plan;
-  if (bt->graph_exec && bt->graph_fp32 == fp32 && bt->graph_shape == U.gemm_shape &&
-      bt->graph_order == U.gemm_order && bt->graph_streaming == U.panel_streaming)
-    return TAILOR_OK;
-  release_graph(bt);
-  BATCH_OK(hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
-  hipError_t e = tailor::lu_factor_batch(U, parts_of(bt), bt->fbase, inputs_of(bt),
-                                         partials_of(bt), fp32, bt->stream, nullptr);
-  hipGraph_t g = nullptr;
-  const hipError_t e_end = hipStreamEndCapture(bt->stream, &g);
-  if (e == hipSuccess) e = e_end;
-  hipGraphExec_t x = nullptr;
-  if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
-  if (e != hipSuccess) {
-    if (g) (void)hipGraphDestroy(g);
-    return bhip(bt, e, "capture of the batch factorization graph");
-  }
-  bt->graph = g;
-  bt->graph_exec = x;
-  bt->graph_fp32 = fp32;
-  bt->graph_shape = U.gemm_shape;
-  bt->graph_order = U.gemm_order;
-  bt->graph_streaming = U.panel_streaming;
-  return TAILOR_OK;
-}
-
 // After the batch's factorization has completed: every context holds its
 // factors at its shift.
 int mark_factored(tailor_batch *bt, const tailor::cplx *sig, bool fp32, double ms) {
@@ -181,6 +166,81 @@ int mark_factored(tailor_batch *bt, const tailor::cplx *sig, bool fp32, double m
   for (size_t k = 0; k < bt->ctxs.size(); ++k)
     tailor::solver_mark_factored(bt->ctxs[k], sig[k], fp32, counters[k], ms);
   return TAILOR_OK;
+}
+
+// The batch's work as a launch sequence on its stream: with `assembles`,
+// every context's assembly (its wavenumber and period already in its
+// parameter block), then the joined factorization (the shifts already in the
+// batch's parameter blocks).
+hipError_t enqueue(tailor_batch *bt, bool assembles, bool fp32) {
+  if (assembles)
+    for (tailor_ctx *c : bt->ctxs) {
+      hipError_t e = tailor::enqueue_assembly(c, true, bt->stream);
+      if (e != hipSuccess) return e;
+    }
+  return tailor::lu_factor_batch(bt->plan, parts_of(bt), bt->fbase, inputs_of(bt),
+                                 partials_of(bt), fp32, bt->stream, nullptr);
+}
+
+int graph_ready(tailor_batch *bt, bool assembles, bool fp32) {
+  const tailor::LuPlan &U = bt->plan;
+  auto &G = bt->graphs[assembles ? 1 : 0];
+  if (G.exec && G.fp32 == fp32 && G.shape == U.gemm_shape && G.order == U.gemm_order &&
+      G.streaming == U.panel_streaming)
+    return TAILOR_OK;
+  if (G.exec) (void)hipGraphExecDestroy(G.exec);
+  if (G.graph) (void)hipGraphDestroy(G.graph);
+  G.exec = nullptr;
+  G.graph = nullptr;
+  BATCH_OK(hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = enqueue(bt, assembles, fp32);
+  hipGraph_t g = nullptr;
+  const hipError_t e_end = hipStreamEndCapture(bt->stream, &g);
+  if (e == hipSuccess) e = e_end;
+  hipGraphExec_t x = nullptr;
+  if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return bhip(bt, e, "capture of the batch graph");
+  }
+  G.graph = g;
+  G.exec = x;
+  G.fp32 = fp32;
+  G.shape = U.gemm_shape;
+  G.order = U.gemm_order;
+  G.streaming = U.panel_streaming;
+  return TAILOR_OK;
+}
+
+// One factorization of the batch at `sig`, after (with `assembles`) the
+// assembly of every context; blocks until done.
+int run(tailor_batch *bt, const tailor::cplx *sig, bool assembles, bool use_graph,
+        double *batch_ms) {
+  BATCH_OK(hipSetDevice(bt->device));
+  bool fp32 = false;
+  if (int st = batch_ready(bt, assembles, &fp32)) return st;
+  if (use_graph)
+    if (int st = graph_ready(bt, assembles, fp32)) return st;
+  if (int st = wait_for_contexts(bt)) return st;
+  BATCH_OK(hipEventRecord(bt->ev_start, bt->stream));
+  if (assembles)
+    for (tailor_ctx *c : bt->ctxs) BATCH_OK(tailor::launch_case_scalars(c, bt->stream));
+  if (int st = set_shifts(bt, sig)) return st;
+  if (use_graph)
+    BATCH_OK(hipGraphLaunch(bt->graphs[assembles ? 1 : 0].exec, bt->stream));
+  else
+    BATCH_OK(enqueue(bt, assembles, fp32));
+  BATCH_OK(hipEventRecord(bt->ev_done, bt->stream));
+  BATCH_OK(hipEventSynchronize(bt->ev_done));
+  float ms = 0.f;
+  BATCH_OK(hipEventElapsedTime(&ms, bt->ev_start, bt->ev_done));
+  if (batch_ms) *batch_ms = ms;
+  if (assembles)
+    for (size_t k = 0; k < bt->ctxs.size(); ++k)
+      if (int st = tailor::finish_assembly(bt->ctxs[k], true, nullptr))
+        return bfail(bt, st, "context " + std::to_string(k) + " of the batch: " +
+                                 bt->ctxs[k]->error);
+  return mark_factored(bt, sig, fp32, ms);
 }
 
 int check_shifts(tailor_batch *bt, const double *sigmas, std::vector<tailor::cplx> &sig) {
@@ -262,25 +322,15 @@ int tailor_batch_factor(tailor_batch *bt, const double *sigmas, int32_t use_grap
   if (!bt) return TAILOR_ERR_USER_INPUT;
   std::vector<tailor::cplx> sig;
   if (int st = check_shifts(bt, sigmas, sig)) return st;
-  BATCH_OK(hipSetDevice(bt->device));
-  bool fp32 = false;
-  if (int st = batch_ready(bt, &fp32)) return st;
-  if (use_graph)
-    if (int st = graph_ready(bt, fp32)) return st;
-  if (int st = wait_for_contexts(bt)) return st;
-  BATCH_OK(hipEventRecord(bt->ev_start, bt->stream));
-  if (int st = set_shifts(bt, sig.data())) return st;
-  if (use_graph)
-    BATCH_OK(hipGraphLaunch(bt->graph_exec, bt->stream));
-  else
-    BATCH_OK(tailor::lu_factor_batch(bt->plan, parts_of(bt), bt->fbase, inputs_of(bt),
-                                     partials_of(bt), fp32, bt->stream, nullptr));
-  BATCH_OK(hipEventRecord(bt->ev_done, bt->stream));
-  BATCH_OK(hipEventSynchronize(bt->ev_done));
-  float ms = 0.f;
-  BATCH_OK(hipEventElapsedTime(&ms, bt->ev_start, bt->ev_done));
-  if (batch_ms) *batch_ms = ms;
-  return mark_factored(bt, sig.data(), fp32, ms);
+  return run(bt, sig.data(), false, use_graph != 0, batch_ms);
+}
+
+int tailor_batch_assemble_factor(tailor_batch *bt, const double *sigmas, int32_t use_graph,
+                                 double *batch_ms) {
+  if (!bt) return TAILOR_ERR_USER_INPUT;
+  std::vector<tailor::cplx> sig;
+  if (int st = check_shifts(bt, sigmas, sig)) return st;
+  return run(bt, sig.data(), true, use_graph != 0, batch_ms);
 }
 
 int tailor_batch_autotune(tailor_batch *bt, double sigma_re, double sigma_im,
@@ -290,7 +340,7 @@ int tailor_batch_autotune(tailor_batch *bt, double sigma_re, double sigma_im,
     return bfail(bt, TAILOR_ERR_USER_INPUT, "shift must be finite");
   BATCH_OK(hipSetDevice(bt->device));
   bool fp32 = false;
-  if (int st = batch_ready(bt, &fp32)) return st;
+  if (int st = batch_ready(bt, false, &fp32)) return st;
   if (int st = wait_for_contexts(bt)) return st;
   std::vector<tailor::cplx> sig(bt->ctxs.size(), tailor::cplx(sigma_re, sigma_im));
   tailor::LuPlan &U = bt->plan;

@@ -185,7 +185,7 @@ int tailor_create(tailor_ctx **out, int device) {
   if (e == hipSuccess) e = tailor::noalloc_setup(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = dalloc(&ctx->d_diag, 1);
-  if (e == hipSuccess) e = dalloc(&ctx->d_recipe, 8);
+  if (e == hipSuccess) e = dalloc(&ctx->d_recipe, tailor::kRecipeDoubles);
   if (e == hipSuccess)
     e = dalloc(&ctx->d_partials,
                static_cast<size_t>(tailor::kPartialBlocks) * tailor::kPartialComps);
@@ -412,28 +412,47 @@ static int check_assembly_flags(tailor_ctx *ctx, const tailor::DeviceDiag &g) {
   return TAILOR_OK;
 }
 
+}  // extern "C"
+
+namespace tailor {
+
+hipError_t enqueue_assembly(tailor_ctx *ctx, bool constrained, hipStream_t s) {
+  hipError_t e;
+  if ((e = launch_reset_diag(ctx, s)) != hipSuccess) return e;
+  if ((e = launch_metrics(ctx, s)) != hipSuccess) return e;
+  if ((e = launch_derivatives(ctx, s)) != hipSuccess) return e;
+  if (constrained && (e = launch_characteristics(ctx, s)) != hipSuccess) return e;
+  if ((e = launch_coefficients(ctx, s)) != hipSuccess) return e;
+  return launch_operator(ctx, constrained, s);
+}
+
+int finish_assembly(tailor_ctx *ctx, bool constrained, DeviceDiag *host_diag) {
+  DeviceDiag g{};
+  HIP_TRY(ctx, hipMemcpy(&g, ctx->d_diag, sizeof g, hipMemcpyDeviceToHost));
+  if (host_diag) *host_diag = g;
+  ctx->vals_kind = constrained ? TAILOR_OP_A : TAILOR_OP_L;
+  ctx->mvals_kind = constrained ? TAILOR_OP_B : TAILOR_OP_M;
+  if (ctx->solver) ctx->solver->factored = false;
+  ctx->assembled = false;
+  if (int st = check_assembly_flags(ctx, g)) return st;
+  ctx->assembled = true;
+  return TAILOR_OK;
+}
+
+}  // namespace tailor
+
+extern "C" {
+
 static int run_assembly(tailor_ctx *ctx, bool constrained, tailor::DeviceDiag *host_diag) {
   if (!ctx) return TAILOR_ERR_USER_INPUT;
   if (!ctx->loaded)
     return fail(ctx, TAILOR_ERR_ARG_WRONGSTATE,
                 "Prepared field data are required before matrix assembly");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
-  HIP_TRY(ctx, tailor::launch_reset_diag(ctx));
-  HIP_TRY(ctx, tailor::launch_metrics(ctx));
-  HIP_TRY(ctx, tailor::launch_derivatives(ctx));
-  if (constrained) HIP_TRY(ctx, tailor::launch_characteristics(ctx));
-  HIP_TRY(ctx, tailor::launch_coefficients(ctx));
-  HIP_TRY(ctx, tailor::launch_operator(ctx, constrained));
-  tailor::DeviceDiag g{};
-  HIP_TRY(ctx, hipMemcpyAsync(&g, ctx->d_diag, sizeof g, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, tailor::launch_case_scalars(ctx, ctx->stream));
+  HIP_TRY(ctx, tailor::enqueue_assembly(ctx, constrained, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  if (host_diag) *host_diag = g;
-  ctx->vals_kind = constrained ? TAILOR_OP_A : TAILOR_OP_L;
-  ctx->mvals_kind = constrained ? TAILOR_OP_B : TAILOR_OP_M;
-  if (ctx->solver) ctx->solver->factored = false;
-  if (int st = check_assembly_flags(ctx, g)) return st;
-  ctx->assembled = true;
-  return TAILOR_OK;
+  return tailor::finish_assembly(ctx, constrained, host_diag);
 }
 
 int tailor_assemble(tailor_ctx *ctx, tailor_assembly_diag *diag) {
@@ -543,15 +562,9 @@ int tailor_time_assembly(tailor_ctx *ctx, int reps, double *write_kernel_ms,
   float ms = 0.0f;
   // full assembly: diag reset, metrics, derivatives, characteristics,
   // coefficient records, operator writer
+  HIP_TRY(ctx, tailor::launch_case_scalars(ctx, ctx->stream));
   HIP_TRY(ctx, hipEventRecord(e0, ctx->stream));
-  for (int r = 0; r < reps; ++r) {
-    HIP_TRY(ctx, tailor::launch_reset_diag(ctx));
-    HIP_TRY(ctx, tailor::launch_metrics(ctx));
-    HIP_TRY(ctx, tailor::launch_derivatives(ctx));
-    HIP_TRY(ctx, tailor::launch_characteristics(ctx));
-    HIP_TRY(ctx, tailor::launch_coefficients(ctx));
-    HIP_TRY(ctx, tailor::launch_operator(ctx, true));
-  }
+  for (int r = 0; r < reps; ++r) HIP_TRY(ctx, tailor::enqueue_assembly(ctx, true, ctx->stream));
   HIP_TRY(ctx, hipEventRecord(e1, ctx->stream));
   HIP_TRY(ctx, hipEventSynchronize(e1));
   HIP_TRY(ctx, hipEventElapsedTime(&ms, e0, e1));
@@ -559,7 +572,7 @@ int tailor_time_assembly(tailor_ctx *ctx, int reps, double *write_kernel_ms,
   // operator writer alone (k_operator; its coefficient records are resident
   // from above)
   HIP_TRY(ctx, hipEventRecord(e0, ctx->stream));
-  for (int r = 0; r < reps; ++r) HIP_TRY(ctx, tailor::launch_operator(ctx, true));
+  for (int r = 0; r < reps; ++r) HIP_TRY(ctx, tailor::launch_operator(ctx, true, ctx->stream));
   HIP_TRY(ctx, hipEventRecord(e1, ctx->stream));
   HIP_TRY(ctx, hipEventSynchronize(e1));
   HIP_TRY(ctx, hipEventElapsedTime(&ms, e0, e1));

@@ -83,7 +83,7 @@ struct tailor_ctx {
   int *d_coef_status = nullptr;  // nodes
   tailor::DeviceDiag *d_diag = nullptr;
   double *d_bcdata = nullptr;    // nz * 106: P, R^-1, incoming mask, status
-  double *d_recipe = nullptr;    // Re, Ma, Pr, gamma, Tref, S
+  double *d_recipe = nullptr;    // kRecipeDoubles: Re, Ma, Pr, gamma, Tref, S, alpha, period
   double *d_partials = nullptr;  // reduction scratch, kPartials doubles
 
   // host copy of the structure and of the rule offsets
@@ -95,6 +95,15 @@ struct tailor_ctx {
 };
 
 namespace tailor {
+
+// The case's device parameter block (d_recipe): the physics of the recipe,
+// then the streamwise wavenumber and the spanwise period, which the assembly
+// kernels read from it (not from launch arguments), so a captured assembly
+// replays with the values written before the replay.
+enum {
+  kRecipeRe = 0, kRecipeMa = 1, kRecipePr = 2, kRecipeGamma = 3, kRecipeTref = 4, kRecipeS = 5,
+  kRecipeAlphaRe = 6, kRecipeAlphaIm = 7, kRecipePeriod = 8, kRecipeDoubles = 16
+};
 
 // Widest stencils (points per 1-D rule) the operator writer takes: its
 // run-time-sized instance holds this many per node in LDS; the derivative
@@ -108,13 +117,22 @@ constexpr int kPartialComps = 32;
 int fail(tailor_ctx *ctx, int code, const std::string &msg);
 int hip_fail(tailor_ctx *ctx, hipError_t e, const char *what);
 
-// assembly launchers (assembly.hip)
-hipError_t launch_reset_diag(tailor_ctx *ctx);
-hipError_t launch_metrics(tailor_ctx *ctx);
-hipError_t launch_derivatives(tailor_ctx *ctx);
-hipError_t launch_characteristics(tailor_ctx *ctx);
-hipError_t launch_coefficients(tailor_ctx *ctx);  // per-node LNS coefficient records
-hipError_t launch_operator(tailor_ctx *ctx, bool constrained);
+// assembly launchers (assembly.hip), on stream s
+hipError_t launch_case_scalars(tailor_ctx *ctx, hipStream_t s);  // alpha, period -> d_recipe
+hipError_t launch_reset_diag(tailor_ctx *ctx, hipStream_t s);
+hipError_t launch_metrics(tailor_ctx *ctx, hipStream_t s);
+hipError_t launch_derivatives(tailor_ctx *ctx, hipStream_t s);
+hipError_t launch_characteristics(tailor_ctx *ctx, hipStream_t s);
+hipError_t launch_coefficients(tailor_ctx *ctx, hipStream_t s);  // per-node LNS coefficient records
+hipError_t launch_operator(tailor_ctx *ctx, bool constrained, hipStream_t s);
+
+// The assembly of A_bc, B_bc (constrained) or L, M as a launch sequence on
+// s, after launch_case_scalars; nothing is read back (capi.cpp).
+hipError_t enqueue_assembly(tailor_ctx *ctx, bool constrained, hipStream_t s);
+// After an enqueued assembly has completed: read its status word back,
+// record which operators the value buffers hold and turn the status bits into
+// the reference's messages (capi.cpp).
+int finish_assembly(tailor_ctx *ctx, bool constrained, DeviceDiag *host_diag);
 hipError_t launch_sumsq(tailor_ctx *ctx, const double *data, long long rows, int ncomp,
                         long long stride, double *host_out);
 

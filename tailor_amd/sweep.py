@@ -32,10 +32,11 @@ streams, so the process needs at least 2 C hardware queues
 unrelated streams wait for each other.
 
 With `batch=B > 0` every form runs through joined factorizations instead:
-the instances go in groups of B, each group on B contexts whose fronts are
-factored together by one launch sequence over their joined elimination
-levels (a tailor_batch, replayed from its HIP graph), so every level launch
-covers the fronts of the whole group.  Two groups of contexts alternate: a
+the instances go in groups of B, each group on B contexts that are
+assembled and then factored together by one launch sequence over their
+joined elimination levels (a tailor_batch, assembly and factorization
+replayed from one HIP graph), so every level launch covers the fronts of
+the whole group.  Two groups of contexts alternate: a
 worker thread assembles and factors group j + 1 while the main thread runs
 group j's Krylov-Schur, one instance after another.  Groups may mix grid
 sizes; each context keeps one size, so warm starts still follow a context's
@@ -407,9 +408,9 @@ class Sweeper:
                 self.batches[bk] = Batch(ctxs)
                 for ctx, k in zip(ctxs, groups[j]):
                     ctx.load_case(cases[k][1])
-                    ctx.assemble(diagnostics=False)
                     ctx.set_precision(precision)
-                self.batches[bk].factor([complex(self.omega_s.imag, -self.omega_s.real)] * len(ctxs))
+                self.batches[bk].assemble_factor(
+                    [complex(self.omega_s.imag, -self.omega_s.real)] * len(ctxs))
         solved = set()  # contexts that hold Ritz vectors of an earlier instance
         errors = []
 
@@ -418,10 +419,10 @@ class Sweeper:
                 ctxs = group_contexts(j)
                 for ctx, k in zip(ctxs, groups[j]):
                     ctx.load_case(cases[k][1])
-                    ctx.assemble(diagnostics=False)
                     ctx.set_precision(precision)
                 sig = [complex(complex(shifts[k]).imag, -complex(shifts[k]).real) for k in groups[j]]
-                self.batches[tuple(id(c) for c in ctxs)].factor(sig, graph=True)
+                # the group's assembly and joined factorization: one graph replay
+                self.batches[tuple(id(c) for c in ctxs)].assemble_factor(sig, graph=True)
             except Exception as e:  # re-raised by the main thread
                 errors.append(e)
 

@@ -168,3 +168,57 @@ def test_mixed_size_mixed_precision_sweep_through_joined_factorizations():
         b = by[id(a.instance)]
         for w in b.omegas:
             assert np.min(np.abs(a.omegas - w)) <= 1e-8 * abs(w), (a.instance, w)
+
+
+def test_batch_assembly_and_factorization_in_one_graph():
+    """tailor_batch_assemble_factor: every context's assembly and the joined
+    factorization replayed from one graph, across wavenumber changes (the
+    wavenumber is read from each case's parameter block, written before each
+    replay).  A_bc and B_bc equal a separate context's own assembly bit for
+    bit, and the solutions equal the own factorization's."""
+    cases = [small_case(40, 16, alpha=2.43), small_case(40, 16, alpha=1.2, seed=6),
+             small_case(24, 12, alpha=0.8)]
+    rng = np.random.default_rng(13)
+    bs = [rng.standard_normal(5 * c.nodes) + 1j * rng.standard_normal(5 * c.nodes) for c in cases]
+    ctxs = [ta.Context(0) for _ in cases]
+    refs = [ta.Context(0) for _ in cases]
+    try:
+        for ctx, ref, c in zip(ctxs, refs, cases):
+            ctx.load_case(c)
+            ref.load_case(c)
+        with ta.Batch(ctxs) as batch:
+            for r, (alphas, sigmas) in enumerate([([2.43, 1.2, 0.8], [-0.02j, 0.0j, 0.05 - 0.1j]),
+                                                  ([1.9, 0.6, 2.7], [0.0j, -0.02j, -0.02j]),
+                                                  ([1.9, 0.6, 2.7], [0.01j, 0.0j, 0.0j])]):
+                for ctx, ref, a in zip(ctxs, refs, alphas):
+                    ctx.set_alpha(a)
+                    ref.set_alpha(a)
+                    ref.assemble(diagnostics=False)
+                print(f"assemble+factor round {r}: alphas {alphas}, shifts {sigmas}", flush=True)
+                batch.assemble_factor(sigmas, graph=True)
+                for k, (ctx, ref, b, s) in enumerate(zip(ctxs, refs, bs, sigmas)):
+                    for op in (ta.OP_A, ta.OP_B):
+                        assert np.array_equal(ctx.export_bsr(op)[2], ref.export_bsr(op)[2]), (r, k, op)
+                    xb = solve(ctx, b)
+                    ref.factor(s)
+                    assert rel(xb, solve(ref, b)) <= 1e-12, (r, k)
+    finally:
+        for ctx in ctxs + refs:
+            ctx.close()
+
+
+def test_batch_assembly_reports_invalid_cases():
+    good, bad = small_case(24, 12), small_case(24, 12, alpha=1.4)
+    bad.baseflow = bad.baseflow.copy()
+    bad.baseflow[7, 4] = -1.0
+    ctxs = [ta.Context(0), ta.Context(0)]
+    try:
+        ctxs[0].load_case(good)
+        ctxs[1].load_case(bad)
+        with ta.Batch(ctxs) as batch:
+            with pytest.raises(ta.TailorError, match="context 1 of the batch: Base-flow density "
+                                                     "and temperature must both be positive"):
+                batch.assemble_factor([0j, 0j])
+    finally:
+        for ctx in ctxs:
+            ctx.close()
