@@ -17,9 +17,10 @@ The same instances also run
 * in the plain form as the reference runs it (complex FP64, fixed
   omega_s = 0.02, cold start, ncv = max(2 nev, nev + 15) = 23, one instance
   after another);
-* in the plain form at its fastest subspace size through the same
-  eight-context pipeline (the strongest plain form; the headline ratio is
-  taken against it);
+* in the plain form at its fastest subspace size, through the same
+  eight-context pipeline and through joined groups of four (each group's
+  assembly and joined factorization replayed from one HIP graph); the
+  headline ratio is taken against the strongest plain form;
 * with the network switched off (fixed shift, same warm starts, early pass
   ends and pipeline): the learned form's rate over this one is what the
   network itself gains.
@@ -28,7 +29,9 @@ Also reported: the plain single instance at alpha = 2.43 (Re=1000, Ma=0.5)
 at several subspace sizes, the mixed-precision form on that operator
 (per-eigenvalue error against the FP64 result, refinement steps per solve),
 a sweep over both grid sizes with per-instance base flows in mixed
-precision, assembly, triangular-solve and SpMV HBM GB/s, the roofline of the
+precision through joined groups (and through the pipeline beside it), the
+factorization throughput of joined groups against one factorization after
+another, assembly, triangular-solve and SpMV HBM GB/s, the roofline of the
 frontal-update GEMM (the kernel with the largest share of a sweep instance)
 from HIP events on the library stream, and the CPU oracle on the 100x50 case
 beside the GPU on the same case.
@@ -67,6 +70,7 @@ LEARNED_NCV = 40  # subspace of the learned form: the wanted cluster converges i
 LEARNED_CHECK_FROM = 28  # learned form: convergence also tested every second step from step 28
 PIPELINE_CONTEXTS = 8  # Krylov-Schur of one instance, factorization of the next seven
 MIXED_SIZE_INSTANCES = 4  # sweep over both grid sizes in mixed precision
+JOINED_GROUP = 4  # instances per joined assembly + factorization (tailor_batch)
 
 
 _T0 = time.perf_counter()
@@ -386,16 +390,36 @@ def phase_reference(ta, steps, warmup) -> dict:
     return {"plain_ref": plain_ref.totals()}
 
 
-def phase_mixed_size(ta, count, ncv, contexts) -> dict:
+def phase_joined_plain(ta, steps, warmup, best_ncv) -> dict:
+    """The plain form at its fastest subspace through joined groups of
+    JOINED_GROUP instances (assembly and joined factorization of a group
+    replayed from one HIP graph, the group's Krylov-Schur runs side by side
+    while the next group is factored)."""
+    from tailor_amd import sweep as sw
+    insts, cases, shifts, warm, timed = headline_instances(ta, steps, warmup)
+    runner = sw.Sweeper(ncv=best_ncv, omega_s=OMEGA, batch=JOINED_GROUP)
+    runner.run(cases, "plain", precision=ta.PRECISION_FP64, order=warm[:1])  # plans, graphs
+    res = runner.run(cases, "plain", precision=ta.PRECISION_FP64, order=timed)
+    runner.close()
+    log(f"plain sweep (ncv {best_ncv}, joined groups of {JOINED_GROUP}): "
+        f"{res.eigenpairs_per_s:.2f} eigenpairs/s")
+    return {"plain_joined": res.totals()}
+
+
+def phase_mixed_size(ta, count, ncv, contexts, joined=True) -> dict:
     """Wavenumber sweep over both grid sizes (half 100x50, half 200x100),
     base flows from seeds 1000 + k, complex FP32 fronts with FP64
-    refinement of every solve to 1e-12, fixed shift, through the pipeline.
-    Each size's half runs as its own timed sweep, so the per-size rates are
-    wall-clock rates; the whole sweep's rate is their combination."""
+    refinement of every solve to 1e-12, fixed shift: through joined groups
+    of JOINED_GROUP instances of a size (assembly and joined factorization
+    of a group replayed from one HIP graph), or (joined=False) through the
+    pipeline of contexts.  Each size's half runs as its own timed sweep, so
+    the per-size rates are wall-clock rates; the whole sweep's rate is their
+    combination."""
     from tailor_amd import sweep as sw
     c3 = sw.sweep_instances(count + 2, seed=1, sizes=[(100, 50), (200, 100)])
     cases = sw.prepare(c3, OMEGA)
-    runner = sw.Sweeper(ncv=ncv, omega_s=OMEGA, contexts=contexts)
+    runner = (sw.Sweeper(ncv=ncv, omega_s=OMEGA, batch=JOINED_GROUP) if joined
+              else sw.Sweeper(ncv=ncv, omega_s=OMEGA, contexts=contexts))
     runner.run(cases, "mixed", order=[0, 1])  # plans and tuning of both sizes
     per_size, good, secs = {}, 0, 0.0
     totals = {"instances": 0, "factorizations": 0, "restarts": 0, "solves": 0,
@@ -411,11 +435,14 @@ def phase_mixed_size(ta, count, ncv, contexts) -> dict:
         secs += t["seconds"]
         for key in totals:
             totals[key] += t[key]
-        log(f"mixed-precision sweep {ny}x{nz}: {t['eigenpairs_per_s']:.2f} eigenpairs/s")
+        log(f"mixed-precision sweep {ny}x{nz} ({'joined groups' if joined else 'pipeline'}): "
+            f"{t['eigenpairs_per_s']:.2f} eigenpairs/s")
     runner.close()
+    how = ({"joined_group": JOINED_GROUP, "graph": "assembly + joined factorization per group"}
+           if joined else {"contexts": contexts})
     return {"mixed_sweep": {**totals, "good_pairs": good, "seconds": round(secs, 4),
                             "eigenpairs_per_s": round(good / secs, 4) if secs > 0 else None,
-                            "ncv": ncv, "contexts": contexts, "per_size": per_size}}
+                            "ncv": ncv, **how, "per_size": per_size}}
 
 
 def phase_batch(ta, count: int = 4) -> dict:
@@ -504,8 +531,12 @@ def phase_main(args) -> int:
         out = phase_reference(ta, args.steps, args.warmup)
     elif args.phase == "batch":
         out = phase_batch(ta)
+    elif args.phase == "joined":
+        out = phase_joined_plain(ta, args.steps, args.warmup, args.best_ncv)
     elif args.phase == "mixed_size":
-        out = phase_mixed_size(ta, args.mixed_count, args.best_ncv, args.contexts)
+        out = phase_mixed_size(ta, args.mixed_count, args.best_ncv, args.contexts, joined=True)
+    elif args.phase == "mixed_size_pipeline":
+        out = phase_mixed_size(ta, args.mixed_count, args.best_ncv, args.contexts, joined=False)
     else:
         raise SystemExit(f"unknown phase {args.phase}")
     print(json.dumps(out))
@@ -544,13 +575,16 @@ def main() -> int:
     pipe = run_phase("pipeline", args, ("--best-ncv", str(best_ncv)))
     ref = run_phase("reference", args)
     mixed_count = max(MIXED_SIZE_INSTANCES, 2 * (args.steps // 2))  # 64 at the default
-    msz = run_phase("mixed_size", args, ("--best-ncv", str(best_ncv),
-                                         "--mixed-count", str(mixed_count),
-                                         "--contexts", str(min(args.contexts, 4))))
+    joined = run_phase("joined", args, ("--best-ncv", str(best_ncv)))
+    msz_args = ("--best-ncv", str(best_ncv), "--mixed-count", str(mixed_count),
+                "--contexts", str(min(args.contexts, 4)))
+    msz = run_phase("mixed_size", args, msz_args)
+    msz_pipe = run_phase("mixed_size_pipeline", args, msz_args)
     bt = run_phase("batch", args)
     L, F, PB, PR = pipe["learned"], pipe["fixed"], pipe["plain_best"], ref["plain_ref"]
+    PJ = joined["plain_joined"]
     rate = lambda t: t["eigenpairs_per_s"]
-    strongest = max(rate(PB), rate(PR))
+    strongest = max(rate(PB), rate(PR), rate(PJ))
     g = single["gemm"]
     gemm_tf = g["gemm_flops"] / (g["gemm_ms"] * 1e-3) / 1e12
     asm_ms = single["assembly"]["full_ms"]
@@ -581,6 +615,7 @@ def main() -> int:
         "learned_sweep": without_assembly(L, asm_ms),
         "network_off_sweep": without_assembly(F, asm_ms),
         "plain_sweep_best": without_assembly(PB, asm_ms),
+        "plain_sweep_best_joined": without_assembly(PJ, asm_ms),
         "plain_sweep_reference": without_assembly(PR, asm_ms),
         "speedup_vs_strongest_plain": round(rate(L) / max(strongest, 1e-300), 3),
         "speedup_vs_reference_plain": round(rate(L) / max(rate(PR), 1e-300), 3),
@@ -607,6 +642,7 @@ def main() -> int:
         "spmv": single["spmv"],
         "mixed_precision": single["mixed"],
         "mixed_size_sweep": msz["mixed_sweep"],
+        "mixed_size_sweep_pipeline": msz_pipe["mixed_sweep"],
         "joined_factorization": bt["batch"],
         "gpu_100x50_plain": {k: v for k, v in small.items() if k != "lambdas"},
     }

@@ -37,8 +37,8 @@ assembled and then factored together by one launch sequence over their
 joined elimination levels (a tailor_batch, assembly and factorization
 replayed from one HIP graph), so every level launch covers the fronts of
 the whole group.  Two groups of contexts alternate: a
-worker thread assembles and factors group j + 1 while the main thread runs
-group j's Krylov-Schur, one instance after another.  Groups may mix grid
+worker thread assembles and factors group j + 1 while group j's Krylov-Schur
+runs, one thread per context.  Groups may mix grid
 sizes; each context keeps one size, so warm starts still follow a context's
 previous instance.
 """
@@ -439,21 +439,35 @@ class Sweeper:
                 raise errors[0]
             # group j + 1 takes the contexts group j - 1 has finished with
             worker = start(j + 1) if j + 1 < len(groups) else None
-            try:
-                for pos, k in enumerate(g):
-                    inst = cases[k][0]
-                    kk = key(j, pos, k)
-                    t0 = time.perf_counter()
-                    rec = self._krylov_record(self.ctx[kk], inst, complex(shifts[k]), form,
-                                              precision, warm_form and kk in solved, t0)
-                    solved.add(kk)
-                    result.records.append(rec)
-                    if on_record:
-                        on_record(rec)
-            except Exception:
+            # the group's Krylov-Schur runs, one thread per context, so their
+            # latency-bound steps overlap each other and the next group's
+            # factorization
+            recs, errs = [None] * len(g), []
+
+            def krylov(pos, k, warm):
+                try:
+                    recs[pos] = self._krylov_record(self.ctx[key(j, pos, k)], cases[k][0],
+                                                    complex(shifts[k]), form, precision, warm,
+                                                    time.perf_counter())
+                except Exception as e:  # re-raised below
+                    errs.append(e)
+
+            ths = [threading.Thread(target=krylov,
+                                    args=(pos, k, warm_form and key(j, pos, k) in solved))
+                   for pos, k in enumerate(g)]
+            for th in ths:
+                th.start()
+            for th in ths:
+                th.join()
+            if errs:
                 if worker:
                     worker.join()
-                raise
+                raise errs[0]
+            for pos, k in enumerate(g):
+                solved.add(key(j, pos, k))
+                result.records.append(recs[pos])
+                if on_record:
+                    on_record(recs[pos])
         result.seconds = time.perf_counter() - t_all
         return result
 
