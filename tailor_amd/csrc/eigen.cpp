@@ -79,7 +79,7 @@ void solver_release(SolverState *S) {
   double *bufs[] = {S->V, S->work, S->ritz, S->partials, S->small, S->hsteps};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
-  int *ib[] = {S->d_order, S->d_pos};
+  int *ib[] = {S->d_order, S->d_pos, S->d_xcol};
   for (int *b : ib)
     if (b) (void)hipFree(b);
   delete S;
@@ -105,6 +105,10 @@ int solver_prepare(tailor_ctx *ctx) {
   HIP_OK(hipMalloc(reinterpret_cast<void **>(&S.d_pos), pos.size() * sizeof(int)));
   HIP_OK(hipMemcpy(S.d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(S.d_pos, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice));
+  std::vector<int> xcol(ctx->h_colidx.size());
+  for (size_t k = 0; k < xcol.size(); ++k) xcol[k] = pos[ctx->h_colidx[k]];
+  HIP_OK(hipMalloc(reinterpret_cast<void **>(&S.d_xcol), std::max<size_t>(1, xcol.size()) * sizeof(int)));
+  HIP_OK(hipMemcpy(S.d_xcol, xcol.data(), xcol.size() * sizeof(int), hipMemcpyHostToDevice));
   S.n = 5LL * d.nodes;
   S.symbolic_ms = now_ms() - t0;
   S.plan_id = g_next_plan_id.fetch_add(1);
@@ -340,7 +344,7 @@ int refined_solve(tailor_ctx *ctx, const double *br, const double *bi, double *y
   double bnorm = -1.0;
   for (int it = 0; it <= refine_max; ++it) {
     // r = b - K y
-    HIP_OK(vec_kmul(ctx->stream, d.nodes, d.ssx * d.sse, ctx->d_vals, ctx->d_colidx, ctx->d_mvals,
+    HIP_OK(vec_kmul(ctx->stream, d.nodes, d.ssx * d.sse, ctx->d_vals, S.d_xcol, ctx->d_mvals,
                     S.d_pos, S.sigma.real(), S.sigma.imag(), yr, yi, rr, ri));
     HIP_OK(vec_axpby(ctx->stream, n, 1.0, 0.0, br, bi, -1.0, 0.0, rr, ri));
     double nr = 0.0, nb = 0.0;
@@ -460,7 +464,7 @@ int solver_time_kmul(tailor_ctx *ctx, int reps, double *ms) {
   double *xr = S.work, *xi = xr + n, *yr = S.work + 2 * n, *yi = yr + n;
   HIP_OK(lu_fill(ctx->stream, xr, 2 * n, 1.0));
   auto launch = [&]() {
-    return vec_kmul(ctx->stream, d.nodes, d.ssx * d.sse, ctx->d_vals, ctx->d_colidx, ctx->d_mvals,
+    return vec_kmul(ctx->stream, d.nodes, d.ssx * d.sse, ctx->d_vals, S.d_xcol, ctx->d_mvals,
                     S.d_pos, S.sigma.real(), S.sigma.imag(), xr, xi, yr, yi);
   };
   HIP_OK(launch());  // warm-up
@@ -676,7 +680,7 @@ int solver_krylov_schur(tailor_ctx *ctx, const tailor_solve_opts &o, tailor_solv
       HIP_OK(vec_combine(ctx->stream, n, m, ncheck, Vr, Vi, n, S.small, Rr, Ri, n));
       for (int i = 0; i < ncheck; ++i) {
         const cplx lam = S.sigma + 1.0 / theta[i];
-        HIP_OK(vec_kmul(ctx->stream, d.nodes, d.ssx * d.sse, ctx->d_vals, ctx->d_colidx, ctx->d_mvals,
+        HIP_OK(vec_kmul(ctx->stream, d.nodes, d.ssx * d.sse, ctx->d_vals, S.d_xcol, ctx->d_mvals,
                         S.d_pos, lam.real(), lam.imag(), Rr + i * n, Ri + i * n, yr, yi));
         double nr, nx;
         if (int st = solver_norm2(ctx, yr, yi, &nr)) return st;

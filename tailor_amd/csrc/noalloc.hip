@@ -95,7 +95,7 @@ extern "C" int tailor_linear_solve_noalloc(tailor_ctx *ctx, const double *b, dou
   if ((e = lu_solve(S.plan, br, bi, yr, yi, S.fp32, st)) != hipSuccess)
     return hip_fail(ctx, e, "lu_solve");
   for (int it = 0; it < refine_steps; ++it) {
-    if ((e = vec_kmul(st, d.nodes, d.ssx * d.sse, ctx->d_vals, ctx->d_colidx, ctx->d_mvals,
+    if ((e = vec_kmul(st, d.nodes, d.ssx * d.sse, ctx->d_vals, S.d_xcol, ctx->d_mvals,
                       S.d_pos, S.sigma.real(), S.sigma.imag(), yr, yi, rr, ri)) != hipSuccess)
       return hip_fail(ctx, e, "vec_kmul");
     if ((e = vec_axpby(st, n, 1.0, 0.0, br, bi, -1.0, 0.0, rr, ri)) != hipSuccess)

@@ -23,6 +23,7 @@ struct SolverState {
   int perturbed = 0;
   double factor_ms = 0.0, symbolic_ms = 0.0;
   int *d_order = nullptr, *d_pos = nullptr;
+  int *d_xcol = nullptr;  // per stored block: its column node in elimination order (SpMV)
   double *V = nullptr, *work = nullptr, *ritz = nullptr, *partials = nullptr, *small = nullptr;
   size_t V_cap = 0, work_cap = 0, ritz_cap = 0, partials_cap = 0, small_cap = 0;
   double *hsteps = nullptr;  // Gram-Schmidt coefficients of one Krylov pass (device)

@@ -51,10 +51,19 @@ __global__ void k_bmul(long long nodes, const int *__restrict__ order, const dou
 }
 
 // y = A x - mu B x in elimination order.  One wave per block row: the row's
-// 77 x 25 entries are contiguous, each lane accumulates per block-row entry
-// r, then the wave reduces the five row sums.
+// 77 x 25 entries are contiguous and each lane takes every 64th of them,
+// accumulating per block-row entry r; the wave then reduces the five row
+// sums.  xcol[k] is block k's column node in elimination order
+// (pos[colidx[k]], built once per structure), so an entry's x needs one
+// dependent load, and each lane issues kKmulSteps steps' loads (entries
+// idx, idx + 64, ...) before using them; the sums run over a lane's entries
+// in the same order either way.
+// Steps of loads in flight per lane (200x100, 633 MB per application:
+// 2 steps 133 us, 4 steps 139 us, 8 steps 150 us; one step, with the column
+// position looked up through pos[colidx[k]], 174 us)
+constexpr int kKmulSteps = 2;
 __global__ __launch_bounds__(VT) void k_kmul(long long nodes, int per, const double *__restrict__ avals,
-                                             const int *__restrict__ colidx,
+                                             const int *__restrict__ xcol,
                                              const double *__restrict__ bvals,
                                              const int *__restrict__ pos, double mu_re, double mu_im,
                                              const double *__restrict__ xr,
@@ -64,17 +73,12 @@ __global__ __launch_bounds__(VT) void k_kmul(long long nodes, int per, const dou
   const int lane = threadIdx.x & 63;
   if (u >= nodes) return;
   double ar[5] = {0, 0, 0, 0, 0}, ai[5] = {0, 0, 0, 0, 0};
-  const long long base = u * per * 25;
+  const double2 *arow = reinterpret_cast<const double2 *>(avals) + u * per * 25;
+  const int *xrow = xcol + u * per;
   const int total = per * 25;
-  for (int idx = lane; idx < total; idx += 64) {
-    const int k = idx / 25, e = idx - 25 * (idx / 25);
-    const int r = e / 5, c = e - 5 * (e / 5);
-    // one 16-byte load per complex entry (interleaved re, im)
-    const double2 a = reinterpret_cast<const double2 *>(avals)[base + idx];
-    const double a_r = a.x, a_i = a.y;
-    const long long xc = 5LL * pos[colidx[u * per + k]] + c;
-    const double x_r = xr[xc], x_i = xi[xc];
-    const double pr = a_r * x_r - a_i * x_i, pi = a_r * x_i + a_i * x_r;
+  auto add = [&](int idx, double2 a, double x_r, double x_i) {
+    const int e = idx - 25 * (idx / 25), r = e / 5;
+    const double pr = a.x * x_r - a.y * x_i, pi = a.x * x_i + a.y * x_r;
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
       if (q == r) {
@@ -82,6 +86,31 @@ __global__ __launch_bounds__(VT) void k_kmul(long long nodes, int per, const dou
         ai[q] += pi;
       }
     }
+  };
+  constexpr int U = kKmulSteps;
+  int idx = lane;
+  for (; idx + 64 * (U - 1) < total; idx += 64 * U) {
+    double2 a[U];
+    long long xc[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const int i = idx + 64 * q, k = i / 25;
+      a[q] = arow[i];
+      xc[q] = 5LL * xrow[k] + (i - 25 * k) % 5;
+    }
+    double x_r[U], x_i[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      x_r[q] = xr[xc[q]];
+      x_i[q] = xi[xc[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < U; ++q) add(idx + 64 * q, a[q], x_r[q], x_i[q]);
+  }
+  for (; idx < total; idx += 64) {
+    const int k = idx / 25;
+    const long long x0 = 5LL * xrow[k] + (idx - 25 * k) % 5;
+    add(idx, arow[idx], xr[x0], xi[x0]);
   }
 #pragma unroll
   for (int q = 0; q < 5; ++q) {
@@ -311,11 +340,11 @@ hipError_t vec_bmul(hipStream_t s, long long nodes, const int *order, const doub
   return hipGetLastError();
 }
 
-hipError_t vec_kmul(hipStream_t s, long long nodes, int per, const double *avals, const int *colidx,
+hipError_t vec_kmul(hipStream_t s, long long nodes, int per, const double *avals, const int *xcol,
                     const double *bvals, const int *pos, double mu_re, double mu_im,
                     const double *xr, const double *xi, double *yr, double *yi) {
   const unsigned g = static_cast<unsigned>((nodes + VT / 64 - 1) / (VT / 64));
-  hipLaunchKernelGGL(k_kmul, dim3(g), dim3(VT), 0, s, nodes, per, avals, colidx, bvals, pos, mu_re,
+  hipLaunchKernelGGL(k_kmul, dim3(g), dim3(VT), 0, s, nodes, per, avals, xcol, bvals, pos, mu_re,
                      mu_im, xr, xi, yr, yi);
   return hipGetLastError();
 }

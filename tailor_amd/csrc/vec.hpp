@@ -13,7 +13,9 @@ hipError_t vec_elim_to_nat(hipStream_t s, long long n, const int *perm, const do
                            const double *si, double *dst);
 hipError_t vec_bmul(hipStream_t s, long long nodes, const int *order, const double *bvals,
                     const double *xr, const double *xi, double *yr, double *yi);
-hipError_t vec_kmul(hipStream_t s, long long nodes, int per, const double *avals, const int *colidx,
+// y = (A - mu B) x in elimination order; xcol[k]: block k's column node in
+// elimination order, pos: node -> elimination position
+hipError_t vec_kmul(hipStream_t s, long long nodes, int per, const double *avals, const int *xcol,
                     const double *bvals, const int *pos, double mu_re, double mu_im,
                     const double *xr, const double *xi, double *yr, double *yi);
 hipError_t vec_dots(hipStream_t s, long long n, int ncols, const double *Vr, const double *Vi,
