@@ -728,7 +728,8 @@ __global__ __launch_bounds__(64) void k_characteristics(CaseDims d, const double
 // writer (doubles): K0, Kxi, Keta (25 complex each), Vxixi, Vxieta, Vetaeta
 // (25 real each) and Gamma (25 real).
 constexpr int kCoefDoubles = 250;
-constexpr int kCoefNodes = 8;  // nodes per k_coefficients workgroup
+// nodes per k_coefficients workgroup (16 or 32 measured the same at 200x100)
+constexpr int kCoefNodes = 8;
 
 // Physical LNS blocks (LNSCoefficients.cpp:173-404), their finiteness, and
 // the Fourier (StreamwiseFourier.cpp:24-41) and curvilinear
@@ -743,8 +744,8 @@ __global__ __launch_bounds__(kThreads) void k_coefficients(
   const int tid = threadIdx.x;
   const long long first = static_cast<long long>(blockIdx.x) * kCoefNodes;
   // six threads per node evaluate the physical blocks
-  if (tid < 6 * kCoefNodes) {
-    const int n = tid / 6, part = tid - 6 * (tid / 6);
+  for (int t6 = tid; t6 < 6 * kCoefNodes; t6 += kThreads) {
+    const int n = t6 / 6, part = t6 - 6 * (t6 / 6);
     const long long node = first + n;
     if (node < d.nodes) {
       Point p;
@@ -758,9 +759,9 @@ __global__ __launch_bounds__(kThreads) void k_coefficients(
     }
   }
   __syncthreads();
-  if (tid < 5 * kCoefNodes) {
+  for (int t5 = tid; t5 < 5 * kCoefNodes; t5 += kThreads) {
     // finiteness of the physical blocks (NonFiniteOutput of evaluate)
-    const int n = tid / 5, part = tid - 5 * (tid / 5);
+    const int n = t5 / 5, part = t5 - 5 * (t5 / 5);
     if (first + n < d.nodes && st[n] == 0) {
       const int b0 = part == 0 ? 0 : part, b1 = part == 0 ? 1 : part + 1;
       bool ok = true;
@@ -774,8 +775,8 @@ __global__ __launch_bounds__(kThreads) void k_coefficients(
   }
   __syncthreads();
   // 25 threads per node: Fourier and curvilinear entries
-  if (tid < 25 * kCoefNodes) {
-    const int n = tid / 25, e = tid - 25 * (tid / 25);
+  for (int t25 = tid; t25 < 25 * kCoefNodes; t25 += kThreads) {
+    const int n = t25 / 25, e = t25 - 25 * (t25 / 25);
     const long long node = first + n;
     if (node < d.nodes && st[n] == 0) {
       const double alpha_re = recipe[kRecipeAlphaRe], alpha_im = recipe[kRecipeAlphaIm];
@@ -822,7 +823,8 @@ __global__ __launch_bounds__(kThreads) void k_coefficients(
     }
   }
   __syncthreads();
-  if (tid < kCoefNodes && first + tid < d.nodes) status[first + tid] = st[tid];
+  for (int n = tid; n < kCoefNodes; n += kThreads)
+    if (first + n < d.nodes) status[first + n] = st[n];
 }
 
 constexpr int kOperatorRows = 8;  // block rows (waves) per writer workgroup
