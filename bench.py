@@ -520,10 +520,13 @@ def phase_main(args) -> int:
     import tailor_amd as ta
     from tailor_amd import case as cs
     if args.phase == "single":
+        peaks = ta.measure_peaks(0)
+        log(f"measured ceilings: FP64 MFMA {peaks['fp64_mfma_tflops']:.1f} TFLOP/s, "
+            f"HBM read {peaks['read_gbs']:.0f} GB/s")
         single = plain_step(ta, cs)
         small = gpu_small_case(ta, cs, REFERENCE_NCV)
         log("100x50 plain instance done")
-        out = {"single": single, "small": small}
+        out = {"single": single, "small": small, "peaks": peaks}
     elif args.phase == "pipeline":
         out = phase_pipeline(ta, args.steps, args.warmup, args.best_ncv, args.learned_ncv,
                              args.contexts)
@@ -570,7 +573,8 @@ def main() -> int:
 
     # every GPU phase in a child process; this process never touches the GPU
     sp = run_phase("single", args)
-    single, small = sp["single"], sp["small"]
+    single, small, peaks = sp["single"], sp["small"], sp["peaks"]
+    mfma_meas, read_meas = peaks["fp64_mfma_tflops"], peaks["read_gbs"]
     best_ncv = single["plain"]["best_ncv"]
     pipe = run_phase("pipeline", args, ("--best-ncv", str(best_ncv)))
     ref = run_phase("reference", args)
@@ -624,6 +628,12 @@ def main() -> int:
         "roofline": {"bound": "mfma", "kernel": "k_gemm (frontal updates, MFMA f64 16x16x4)",
                      "achieved": round(gemm_tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(gemm_tf / FP64_PEAK_TFLOPS, 4),
+                     "measured_peak": round(mfma_meas, 2),
+                     "frac_of_measured_peak": round(gemm_tf / mfma_meas, 4),
+                     "measured_peak_note": "sustained dense FP64 MFMA rate of this GPU "
+                                           "(tailor_measure_peaks: v_mfma_f64_16x16x4f64 back "
+                                           "to back, two waves per SIMD, every CU); the frontal "
+                                           "GEMM's 3M form issues 6 of the 8 counted flops",
                      "traffic": measured_traffic("k_gemm<double>"),
                      "traffic_note": "mean HBM bytes per k_gemm dispatch (counter runs in profiles/)",
                      "timing": "HIP events around every k_gemm launch of one serialized "
@@ -637,9 +647,15 @@ def main() -> int:
                      "factor_ms": round(g["factor_ms"], 2)},
         "roofline_traced": None,
         "plain_instance": single["plain"],
-        "assembly": single["assembly"],
-        "solve_sweep": single["solve_sweep"],
-        "spmv": single["spmv"],
+        "measured_ceilings": {"fp64_mfma_tflops": round(mfma_meas, 2),
+                              "hbm_read_gbs": round(read_meas, 1),
+                              "published": {"fp64_mfma_tflops": FP64_PEAK_TFLOPS,
+                                            "hbm_gbs": HBM_PEAK_GBS}},
+        "assembly": {**single["assembly"],
+                     "frac_of_measured_read_full": round(single["assembly"]["GBps_full"] / read_meas, 4)},
+        "solve_sweep": {**single["solve_sweep"],
+                        "frac_of_measured_read": round(single["solve_sweep"]["GBps"] / read_meas, 4)},
+        "spmv": {**single["spmv"], "frac_of_measured_read": round(single["spmv"]["GBps"] / read_meas, 4)},
         "mixed_precision": single["mixed"],
         "mixed_size_sweep": msz["mixed_sweep"],
         "mixed_size_sweep_pipeline": msz_pipe["mixed_sweep"],
@@ -651,6 +667,7 @@ def main() -> int:
         tf = g["gemm_flops"] / (tw["ms"] * 1e-3) / 1e12
         out["roofline_traced"] = {**tw, "achieved": round(tf, 3),
                                   "frac": round(tf / FP64_PEAK_TFLOPS, 4),
+                                  "frac_of_measured_peak": round(tf / mfma_meas, 4),
                                   "note": "the same factorization's k_gemm launches timed by a "
                                           "rocprofv3 kernel trace of this benchmark"}
     if not args.no_cpu_baseline:

@@ -364,6 +364,15 @@ int tailor_eigenpair_count(const tailor_ctx *ctx);
 int tailor_get_eigenpair(tailor_ctx *ctx, int32_t k, double *lambda, double *q, double *residual,
                          double *relative_error);
 
+/* The rates device `device` sustains for the two resources the path's
+ * kernels are bound by, each measured with a kernel that does nothing else:
+ * dense FP64 MFMA (v_mfma_f64_16x16x4f64 back to back on independent
+ * accumulators, two waves per SIMD, every CU) in TFLOP/s, and a streaming
+ * HBM read of a 2 GiB buffer in GB/s (either pointer may be NULL).  The
+ * attainable ceilings beside the published peaks; allocates and frees 2 GiB
+ * when read_gbs is requested.  No reference counterpart (measurement only). */
+int tailor_measure_peaks(int device, double *fp64_mfma_tflops, double *read_gbs);
+
 /* ------------------------------------------------------------------------
  * Shift-proposal network (no reference counterpart).  A 3 -> 64 -> 64 -> 16
  * perceptron with GELU activations in FP32: normalised (alpha, Re, Ma) in,

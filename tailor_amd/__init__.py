@@ -164,6 +164,7 @@ def library() -> C.CDLL:
                                                  P(C.c_double)]
     lib.tailor_batch_autotune.argtypes = [C.c_void_p, C.c_double, C.c_double, P(TuneReport)]
     lib.tailor_batch_set_gemm_config.argtypes = [C.c_void_p, C.c_int32]
+    lib.tailor_measure_peaks.argtypes = [C.c_int, P(C.c_double), P(C.c_double)]
     lib.tailor_eigenpair_count.argtypes = [C.c_void_p]
     lib.tailor_get_eigenpair.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double),
                                          P(C.c_double), P(C.c_double)]
@@ -417,6 +418,17 @@ class Batch:
 
     def set_gemm_config(self, config: int) -> None:
         self._check(self.lib.tailor_batch_set_gemm_config(self.ptr, int(config)))
+
+
+def measure_peaks(device: int = 0) -> dict:
+    """The device's sustained dense FP64 MFMA rate (TFLOP/s) and streaming
+    HBM read rate (GB/s), each from a kernel that does nothing else."""
+    lib = library()
+    tf, gbs = C.c_double(), C.c_double()
+    st = lib.tailor_measure_peaks(int(device), C.byref(tf), C.byref(gbs))
+    if st:
+        raise TailorError(st, "tailor_measure_peaks failed")
+    return {"fp64_mfma_tflops": tf.value, "read_gbs": gbs.value}
 
 
 def _linear_solve(self, b: np.ndarray, refine_max: int = 0, refine_tol: float = 1e-14):

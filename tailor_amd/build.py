@@ -27,7 +27,7 @@ PER_FILE = {
     "lu.hip": ["-munsafe-fp-atomics"],
 }
 SOURCES = ["assembly.hip", "lu.hip", "vec.hip", "capi.cpp", "eigen.cpp", "symbolic.cpp",
-           "lu_plan.cpp", "dense.cpp", "mlp.hip", "noalloc.hip", "batch.cpp"]
+           "lu_plan.cpp", "dense.cpp", "mlp.hip", "noalloc.hip", "batch.cpp", "calibrate.hip"]
 
 
 def _compile(src: str) -> Path:

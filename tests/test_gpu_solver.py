@@ -277,3 +277,12 @@ def test_solve_pair_matches_single_solves(oracle_lib, shape, precision):
     fp64 = precision == ta.PRECISION_FP64
     assert d1 <= (1e-12 if fp64 else 1e-6) and d2 <= (1e-12 if fp64 else 1e-6)
     assert r1 <= (1e-9 if fp64 else 1e-2) and r2 <= (1e-9 if fp64 else 1e-2)
+
+
+def test_measured_peaks_are_plausible():
+    """tailor_measure_peaks: the sustained FP64 MFMA and HBM read rates lie
+    between a tenth of and the published peaks (78.6 TFLOP/s, 8 TB/s)."""
+    peaks = ta.measure_peaks(0)
+    print(f"measured: {peaks}", flush=True)
+    assert 7.86 < peaks["fp64_mfma_tflops"] <= 78.6 * 1.05
+    assert 800.0 < peaks["read_gbs"] <= 8000.0 * 1.05
