@@ -390,14 +390,14 @@ def phase_reference(ta, steps, warmup) -> dict:
     return {"plain_ref": plain_ref.totals()}
 
 
-def phase_joined_plain(ta, steps, warmup, best_ncv) -> dict:
+def phase_joined_plain(ta, steps, warmup, best_ncv, threads=True) -> dict:
     """The plain form at its fastest subspace through joined groups of
     JOINED_GROUP instances (assembly and joined factorization of a group
     replayed from one HIP graph, the group's Krylov-Schur runs side by side
     while the next group is factored)."""
     from tailor_amd import sweep as sw
     insts, cases, shifts, warm, timed = headline_instances(ta, steps, warmup)
-    runner = sw.Sweeper(ncv=best_ncv, omega_s=OMEGA, batch=JOINED_GROUP)
+    runner = sw.Sweeper(ncv=best_ncv, omega_s=OMEGA, batch=JOINED_GROUP, threads=threads)
     runner.run(cases, "plain", precision=ta.PRECISION_FP64, order=warm[:1])  # plans, graphs
     res = runner.run(cases, "plain", precision=ta.PRECISION_FP64, order=timed)
     runner.close()
@@ -418,8 +418,8 @@ def phase_mixed_size(ta, count, ncv, contexts, joined=True) -> dict:
     from tailor_amd import sweep as sw
     c3 = sw.sweep_instances(count + 2, seed=1, sizes=[(100, 50), (200, 100)])
     cases = sw.prepare(c3, OMEGA)
-    runner = (sw.Sweeper(ncv=ncv, omega_s=OMEGA, batch=JOINED_GROUP) if joined
-              else sw.Sweeper(ncv=ncv, omega_s=OMEGA, contexts=contexts))
+    runner = (sw.Sweeper(ncv=ncv, omega_s=OMEGA, batch=JOINED_GROUP, threads=contexts > 1)
+              if joined else sw.Sweeper(ncv=ncv, omega_s=OMEGA, contexts=contexts))
     runner.run(cases, "mixed", order=[0, 1])  # plans and tuning of both sizes
     per_size, good, secs = {}, 0, 0.0
     totals = {"instances": 0, "factorizations": 0, "restarts": 0, "solves": 0,
@@ -535,7 +535,8 @@ def phase_main(args) -> int:
     elif args.phase == "batch":
         out = phase_batch(ta)
     elif args.phase == "joined":
-        out = phase_joined_plain(ta, args.steps, args.warmup, args.best_ncv)
+        out = phase_joined_plain(ta, args.steps, args.warmup, args.best_ncv,
+                                 threads=args.contexts > 1)
     elif args.phase == "mixed_size":
         out = phase_mixed_size(ta, args.mixed_count, args.best_ncv, args.contexts, joined=True)
     elif args.phase == "mixed_size_pipeline":
@@ -555,7 +556,8 @@ def main() -> int:
     ap.add_argument("--learned-ncv", type=int, default=LEARNED_NCV)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--contexts", type=int, default=PIPELINE_CONTEXTS,
-                    help="contexts of the pipelined sweeps (1: no worker threads)")
+                    help="contexts of the pipelined sweeps (1: no worker threads anywhere, "
+                         "also in the joined sweeps; for kernel traces)")
     ap.add_argument("--phase", help=argparse.SUPPRESS)
     ap.add_argument("--best-ncv", type=int, default=REFERENCE_NCV, help=argparse.SUPPRESS)
     ap.add_argument("--mixed-count", type=int, default=MIXED_SIZE_INSTANCES, help=argparse.SUPPRESS)

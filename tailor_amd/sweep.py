@@ -145,7 +145,8 @@ _WARM_FORMS = ("learned", "fixed")
 class Sweeper:
     def __init__(self, device: int = 0, nev: int = 8, tol: float = 1e-10, ncv: int = 64,
                  omega_s: complex = 0.02 + 0j, maxit: int = 1000, contexts: int = 1,
-                 check_from: int = 0, graph: bool = False, batch: int = 0):
+                 check_from: int = 0, graph: bool = False, batch: int = 0,
+                 threads: bool = True):
         """`check_from` > 0: the learned form's Krylov-Schur runs also test
         convergence after every second step from that step on and end a pass
         early (the plain form and the fallback test only at pass ends).
@@ -160,6 +161,10 @@ class Sweeper:
             raise ValueError("batch must be 0 to 8")
         self.batch = batch
         self.batches = {}  # joined factorizations, by the contexts they join
+        # joined groups: False runs each group's assembly, factorization and
+        # Krylov-Schur runs one after another on the calling thread (for
+        # profilers that cannot follow launches from several threads)
+        self.threads = threads
         if check_from < 0:
             raise ValueError("check_from must be >= 0")
         self.check_from = check_from
@@ -426,7 +431,14 @@ class Sweeper:
             except Exception as e:  # re-raised by the main thread
                 errors.append(e)
 
+        class _Done:
+            def join(self):
+                pass
+
         def start(j):
+            if not self.threads:
+                stage(j)
+                return _Done()
             th = threading.Thread(target=stage, args=(j,))
             th.start()
             return th
@@ -452,13 +464,16 @@ class Sweeper:
                 except Exception as e:  # re-raised below
                     errs.append(e)
 
-            ths = [threading.Thread(target=krylov,
-                                    args=(pos, k, warm_form and key(j, pos, k) in solved))
-                   for pos, k in enumerate(g)]
-            for th in ths:
-                th.start()
-            for th in ths:
-                th.join()
+            jobs = [(pos, k, warm_form and key(j, pos, k) in solved) for pos, k in enumerate(g)]
+            if self.threads:
+                ths = [threading.Thread(target=krylov, args=job) for job in jobs]
+                for th in ths:
+                    th.start()
+                for th in ths:
+                    th.join()
+            else:
+                for job in jobs:
+                    krylov(*job)
             if errs:
                 if worker:
                     worker.join()
