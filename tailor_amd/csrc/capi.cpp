@@ -184,9 +184,6 @@ int tailor_create(tailor_ctx **out, int device) {
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = tailor::noalloc_setup(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
   if (e == hipSuccess) e = dalloc(&ctx->d_diag, 1);
   if (e == hipSuccess) e = dalloc(&ctx->d_recipe, tailor::kRecipeDoubles);
   if (e == hipSuccess)
@@ -209,10 +206,6 @@ void tailor_destroy(tailor_ctx *ctx) {
   dfree(ctx->d_diag);
   dfree(ctx->d_recipe);
   dfree(ctx->d_partials);
-  if (ctx->aux) (void)hipStreamSynchronize(ctx->aux);
-  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
-  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
-  if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -428,16 +421,12 @@ hipError_t enqueue_assembly(tailor_ctx *ctx, bool constrained, hipStream_t s) {
   if ((e = launch_reset_diag(ctx, s)) != hipSuccess) return e;
   if ((e = launch_metrics(ctx, s)) != hipSuccess) return e;
   if ((e = launch_derivatives(ctx, s)) != hipSuccess) return e;
-  if (constrained) {
-    // the far-field characteristics (one thread per far-field node, a
-    // latency-bound chain) beside the coefficient records
-    if ((e = hipEventRecord(ctx->ev_fork, s)) != hipSuccess) return e;
-    if ((e = hipStreamWaitEvent(ctx->aux, ctx->ev_fork, 0)) != hipSuccess) return e;
-    if ((e = launch_characteristics(ctx, ctx->aux)) != hipSuccess) return e;
-    if ((e = hipEventRecord(ctx->ev_join, ctx->aux)) != hipSuccess) return e;
-  }
+  // (the far-field characteristics on a second stream per context, beside
+  // the coefficient records, saved 10 us per 200x100 assembly but the extra
+  // stream per context slowed the eight-context sweeps: every stream a
+  // process creates takes a hardware queue)
+  if (constrained && (e = launch_characteristics(ctx, s)) != hipSuccess) return e;
   if ((e = launch_coefficients(ctx, s)) != hipSuccess) return e;
-  if (constrained && (e = hipStreamWaitEvent(s, ctx->ev_join, 0)) != hipSuccess) return e;
   return launch_operator(ctx, constrained, s);
 }
 

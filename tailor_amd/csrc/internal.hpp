@@ -53,10 +53,6 @@ struct CaseDims {
 struct tailor_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  // the far-field characteristics of an assembly run here, beside the
-  // coefficient records (both need only the derivatives)
-  hipStream_t aux = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::string error;
   bool loaded = false;
   bool assembled = false;
