@@ -219,6 +219,14 @@ int tailor_set_gemm_config(tailor_ctx *ctx, int32_t config);
 #define TAILOR_PANEL_STREAMING 1
 int tailor_set_panel_kernel(tailor_ctx *ctx, int32_t kernel);
 
+/* Two-level blocking of the factorization: on (the default), pairs of
+ * 128-column blocks share one update of the rows below them, so the
+ * trailing-update GEMMs run with K = 256 instead of 128 (the second block's
+ * panels then run without look-ahead); off, every block updates on its own.
+ * The factors agree to rounding (200x100: frontal GEMM 107 -> 102 ms per
+ * factorization).  No reference counterpart (performance only). */
+int tailor_set_block_pairs(tailor_ctx *ctx, int32_t on);
+
 /* ------------------------------------------------------------------------
  * Batches: the factorizations of several contexts as one launch sequence.
  * A batch joins the fronts of `count` distinct contexts of one device (each

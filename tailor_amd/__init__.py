@@ -146,6 +146,7 @@ def library() -> C.CDLL:
     lib.tailor_autotune.argtypes = [C.c_void_p, C.c_double, C.c_double, P(TuneReport)]
     lib.tailor_set_gemm_config.argtypes = [C.c_void_p, C.c_int32]
     lib.tailor_set_panel_kernel.argtypes = [C.c_void_p, C.c_int32]
+    lib.tailor_set_block_pairs.argtypes = [C.c_void_p, C.c_int32]
     lib.tailor_profile_factor.argtypes = [C.c_void_p, C.c_double, C.c_double, P(C.c_double),
                                           P(C.c_double), P(C.c_double), P(C.c_int32)]
     lib.tailor_time_linear_solve.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double)]
@@ -527,6 +528,7 @@ Context.profile_factor = _profile_factor
 Context.autotune = _autotune
 Context.set_gemm_config = lambda self, c: self._check(self.lib.tailor_set_gemm_config(self.ptr, int(c)))
 Context.set_panel_kernel = lambda self, k: self._check(self.lib.tailor_set_panel_kernel(self.ptr, int(k)))
+Context.set_block_pairs = lambda self, on: self._check(self.lib.tailor_set_block_pairs(self.ptr, 1 if on else 0))
 Context.linear_solve = _linear_solve
 Context.solve = _solve
 Context.eigenpairs = _eigenpairs

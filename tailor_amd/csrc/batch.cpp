@@ -41,7 +41,7 @@ struct tailor_batch {
   struct Graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    bool fp32 = false, streaming = false;
+    bool fp32 = false, streaming = false, pairs = false;
     std::vector<int> shape, order;
   } graphs[2];
   hipEvent_t ev_start = nullptr, ev_done = nullptr;
@@ -186,7 +186,7 @@ int graph_ready(tailor_batch *bt, bool assembles, bool fp32) {
   const tailor::LuPlan &U = bt->plan;
   auto &G = bt->graphs[assembles ? 1 : 0];
   if (G.exec && G.fp32 == fp32 && G.shape == U.gemm_shape && G.order == U.gemm_order &&
-      G.streaming == U.panel_streaming)
+      G.streaming == U.panel_streaming && G.pairs == U.pair_blocks)
     return TAILOR_OK;
   if (G.exec) (void)hipGraphExecDestroy(G.exec);
   if (G.graph) (void)hipGraphDestroy(G.graph);
@@ -209,6 +209,7 @@ int graph_ready(tailor_batch *bt, bool assembles, bool fp32) {
   G.shape = U.gemm_shape;
   G.order = U.gemm_order;
   G.streaming = U.panel_streaming;
+  G.pairs = U.pair_blocks;
   return TAILOR_OK;
 }
 

@@ -714,6 +714,15 @@ int tailor_set_panel_kernel(tailor_ctx *ctx, int32_t kernel) {
   return TAILOR_OK;
 }
 
+int tailor_set_block_pairs(tailor_ctx *ctx, int32_t on) {
+  if (!ctx) return TAILOR_ERR_USER_INPUT;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (int st = tailor::solver_prepare(ctx)) return st;
+  ctx->solver->plan.pair_blocks = on != 0;
+  ctx->solver->factored = false;
+  return TAILOR_OK;
+}
+
 int tailor_set_precision(tailor_ctx *ctx, int32_t precision) {
   if (!ctx) return TAILOR_ERR_USER_INPUT;
   if (precision != TAILOR_PRECISION_FP64 && precision != TAILOR_PRECISION_MIXED)

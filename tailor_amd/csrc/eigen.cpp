@@ -177,7 +177,8 @@ int graph_ready(tailor_ctx *ctx) {
   SolverState &S = *ctx->solver;
   const bool fp32 = ctx->precision == TAILOR_PRECISION_MIXED;
   if (S.graph_exec && S.graph_fp32 == fp32 && S.graph_shape == S.plan.gemm_shape &&
-      S.graph_order == S.plan.gemm_order && S.graph_streaming == S.plan.panel_streaming)
+      S.graph_order == S.plan.gemm_order && S.graph_streaming == S.plan.panel_streaming &&
+      S.graph_pairs == S.plan.pair_blocks)
     return TAILOR_OK;
   release_graphs(&S);
   const LuInputs in = factor_inputs(ctx);
@@ -200,6 +201,7 @@ int graph_ready(tailor_ctx *ctx) {
   S.graph_shape = S.plan.gemm_shape;
   S.graph_order = S.plan.gemm_order;
   S.graph_streaming = S.plan.panel_streaming;
+  S.graph_pairs = S.plan.pair_blocks;
   return TAILOR_OK;
 }
 

@@ -1174,22 +1174,30 @@ __device__ inline void trailing_column(const FrontDev &d, int K1, int col, T *ar
   }
 }
 
-// The listed row moves on F columns [K1, m) and E: a wave per column, lanes
-// over the moved rows; all of a column's loads complete before its stores.
+// The listed row moves on F columns [K1, m) and E, or (c1 > c0) on F
+// columns [c0, c1) only (an earlier block's L columns): a wave per column,
+// lanes over the moved rows; all of a column's loads complete before its
+// stores.
 template <typename T>
 __global__ __launch_bounds__(256) void k_apply_perm(const int *__restrict__ list, int nlist,
                                                     const FrontDev *__restrict__ fr, int K0, int NB,
                                                     const int *__restrict__ bperm,
-                                                    T *__restrict__ arena) {
+                                                    T *__restrict__ arena, int c0, int c1) {
   const FrontDev d = fr[list[blockIdx.y]];
   if (K0 >= d.p) return;
   const int K1 = min(K0 + NB, d.p);
   const int *pl = bperm + static_cast<long long>(blockIdx.y) * kPermStride;
   const int n = pl[0];
   const int col = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
-  if (n == 0 || col >= d.m - K1 + d.p) return;
+  const bool range = c1 > c0;
+  if (n == 0 || col >= (range ? c1 - c0 : d.m - K1 + d.p)) return;
   T *Xr, *Xi;
-  trailing_column(d, K1, col, arena, Xr, Xi);
+  if (range) {
+    Xr = arena + d.off + static_cast<long long>(c0 + col) * d.m;
+    Xi = Xr + static_cast<long long>(d.m) * d.m;
+  } else {
+    trailing_column(d, K1, col, arena, Xr, Xi);
+  }
   constexpr int kPer = 2 * kLuNbOuter / 64;
   T vr[kPer], vi[kPer];
 #pragma unroll
@@ -1214,12 +1222,13 @@ __global__ __launch_bounds__(256) void k_apply_perm(const int *__restrict__ list
 }
 
 // Fallback for fronts whose rows [K0, p) exceed the row map: the exchanges
-// of the block applied in order, a thread per trailing column.
+// of the block applied in order, a thread per trailing column (or, c1 > c0,
+// per F column of [c0, c1)).
 template <typename T>
 __global__ __launch_bounds__(256) void k_block_swaps(const int *__restrict__ list, int nlist,
                                                      const FrontDev *__restrict__ fr, int K0, int NB,
                                                      const int *__restrict__ ipiv,
-                                                     T *__restrict__ arena) {
+                                                     T *__restrict__ arena, int c0, int c1) {
   const FrontDev d = fr[list[blockIdx.y]];
   if (K0 >= d.p || d.p - K0 <= kPanelMapRows) return;
   const int K1 = min(K0 + NB, d.p);
@@ -1227,9 +1236,15 @@ __global__ __launch_bounds__(256) void k_block_swaps(const int *__restrict__ lis
   if (threadIdx.x < K1 - K0) pv[threadIdx.x] = ipiv[d.piv_off + K0 + threadIdx.x];
   __syncthreads();
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= d.m - K1 + d.p) return;
+  const bool range = c1 > c0;
+  if (col >= (range ? c1 - c0 : d.m - K1 + d.p)) return;
   T *Xr, *Xi;
-  trailing_column(d, K1, col, arena, Xr, Xi);
+  if (range) {
+    Xr = arena + d.off + static_cast<long long>(c0 + col) * d.m;
+    Xi = Xr + static_cast<long long>(d.m) * d.m;
+  } else {
+    trailing_column(d, K1, col, arena, Xr, Xi);
+  }
   for (int r = K0; r < K1; ++r) {
     const int q = pv[r - K0];
     if (q != r) {
@@ -1245,9 +1260,15 @@ __global__ __launch_bounds__(256) void k_block_swaps(const int *__restrict__ lis
 // built block by block from the last: within block [K0, K1) each panel's
 // rows are finished and ZIN updates the block's rows above the panel
 // (K = 32); ZBLK then updates all rows above the block at once (K = 128).
+// With paired blocks (two-level blocking, LuPlan::pair_blocks), block A's
+// update of the rows below it is deferred until block B = [K1, K1n) is
+// factored: PAIRROWS applies it to B's rows of the columns right of B and
+// of E (K = 128), and the pair's TRAILHEAD / TRAILREST / TRAIL then update
+// everything below B with K = 256 (the two blocks' L columns and U rows are
+// contiguous).
 enum GemmRegion {
   kGemmInner = 0, kGemmBlkRows = 1, kGemmTrail = 2, kGemmZ = 3, kGemmTrailHead = 4,
-  kGemmTrailRest = 5, kGemmZBlk = 6
+  kGemmTrailRest = 5, kGemmZBlk = 6, kGemmPairRows = 7
 };
 
 // Operands of one front's update.  Columns [0, N1) of B and C come from the
@@ -1273,11 +1294,11 @@ __host__ __device__ inline GemmShape gemm_shape(int region, const FrontDev &d, i
                                                 int K0, int NB) {
   GemmShape g{0, 0, 0, 0, 0};
   const bool trail = region == kGemmTrail || region == kGemmTrailHead ||
-                     region == kGemmTrailRest || region == kGemmZBlk;
+                     region == kGemmTrailRest || region == kGemmZBlk || region == kGemmPairRows;
   if (trail ? K0 >= d.p : k0 >= d.p) return g;
   const int kb = min(nb, d.p - k0);
-  const int K1 = min(K0 + NB, d.p);
-  const int K1n = min(K1 + NB, d.p);  // end of the next block
+  const int K1 = min(K0 + NB, d.p);           // end of the block (or pair) [K0, K1)
+  const int K1n = min(K1 + kLuNbOuter, d.p);  // end of the next block
   switch (region) {
     case kGemmInner:
       g = {d.m - k0 - kb, d.m - k0 - kb, K1 - k0 - kb, K1 - k0 - kb, kb};
@@ -1294,6 +1315,9 @@ __host__ __device__ inline GemmShape gemm_shape(int region, const FrontDev &d, i
     case kGemmTrailRest:
       g = {d.m - K1, d.p - K1, d.m - K1n + d.p, d.m - K1n, K1 - K0};
       break;
+    case kGemmPairRows:
+      g = {K1n - K1, K1n - K1, d.m - K1n + d.p, d.m - K1n, K1 - K0};
+      break;
     case kGemmZBlk:
       g = {K0, K0, d.p - K0, d.p - K0, K1 - K0};
       break;
@@ -1308,7 +1332,7 @@ template <typename T>
 __device__ inline bool gemm_view(int region, const FrontDev &d, int k0, int nb, int K0, int NB,
                                  T *arena, GemmView<T> &v) {
   const bool trail = region == kGemmTrail || region == kGemmTrailHead ||
-                     region == kGemmTrailRest || region == kGemmZBlk;
+                     region == kGemmTrailRest || region == kGemmZBlk || region == kGemmPairRows;
   if (trail ? K0 >= d.p : k0 >= d.p) return false;
   const long long m = d.m, p = d.p, mm = m * m, pp = p * p;
   T *Fr = arena + d.off, *Fi = Fr + mm;
@@ -1316,7 +1340,7 @@ __device__ inline bool gemm_view(int region, const FrontDev &d, int k0, int nb, 
   T *Zr = Ei + pp, *Zi = Zr + pp;
   const int kb = min(nb, d.p - k0);
   const int K1 = min(K0 + NB, d.p);
-  const int K1n = min(K1 + NB, d.p);
+  const int K1n = min(K1 + kLuNbOuter, d.p);
   switch (region) {
     case kGemmTrailHead: {
       const long long a = K0 * m + K1, b = K1 * m + K0, c = K1 * m + K1;
@@ -1331,6 +1355,14 @@ __device__ inline bool gemm_view(int region, const FrontDev &d, int k0, int nb, 
       v = {Fr + a, Fi + a, m, Fr + b, Fi + b, Er + K0, Ei + K0, m, p,
            Fr + c, Fi + c, Er + K1, Ei + K1, m, p,
            d.m - K1, d.p - K1, d.m - K1n + d.p, d.m - K1n, K1 - K0};
+      break;
+    }
+    case kGemmPairRows: {  // TRAILREST restricted to the next block's rows [K1, K1n)
+      const long long a = K0 * m + K1, b = static_cast<long long>(K1n) * m + K0,
+                      c = static_cast<long long>(K1n) * m + K1;
+      v = {Fr + a, Fi + a, m, Fr + b, Fi + b, Er + K0, Ei + K0, m, p,
+           Fr + c, Fi + c, Er + K1, Ei + K1, m, p,
+           K1n - K1, K1n - K1, d.m - K1n + d.p, d.m - K1n, K1 - K0};
       break;
     }
     case kGemmInner: {
@@ -2028,7 +2060,9 @@ static hipError_t factor_levels(const LuPlan &P, hipStream_t s, FactorTiming *tm
     const int *list = P.d_level_list + L.list_off;
     const int order = lev < P.gemm_order.size() ? P.gemm_order[lev] : kOrderLinear;
     const int shape = lev < P.gemm_shape.size() ? P.gemm_shape[lev] : kShape64x64;
-    auto gemm = [&](long long rows, long long cols, int region, int k0, int K0,
+    // one batched GEMM of `region`; W: the K-block width of the trailing
+    // regions (NB, or 2 NB for a pair of blocks)
+    auto gemm = [&](long long rows, long long cols, int region, int k0, int K0, int W,
                     hipStream_t st) -> hipError_t {
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (tm) {
@@ -2040,7 +2074,7 @@ static hipError_t factor_levels(const LuPlan &P, hipStream_t s, FactorTiming *tm
         event_level.push_back(static_cast<int>(lev));
         if ((r = hipEventRecord(e0, st)) != hipSuccess) return r;
         for (int q = 0; q < L.count; ++q) {
-          const GemmShape g = gemm_shape(region, P.h_front[P.h_level_list[L.list_off + q]], k0, nb, K0, NB);
+          const GemmShape g = gemm_shape(region, P.h_front[P.h_level_list[L.list_off + q]], k0, nb, K0, W);
           if (g.M > 0 && g.N > 0 && g.K > 0)
             tm->gemm_flops += 8.0 * g.K * (static_cast<double>(g.M) * g.N1 +
                                            static_cast<double>(g.M2) * (g.N - g.N1));
@@ -2048,7 +2082,7 @@ static hipError_t factor_levels(const LuPlan &P, hipStream_t s, FactorTiming *tm
         ++tm->gemm_launches;
       }
       hipError_t r = launch_gemm<T>(shape, order, rows, cols, L.count, st, list, P.d_front,
-                                    region, k0, nb, K0, NB, arena);
+                                    region, k0, nb, K0, W, arena);
       if (r != hipSuccess) return r;
       if (tm) return hipEventRecord(e1, st);
       return hipSuccess;
@@ -2071,58 +2105,94 @@ static hipError_t factor_levels(const LuPlan &P, hipStream_t s, FactorTiming *tm
         if (k0 + nb < Kend) {
           hipLaunchKernelGGL(k_left_apply<T>, dim3(cdivu(NB - nb, 64), L.count), dim3(256), 0,
                              st, list, L.count, P.d_front, k0, nb, K0, NB, 0, dinv, arena);
-          if ((r = gemm(L.mmax, NB - nb, kGemmInner, k0, K0, st)) != hipSuccess) return r;
+          if ((r = gemm(L.mmax, NB - nb, kGemmInner, k0, K0, NB, st)) != hipSuccess) return r;
         }
         if ((r = hipGetLastError()) != hipSuccess) return r;
       }
       return hipSuccess;
     };
-    // Look-ahead: the panels of block i+1 run on the side stream once the
-    // TRAILHEAD update of block i has produced their columns, while the main
-    // stream applies TRAILREST (later columns and E).  The two touch
-    // disjoint columns; the main stream waits for the panels before the next
-    // block's row exchanges and U rows.  A timed factorization runs the same
-    // launches in order on one stream, so each kernel's duration is its own.
-    const bool lookahead = tm == nullptr;
-    if ((e = panel_phase(0, s)) != hipSuccess) return e;
-    for (int K0 = 0; K0 < L.pmax; K0 += NB) {
+    // the row exchanges of block [K0, K0 + NB) on the trailing F columns and
+    // E (composed first), or, c1 > c0, on F columns [c0, c1) with the
+    // permutation composed by the previous call for the same block
+    auto exchanges = [&](int K0, int c0, int c1) -> hipError_t {
+      const bool range = c1 > c0;
+      const long long ncols = range ? c1 - c0 : L.mmax + L.pmax;
+      if (!range)
+        hipLaunchKernelGGL(k_block_perm, dim3(L.count), dim3(256), 0, s, list, L.count,
+                           P.d_front, K0, NB, P.d_ipiv, P.d_bperm);
+      hipLaunchKernelGGL(k_apply_perm<T>, dim3(cdivu(ncols, 4), L.count), dim3(256), 0, s,
+                         list, L.count, P.d_front, K0, NB, P.d_bperm, arena, c0, c1);
+      if (L.pmax > kPanelMapRows)
+        hipLaunchKernelGGL(k_block_swaps<T>, dim3(cdivu(ncols, 256), L.count), dim3(256), 0, s,
+                           list, L.count, P.d_front, K0, NB, P.d_ipiv, arena, c0, c1);
+      return hipGetLastError();
+    };
+    // U rows of block [K0, K0 + NB) for the trailing F columns and E
+    auto u_rows = [&](int K0) -> hipError_t {
       const int Kend = std::min(K0 + NB, L.pmax);
       for (int k0 = K0; k0 < Kend; k0 += nb) {
-        if (k0 == K0) {
-          // the block's row exchanges on the trailing F columns and E
-          hipLaunchKernelGGL(k_block_perm, dim3(L.count), dim3(256), 0, s, list, L.count,
-                             P.d_front, K0, NB, P.d_ipiv, P.d_bperm);
-          hipLaunchKernelGGL(k_apply_perm<T>, dim3(cdivu(L.mmax + L.pmax, 4), L.count),
-                             dim3(256), 0, s, list, L.count, P.d_front, K0, NB, P.d_bperm, arena);
-          if (L.pmax > kPanelMapRows)
-            hipLaunchKernelGGL(k_block_swaps<T>, dim3(cdivu(L.mmax + L.pmax, 256), L.count),
-                               dim3(256), 0, s, list, L.count, P.d_front, K0, NB, P.d_ipiv,
-                               arena);
-        }
         hipLaunchKernelGGL(k_left_apply<T>, dim3(cdivu(L.mmax + L.pmax, 64), L.count),
                            dim3(256), 0, s, list, L.count, P.d_front, k0, nb, K0, NB, 1, dinv,
                            arena);
-        if (k0 + nb < Kend)
-          if ((e = gemm(NB - nb, L.mmax + L.pmax, kGemmBlkRows, k0, K0, s)) != hipSuccess) return e;
+        if (k0 + nb < Kend) {
+          hipError_t r = gemm(NB - nb, L.mmax + L.pmax, kGemmBlkRows, k0, K0, NB, s);
+          if (r != hipSuccess) return r;
+        }
       }
-      const bool next = K0 + NB < L.pmax;
+      return hipGetLastError();
+    };
+    // Look-ahead: the panels of the next block run on the side stream once
+    // the TRAILHEAD update has produced their columns, while the main stream
+    // applies TRAILREST (later columns and E).  The two touch disjoint
+    // columns; the main stream waits for the panels before the next block's
+    // row exchanges and U rows.  A timed factorization runs the same launches
+    // in order on one stream, so each kernel's duration is its own.
+    //
+    // Paired blocks (P.pair_blocks): blocks A = [K0, K0 + NB) and
+    // B = [K0 + NB, K0 + 2 NB) share one update of the rows below B
+    // (K = 2 NB).  After A's exchanges and U rows, TRAILHEAD brings B's
+    // columns up to date and B is factored (its panels are not overlapped);
+    // B's exchanges then also permute A's L columns, so that the deferred
+    // update of A sees the rows in B's order, PAIRROWS applies A's update to
+    // B's rows, and B's U rows follow.  Rows below B get both blocks' update
+    // in the pair's TRAILHEAD / TRAILREST / TRAIL.
+    const bool lookahead = tm == nullptr;
+    if ((e = panel_phase(0, s)) != hipSuccess) return e;
+    for (int K0 = 0; K0 < L.pmax;) {
+      const bool pair = P.pair_blocks && K0 + NB < L.pmax;
+      const int W = pair ? 2 * NB : NB;
+      if ((e = exchanges(K0, 0, 0)) != hipSuccess) return e;
+      if ((e = u_rows(K0)) != hipSuccess) return e;
+      if (pair) {
+        const int KB = K0 + NB;
+        if ((e = gemm(L.mmax, NB, kGemmTrailHead, K0, K0, NB, s)) != hipSuccess) return e;
+        if ((e = panel_phase(KB, s)) != hipSuccess) return e;
+        if ((e = exchanges(KB, 0, 0)) != hipSuccess) return e;
+        if ((e = exchanges(KB, K0, KB)) != hipSuccess) return e;
+        if ((e = gemm(NB, L.mmax + L.pmax, kGemmPairRows, K0, K0, NB, s)) != hipSuccess) return e;
+        if ((e = u_rows(KB)) != hipSuccess) return e;
+      }
+      const bool next = K0 + W < L.pmax;
       if (next) {
-        if ((e = gemm(L.mmax, NB, kGemmTrailHead, K0, K0, s)) != hipSuccess) return e;
+        if ((e = gemm(L.mmax, NB, kGemmTrailHead, K0, K0, W, s)) != hipSuccess) return e;
         if (lookahead) {
           if ((e = hipEventRecord(P.ev_head, s)) != hipSuccess) return e;
           if ((e = hipStreamWaitEvent(P.side, P.ev_head, 0)) != hipSuccess) return e;
-          if ((e = panel_phase(K0 + NB, P.side)) != hipSuccess) return e;
+          if ((e = panel_phase(K0 + W, P.side)) != hipSuccess) return e;
           if ((e = hipEventRecord(P.ev_panel, P.side)) != hipSuccess) return e;
-          if ((e = gemm(L.mmax, L.mmax + L.pmax, kGemmTrailRest, K0, K0, s)) != hipSuccess) return e;
+          if ((e = gemm(L.mmax, L.mmax + L.pmax, kGemmTrailRest, K0, K0, W, s)) != hipSuccess)
+            return e;
           if ((e = hipStreamWaitEvent(s, P.ev_panel, 0)) != hipSuccess) return e;
         } else {
-          if ((e = panel_phase(K0 + NB, s)) != hipSuccess) return e;
-          if ((e = gemm(L.mmax, L.mmax + L.pmax, kGemmTrailRest, K0, K0, s)) != hipSuccess) return e;
+          if ((e = panel_phase(K0 + W, s)) != hipSuccess) return e;
+          if ((e = gemm(L.mmax, L.mmax + L.pmax, kGemmTrailRest, K0, K0, W, s)) != hipSuccess)
+            return e;
         }
       } else {
-        if ((e = gemm(L.mmax, L.mmax + L.pmax, kGemmTrail, K0, K0, s)) != hipSuccess) return e;
+        if ((e = gemm(L.mmax, L.mmax + L.pmax, kGemmTrail, K0, K0, W, s)) != hipSuccess) return e;
       }
       if ((e = hipGetLastError()) != hipSuccess) return e;
+      K0 += W;
     }
     // Z = U11^-1 by backward block-Jordan steps: within each block the
     // panels from the last (panel rows by back substitution, ZIN for the
@@ -2133,11 +2203,11 @@ static hipError_t factor_levels(const LuPlan &P, hipStream_t s, FactorTiming *tm
         hipLaunchKernelGGL(k_upper_solve<T>, dim3(cdivu(L.pmax, 8), L.count), dim3(256), 0, s,
                            list, L.count, P.d_front, k0, nb, dinv, arena);
         if (k0 > K0)
-          if ((e = gemm(NB - nb, L.pmax, kGemmZ, k0, K0, s)) != hipSuccess) return e;
+          if ((e = gemm(NB - nb, L.pmax, kGemmZ, k0, K0, NB, s)) != hipSuccess) return e;
         if ((e = hipGetLastError()) != hipSuccess) return e;
       }
       if (K0 > 0)
-        if ((e = gemm(L.pmax, L.pmax, kGemmZBlk, K0, K0, s)) != hipSuccess) return e;
+        if ((e = gemm(L.pmax, L.pmax, kGemmZBlk, K0, K0, NB, s)) != hipSuccess) return e;
     }
   }
   if (tm) {

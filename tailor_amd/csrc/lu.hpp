@@ -39,6 +39,7 @@ struct LuPlan {
   std::vector<int> gemm_order;     // per level (kOrder*); empty = linear everywhere
   std::vector<int> gemm_shape;     // per level tile shape (0: 64x64, 1: 128x64, 2: 128x128)
   bool panel_streaming = false;    // force the streaming panel kernel on every level
+  bool pair_blocks = true;         // two-level blocking: pairs of blocks share one update (K = 256)
   std::vector<FrontDev> h_front;   // host copies (flop accounting)
   std::vector<int> h_level_list;
   FrontDev *d_front = nullptr;

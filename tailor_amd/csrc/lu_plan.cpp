@@ -243,6 +243,7 @@ hipError_t lu_plan_batch(const std::vector<BatchPart> &parts, bool fp32, LuPlan 
   U.gemm_shape = parts[deepest].plan->gemm_shape;
   U.gemm_order = parts[deepest].plan->gemm_order;
   U.panel_streaming = parts[deepest].plan->panel_streaming;
+  U.pair_blocks = parts[deepest].plan->pair_blocks;
   U.h_front = fr;
   U.h_level_list = level_list;
   U.arena = reinterpret_cast<double *>(const_cast<char *>(base));

@@ -34,7 +34,7 @@ struct SolverState {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   unsigned long long graph_id = 0;  // unique over the process
-  bool graph_fp32 = false, graph_streaming = false;
+  bool graph_fp32 = false, graph_streaming = false, graph_pairs = true;
   std::vector<int> graph_shape, graph_order;
   // results
   int nritz = 0;

@@ -146,7 +146,7 @@ class Sweeper:
     def __init__(self, device: int = 0, nev: int = 8, tol: float = 1e-10, ncv: int = 64,
                  omega_s: complex = 0.02 + 0j, maxit: int = 1000, contexts: int = 1,
                  check_from: int = 0, graph: bool = False, batch: int = 0,
-                 threads: bool = True):
+                 threads: bool = True, block_pairs: bool | None = None):
         """`check_from` > 0: the learned form's Krylov-Schur runs also test
         convergence after every second step from that step on and end a pass
         early (the plain form and the fallback test only at pass ends).
@@ -165,6 +165,10 @@ class Sweeper:
         # Krylov-Schur runs one after another on the calling thread (for
         # profilers that cannot follow launches from several threads)
         self.threads = threads
+        # two-level blocking of every context's factorization (K = 256
+        # trailing updates; tailor_set_block_pairs); None: the library's
+        # default (on)
+        self.block_pairs = block_pairs
         if check_from < 0:
             raise ValueError("check_from must be >= 0")
         self.check_from = check_from
@@ -196,6 +200,8 @@ class Sweeper:
         ctx.load_case(case)
         ctx.assemble(diagnostics=False)
         ctx.set_precision(precision)
+        if self.block_pairs is not None:
+            ctx.set_block_pairs(self.block_pairs)
         return ctx.autotune(complex(self.omega_s.imag, -self.omega_s.real))
 
     def _factor(self, ctx, omega, precision):
@@ -260,6 +266,8 @@ class Sweeper:
                 # first instance of a size: pick the frontal GEMM tile order
                 tt = time.perf_counter()
                 ctx.set_precision(precision)
+                if self.block_pairs is not None:
+                    ctx.set_block_pairs(self.block_pairs)
                 self.tuned[(inst.ny, inst.nz)] = ctx.autotune(
                     complex(self.omega_s.imag, -self.omega_s.real))
                 t_tune += time.perf_counter() - tt
