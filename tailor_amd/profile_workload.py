@@ -15,6 +15,15 @@ them.
 
     python -m tailor_amd.profile_workload summarize <kernel_trace.csv>... --command "..." [--out f.md]
 
+    python -m tailor_amd.profile_workload summarize-pmc --fetch <counter_collection.csv>...
+        --write <counter_collection.csv>... --command "..." --out-md f.md --out-json f.json
+
+Summarises two `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs of
+this workload: per kernel family the mean KiB per dispatch of each counter
+and the HBM bytes per dispatch after scaling both counters by what they
+read for the workload's 1 GiB device copy (written as markdown and as the
+JSON that bench.py reads for the roofline's `traffic`).
+
 Summarises a `rocprofv3 --kernel-trace --output-format csv` trace (one file
 per traced process, all on the same clock) as a
 markdown table of per-kernel launch counts and durations, and separately
@@ -90,8 +99,90 @@ def summarize(csv_paths, command: str) -> str:
     return "\n".join(lines) + "\n"
 
 
+def _family(name: str) -> str:
+    """Counter rows are grouped by kernel family: k_gemm by element type (all
+    tile shapes), k_gemv by type and columns per wave, others by full name."""
+    import re
+    m = re.match(r"(k_gemm|k_gemv)<([^>]*)>", name)
+    if not m:
+        return name
+    args = [a.strip() for a in m.group(2).split(",")]
+    return f"{m.group(1)}<{', '.join(args[:1 if m.group(1) == 'k_gemm' else 2])}>"
+
+
+def _pmc_rows(paths, counter):
+    """{dispatch id: (kernel family, summed counter value)} of one counter's runs."""
+    import csv
+    out = {}
+    for path in paths:
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                if r.get("Counter_Name") != counter:
+                    continue
+                key = (path, r["Dispatch_Id"])
+                fam = _family(_short_name(r["Kernel_Name"]))
+                val = float(r["Counter_Value"])
+                prev = out.get(key, (fam, 0.0))
+                out[key] = (fam, prev[1] + val)
+    return out
+
+
+def summarize_pmc(fetch_paths, write_paths, command: str):
+    """(markdown, json dict) of the FETCH_SIZE and WRITE_SIZE runs (module
+    docstring); the counters are in KiB."""
+    fetch, write = _pmc_rows(fetch_paths, "FETCH_SIZE"), _pmc_rows(write_paths, "WRITE_SIZE")
+
+    def per_family(rows):
+        agg = {}
+        for fam, v in rows.values():
+            n, tot = agg.get(fam, (0, 0.0))
+            agg[fam] = (n + 1, tot + v)
+        return agg
+
+    fa, wa = per_family(fetch), per_family(write)
+    copy = "__amd_rocclr_copyBuffer"
+    gib_kib = float(1 << 20)
+    fscale = round(gib_kib / fa[copy][1], 1) if copy in fa and fa[copy][1] > 0 else 1.0
+    wscale = round(gib_kib / wa[copy][1], 1) if copy in wa and wa[copy][1] > 0 else 1.0
+    fams = sorted(set(fa) | set(wa), key=lambda k: -(fa.get(k, (1, 0.0))[1] + wa.get(k, (1, 0.0))[1]))
+    lines = ["Commands (GPU box, repo root), one counter per run:", "", f"    {command}", "",
+             f"Scale factors from the 1 GiB device copy ({copy}): FETCH_SIZE summed to "
+             f"{fa.get(copy, (0, 0.0))[1]:,.0f} KiB (scale {fscale}), WRITE_SIZE to "
+             f"{wa.get(copy, (0, 0.0))[1]:,.0f} KiB (scale {wscale}).", "",
+             "| kernel | dispatches | FETCH KiB/dispatch | WRITE KiB/dispatch | "
+             "HBM bytes/dispatch (scaled) |", "|---|---|---|---|---|"]
+    kernels = {}
+    for fam in fams:
+        nf, tf = fa.get(fam, (0, 0.0))
+        nw, tw = wa.get(fam, (0, 0.0))
+        n = max(nf, nw)
+        fk = tf / nf if nf else 0.0
+        wk = tw / nw if nw else 0.0
+        traffic = (fk * fscale + wk * wscale) * 1024.0
+        lines.append(f"| {fam} | {n} | {fk:.1f} | {wk:.1f} | {traffic / 1e6:.1f} MB |")
+        kernels[fam] = {"dispatches": n, "traffic_bytes_per_dispatch": int(round(traffic)),
+                        "fetch_kib": round(fk, 1), "write_kib": round(wk, 1)}
+    js = {"fetch_scale": fscale, "write_scale": wscale, "kernels": kernels}
+    return "\n".join(lines) + "\n", js
+
+
 def main(argv=None) -> int:
     argv = sys.argv[1:] if argv is None else list(argv)
+    if argv and argv[0] == "summarize-pmc":
+        import json
+        sp = argparse.ArgumentParser(prog="profile_workload summarize-pmc")
+        sp.add_argument("--fetch", nargs="+", required=True)
+        sp.add_argument("--write", nargs="+", required=True)
+        sp.add_argument("--command", required=True)
+        sp.add_argument("--out-md", required=True)
+        sp.add_argument("--out-json", required=True)
+        b = sp.parse_args(argv[1:])
+        text, js = summarize_pmc(b.fetch, b.write, b.command)
+        with open(b.out_md, "w") as f:
+            f.write(text)
+        with open(b.out_json, "w") as f:
+            json.dump(js, f, indent=1)
+        return 0
     if argv and argv[0] == "summarize":
         sp = argparse.ArgumentParser(prog="profile_workload summarize")
         sp.add_argument("csv", nargs="+")
@@ -122,12 +213,15 @@ def main(argv=None) -> int:
         ctx.set_gemm_config(a.gemm_config)
     fd = ctx.factor(c.lambda_shift)
     ms, nbytes = ctx.time_linear_solve(a.solves)
+    kms, kbytes = ctx.time_operator(a.solves)
     nodes = a.ny * a.nz
     asm = 16 * 25 * (nodes * 77 + nodes) + 128 * nodes + 3 * 8 * (a.ny * 11 + a.nz * 7)
     print(f"assembly algorithmic bytes {asm}")
     print(f"factorization LU flops {fd['flops']:.6e}, arena bytes {fd['arena_bytes']}")
     print(f"solve algorithmic bytes {nbytes:.6e} per solve, {ms:.4f} ms per solve "
           f"({a.solves + 1} solves incl. warm-up)")
+    print(f"SpMV algorithmic bytes {kbytes:.6e} per application, {kms:.4f} ms "
+          f"({a.solves + 1} applications incl. warm-up)")
     ctx.close()
     n = (1 << 30) // 8
     src = torch.ones(n, dtype=torch.float64, device="cuda")
