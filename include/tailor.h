@@ -190,12 +190,23 @@ int tailor_factor_batch(tailor_ctx *const *ctxs, int32_t count, const double *si
 int tailor_profile_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, double *factor_ms,
                           double *gemm_ms, double *gemm_flops, int32_t *gemm_launches);
 
+/* The frontal GEMM time of one factorization at `sigma` by elimination
+ * level and by update region, timed as in tailor_profile_factor.
+ * level_ms / level_flops receive levels 0 .. max_levels - 1 (zero past the
+ * plan's levels; *levels gets the plan's count), region_ms / region_flops
+ * (8 entries each, may be NULL) the regions 0 INNER, 1 BLKROWS, 2 TRAIL,
+ * 3 ZIN, 4 TRAILHEAD, 5 TRAILREST, 6 ZBLK, 7 PAIRROWS; flops count 8 per
+ * complex multiply-add.  No reference counterpart (profiling only). */
+int tailor_profile_gemm(tailor_ctx *ctx, double sigma_re, double sigma_im, int32_t max_levels,
+                        int32_t *levels, double *level_ms, double *level_flops,
+                        double *region_ms, double *region_flops);
+
 /* Autotuning of the frontal GEMM: factor at `sigma` once per configuration
  * c = 2 * shape + order, shape 0: 64x64 complex tile (256 threads, K
- * chunk 16), 1: 64x64 (K chunk 4, four waves per SIMD), 2: 128x128 (512
- * threads), 3: 64x64 (K chunk 8, three waves per SIMD); order 0: tiles in
- * launch order,
- * 1: XCD-aware grouped order.  The GEMM launches of each elimination level
+ * chunk 16), 1: 64x64 (K chunk 4, three waves per SIMD), 2: 64x64 (K chunk
+ * 8, operands loaded straight into a three-stage LDS ring; FP32 fronts use
+ * shape 1's tile), 3: 64x64 (K chunk 8, two waves per SIMD); order 0: tiles
+ * in launch order, 1: XCD-aware grouped order.  The GEMM launches of each elimination level
  * are timed with HIP events and each level keeps its fastest configuration
  * for later factorizations of this context.  No reference counterpart
  * (performance only; results are unchanged up to rounding). */

@@ -149,6 +149,9 @@ def library() -> C.CDLL:
     lib.tailor_set_block_pairs.argtypes = [C.c_void_p, C.c_int32]
     lib.tailor_profile_factor.argtypes = [C.c_void_p, C.c_double, C.c_double, P(C.c_double),
                                           P(C.c_double), P(C.c_double), P(C.c_int32)]
+    lib.tailor_profile_gemm.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int32,
+                                        P(C.c_int32), P(C.c_double), P(C.c_double),
+                                        P(C.c_double), P(C.c_double)]
     lib.tailor_time_linear_solve.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double)]
     lib.tailor_time_operator.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_double)]
     lib.tailor_linear_solve_pair.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double),
@@ -303,6 +306,25 @@ def _profile_factor(self, sigma: complex) -> dict:
                                                C.byref(g), C.byref(fl), C.byref(n)))
     return {"factor_ms": f.value, "gemm_ms": g.value, "gemm_flops": fl.value,
             "gemm_launches": n.value}
+
+
+GEMM_REGIONS = ("inner", "blkrows", "trail", "zin", "trailhead", "trailrest", "zblk", "pairrows")
+
+
+def _profile_gemm(self, sigma: complex, max_levels: int = 64) -> dict:
+    """Factor with HIP events around every frontal GEMM launch and return
+    the GEMM time and flops (8 per complex multiply-add) per elimination
+    level and per update region."""
+    sigma = complex(sigma)
+    n = C.c_int32()
+    lms, lfl = (C.c_double * max_levels)(), (C.c_double * max_levels)()
+    rms, rfl = (C.c_double * 8)(), (C.c_double * 8)()
+    self._check(self.lib.tailor_profile_gemm(self.ptr, sigma.real, sigma.imag, int(max_levels),
+                                             C.byref(n), lms, lfl, rms, rfl))
+    nl = min(n.value, max_levels)
+    return {"level_ms": list(lms)[:nl], "level_flops": list(lfl)[:nl],
+            "region_ms": dict(zip(GEMM_REGIONS, list(rms))),
+            "region_flops": dict(zip(GEMM_REGIONS, list(rfl)))}
 
 
 def _autotune(self, sigma: complex) -> dict:
@@ -525,6 +547,7 @@ Context.factor = _factor
 Context.factor_graph = _factor_graph
 Context.set_precision = _set_precision
 Context.profile_factor = _profile_factor
+Context.profile_gemm = _profile_gemm
 Context.autotune = _autotune
 Context.set_gemm_config = lambda self, c: self._check(self.lib.tailor_set_gemm_config(self.ptr, int(c)))
 Context.set_panel_kernel = lambda self, k: self._check(self.lib.tailor_set_panel_kernel(self.ptr, int(k)))

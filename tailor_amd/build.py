@@ -25,6 +25,10 @@ COMMON = ["-O3", "-std=c++20", "-fPIC", "--offload-arch=gfx950",
 PER_FILE = {
     "assembly.hip": ["-ffp-contract=off"],
     "lu.hip": ["-munsafe-fp-atomics"],
+    # keep MFMA accumulators in VGPRs: with the default AGPR form the
+    # calibration loop copies every accumulator to and from AGPRs around
+    # each round of MFMAs and measures those copies, not the MFMA rate
+    "calibrate.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
 }
 SOURCES = ["assembly.hip", "lu.hip", "vec.hip", "capi.cpp", "eigen.cpp", "symbolic.cpp",
            "lu_plan.cpp", "dense.cpp", "mlp.hip", "noalloc.hip", "batch.cpp", "calibrate.hip"]

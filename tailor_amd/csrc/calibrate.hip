@@ -7,10 +7,12 @@
 //              far larger than the caches.
 //
 // They give the attainable ceilings beside the published peaks (78.6 TFLOP/s
-// dense FP64, 8 TB/s): on the MI355X measured here the MFMA issues one
-// 16x16x4 FP64 instruction per ~100-110 shader cycles per SIMD with two or
-// more waves (~140 with one), at a steady 2.35-2.4 GHz, i.e. ~47 TFLOP/s,
-// and a streaming read reaches ~6 TB/s.
+// dense FP64, 8 TB/s).  The accumulators must stay in the registers the MFMA
+// writes (the build compiles this file with the VGPR form of the MFMA
+// builtins): copied to and from AGPRs around every round of MFMAs, the same
+// loop measures ~47 TFLOP/s instead.  On the MI355X measured here the MFMA
+// issues one 16x16x4 FP64 instruction per 64 shader cycles per SIMD with one
+// or more waves, ~77 TFLOP/s at 2.4 GHz, and a streaming read reaches ~6 TB/s.
 #include <hip/hip_runtime.h>
 
 #include "../../include/tailor.h"

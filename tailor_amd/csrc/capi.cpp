@@ -679,6 +679,29 @@ int tailor_profile_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, dou
   return TAILOR_OK;
 }
 
+int tailor_profile_gemm(tailor_ctx *ctx, double sigma_re, double sigma_im, int32_t max_levels,
+                        int32_t *levels, double *level_ms, double *level_flops,
+                        double *region_ms, double *region_flops) {
+  if (!ctx || max_levels < 0 || (max_levels > 0 && (!level_ms || !level_flops)))
+    return TAILOR_ERR_USER_INPUT;
+  if (!std::isfinite(sigma_re) || !std::isfinite(sigma_im))
+    return fail(ctx, TAILOR_ERR_USER_INPUT, "shift must be finite");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  tailor::FactorTiming tm;
+  if (int st = tailor::solver_factor(ctx, tailor::cplx(sigma_re, sigma_im), &tm)) return st;
+  const int nl = static_cast<int>(tm.level_gemm_ms.size());
+  if (levels) *levels = nl;
+  for (int l = 0; l < max_levels; ++l) {
+    level_ms[l] = l < nl ? tm.level_gemm_ms[l] : 0.0;
+    level_flops[l] = l < nl ? tm.level_gemm_flops[l] : 0.0;
+  }
+  for (int r = 0; r < 8; ++r) {
+    if (region_ms) region_ms[r] = tm.region_gemm_ms[r];
+    if (region_flops) region_flops[r] = tm.region_gemm_flops[r];
+  }
+  return TAILOR_OK;
+}
+
 int tailor_autotune(tailor_ctx *ctx, double sigma_re, double sigma_im, tailor_tune_report *report) {
   if (!ctx) return TAILOR_ERR_USER_INPUT;
   if (!std::isfinite(sigma_re) || !std::isfinite(sigma_im))

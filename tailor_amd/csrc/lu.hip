@@ -1414,12 +1414,48 @@ constexpr int GKC = 16;  // K chunk staged in LDS
 //   shape 0: 64 x 64, 256 threads, K chunk 16, 3M form, two waves per SIMD;
 //   shape 1: 64 x 64, 256 threads, K chunk 4, 3M form, registers held to
 //            three waves per SIMD (a quarter of shape 0's LDS);
-//   shape 2: 128 x 128, 512 threads, four real products;
+//   shape 2: 64 x 64, 256 threads, K chunk 8, 3M form, operands loaded
+//            straight into a three-stage LDS ring (k_gemm_lds; FP64 only,
+//            FP32 fronts use shape 1's tile);
 //   shape 3: 64 x 64, 256 threads, K chunk 8, 3M form, two waves per SIMD.
 // The shorter K chunks trade more barriers for more resident workgroups,
 // which hide the first chunk's and C's load latency of each tile.
-enum { kShape64x64 = 0, kShape64x64k4 = 1, kShape128x128 = 2, kShape64x64k8 = 3, kShapeCount = 4 };
+enum { kShape64x64 = 0, kShape64x64k4 = 1, kShapeLds = 2, kShape64x64k8 = 3, kShapeCount = 4 };
 static_assert(kShapeCount == kGemmShapes, "shape count");
+
+// Output tile (row0, col0) of this workgroup among the (M / TM) x (N / TN)
+// tiles of its front; false when it has none.
+__device__ inline bool gemm_tile(int M, int N, int TM, int TN, int order, int &row0, int &col0) {
+  const int tiles_m = (M + TM - 1) / TM, tiles_n = (N + TN - 1) / TN;
+  if (order == kOrderLinear) {
+    // tile = blockIdx.x, row blocks fastest
+    const int tile = blockIdx.x;
+    if (tile >= tiles_m * tiles_n) return false;
+    row0 = (tile % tiles_m) * TM;
+    col0 = (tile / tiles_m) * TN;
+    return true;
+  }
+  // XCD-aware order: workgroups are dispatched round-robin over the 8
+  // XCDs, so XCD x (= blockIdx.x % 8, gridDim.x being a multiple of 8)
+  // receives a contiguous eighth of this front's tiles (rotated by front);
+  // within it, tiles go kGemmGroup row blocks at a time so that
+  // co-resident workgroups share A and B tiles in that XCD's L2.
+  const int ntiles = tiles_m * tiles_n;
+  const int per = (ntiles + kXcds - 1) / kXcds;
+  const int q = static_cast<int>(blockIdx.x / kXcds);
+  if (q >= per) return false;
+  const int slot = (static_cast<int>(blockIdx.x % kXcds) - static_cast<int>(blockIdx.y % kXcds) +
+                    kXcds) % kXcds;
+  const int tile = slot * per + q;
+  if (tile >= ntiles) return false;
+  const int in_group = kGemmGroup * tiles_n;
+  const int first_m = (tile / in_group) * kGemmGroup;
+  const int group_m = min(tiles_m - first_m, kGemmGroup);
+  const int tm = first_m + (tile % in_group) % group_m, tn = (tile % in_group) / group_m;
+  row0 = tm * TM;
+  col0 = tn * TN;
+  return true;
+}
 
 template <typename T, int WMW, int WNW, int WMB, int WNB, int KC = GKC, int MINW = 1, bool M3 = false>
 __global__ __launch_bounds__(64 * WMW * WNW, MINW) void k_gemm(const int *__restrict__ list,
@@ -1442,35 +1478,8 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void k_gemm(const int *__rest
   const FrontDev d = fr[f];
   GemmView<T> v;
   if (!gemm_view(region, d, k0, nb, K0, NB, arena, v)) return;
-  const int tiles_m = (v.M + TM - 1) / TM, tiles_n = (v.N + TN - 1) / TN;
   int row0, col0;
-  if (order == kOrderLinear) {
-    // tile = blockIdx.x, row blocks fastest
-    const int tile = blockIdx.x;
-    if (tile >= tiles_m * tiles_n) return;
-    row0 = (tile % tiles_m) * TM;
-    col0 = (tile / tiles_m) * TN;
-  } else {
-    // XCD-aware order: workgroups are dispatched round-robin over the 8
-    // XCDs, so XCD x (= blockIdx.x % 8, gridDim.x being a multiple of 8)
-    // receives a contiguous eighth of this front's tiles (rotated by front);
-    // within it, tiles go kGemmGroup row blocks at a time so that
-    // co-resident workgroups share A and B tiles in that XCD's L2.
-    const int ntiles = tiles_m * tiles_n;
-    const int per = (ntiles + kXcds - 1) / kXcds;
-    const int q = static_cast<int>(blockIdx.x / kXcds);
-    if (q >= per) return;
-    const int slot = (static_cast<int>(blockIdx.x % kXcds) - static_cast<int>(blockIdx.y % kXcds) +
-                      kXcds) % kXcds;
-    const int tile = slot * per + q;
-    if (tile >= ntiles) return;
-    const int in_group = kGemmGroup * tiles_n;
-    const int first_m = (tile / in_group) * kGemmGroup;
-    const int group_m = min(tiles_m - first_m, kGemmGroup);
-    const int tm = first_m + (tile % in_group) % group_m, tn = (tile % in_group) / group_m;
-    row0 = tm * TM;
-    col0 = tn * TN;
-  }
+  if (!gemm_tile(v.M, v.N, TM, TN, order, row0, col0)) return;
   // two LDS buffers: the next K chunk is stored while the current one feeds
   // the MFMAs, so each chunk needs one barrier
   __shared__ T sAr[2][GKC][LDA_], sAi[2][GKC][LDA_], sBr[2][TN][LDB_], sBi[2][TN][LDB_];
@@ -1664,6 +1673,229 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void k_gemm(const int *__rest
   }
 }
 
+// ----------------------------------------- frontal GEMM, direct LDS loads
+//
+// Tile shape 2 (FP64): the 64 x 64 3M tile of shape 3 (2 x 2 waves, each
+// 2 x 2 MFMA blocks, K chunk 8), fed by global -> LDS loads that write
+// LDS without passing through registers (global_load_lds_dwordx4: each lane
+// moves 16 bytes, a wave's 1 KB lands contiguously).  The chunks rotate
+// through a ring of kLdsStages LDS stages, so chunk c + 2 is in flight while
+// chunk c feeds the MFMAs, and a chunk costs one barrier and no VALU work
+// to stage.  Per chunk each wave issues four loads: A columns k = 2w, 2w + 1
+// (lanes 0-31 the real plane's 64 rows, two per lane; lanes 32-63 the
+// imaginary), and the real and imaginary planes of B's column group w
+// (16 columns x 8 k; lane l: column l % 16, k pair l / 16).  Stage layout:
+//   A  [k (8)][plane (2)][row (64)], 16 doubles of padding per k;
+//   B  [plane (2)][column group (4)][k pair (4)][column (16)][2].
+// Loads never stray outside the front's planes: a row pair starting beyond
+// M, or a column beyond N, is redirected to row pair 0 or the last column
+// (such rows and columns only reach outputs that are not stored), and a row
+// pair (M - 1, M) reads the element after its column's last row, which is
+// the next column's first (or the next plane's).  A K tail (K mod 8) is
+// staged through registers with zeros past K, as in k_gemm.  The stages are
+// separate arrays, which lets the compiler see that the MFMA operand reads
+// of one stage do not depend on the loads in flight into another.
+constexpr int kLdsStages = 3, kLdsKc = 8, kLdsAk = 2 * 64 + 16;
+constexpr int kLdsASize = kLdsKc * kLdsAk, kLdsBSize = 4 * 128;  // doubles per stage (B: per plane)
+
+// One global -> LDS load of 16 bytes per lane; the wave's 1 KB lands at the
+// LDS address of l (wave-uniform) + 16 x lane.  Written out rather than
+// through the compiler builtin: the compiler's tracking of such loads puts a
+// wait for every load in flight before operand reads it cannot tell apart
+// from the stage being filled, which serializes the ring.  The kernel waits
+// for these loads itself (s_waitcnt vmcnt, then a barrier) before any wave
+// reads the stage; the loads have no register results, and a compiler wait
+// for its own loads only becomes stricter with them in flight, since loads
+// complete in issue order.  M0 carries the LDS address; the compiled
+// kernel keeps nothing in M0 (its only M0 writes are these).
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ inline void lds_load16(const double *g, double *l) {
+  typedef __attribute__((address_space(3))) double lds_double;
+  const unsigned lds = __builtin_amdgcn_readfirstlane(
+      static_cast<unsigned>(reinterpret_cast<uintptr_t>((lds_double *)l)));
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+               :
+               : "v"(g), "s"(lds)
+               : "memory", "m0");
+}
+#pragma clang diagnostic pop
+
+// s_waitcnt immediate that waits until at most n vector-memory operations
+// of this wave are outstanding (other counters untouched)
+constexpr int wait_vm_imm(int n) { return (n & 0xF) | ((n >> 4) << 14) | (0x7 << 4) | (0xF << 8); }
+
+template <int MINW>
+__global__ __launch_bounds__(256, MINW) void k_gemm_lds(const int *__restrict__ list, int nlist,
+                                                      const FrontDev *__restrict__ fr, int region,
+                                                      int k0, int nb, int K0, int NB, int order,
+                                                      double *__restrict__ arena) {
+  constexpr int TM = 64, TN = 64, KC = kLdsKc, S = kLdsStages, AK = kLdsAk;
+  static_assert(S == 3, "the stage arrays below are written out for three stages");
+  const int f = list[blockIdx.y];
+  const FrontDev d = fr[f];
+  GemmView<double> v;
+  if (!gemm_view(region, d, k0, nb, K0, NB, arena, v)) return;
+  int row0, col0;
+  if (!gemm_tile(v.M, v.N, TM, TN, order, row0, col0)) return;
+  __shared__ __attribute__((aligned(16))) double sa0[kLdsASize], sb0[2 * kLdsBSize];
+  __shared__ __attribute__((aligned(16))) double sa1[kLdsASize], sb1[2 * kLdsBSize];
+  __shared__ __attribute__((aligned(16))) double sa2[kLdsASize], sb2[2 * kLdsBSize];
+  const int t = threadIdx.x, l = t % 64, w = t / 64, wm = w % 2, wn = w / 2;
+  // load sources: A row pair of this lane (plane l / 32) and B columns
+  const int arow = row0 + 2 * (l % 32);
+  const double *a_src = (l < 32 ? v.Ar : v.Ai) + (arow < v.M ? arow : 0);
+  const int bcol = min(col0 + 16 * w + l % 16, v.N - 1);
+  const double *b_src_r, *b_src_i;
+  if (bcol < v.N1) {
+    b_src_r = v.Br + bcol * v.ldb;
+    b_src_i = v.Bi + bcol * v.ldb;
+  } else {
+    b_src_r = v.B2r + (bcol - v.N1) * v.ldb2;
+    b_src_i = v.B2i + (bcol - v.N1) * v.ldb2;
+  }
+  const int bk = 2 * (l / 16);
+  auto issue = [&](double *sa, double *sb, int kc) {
+    lds_load16(a_src + static_cast<long long>(kc + 2 * w) * v.lda, sa + (2 * w) * AK);
+    lds_load16(a_src + static_cast<long long>(kc + 2 * w + 1) * v.lda, sa + (2 * w + 1) * AK);
+    lds_load16(b_src_r + kc + bk, sb + 128 * w);
+    lds_load16(b_src_i + kc + bk, sb + kLdsBSize + 128 * w);
+  };
+  const int nfull = v.K / KC, ktail = v.K - nfull * KC;
+  if (nfull > 0) issue(sa0, sb0, 0);
+  asm volatile("" ::: "memory");  // keeps chunk 0's loads ahead of chunk 1's
+  if (nfull > 1) issue(sa1, sb1, KC);
+  // accumulators of the 3M form with B not negated: x = P1 - Cr,
+  // y = P2, z = P3 - Cr - Ci, where P1 = sum Ar Br, P2 = sum Ai Bi and
+  // P3 = sum (Ar + Ai)(Br + Bi); then C - AB = (y - x) + i (x + y - z)
+  typedef Mfma<double>::acc_t acc_t;
+  acc_t cx[2][2], cy[2][2], cz[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int row = row0 + wm * 32 + a * 16 + l % 16;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int col = col0 + wn * 32 + b * 16 + Mfma<double>::ocol(l, e);
+        double xr = 0.0, xi = 0.0;
+        if (col < v.N1) {
+          if (row < v.M) {
+            const long long ic = static_cast<long long>(col) * v.ldc + row;
+            xr = v.Cr[ic];
+            xi = v.Ci[ic];
+          }
+        } else if (col < v.N && row < v.M2) {
+          const long long ic = static_cast<long long>(col - v.N1) * v.ldc2 + row;
+          xr = v.C2r[ic];
+          xi = v.C2i[ic];
+        }
+        cx[a][b][e] = -xr;
+        cy[a][b][e] = 0.0;
+        cz[a][b][e] = -(xr + xi);
+      }
+    }
+  }
+  auto compute = [&](const double *A, const double *B) {
+#pragma unroll
+    for (int kk = 0; kk < KC; kk += 4) {
+      const int k = kk + l / 16;
+      double a_r[2], a_i[2], a_s[2], b_r[2], b_i[2], b_s[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int row = wm * 32 + q * 16 + l % 16;
+        a_r[q] = A[k * AK + row];
+        a_i[q] = A[k * AK + 64 + row];
+        a_s[q] = a_r[q] + a_i[q];
+        const int off = (wn * 2 + q) * 128 + (k / 2) * 32 + (l % 16) * 2 + (k % 2);
+        b_r[q] = B[off];
+        b_i[q] = B[kLdsBSize + off];
+        b_s[q] = b_r[q] + b_i[q];
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          cx[a][b] = Mfma<double>::mma(b_r[b], a_r[a], cx[a][b]);
+          cy[a][b] = Mfma<double>::mma(b_i[b], a_i[a], cy[a][b]);
+          cz[a][b] = Mfma<double>::mma(b_s[b], a_s[a], cz[a][b]);
+        }
+    }
+  };
+  // chunk c of the ring: wait for its loads (this wave's later loads, of at
+  // most one more chunk, may stay in flight), barrier, refill the stage
+  // chunk c - 1 used with chunk c + 2, then the MFMAs
+  auto step = [&](int c, double *sa, double *sb, double *na, double *nb) {
+    if (c + 1 < nfull) __builtin_amdgcn_s_waitcnt(wait_vm_imm(4));
+    else __builtin_amdgcn_s_waitcnt(wait_vm_imm(0));
+    // a plain barrier: each wave has waited for its own loads of chunk c,
+    // and had consumed its reads of the stage refilled next before arriving
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (c + 2 < nfull) issue(na, nb, (c + 2) * KC);
+    compute(sa, sb);
+  };
+  for (int c = 0; c < nfull; c += S) {
+    step(c, sa0, sb0, sa2, sb2);
+    if (c + 1 < nfull) step(c + 1, sa1, sb1, sa0, sb0);
+    if (c + 2 < nfull) step(c + 2, sa2, sb2, sa1, sb1);
+  }
+  if (ktail > 0) {
+    // the stage after the last full chunk's; no loads are in flight
+    double *sa = nfull % S == 0 ? sa0 : nfull % S == 1 ? sa1 : sa2;
+    double *sb = nfull % S == 0 ? sb0 : nfull % S == 1 ? sb1 : sb2;
+    const int kc = nfull * KC;
+    // A: 8 k x 2 planes x 64 rows; B: 2 planes x 64 columns x 8 k (four each per thread)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int e = t + 256 * s, r = e % 64, pl = (e / 64) % 2, k = e / 128;
+      double x = 0.0;
+      if (k < ktail && row0 + r < v.M)
+        x = (pl ? v.Ai : v.Ar)[static_cast<long long>(kc + k) * v.lda + row0 + r];
+      sa[k * AK + pl * 64 + r] = x;
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int e = t + 256 * s, k = e % 8, col = (e / 8) % 64, pl = e / 512;
+      const int gc = col0 + col;
+      double x = 0.0;
+      if (k < ktail && gc < v.N) {
+        if (gc < v.N1) x = (pl ? v.Bi : v.Br)[static_cast<long long>(gc) * v.ldb + kc + k];
+        else x = (pl ? v.B2i : v.B2r)[static_cast<long long>(gc - v.N1) * v.ldb2 + kc + k];
+      }
+      sb[pl * kLdsBSize + (col / 16) * 128 + (k / 2) * 32 + (col % 16) * 2 + (k % 2)] = x;
+    }
+    __syncthreads();
+    compute(sa, sb);
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int row = row0 + wm * 32 + a * 16 + l % 16;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int col = col0 + wn * 32 + b * 16 + Mfma<double>::ocol(l, e);
+        if (col >= v.N) continue;
+        const double x = cx[a][b][e], y = cy[a][b][e];
+        const double re = y - x, im = (x + y) - cz[a][b][e];
+        if (col < v.N1) {
+          if (row >= v.M) continue;
+          const long long ic = static_cast<long long>(col) * v.ldc + row;
+          v.Cr[ic] = re;
+          v.Ci[ic] = im;
+        } else {
+          if (row >= v.M2) continue;
+          const long long ic = static_cast<long long>(col - v.N1) * v.ldc2 + row;
+          v.C2r[ic] = re;
+          v.C2i[ic] = im;
+        }
+      }
+    }
+  }
+}
+
 // Product form and waves-per-SIMD bound of each tile shape.  The 64x64
 // tiles use the 3M form; its third accumulator set spills at four waves per
 // SIMD (K chunk 4) and three (K chunk 8), so their bounds are three and two,
@@ -1671,10 +1903,11 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void k_gemm(const int *__rest
 // per factorization, XCD-grouped order): K chunk 4: 3M at three waves 106.8,
 // four products at four waves 116.1; K chunk 8: 3M at two 110.5, four
 // products at three 118.6; K chunk 16: 3M at two 110.5, four products at
-// one 126.8.  The 128x128 tiles keep four products (3M spills at 256
-// registers).
-constexpr int kGemmK4Waves = 3, kGemmK8Waves = 2, kGemmK16Waves = 2;
-constexpr bool kGemmK4M3 = true, kGemmK8M3 = true, kGemmK16M3 = true, kGemm128M3 = false;
+// one 126.8.  The LDS-ring tile (shape 2, 52 KB of LDS per workgroup) at two
+// waves per SIMD: 92.0 ms against 101.6 for the best register-staged tile;
+// at three (spilling at the 168-register bound) 98.9.
+constexpr int kGemmK4Waves = 3, kGemmK8Waves = 2, kGemmK16Waves = 2, kGemmLdsWaves = 2;
+constexpr bool kGemmK4M3 = true, kGemmK8M3 = true, kGemmK16M3 = true;
 
 // Launch one batched GEMM with the given tile shape; rows/cols are upper
 // bounds of M and N over the level's fronts.
@@ -1682,11 +1915,7 @@ template <typename T>
 hipError_t launch_gemm(int shape, int order, long long rows, long long cols, int count,
                        hipStream_t st, const int *list, const FrontDev *fr, int region, int k0,
                        int nb, int K0, int NB, T *arena) {
-  int tm = 64, tn = 64, threads = 256;
-  if (shape == kShape128x128) {
-    tm = tn = 128;
-    threads = 512;
-  }
+  const int tm = 64, tn = 64, threads = 256;
   const long long tr = rows <= 0 ? 1 : (rows + tm - 1) / tm, tc = cols <= 0 ? 1 : (cols + tn - 1) / tn;
   unsigned tiles = static_cast<unsigned>(tr * tc);
   if (order != kOrderLinear) tiles = (tiles + kXcds - 1) / kXcds * kXcds;  // XCD order: multiple of 8
@@ -1696,9 +1925,13 @@ hipError_t launch_gemm(int shape, int order, long long rows, long long cols, int
       hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 2, 4, kGemmK4Waves, kGemmK4M3>), grid, block, 0, st,
                          list, count, fr, region, k0, nb, K0, NB, order, arena);
       break;
-    case kShape128x128:
-      hipLaunchKernelGGL((k_gemm<T, 2, 4, 4, 2, GKC, 1, kGemm128M3>), grid, block, 0, st, list,
-                         count, fr, region, k0, nb, K0, NB, order, arena);
+    case kShapeLds:
+      if constexpr (sizeof(T) == 8)
+        hipLaunchKernelGGL(k_gemm_lds<kGemmLdsWaves>, grid, block, 0, st, list, count, fr, region,
+                           k0, nb, K0, NB, order, arena);
+      else
+        hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 2, 4, kGemmK4Waves, kGemmK4M3>), grid, block, 0, st,
+                           list, count, fr, region, k0, nb, K0, NB, order, arena);
       break;
     case kShape64x64k8:
       hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 2, 8, kGemmK8Waves, kGemmK8M3>), grid, block, 0, st,
@@ -2040,6 +2273,8 @@ static hipError_t factor_levels(const LuPlan &P, hipStream_t s, FactorTiming *tm
   hipError_t e;
   std::vector<hipEvent_t> events;  // start/stop pairs around GEMM launches when timing
   std::vector<int> event_level;    // level of each pair
+  std::vector<int> event_region;   // update region of each pair
+  if (tm) tm->level_gemm_flops.assign(P.levels.size(), 0.0);
   T *arena = reinterpret_cast<T *>(P.arena);
   hipLaunchKernelGGL(k_init_identity<T>, dim3(8, P.nsn), dim3(256), 0, s, P.d_front, P.nsn, arena);
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -2072,12 +2307,17 @@ static hipError_t factor_levels(const LuPlan &P, hipStream_t s, FactorTiming *tm
         events.push_back(e0);
         events.push_back(e1);
         event_level.push_back(static_cast<int>(lev));
+        event_region.push_back(region);
         if ((r = hipEventRecord(e0, st)) != hipSuccess) return r;
         for (int q = 0; q < L.count; ++q) {
           const GemmShape g = gemm_shape(region, P.h_front[P.h_level_list[L.list_off + q]], k0, nb, K0, W);
-          if (g.M > 0 && g.N > 0 && g.K > 0)
-            tm->gemm_flops += 8.0 * g.K * (static_cast<double>(g.M) * g.N1 +
+          if (g.M > 0 && g.N > 0 && g.K > 0) {
+            const double fl = 8.0 * g.K * (static_cast<double>(g.M) * g.N1 +
                                            static_cast<double>(g.M2) * (g.N - g.N1));
+            tm->gemm_flops += fl;
+            tm->level_gemm_flops[lev] += fl;
+            tm->region_gemm_flops[region] += fl;
+          }
         }
         ++tm->gemm_launches;
       }
@@ -2218,6 +2458,7 @@ static hipError_t factor_levels(const LuPlan &P, hipStream_t s, FactorTiming *tm
       if ((e = hipEventElapsedTime(&ms, events[q], events[q + 1])) != hipSuccess) return e;
       tm->gemm_ms += ms;
       tm->level_gemm_ms[event_level[q / 2]] += ms;
+      tm->region_gemm_ms[event_region[q / 2]] += ms;
     }
     for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
   }

@@ -37,7 +37,7 @@ struct LuPlan {
   long long w_len = 0;          // per-plane length of the front workspace
   std::vector<LevelPlan> levels;
   std::vector<int> gemm_order;     // per level (kOrder*); empty = linear everywhere
-  std::vector<int> gemm_shape;     // per level tile shape (0: 64x64, 1: 128x64, 2: 128x128)
+  std::vector<int> gemm_shape;     // per level tile shape (the kShape* codes of lu.hip)
   bool panel_streaming = false;    // force the streaming panel kernel on every level
   bool pair_blocks = true;         // two-level blocking: pairs of blocks share one update (K = 256)
   std::vector<FrontDev> h_front;   // host copies (flop accounting)
@@ -118,7 +118,10 @@ struct FactorTiming {
   double gemm_ms = 0.0;
   double gemm_flops = 0.0;
   int gemm_launches = 0;
-  std::vector<double> level_gemm_ms;  // GEMM time per elimination level
+  std::vector<double> level_gemm_ms;     // GEMM time per elimination level
+  std::vector<double> level_gemm_flops;  // and its flops (8 per complex multiply-add)
+  double region_gemm_ms[8] = {};         // by update region (the kGemm* codes of lu.hip)
+  double region_gemm_flops[8] = {};
 };
 
 // Tile orders and shapes of the frontal GEMM (chosen per level by the

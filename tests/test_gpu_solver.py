@@ -227,6 +227,59 @@ def test_panel_kernels_agree(oracle_lib, shape, precision):
     assert diff <= (1e-8 if precision == 0 else 1e-2)
 
 
+@pytest.mark.parametrize("shape", [(64, 32), (120, 60), (200, 100)])
+def test_lds_ring_tile_matches_register_tile(oracle_lib, shape):
+    """Tile shape 2 (operands loaded straight into a three-stage LDS ring)
+    against shape 3 (the same 64 x 64 3M tile staged through registers):
+    the same factorization up to rounding, so the unrefined solutions agree
+    far below the solve bar.  Fronts at these sizes have row and column
+    counts that are not multiples of the tile and trailing updates whose
+    depth leaves a K tail (p - K0 mod 8 != 0, e.g. p = 475 or 1375)."""
+    c = cs.synthetic_case(*shape, omega_shift=0.02,
+                          nodes=None if shape[0] in (100, 200) else np.linspace(-1.0, 1.0, shape[0]))
+    A, B = operators(oracle_lib, c)
+    sigma = 0.1 - 0.05j
+    K = (A - sigma * B).tocsr()
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(K.shape[0]) + 1j * rng.standard_normal(K.shape[0])
+    xs = []
+    with ta.Context(0) as ctx:
+        ctx.load_case(c)
+        ctx.assemble(diagnostics=False)
+        for config in (2 * 2 + 1, 2 * 3 + 1, 2 * 2):
+            ctx.set_gemm_config(config)
+            ctx.factor(sigma)
+            x, _, _ = ctx.linear_solve(b, refine_max=0)
+            xs.append(x)
+    res = [np.linalg.norm(K @ x - b) / np.linalg.norm(b) for x in xs]
+    diffs = [np.linalg.norm(x - xs[1]) / np.linalg.norm(xs[1]) for x in (xs[0], xs[2])]
+    print(f"{shape}: residuals {[f'{r:.2e}' for r in res]}, rel diff vs shape 3 "
+          f"{[f'{d:.2e}' for d in diffs]}", flush=True)
+    assert max(res) <= 1e-9
+    assert max(diffs) <= 1e-8
+
+
+def test_gemm_profile_by_level_and_region():
+    """tailor_profile_gemm: the per-level and per-region GEMM figures of one
+    factorization cover the same launches (equal sums) and the flops equal
+    tailor_profile_factor's algorithmic count."""
+    c = cs.synthetic_case(100, 50, omega_shift=0.02)
+    with ta.Context(0) as ctx:
+        ctx.load_case(c)
+        ctx.assemble(diagnostics=False)
+        f = ctx.profile_factor(c.lambda_shift)
+        g = ctx.profile_gemm(c.lambda_shift)
+    lf, rf = sum(g["level_flops"]), sum(g["region_flops"].values())
+    lm, rm = sum(g["level_ms"]), sum(g["region_ms"].values())
+    print(f"levels {len(g['level_ms'])}: {lm:.2f} ms, {lf:.3e} flops; regions {g['region_ms']}",
+          flush=True)
+    assert len(g["level_ms"]) >= 5
+    assert lf == pytest.approx(rf, rel=1e-12) and lf == pytest.approx(f["gemm_flops"], rel=1e-12)
+    assert lm == pytest.approx(rm, rel=1e-6) and lm > 0.0
+    assert all(x > 0.0 for x in g["level_flops"])
+    assert g["region_flops"]["trailrest"] > 0.0 and g["region_flops"]["inner"] > 0.0
+
+
 @pytest.mark.parametrize("config", range(8))
 def test_every_gemm_configuration(oracle_lib, config):
     """Each frontal-GEMM tile shape and order (the autotuner's choices) gives
@@ -281,10 +334,13 @@ def test_solve_pair_matches_single_solves(oracle_lib, shape, precision):
 
 def test_measured_peaks_are_plausible():
     """tailor_measure_peaks: the sustained FP64 MFMA and HBM read rates lie
-    between a tenth of and the published peaks (78.6 TFLOP/s, 8 TB/s)."""
+    below the published peaks (78.6 TFLOP/s, 8 TB/s), the MFMA rate within a
+    quarter of its peak (an MI355X issues one 16x16x4 FP64 MFMA per 64
+    cycles per SIMD; a calibration loop that copies its accumulators between
+    register files reads ~0.6 of the peak) and the read rate above a tenth."""
     peaks = ta.measure_peaks(0)
     print(f"measured: {peaks}", flush=True)
-    assert 7.86 < peaks["fp64_mfma_tflops"] <= 78.6 * 1.05
+    assert 0.75 * 78.6 < peaks["fp64_mfma_tflops"] <= 78.6 * 1.05
     assert 800.0 < peaks["read_gbs"] <= 8000.0 * 1.05
 
 
