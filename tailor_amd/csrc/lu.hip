@@ -1415,7 +1415,7 @@ constexpr int GKC = 16;  // K chunk staged in LDS
 //   shape 1: 64 x 64, 256 threads, K chunk 4, 3M form, registers held to
 //            three waves per SIMD (a quarter of shape 0's LDS);
 //   shape 2: 64 x 64, 256 threads, K chunk 8, 3M form, operands loaded
-//            straight into a three-stage LDS ring (k_gemm_lds; FP64 only,
+//            straight into a ring of LDS stages (k_gemm_lds; FP64 only,
 //            FP32 fronts use shape 1's tile);
 //   shape 3: 64 x 64, 256 threads, K chunk 8, 3M form, two waves per SIMD.
 // The shorter K chunks trade more barriers for more resident workgroups,
@@ -1679,92 +1679,85 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void k_gemm(const int *__rest
 // 2 x 2 MFMA blocks, K chunk 8), fed by global -> LDS loads that write
 // LDS without passing through registers (global_load_lds_dwordx4: each lane
 // moves 16 bytes, a wave's 1 KB lands contiguously).  The chunks rotate
-// through a ring of kLdsStages LDS stages, so chunk c + 2 is in flight while
-// chunk c feeds the MFMAs, and a chunk costs one barrier and no VALU work
-// to stage.  Per chunk each wave issues four loads: A columns k = 2w, 2w + 1
+// through a ring of S LDS stages (kGemmLdsStages = 2: chunk c + 1 is in
+// flight while chunk c feeds the MFMAs; with three, chunk c + 2 as well),
+// and a chunk costs one barrier and no VALU work to stage.  Per chunk each
+// wave issues four loads: A columns k = 2w, 2w + 1
 // (lanes 0-31 the real plane's 64 rows, two per lane; lanes 32-63 the
 // imaginary), and the real and imaginary planes of B's column group w
 // (16 columns x 8 k; lane l: column l % 16, k pair l / 16).  Stage layout:
 //   A  [k (8)][plane (2)][row (64)], 16 doubles of padding per k;
-//   B  [plane (2)][column group (4)][k pair (4)][column (16)][2].
+//   B  per plane [column group (4)][k pair (4)][column (16)][2].
 // Loads never stray outside the front's planes: a row pair starting beyond
 // M, or a column beyond N, is redirected to row pair 0 or the last column
 // (such rows and columns only reach outputs that are not stored), and a row
 // pair (M - 1, M) reads the element after its column's last row, which is
 // the next column's first (or the next plane's).  A K tail (K mod 8) is
-// staged through registers with zeros past K, as in k_gemm.  The stages are
-// separate arrays, which lets the compiler see that the MFMA operand reads
-// of one stage do not depend on the loads in flight into another.
-constexpr int kLdsStages = 3, kLdsKc = 8, kLdsAk = 2 * 64 + 16;
+// staged through registers with zeros past K, as in k_gemm.
+//
+// Each stage's A and B planes are arrays of their own, addressed with
+// compile-time stage indices: the compiler then tells the MFMA operand reads
+// of one stage from the loads in flight into the others and waits only for
+// those it must (the waits the ring needs are written out below).
+constexpr int kLdsKc = 8, kLdsAk = 2 * 64 + 16;
 constexpr int kLdsASize = kLdsKc * kLdsAk, kLdsBSize = 4 * 128;  // doubles per stage (B: per plane)
-
-// One global -> LDS load of 16 bytes per lane; the wave's 1 KB lands at the
-// LDS address of l (wave-uniform) + 16 x lane.  Written out rather than
-// through the compiler builtin: the compiler's tracking of such loads puts a
-// wait for every load in flight before operand reads it cannot tell apart
-// from the stage being filled, which serializes the ring.  The kernel waits
-// for these loads itself (s_waitcnt vmcnt, then a barrier) before any wave
-// reads the stage; the loads have no register results, and a compiler wait
-// for its own loads only becomes stricter with them in flight, since loads
-// complete in issue order.  M0 carries the LDS address; the compiled
-// kernel keeps nothing in M0 (its only M0 writes are these).
-#pragma clang diagnostic push
-#pragma clang diagnostic ignored "-Winline-asm"
-__device__ inline void lds_load16(const double *g, double *l) {
-  typedef __attribute__((address_space(3))) double lds_double;
-  const unsigned lds = __builtin_amdgcn_readfirstlane(
-      static_cast<unsigned>(reinterpret_cast<uintptr_t>((lds_double *)l)));
-  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
-               :
-               : "v"(g), "s"(lds)
-               : "memory", "m0");
-}
-#pragma clang diagnostic pop
 
 // s_waitcnt immediate that waits until at most n vector-memory operations
 // of this wave are outstanding (other counters untouched)
 constexpr int wait_vm_imm(int n) { return (n & 0xF) | ((n >> 4) << 14) | (0x7 << 4) | (0xF << 8); }
 
-template <int MINW>
+// One global -> LDS load of 16 bytes per lane (the wave's 1 KB lands at l,
+// wave-uniform, + 16 x lane).  A device function of its own: the builtin
+// exists for the device only.
+__device__ inline void lds_load16(const double *g, double *l) {
+  __builtin_amdgcn_global_load_lds(g, l, 16, 0, 0);
+}
+
+template <int MINW, int S>
 __global__ __launch_bounds__(256, MINW) void k_gemm_lds(const int *__restrict__ list, int nlist,
                                                       const FrontDev *__restrict__ fr, int region,
                                                       int k0, int nb, int K0, int NB, int order,
                                                       double *__restrict__ arena) {
-  constexpr int TM = 64, TN = 64, KC = kLdsKc, S = kLdsStages, AK = kLdsAk;
-  static_assert(S == 3, "the stage arrays below are written out for three stages");
+  constexpr int TM = 64, TN = 64, KC = kLdsKc, AK = kLdsAk;
+  static_assert(S == 2 || S == 3, "the stage arrays below are written out for two or three stages");
   const int f = list[blockIdx.y];
   const FrontDev d = fr[f];
   GemmView<double> v;
   if (!gemm_view(region, d, k0, nb, K0, NB, arena, v)) return;
   int row0, col0;
   if (!gemm_tile(v.M, v.N, TM, TN, order, row0, col0)) return;
-  __shared__ __attribute__((aligned(16))) double sa0[kLdsASize], sb0[2 * kLdsBSize];
-  __shared__ __attribute__((aligned(16))) double sa1[kLdsASize], sb1[2 * kLdsBSize];
-  __shared__ __attribute__((aligned(16))) double sa2[kLdsASize], sb2[2 * kLdsBSize];
+  __shared__ __attribute__((aligned(16))) double sa0[kLdsASize], sbr0[kLdsBSize], sbi0[kLdsBSize];
+  __shared__ __attribute__((aligned(16))) double sa1[kLdsASize], sbr1[kLdsBSize], sbi1[kLdsBSize];
+  __shared__ __attribute__((aligned(16))) double sa2[S > 2 ? kLdsASize : 2], sbr2[S > 2 ? kLdsBSize : 2],
+      sbi2[S > 2 ? kLdsBSize : 2];
+  auto stage_a = [&](int st) -> double * { return st == 0 ? sa0 : st == 1 ? sa1 : sa2; };
+  auto stage_br = [&](int st) -> double * { return st == 0 ? sbr0 : st == 1 ? sbr1 : sbr2; };
+  auto stage_bi = [&](int st) -> double * { return st == 0 ? sbi0 : st == 1 ? sbi1 : sbi2; };
   const int t = threadIdx.x, l = t % 64, w = t / 64, wm = w % 2, wn = w / 2;
   // load sources: A row pair of this lane (plane l / 32) and B columns
   const int arow = row0 + 2 * (l % 32);
   const double *a_src = (l < 32 ? v.Ar : v.Ai) + (arow < v.M ? arow : 0);
   const int bcol = min(col0 + 16 * w + l % 16, v.N - 1);
-  const double *b_src_r, *b_src_i;
+  const double *b_src[2];
   if (bcol < v.N1) {
-    b_src_r = v.Br + bcol * v.ldb;
-    b_src_i = v.Bi + bcol * v.ldb;
+    b_src[0] = v.Br + bcol * v.ldb;
+    b_src[1] = v.Bi + bcol * v.ldb;
   } else {
-    b_src_r = v.B2r + (bcol - v.N1) * v.ldb2;
-    b_src_i = v.B2i + (bcol - v.N1) * v.ldb2;
+    b_src[0] = v.B2r + (bcol - v.N1) * v.ldb2;
+    b_src[1] = v.B2i + (bcol - v.N1) * v.ldb2;
   }
   const int bk = 2 * (l / 16);
-  auto issue = [&](double *sa, double *sb, int kc) {
-    lds_load16(a_src + static_cast<long long>(kc + 2 * w) * v.lda, sa + (2 * w) * AK);
-    lds_load16(a_src + static_cast<long long>(kc + 2 * w + 1) * v.lda, sa + (2 * w + 1) * AK);
-    lds_load16(b_src_r + kc + bk, sb + 128 * w);
-    lds_load16(b_src_i + kc + bk, sb + kLdsBSize + 128 * w);
+  auto issue = [&](int st, int kc) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      lds_load16(a_src + static_cast<long long>(kc + 2 * w + u) * v.lda, stage_a(st) + (2 * w + u) * AK);
+    lds_load16(b_src[0] + kc + bk, stage_br(st) + 128 * w);
+    lds_load16(b_src[1] + kc + bk, stage_bi(st) + 128 * w);
   };
   const int nfull = v.K / KC, ktail = v.K - nfull * KC;
-  if (nfull > 0) issue(sa0, sb0, 0);
-  asm volatile("" ::: "memory");  // keeps chunk 0's loads ahead of chunk 1's
-  if (nfull > 1) issue(sa1, sb1, KC);
+#pragma unroll
+  for (int st = 0; st < S - 1; ++st)
+    if (st < nfull) issue(st, st * KC);
   // accumulators of the 3M form with B not negated: x = P1 - Cr,
   // y = P2, z = P3 - Cr - Ci, where P1 = sum Ar Br, P2 = sum Ai Bi and
   // P3 = sum (Ar + Ai)(Br + Bi); then C - AB = (y - x) + i (x + y - z)
@@ -1796,7 +1789,7 @@ __global__ __launch_bounds__(256, MINW) void k_gemm_lds(const int *__restrict__ 
       }
     }
   }
-  auto compute = [&](const double *A, const double *B) {
+  auto compute = [&](const double *A, const double *Br, const double *Bi) {
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 4) {
       const int k = kk + l / 16;
@@ -1808,8 +1801,8 @@ __global__ __launch_bounds__(256, MINW) void k_gemm_lds(const int *__restrict__ 
         a_i[q] = A[k * AK + 64 + row];
         a_s[q] = a_r[q] + a_i[q];
         const int off = (wn * 2 + q) * 128 + (k / 2) * 32 + (l % 16) * 2 + (k % 2);
-        b_r[q] = B[off];
-        b_i[q] = B[kLdsBSize + off];
+        b_r[q] = Br[off];
+        b_i[q] = Bi[off];
         b_s[q] = b_r[q] + b_i[q];
       }
 #pragma unroll
@@ -1822,29 +1815,30 @@ __global__ __launch_bounds__(256, MINW) void k_gemm_lds(const int *__restrict__ 
         }
     }
   };
-  // chunk c of the ring: wait for its loads (this wave's later loads, of at
-  // most one more chunk, may stay in flight), barrier, refill the stage
-  // chunk c - 1 used with chunk c + 2, then the MFMAs
-  auto step = [&](int c, double *sa, double *sb, double *na, double *nb) {
-    if (c + 1 < nfull) __builtin_amdgcn_s_waitcnt(wait_vm_imm(4));
-    else __builtin_amdgcn_s_waitcnt(wait_vm_imm(0));
-    // a plain barrier: each wave has waited for its own loads of chunk c,
-    // and had consumed its reads of the stage refilled next before arriving
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-    if (c + 2 < nfull) issue(na, nb, (c + 2) * KC);
-    compute(sa, sb);
-  };
-  for (int c = 0; c < nfull; c += S) {
-    step(c, sa0, sb0, sa2, sb2);
-    if (c + 1 < nfull) step(c + 1, sa1, sb1, sa0, sb0);
-    if (c + 2 < nfull) step(c + 2, sa2, sb2, sa1, sb1);
+  for (int c0 = 0; c0 < nfull; c0 += S) {
+#pragma unroll
+    for (int st = 0; st < S; ++st) {
+      const int c = c0 + st;
+      if (c < nfull) {
+        // chunk c's loads are done once at most the later chunk's (c + 1,
+        // four loads, issued with three stages) are in flight
+        if (S == 3 && c + 1 < nfull) __builtin_amdgcn_s_waitcnt(wait_vm_imm(4));
+        else __builtin_amdgcn_s_waitcnt(wait_vm_imm(0));
+        // a plain barrier: each wave has waited for its own loads of chunk
+        // c, and consumed its reads of the stage refilled next (chunk
+        // c - 1's) before arriving
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (c + S - 1 < nfull) issue((st + S - 1) % S, (c + S - 1) * KC);
+        compute(stage_a(st), stage_br(st), stage_bi(st));
+      }
+    }
   }
   if (ktail > 0) {
     // the stage after the last full chunk's; no loads are in flight
-    double *sa = nfull % S == 0 ? sa0 : nfull % S == 1 ? sa1 : sa2;
-    double *sb = nfull % S == 0 ? sb0 : nfull % S == 1 ? sb1 : sb2;
+    const int st = nfull % S;
+    double *sa = stage_a(st), *sbr = stage_br(st), *sbi = stage_bi(st);
     const int kc = nfull * KC;
     // A: 8 k x 2 planes x 64 rows; B: 2 planes x 64 columns x 8 k (four each per thread)
 #pragma unroll
@@ -1864,10 +1858,10 @@ __global__ __launch_bounds__(256, MINW) void k_gemm_lds(const int *__restrict__ 
         if (gc < v.N1) x = (pl ? v.Bi : v.Br)[static_cast<long long>(gc) * v.ldb + kc + k];
         else x = (pl ? v.B2i : v.B2r)[static_cast<long long>(gc - v.N1) * v.ldb2 + kc + k];
       }
-      sb[pl * kLdsBSize + (col / 16) * 128 + (k / 2) * 32 + (col % 16) * 2 + (k % 2)] = x;
+      (pl ? sbi : sbr)[(col / 16) * 128 + (k / 2) * 32 + (col % 16) * 2 + (k % 2)] = x;
     }
     __syncthreads();
-    compute(sa, sb);
+    compute(sa, sbr, sbi);
   }
 #pragma unroll
   for (int a = 0; a < 2; ++a) {
@@ -1903,10 +1897,13 @@ __global__ __launch_bounds__(256, MINW) void k_gemm_lds(const int *__restrict__ 
 // per factorization, XCD-grouped order): K chunk 4: 3M at three waves 106.8,
 // four products at four waves 116.1; K chunk 8: 3M at two 110.5, four
 // products at three 118.6; K chunk 16: 3M at two 110.5, four products at
-// one 126.8.  The LDS-ring tile (shape 2, 52 KB of LDS per workgroup) at two
-// waves per SIMD: 92.0 ms against 101.6 for the best register-staged tile;
-// at three (spilling at the 168-register bound) 98.9.
-constexpr int kGemmK4Waves = 3, kGemmK8Waves = 2, kGemmK16Waves = 2, kGemmLdsWaves = 2;
+// one 126.8.  The LDS-ring tile (shape 2): against 101.6 ms for the best
+// register-staged tile, three stages (52 KB of LDS per workgroup) with the
+// loads written as inline assembly 92.0 at two waves per SIMD and 98.9 at
+// three (spilling); on another GPU, three stages 93.0 (inline assembly, two
+// waves) and 92.8 (builtin loads, three waves), two stages (35 KB) 89.9 at
+// three waves (two compile to the same code).
+constexpr int kGemmK4Waves = 3, kGemmK8Waves = 2, kGemmK16Waves = 2, kGemmLdsWaves = 3, kGemmLdsStages = 2;
 constexpr bool kGemmK4M3 = true, kGemmK8M3 = true, kGemmK16M3 = true;
 
 // Launch one batched GEMM with the given tile shape; rows/cols are upper
@@ -1927,7 +1924,7 @@ hipError_t launch_gemm(int shape, int order, long long rows, long long cols, int
       break;
     case kShapeLds:
       if constexpr (sizeof(T) == 8)
-        hipLaunchKernelGGL(k_gemm_lds<kGemmLdsWaves>, grid, block, 0, st, list, count, fr, region,
+        hipLaunchKernelGGL((k_gemm_lds<kGemmLdsWaves, kGemmLdsStages>), grid, block, 0, st, list, count, fr, region,
                            k0, nb, K0, NB, order, arena);
       else
         hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 2, 4, kGemmK4Waves, kGemmK4M3>), grid, block, 0, st,
