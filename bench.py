@@ -107,7 +107,8 @@ def traced_gemm_window():
     path = ROOT / "profiles" / "bench_sweep_200x100_kernel_stats.md"
     if not path.exists():
         return None
-    m = re.search(r"Frontal GEMM \(k_gemm, all tile shapes\): (\d+) launches, ([0-9.]+) ms, "
+    m = re.search(r"Frontal GEMM \(k_gemm and k_gemm_lds, all tile shapes\): (\d+) launches, "
+                  r"([0-9.]+) ms, "
                   r"mean ([0-9.]+) us per launch", path.read_text())
     if not m:
         return None
@@ -627,7 +628,7 @@ def main() -> int:
         "speedup_vs_reference_plain": round(rate(L) / max(rate(PR), 1e-300), 3),
         "network_gain": round(rate(L) / max(rate(F), 1e-300), 3),
         "learned_pairs_found_by_plain": pipe["overlap"],
-        "roofline": {"bound": "mfma", "kernel": "k_gemm (frontal updates, MFMA f64 16x16x4)",
+        "roofline": {"bound": "mfma", "kernel": "k_gemm / k_gemm_lds (frontal updates, MFMA f64 16x16x4)",
                      "achieved": round(gemm_tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(gemm_tf / FP64_PEAK_TFLOPS, 4),
                      "measured_peak": round(mfma_meas, 2),

@@ -1423,37 +1423,57 @@ constexpr int GKC = 16;  // K chunk staged in LDS
 enum { kShape64x64 = 0, kShape64x64k4 = 1, kShapeLds = 2, kShape64x64k8 = 3, kShapeCount = 4 };
 static_assert(kShapeCount == kGemmShapes, "shape count");
 
-// Output tile (row0, col0) of this workgroup among the (M / TM) x (N / TN)
-// tiles of its front; false when it has none.
-__device__ inline bool gemm_tile(int M, int N, int TM, int TN, int order, int &row0, int &col0) {
-  const int tiles_m = (M + TM - 1) / TM, tiles_n = (N + TN - 1) / TN;
+// Output tile (row0, col0) of this workgroup in its front's update v.  The
+// tiles holding output are enumerated as two blocks: every row tile of the
+// column tiles that start in the first column segment, then the row tiles
+// above M2 of the column tiles in the second segment (the E columns of a
+// trailing update hold output in rows [0, p - K1) only; a tile below would
+// run its whole K loop to store nothing).  False for a workgroup past the
+// front's tiles.
+template <typename T>
+__device__ inline bool gemm_tile(const GemmView<T> &v, int TM, int TN, int order, int &row0,
+                                 int &col0) {
+  const int tiles_m = (v.M + TM - 1) / TM, tiles_n = (v.N + TN - 1) / TN;
+  const int split = min((v.N1 + TN - 1) / TN, tiles_n);
+  const int tiles_m2 = min(max(v.M2, 0) + TM - 1, v.M + TM - 1) / TM;
+  const int na = tiles_m * split, ntiles = na + tiles_m2 * (tiles_n - split);
+  int tile;
   if (order == kOrderLinear) {
-    // tile = blockIdx.x, row blocks fastest
-    const int tile = blockIdx.x;
-    if (tile >= tiles_m * tiles_n) return false;
-    row0 = (tile % tiles_m) * TM;
-    col0 = (tile / tiles_m) * TN;
-    return true;
+    tile = blockIdx.x;  // row tiles fastest
+  } else {
+    // XCD-aware order: workgroups are dispatched round-robin over the 8
+    // XCDs, so XCD x (= blockIdx.x % 8, gridDim.x being a multiple of 8)
+    // receives a contiguous eighth of this front's tiles (rotated by front);
+    // within it, tiles go kGemmGroup row tiles at a time so that
+    // co-resident workgroups share A and B tiles in that XCD's L2.
+    const int per = (ntiles + kXcds - 1) / kXcds;
+    const int q = static_cast<int>(blockIdx.x / kXcds);
+    if (q >= per) return false;
+    const int slot = (static_cast<int>(blockIdx.x % kXcds) - static_cast<int>(blockIdx.y % kXcds) +
+                      kXcds) % kXcds;
+    tile = slot * per + q;
   }
-  // XCD-aware order: workgroups are dispatched round-robin over the 8
-  // XCDs, so XCD x (= blockIdx.x % 8, gridDim.x being a multiple of 8)
-  // receives a contiguous eighth of this front's tiles (rotated by front);
-  // within it, tiles go kGemmGroup row blocks at a time so that
-  // co-resident workgroups share A and B tiles in that XCD's L2.
-  const int ntiles = tiles_m * tiles_n;
-  const int per = (ntiles + kXcds - 1) / kXcds;
-  const int q = static_cast<int>(blockIdx.x / kXcds);
-  if (q >= per) return false;
-  const int slot = (static_cast<int>(blockIdx.x % kXcds) - static_cast<int>(blockIdx.y % kXcds) +
-                    kXcds) % kXcds;
-  const int tile = slot * per + q;
   if (tile >= ntiles) return false;
-  const int in_group = kGemmGroup * tiles_n;
-  const int first_m = (tile / in_group) * kGemmGroup;
-  const int group_m = min(tiles_m - first_m, kGemmGroup);
-  const int tm = first_m + (tile % in_group) % group_m, tn = (tile % in_group) / group_m;
+  int rows = tiles_m, cols = split, coff = 0;
+  if (tile >= na) {
+    tile -= na;
+    rows = tiles_m2;
+    cols = tiles_n - split;
+    coff = split;
+  }
+  int tm, tn;
+  if (order == kOrderLinear) {
+    tm = tile % rows;
+    tn = tile / rows;
+  } else {
+    const int in_group = kGemmGroup * cols;
+    const int first_m = (tile / in_group) * kGemmGroup;
+    const int group_m = min(rows - first_m, kGemmGroup);
+    tm = first_m + (tile % in_group) % group_m;
+    tn = (tile % in_group) / group_m;
+  }
   row0 = tm * TM;
-  col0 = tn * TN;
+  col0 = (tn + coff) * TN;
   return true;
 }
 
@@ -1479,7 +1499,7 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void k_gemm(const int *__rest
   GemmView<T> v;
   if (!gemm_view(region, d, k0, nb, K0, NB, arena, v)) return;
   int row0, col0;
-  if (!gemm_tile(v.M, v.N, TM, TN, order, row0, col0)) return;
+  if (!gemm_tile(v, TM, TN, order, row0, col0)) return;
   // two LDS buffers: the next K chunk is stored while the current one feeds
   // the MFMAs, so each chunk needs one barrier
   __shared__ T sAr[2][GKC][LDA_], sAi[2][GKC][LDA_], sBr[2][TN][LDB_], sBi[2][TN][LDB_];
@@ -1725,7 +1745,7 @@ __global__ __launch_bounds__(256, MINW) void k_gemm_lds(const int *__restrict__ 
   GemmView<double> v;
   if (!gemm_view(region, d, k0, nb, K0, NB, arena, v)) return;
   int row0, col0;
-  if (!gemm_tile(v.M, v.N, TM, TN, order, row0, col0)) return;
+  if (!gemm_tile(v, TM, TN, order, row0, col0)) return;
   __shared__ __attribute__((aligned(16))) double sa0[kLdsASize], sbr0[kLdsBSize], sbi0[kLdsBSize];
   __shared__ __attribute__((aligned(16))) double sa1[kLdsASize], sbr1[kLdsBSize], sbi1[kLdsBSize];
   __shared__ __attribute__((aligned(16))) double sa2[S > 2 ? kLdsASize : 2], sbr2[S > 2 ? kLdsBSize : 2],

@@ -89,11 +89,12 @@ def summarize(csv_paths, command: str) -> str:
                 c, ns = win.get(name, (0, 0))
                 win[name] = (c + 1, ns + (t1 - t0))
                 wtot += t1 - t0
-        g_calls = sum(c for n, (c, _) in win.items() if n.startswith("k_gemm<"))
-        g_ns = sum(ns for n, (_, ns) in win.items() if n.startswith("k_gemm<"))
+        gemm = lambda n: n.startswith("k_gemm<") or n.startswith("k_gemm_lds<")
+        g_calls = sum(c for n, (c, _) in win.items() if gemm(n))
+        g_ns = sum(ns for n, (_, ns) in win.items() if gemm(n))
         lines += ["", "## The event-timed factorization (tailor_profile_factor window)", "",
                   f"Window {(hi - lo) / 1e6:.2f} ms, {wtot / 1e6:.2f} ms of kernel time.  "
-                  f"Frontal GEMM (k_gemm, all tile shapes): {g_calls} launches, "
+                  f"Frontal GEMM (k_gemm and k_gemm_lds, all tile shapes): {g_calls} launches, "
                   f"{g_ns / 1e6:.2f} ms, mean {g_ns / max(g_calls, 1) / 1e3:.2f} us per launch.", ""]
         lines += _table(win, wtot, limit=25)
     return "\n".join(lines) + "\n"
@@ -101,8 +102,11 @@ def summarize(csv_paths, command: str) -> str:
 
 def _family(name: str) -> str:
     """Counter rows are grouped by kernel family: k_gemm by element type (all
-    tile shapes), k_gemv by type and columns per wave, others by full name."""
+    tile shapes, the FP64 LDS-ring tile k_gemm_lds included), k_gemv by type
+    and columns per wave, others by full name."""
     import re
+    if name.startswith("k_gemm_lds<"):
+        return "k_gemm<double>"
     m = re.match(r"(k_gemm|k_gemv)<([^>]*)>", name)
     if not m:
         return name
