@@ -91,10 +91,10 @@ __global__ void k_init_identity(const FrontDev *__restrict__ fr, int nf, T *__re
   if (f >= nf) return;
   const FrontDev d = fr[f];
   T *E = arena + d.off + 2LL * d.m * d.m;
-  T *Z = E + 2LL * d.p * d.p;
+  T *Z = E + 2LL * d.pe * d.p;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < d.p; k += gridDim.x * blockDim.x) {
-    E[static_cast<long long>(k) * d.p + k] = T(1);
-    Z[static_cast<long long>(k) * d.p + k] = T(1);
+    E[static_cast<long long>(k) * d.pe + k] = T(1);
+    Z[static_cast<long long>(k) * d.pe + k] = T(1);
   }
 }
 
@@ -878,7 +878,7 @@ __global__ __launch_bounds__(256) void k_left_apply(const int *__restrict__ list
   if (k0 >= d.p) return;
   const int kb = min(nb, d.p - k0);
   const int K1 = min(K0 + NB, d.p);
-  const long long m = d.m, p = d.p, mm = m * m, pp = p * p;
+  const long long m = d.m, pe = d.pe, mm = m * m, pp = pe * d.p;
   T *Fr = arena + d.off, *Fi = Fr + mm;
   T *Er = Fi + mm, *Ei = Er + pp;
   T *Zr = Ei + pp, *Zi = Zr + pp;
@@ -906,11 +906,11 @@ __global__ __launch_bounds__(256) void k_left_apply(const int *__restrict__ list
       xr = Fr + base;
       xi = Fi + base;
     } else if (mode == 1) {
-      base = static_cast<long long>(j - nF) * p + k0;
+      base = static_cast<long long>(j - nF) * pe + k0;
       xr = Er + base;
       xi = Ei + base;
     } else {
-      base = static_cast<long long>(k0 + j) * p + k0;
+      base = static_cast<long long>(k0 + j) * pe + k0;
       xr = Zr + base;
       xi = Zi + base;
     }
@@ -1060,9 +1060,9 @@ __global__ __launch_bounds__(256) void k_upper_solve(const int *__restrict__ lis
   const FrontDev d = fr[list[blockIdx.y]];
   if (k0 >= d.p) return;
   const int kb = min(nb, d.p - k0);
-  const long long m = d.m, p = d.p;
+  const long long m = d.m, pe = d.pe, pp = pe * d.p;
   const T *Fr = arena + d.off, *Fi = Fr + m * m;
-  T *Zr = arena + d.off + 2 * m * m + 2 * p * p, *Zi = Zr + p * p;
+  T *Zr = arena + d.off + 2 * m * m + 2 * pp, *Zi = Zr + pp;
   const int col0 = k0 + blockIdx.x * 8;  // 8 columns per workgroup
   if (col0 >= d.p) return;
   __shared__ T Ur[kLuNb][kLuNb + 1], Ui[kLuNb][kLuNb + 1];  // U[i][q] = F[k0+i, k0+q]
@@ -1082,8 +1082,8 @@ __global__ __launch_bounds__(256) void k_upper_solve(const int *__restrict__ lis
   const int lane = threadIdx.x % 64, h = lane / 32, i = lane % 32;
   const int col = col0 + 2 * (threadIdx.x / 64) + h;
   const bool live = col < d.p && i < kb;
-  T *xr_p = Zr + static_cast<long long>(min(col, d.p - 1)) * p + k0;
-  T *xi_p = Zi + static_cast<long long>(min(col, d.p - 1)) * p + k0;
+  T *xr_p = Zr + static_cast<long long>(min(col, d.p - 1)) * pe + k0;
+  T *xi_p = Zi + static_cast<long long>(min(col, d.p - 1)) * pe + k0;
   T xr = T(0), xi = T(0);
   if (live) {
     xr = xr_p[i];
@@ -1163,14 +1163,14 @@ __global__ __launch_bounds__(256) void k_block_perm(const int *__restrict__ list
 template <typename T>
 __device__ inline void trailing_column(const FrontDev &d, int K1, int col, T *arena, T *&xr,
                                        T *&xi) {
-  const long long m = d.m, p = d.p, mm = m * m;
+  const long long m = d.m, pe = d.pe, mm = m * m;
   const int nF = d.m - K1;
   if (col < nF) {
     xr = arena + d.off + static_cast<long long>(K1 + col) * m;
     xi = xr + mm;
   } else {
-    xr = arena + d.off + 2 * mm + static_cast<long long>(col - nF) * p;
-    xi = xr + p * p;
+    xr = arena + d.off + 2 * mm + static_cast<long long>(col - nF) * pe;
+    xi = xr + pe * d.p;
   }
 }
 
@@ -1334,7 +1334,8 @@ __device__ inline bool gemm_view(int region, const FrontDev &d, int k0, int nb, 
   const bool trail = region == kGemmTrail || region == kGemmTrailHead ||
                      region == kGemmTrailRest || region == kGemmZBlk || region == kGemmPairRows;
   if (trail ? K0 >= d.p : k0 >= d.p) return false;
-  const long long m = d.m, p = d.p, mm = m * m, pp = p * p;
+  // p: the leading dimension of E and Z (counts below use d.p)
+  const long long m = d.m, p = d.pe, mm = m * m, pp = p * d.p;
   T *Fr = arena + d.off, *Fi = Fr + mm;
   T *Er = Fi + mm, *Ei = Er + pp;
   T *Zr = Ei + pp, *Zi = Zr + pp;
@@ -2048,6 +2049,7 @@ template <typename T> struct GemvView {
   const double *addr, *addi;
   double sign;
   bool upper;  // A is upper triangular (skip tiles below the diagonal)
+  int rshift;  // row tiles start at row -rshift (so that they start on a 128-byte line)
 };
 
 struct SolveMaps {
@@ -2059,32 +2061,34 @@ template <typename T>
 __device__ inline bool gemv_view(int op, const FrontDev *fr, int f, const FrontDev &d,
                                  const SolveMaps &mp, const T *arena, double *Wr, double *Wi,
                                  double *Yr, double *Yi, double *Xr, double *Xi, GemvView<T> &v) {
-  const long long mm = static_cast<long long>(d.m) * d.m, pp = static_cast<long long>(d.p) * d.p;
+  const long long mm = static_cast<long long>(d.m) * d.m, pp = static_cast<long long>(d.pe) * d.p;
   const T *Fr = arena + d.off, *Fi = Fr + mm;
   const T *Er = Fi + mm, *Ei = Er + pp;
   const T *Zr = Ei + pp, *Zi = Zr + pp;
   switch (op) {
     case 0:
-      v = {Er, Ei, d.p, d.p, d.p, Wr + d.woff, Wi + d.woff, nullptr, Yr + d.dof0, Yi + d.dof0,
-           nullptr, nullptr, nullptr, 1.0, false};
+      v = {Er, Ei, d.pe, d.p, d.p, Wr + d.woff, Wi + d.woff, nullptr, Yr + d.dof0, Yi + d.dof0,
+           nullptr, nullptr, nullptr, 1.0, false, 0};
       return d.p > 0;
     case 1: {
       if (d.c == 0 || d.p == 0) return false;
       const long long pw = fr[mp.parent[f]].woff;
+      // L21 starts at row p of F's line-aligned columns
+      constexpr int line = 128 / static_cast<int>(sizeof(T));
       v = {Fr + d.p, Fi + d.p, d.m, d.c, d.p, Yr + d.dof0, Yi + d.dof0, nullptr, Wr + pw, Wi + pw,
-           mp.rel + mp.rel_off[f], Wr + d.woff + d.p, Wi + d.woff + d.p, -1.0, false};
+           mp.rel + mp.rel_off[f], Wr + d.woff + d.p, Wi + d.woff + d.p, -1.0, false, d.p % line};
       return true;
     }
     case 2: {
       if (d.c == 0 || d.p == 0) return false;
       const long long a = static_cast<long long>(d.p) * d.m;
       v = {Fr + a, Fi + a, d.m, d.p, d.c, Xr, Xi, mp.upd_pos + mp.upd_off[f], Yr + d.dof0,
-           Yi + d.dof0, nullptr, nullptr, nullptr, -1.0, false};
+           Yi + d.dof0, nullptr, nullptr, nullptr, -1.0, false, 0};
       return true;
     }
     default:
-      v = {Zr, Zi, d.p, d.p, d.p, Yr + d.dof0, Yi + d.dof0, nullptr, Xr + d.dof0, Xi + d.dof0,
-           nullptr, nullptr, nullptr, 1.0, true};
+      v = {Zr, Zi, d.pe, d.p, d.p, Yr + d.dof0, Yi + d.dof0, nullptr, Xr + d.dof0, Xi + d.dof0,
+           nullptr, nullptr, nullptr, 1.0, true, 0};
       return d.p > 0;
   }
 }
@@ -2125,7 +2129,7 @@ __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list
       return;
   const GemvView<T> &V = v[0];  // A, its shape and the maps are shared
   const int lane = threadIdx.x % GVR, wave = threadIdx.x / GVR;
-  const int r0 = blockIdx.x * GVR;
+  const int r0 = blockIdx.x * GVR - V.rshift;
   const int row = r0 + lane;
   const int cw = blockIdx.z * GVW * GVC;  // first column of this workgroup
   if (r0 >= V.M || cw >= V.N) return;
@@ -2134,8 +2138,9 @@ __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list
   const int c1 = min(V.N, c0 + GVC);
   __shared__ double sxr[NR][GVW * GVC], sxi[NR][GVW * GVC];
   __shared__ double pr[NR][GVW][GVR], pi[NR][GVW][GVR];
-  const bool active = row < V.M && !(V.upper && c1 <= r0);
-  const T *ar = V.Ar + (row < V.M ? row : 0), *ai = V.Ai + (row < V.M ? row : 0);
+  const bool inside = row >= 0 && row < V.M;
+  const bool active = inside && !(V.upper && c1 <= r0);
+  const T *ar = V.Ar + (inside ? row : 0), *ai = V.Ai + (inside ? row : 0);
   // KS columns per step: every lane keeps 2 * KS factor loads in flight,
   // 256 B with FP64 entries (KS = 16) and, with KS = 32, also with FP32
   // entries, so both precisions hold the same bytes in flight per wave.
@@ -2231,7 +2236,7 @@ __global__ __launch_bounds__(GVR * GVW) void k_gemv(const int *__restrict__ list
     pi[r][wave][lane] = acci[r];
   }
   __syncthreads();
-  if (wave == 0 && row < V.M) {
+  if (wave == 0 && inside) {
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       double sr = 0.0, si = 0.0;
@@ -2582,7 +2587,8 @@ static hipError_t solve_impl(const LuPlan &P, const double *const *b, double *co
   for (size_t lev = 0; lev < P.levels.size(); ++lev) {
     const LevelPlan &L = P.levels[lev];
     gemv(L, 0, L.pmax, L.pmax);
-    if (L.cmax > 0) gemv(L, 1, L.cmax, L.pmax);
+    // op 1's row tiles start up to one line before row 0 (GemvView::rshift)
+    if (L.cmax > 0) gemv(L, 1, L.cmax + 128 / static_cast<int>(sizeof(T)), L.pmax);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   for (int lev = static_cast<int>(P.levels.size()) - 1; lev >= 0; --lev) {

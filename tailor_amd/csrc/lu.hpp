@@ -14,7 +14,8 @@ namespace tailor {
 struct FrontDev {
   long long off;   // arena offset (doubles)
   long long woff;  // workspace offset (per-front vector of m entries)
-  int m, p, c;
+  int m, p, c;     // order (p + c rounded up to kFrontAlign), pivots, update rows
+  int pe;          // leading dimension of the E and Z planes (p rounded up to kFrontAlign)
   int piv_off;     // offset into the pivot array
   int dof0;        // first elimination-order dof of the front's pivots
   int dinv_off;    // first panel slot of the front in its level's inverse workspace
@@ -85,6 +86,14 @@ struct LuInputs {
 enum { kFpSigRe = 0, kFpSigIm = 1, kFpTiny = 2, kFpPerturb = 3, kFpCount = 4 };
 
 constexpr int kLuNb = 32;        // panel width
+// Front orders are rounded up to, and front offsets aligned to, multiples of
+// kFrontAlign elements, so that every column of F starts on a 128-byte line
+// (FP32 fronts; 256 bytes with FP64).  The extra rows and columns belong to
+// the update part (rows and columns [p + c, m)): they start as zeros, no
+// pivot, extend-add, scatter or solve reaches them, and the updates keep
+// them zero.  The p x p planes E and Z are stored with leading dimension pe
+// (p rounded up likewise); their rows [p, pe) stay zero and unused.
+constexpr int kFrontAlign = 32;
 constexpr int kLuNbOuter = 128;  // outer block width (K of the trailing update)
 constexpr int kLuInvSlot = 4 * kLuNb * kLuNb;      // inverted pivot blocks of one panel
 constexpr int kLuPermStride = 1 + 4 * kLuNbOuter;  // one front's block row permutation

@@ -37,16 +37,18 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
   int poff = 0;
   for (int f = 0; f < S.nsn; ++f) {
     FrontDev &d = fr[f];
-    d.m = S.m[f];
+    d.m = (S.m[f] + kFrontAlign - 1) / kFrontAlign * kFrontAlign;
     d.p = S.p[f];
     d.c = S.c[f];
+    d.pe = (d.p + kFrontAlign - 1) / kFrontAlign * kFrontAlign;
+    off = (off + kFrontAlign - 1) / kFrontAlign * kFrontAlign;
     d.off = off;
     d.woff = woff;
     d.piv_off = poff;
     d.dof0 = 5 * S.sn_start[f];
     d.dinv_off = 0;
     d.inst = 0;
-    off += 2LL * d.m * d.m + 4LL * d.p * d.p;
+    off += 2LL * d.m * d.m + 4LL * d.pe * d.p;
     woff += d.m;
     poff += d.p;
   }
@@ -64,7 +66,7 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
       level_list.push_back(f);
       L.pmax = std::max(L.pmax, S.p[f]);
       L.cmax = std::max(L.cmax, S.c[f]);
-      L.mmax = std::max(L.mmax, S.m[f]);
+      L.mmax = std::max(L.mmax, fr[f].m);
     }
     for (int slot = 0; slot < 2; ++slot) {
       L.child_slot_off[slot] = static_cast<int>(child_slots.size());
