@@ -1,12 +1,12 @@
 """A short, fixed workload for hardware-counter profiles.
 
-    python -m tailor_amd.profile_workload [--ny 200 --nz 100 --solves 4 --gemm-config 3]
+    python -m tailor_amd.profile_workload [--ny 200 --nz 100 --solves 4 --gemm-config 5]
 
 Runs on the synthetic case: one assembly of A_bc and B_bc, one FP64
 factorization of A_bc - sigma B_bc (sigma = lambda_s for omega_s = 0.02) with
-the frontal GEMM tiles forced to one configuration (default 3: 64x64 tiles,
-K chunk 4, XCD-grouped order, the autotuner's choice on most levels at
-200x100; -1 keeps the untuned default),
+the frontal GEMM tiles forced to one configuration (default 5: the 64x64
+tile fed through the LDS ring, XCD-grouped order, the autotuner's choice on
+most levels at 200x100; -1 keeps the untuned default),
 `--solves` triangular-solve pairs, and one device-to-device copy of exactly
 1 GiB (a kernel whose HBM traffic is known, used to scale the FETCH_SIZE /
 WRITE_SIZE counters of the other kernels).  Prints the algorithmic bytes or
@@ -23,6 +23,13 @@ this workload: per kernel family the mean KiB per dispatch of each counter
 and the HBM bytes per dispatch after scaling both counters by what they
 read for the workload's 1 GiB device copy (written as markdown and as the
 JSON that bench.py reads for the roofline's `traffic`).
+
+    python -m tailor_amd.profile_workload summarize-solves <kernel_trace.csv> --command "..." [--out f.md]
+
+Splits a kernel trace of this workload into its triangular solves (at each
+k_solve_init) and reports the wall time, summed kernel time and idle gaps
+per solve, and one solve's launches in order with their grids and
+durations.
 
 Summarises a `rocprofv3 --kernel-trace --output-format csv` trace (one file
 per traced process, all on the same clock) as a
@@ -97,6 +104,46 @@ def summarize(csv_paths, command: str) -> str:
                   f"Frontal GEMM (k_gemm and k_gemm_lds, all tile shapes): {g_calls} launches, "
                   f"{g_ns / 1e6:.2f} ms, mean {g_ns / max(g_calls, 1) / 1e3:.2f} us per launch.", ""]
         lines += _table(win, wtot, limit=25)
+    return "\n".join(lines) + "\n"
+
+
+def summarize_solves(csv_path: str, command: str) -> str:
+    """Markdown summary of the triangular solves in a kernel trace of this
+    workload (see the module docstring)."""
+    import csv
+    launches = []
+    with open(csv_path, newline="") as f:
+        for r in csv.DictReader(f):
+            grid = tuple(int(r.get(k, 0) or 0) for k in ("Grid_Size_X", "Grid_Size_Y", "Grid_Size_Z"))
+            launches.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]),
+                             _short_name(r["Kernel_Name"]), grid))
+    launches.sort()
+    starts = [i for i, x in enumerate(launches) if x[2] == "k_solve_init"]
+    solves = []
+    for a, b in zip(starts, starts[1:]):
+        seg = launches[a:b]
+        # a solve ends with its last GEMV; later launches (the next workload
+        # phase) are not part of it
+        last = max(i for i, x in enumerate(seg) if x[2].startswith("k_gemv") or i == 0)
+        solves.append(seg[:last + 1])
+    solves = solves[1:]  # the warm-up solve
+    if not solves:
+        return "no solves found\n"
+    wall = sorted(s[-1][1] - s[0][0] for s in solves)
+    busy = sorted(sum(t1 - t0 for t0, t1, _, _ in s) for s in solves)
+    gaps = sorted(sum(max(0, s[i + 1][0] - s[i][1]) for i in range(len(s) - 1)) for s in solves)
+    med = lambda v: v[len(v) // 2] / 1e3
+    one = solves[len(solves) // 2]
+    lines = ["Command (GPU box, repo root):", "", f"    {command}", "",
+             f"* solves analysed: {len(solves)} (the warm-up solve excluded)",
+             f"* wall per solve (first start to last end): median {med(wall):.1f} us, "
+             f"range {wall[0] / 1e3:.1f}-{wall[-1] / 1e3:.1f}",
+             f"* summed kernel durations per solve: median {med(busy):.1f} us",
+             f"* summed idle gaps between consecutive launches: median {med(gaps):.1f} us", "",
+             "One solve (the median one by position), launches in start order:", "",
+             "| # | kernel | grid (work-items x, fronts, column tiles) | us |", "|---|---|---|---|"]
+    for i, (t0, t1, name, g) in enumerate(one):
+        lines.append(f"| {i} | {name} | {g[0]} x {g[1]} x {g[2]} | {(t1 - t0) / 1e3:.1f} |")
     return "\n".join(lines) + "\n"
 
 
@@ -187,6 +234,19 @@ def main(argv=None) -> int:
         with open(b.out_json, "w") as f:
             json.dump(js, f, indent=1)
         return 0
+    if argv and argv[0] == "summarize-solves":
+        sp = argparse.ArgumentParser(prog="profile_workload summarize-solves")
+        sp.add_argument("csv")
+        sp.add_argument("--command", required=True, help="the profiled command, for the record")
+        sp.add_argument("--out")
+        b = sp.parse_args(argv[1:])
+        text = summarize_solves(b.csv, b.command)
+        if b.out:
+            with open(b.out, "w") as f:
+                f.write(text)
+        else:
+            print(text)
+        return 0
     if argv and argv[0] == "summarize":
         sp = argparse.ArgumentParser(prog="profile_workload summarize")
         sp.add_argument("csv", nargs="+")
@@ -204,7 +264,7 @@ def main(argv=None) -> int:
     ap.add_argument("--ny", type=int, default=200)
     ap.add_argument("--nz", type=int, default=100)
     ap.add_argument("--solves", type=int, default=4)
-    ap.add_argument("--gemm-config", type=int, default=3)
+    ap.add_argument("--gemm-config", type=int, default=5)
     a = ap.parse_args(argv)
     import torch
     import tailor_amd as ta
