@@ -2332,7 +2332,11 @@ static hipError_t factor_levels(const LuPlan &P, hipStream_t s, FactorTiming *tm
         event_region.push_back(region);
         if ((r = hipEventRecord(e0, st)) != hipSuccess) return r;
         for (int q = 0; q < L.count; ++q) {
-          const GemmShape g = gemm_shape(region, P.h_front[P.h_level_list[L.list_off + q]], k0, nb, K0, W);
+          // algorithmic flops: the front's own order, not the rows and
+          // columns that round it up to whole 128-byte lines
+          FrontDev dq = P.h_front[P.h_level_list[L.list_off + q]];
+          dq.m = dq.p + dq.c;
+          const GemmShape g = gemm_shape(region, dq, k0, nb, K0, W);
           if (g.M > 0 && g.N > 0 && g.K > 0) {
             const double fl = 8.0 * g.K * (static_cast<double>(g.M) * g.N1 +
                                            static_cast<double>(g.M2) * (g.N - g.N1));
