@@ -308,7 +308,8 @@ def _profile_factor(self, sigma: complex) -> dict:
             "gemm_launches": n.value}
 
 
-GEMM_REGIONS = ("inner", "blkrows", "trail", "zin", "trailhead", "trailrest", "zblk", "pairrows")
+# index 1 is unused (the block rows are finished by a triangular sweep)
+GEMM_REGIONS = ("inner", "unused", "trail", "zin", "trailhead", "trailrest", "zblk", "pairrows")
 
 
 def _profile_gemm(self, sigma: complex, max_levels: int = 64) -> dict:

@@ -212,9 +212,8 @@ __global__ __launch_bounds__(256) void k_extend_add(const int *__restrict__ task
 //       k_gemm INNER   F[k0+kb:m, k0+kb:K1] -= L_panel U_panel
 //     k_block_perm,    the block's row exchanges on the trailing F columns
 //     k_apply_perm     and on E, as one permutation
-//     per panel:
-//       k_left_apply   (mode 1) panel rows of the trailing F columns and E
-//       k_gemm BLKROWS remaining block rows of those columns
+//     k_block_rows     the block's rows of the trailing F columns and E:
+//                      X <- L_blk^-1 X, all panels in one sweep
 //     k_gemm TRAIL     F[K1:m, K1:m] -= L21_blk U12_blk and
 //                      E[K1:p, :]    -= L21_blk E[K0:K1, :]   (K = K1 - K0)
 //   then Z = U11^-1 block by block from the last: per panel k_upper_solve
@@ -756,6 +755,8 @@ template <> struct Mfma<double> {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
   }
   static __device__ inline int ocol(int l, int v) { return 4 * v + l / 16; }
+  // the same as a function of the lane's group of 16 (g = l / 16)
+  static __device__ constexpr int orow(int g, int v) { return 4 * v + g; }
 };
 template <> struct Mfma<float> {
   typedef float acc_t __attribute__((ext_vector_type(4)));
@@ -763,6 +764,7 @@ template <> struct Mfma<float> {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0);
   }
   static __device__ inline int ocol(int l, int v) { return 4 * (l / 16) + v; }
+  static __device__ constexpr int orow(int g, int v) { return 4 * g + v; }
 };
 
 // ------------------------------------------- inverted pivot blocks (MFMA)
@@ -780,87 +782,184 @@ __device__ inline T *inv_slot(T *dinv, const FrontDev &d, int k0) {
   return dinv + static_cast<long long>(d.dinv_off + k0 / kLuNb) * kInvSlot;
 }
 
-// One front per workgroup: wave 0 computes L_D^-1 (lane l: column l by
-// forward substitution), wave 1 U_D^-1 (lane l: column l by back
-// substitution); lanes 32..63 only help load the block.
+// One kLuNb x kLuNb plane pair of a slot (column-major) into LDS as
+// Mr[i][q], Mi[i][q], by NT threads; each thread issues all of its loads
+// before its first LDS store, so the loads are in flight together.
+template <int NT, typename T>
+__device__ inline void stage_slot(const T *slot, T (*Mr)[kLuNb + 1], T (*Mi)[kLuNb + 1]) {
+  constexpr int kPer = kLuNb * kLuNb / NT;
+  T vr[kPer], vi[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    vr[u] = slot[threadIdx.x + NT * u];
+    vi[u] = slot[kLuNb * kLuNb + threadIdx.x + NT * u];
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int e = threadIdx.x + NT * u;
+    Mr[e % kLuNb][e / kLuNb] = vr[u];
+    Mi[e % kLuNb][e / kLuNb] = vi[u];
+  }
+}
+
+// One front per workgroup: wave 0 computes L_D^-1, wave 1 U_D^-1, each by
+// 2 x 2 blocks of 16:
+//   L = [A 0; C B]:  L^-1 = [A^-1 0; -B^-1 (C A^-1)  B^-1]
+//   U = [A C; 0 B]:  U^-1 = [A^-1  -A^-1 (C B^-1); 0 B^-1]
+// Lanes 0..15 invert A and lanes 16..31 B (lane l: column l, by forward or
+// back substitution over 16 rows), then the whole wave forms the
+// off-diagonal block with two 16 x 16 x 16 complex products on MFMA.  The
+// inverses are those of the leading kb x kb block, zero-padded: the zeros
+// of D outside it carry through the products.
 template <typename T>
 __global__ __launch_bounds__(128) void k_diag_inv(const int *__restrict__ list, int nlist,
                                                   const FrontDev *__restrict__ fr, int k0, int nb,
                                                   const T *__restrict__ arena,
                                                   T *__restrict__ dinv) {
+  using acc_t = typename Mfma<T>::acc_t;
+  constexpr int H = kLuNb / 2;
+  static_assert(H == 16, "16 x 16 diagonal blocks");
   const FrontDev d = fr[list[blockIdx.x]];
   if (k0 >= d.p) return;
   const int kb = min(nb, d.p - k0);
   const long long m = d.m;
   const T *Fr = arena + d.off, *Fi = Fr + m * m;
   __shared__ T Dr[kLuNb][kLuNb + 1], Di[kLuNb][kLuNb + 1];  // D[i][q] = F[k0+i, k0+q]
-  for (int e = threadIdx.x; e < kLuNb * kLuNb; e += blockDim.x) {
-    const int i = e % kLuNb, q = e / kLuNb;
-    T vr = T(0), vi = T(0);
-    if (i < kb && q < kb) {
-      const long long idx = static_cast<long long>(k0 + q) * m + k0 + i;
-      vr = Fr[idx];
-      vi = Fi[idx];
+  __shared__ T Xr[2][kLuNb][kLuNb + 1], Xi[2][kLuNb][kLuNb + 1];  // wave w's inverse
+  __shared__ T Pr[2][H][H + 1], Pi[2][H][H + 1];                  // C A^-1 or C B^-1
+  {
+    // all loads issued before the first LDS store (clamped addresses, zeroed
+    // outside the kb x kb block)
+    constexpr int kPer = kLuNb * kLuNb / 128;  // 128 threads
+    T vr[kPer], vi[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = threadIdx.x + 128 * u, i = e % kLuNb, q = e / kLuNb;
+      const long long idx = static_cast<long long>(k0 + min(q, kb - 1)) * m + k0 + min(i, kb - 1);
+      const bool ok = i < kb && q < kb;
+      const T a = Fr[idx], b = Fi[idx];
+      vr[u] = ok ? a : T(0);
+      vi[u] = ok ? b : T(0);
     }
-    Dr[i][q] = vr;
-    Di[i][q] = vi;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = threadIdx.x + 128 * u;
+      Dr[e % kLuNb][e / kLuNb] = vr[u];
+      Di[e % kLuNb][e / kLuNb] = vi[u];
+    }
   }
   __syncthreads();
-  const int l = threadIdx.x % 64, wave = threadIdx.x / 64;
-  if (l >= kLuNb) return;  // no barrier below
+  const int l = threadIdx.x % 64, w = threadIdx.x / 64;
+  if (l < kLuNb) {
+    // column l of the diagonal block's inverse: rows o .. o + 15
+    const int o = l < H ? 0 : H;
+    const bool col_in = l < kb;
+    T xr[H], xi[H];
+    if (w == 0) {
+      // unit lower: x_i = [o + i == l] - sum_{q < i} L[o+i][o+q] x_q
+#pragma unroll
+      for (int i = 0; i < H; ++i) {
+        T ar = o + i == l ? T(1) : T(0), ai = T(0);
+#pragma unroll
+        for (int q = 0; q < i; ++q) {
+          const T lr = Dr[o + i][o + q], li = Di[o + i][o + q];
+          ar -= lr * xr[q] - li * xi[q];
+          ai -= lr * xi[q] + li * xr[q];
+        }
+        const bool in = o + i < kb && col_in;
+        xr[i] = in ? ar : T(0);
+        xi[i] = in ? ai : T(0);
+      }
+    } else {
+      // upper: x_i = ([o + i == l] - sum_{q > i} U[o+i][o+q] x_q) / U[o+i][o+i]
+#pragma unroll
+      for (int i = H - 1; i >= 0; --i) {
+        T ar = o + i == l ? T(1) : T(0), ai = T(0);
+#pragma unroll
+        for (int q = i + 1; q < H; ++q) {
+          const T ur = Dr[o + i][o + q], ui = Di[o + i][o + q];
+          ar -= ur * xr[q] - ui * xi[q];
+          ai -= ur * xi[q] + ui * xr[q];
+        }
+        xr[i] = T(0);
+        xi[i] = T(0);
+        if (o + i < kb && col_in) {
+          const double dr = Dr[o + i][o + i], di = Di[o + i][o + i];
+          const double den = dr * dr + di * di;
+          xr[i] = static_cast<T>((ar * dr + ai * di) / den);
+          xi[i] = static_cast<T>((ai * dr - ar * di) / den);
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+      Xr[w][o + i][l] = xr[i];
+      Xi[w][o + i][l] = xi[i];
+      Xr[w][(o + i + H) % kLuNb][l] = T(0);  // the other off-diagonal rows
+      Xi[w][(o + i + H) % kLuNb][l] = T(0);
+    }
+  }
+  __syncthreads();
+  // MFMA products: lane l supplies M[l%16][4s + g] and N[4s + g][l%16] and
+  // receives rows orow(g, v) of column l%16
+  const int r = l % 16, g = l / 16;
+  // P = C X1:  L: C = D[16:32, 0:16], X1 = A^-1;  U: C = D[0:16, 16:32], X1 = B^-1
+  {
+    const int cr0 = w == 0 ? H : 0, cc0 = w == 0 ? 0 : H, x0 = w == 0 ? 0 : H;
+    acc_t pr = acc_t{0, 0, 0, 0}, pi = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < H / 4; ++s) {
+      const int k = 4 * s + g;
+      const T ar = Dr[cr0 + r][cc0 + k], ai = Di[cr0 + r][cc0 + k];
+      const T br = Xr[w][x0 + k][x0 + r], bi = Xi[w][x0 + k][x0 + r];
+      pr = Mfma<T>::mma(ar, br, pr);
+      pr = Mfma<T>::mma(ai, -bi, pr);
+      pi = Mfma<T>::mma(ar, bi, pi);
+      pi = Mfma<T>::mma(ai, br, pi);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      Pr[w][Mfma<T>::orow(g, v)][r] = pr[v];
+      Pi[w][Mfma<T>::orow(g, v)][r] = pi[v];
+    }
+  }
+  __syncthreads();
+  // off-diagonal block -X2 P:  L: X2 = B^-1, rows 16:32, columns 0:16;
+  //                            U: X2 = A^-1, rows 0:16, columns 16:32
+  {
+    const int x0 = w == 0 ? H : 0, qr0 = w == 0 ? H : 0, qc0 = w == 0 ? 0 : H;
+    acc_t qr = acc_t{0, 0, 0, 0}, qi = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < H / 4; ++s) {
+      const int k = 4 * s + g;
+      const T ar = Xr[w][x0 + r][x0 + k], ai = Xi[w][x0 + r][x0 + k];
+      const T br = Pr[w][k][r], bi = Pi[w][k][r];
+      qr = Mfma<T>::mma(ar, -br, qr);
+      qr = Mfma<T>::mma(ai, bi, qr);
+      qi = Mfma<T>::mma(ar, -bi, qi);
+      qi = Mfma<T>::mma(ai, -br, qi);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      Xr[w][qr0 + Mfma<T>::orow(g, v)][qc0 + r] = qr[v];
+      Xi[w][qr0 + Mfma<T>::orow(g, v)][qc0 + r] = qi[v];
+    }
+  }
+  __syncthreads();
+  // slot planes Lr, Li, Ur, Ui, column-major
   T *slot = inv_slot(dinv, d, k0);
-  T xr[kLuNb], xi[kLuNb];
-  if (wave == 0) {
-    // L x = e_l with unit diagonal: x_i = [i == l] - sum_{q < i} L[i][q] x_q
 #pragma unroll
-    for (int i = 0; i < kLuNb; ++i) {
-      T ar = i == l ? T(1) : T(0), ai = T(0);
-#pragma unroll
-      for (int q = 0; q < i; ++q) {
-        const T lr = Dr[i][q], li = Di[i][q];
-        ar -= lr * xr[q] - li * xi[q];
-        ai -= lr * xi[q] + li * xr[q];
-      }
-      const bool in = i < kb && l < kb;
-      xr[i] = in ? ar : T(0);
-      xi[i] = in ? ai : T(0);
-    }
-#pragma unroll
-    for (int i = 0; i < kLuNb; ++i) {
-      slot[l * kLuNb + i] = xr[i];
-      slot[kLuNb * kLuNb + l * kLuNb + i] = xi[i];
-    }
-  } else {
-    // U x = e_l: x_i = ([i == l] - sum_{q > i} U[i][q] x_q) / U[i][i]
-#pragma unroll
-    for (int i = kLuNb - 1; i >= 0; --i) {
-      T ar = i == l ? T(1) : T(0), ai = T(0);
-#pragma unroll
-      for (int q = i + 1; q < kLuNb; ++q) {
-        const T ur = Dr[i][q], ui = Di[i][q];
-        ar -= ur * xr[q] - ui * xi[q];
-        ai -= ur * xi[q] + ui * xr[q];
-      }
-      xr[i] = T(0);
-      xi[i] = T(0);
-      if (i < kb && l < kb) {
-        const double dr = Dr[i][i], di = Di[i][i];
-        const double den = dr * dr + di * di;
-        xr[i] = static_cast<T>((ar * dr + ai * di) / den);
-        xi[i] = static_cast<T>((ai * dr - ar * di) / den);
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < kLuNb; ++i) {
-      slot[2 * kLuNb * kLuNb + l * kLuNb + i] = xr[i];
-      slot[3 * kLuNb * kLuNb + l * kLuNb + i] = xi[i];
-    }
+  for (int u = 0; u < kLuNb * kLuNb / 128; ++u) {
+    const int e = threadIdx.x + 128 * u, i = e % kLuNb, c = e / kLuNb;
+    slot[e] = Xr[0][i][c];
+    slot[kLuNb * kLuNb + e] = Xi[0][i][c];
+    slot[2 * kLuNb * kLuNb + e] = Xr[1][i][c];
+    slot[3 * kLuNb * kLuNb + e] = Xi[1][i][c];
   }
 }
 
 // X[k0:k0+kb, cols] <- M X (M kb x kb from the panel's slot):
 //   mode 0: F columns [k0+kb, K1) of the current block, M = L_D^-1;
-//   mode 1: F columns [K1, m) and all p columns of E, M = L_D^-1;
 //   mode 2: Z columns [k0, p), M = U_D^-1.
 // Four waves per workgroup, 16 columns per wave.  The MFMAs compute
 // (M X)^T = X^T M^T: lane l supplies X[4s + l/16][l%16] and
@@ -882,16 +981,11 @@ __global__ __launch_bounds__(256) void k_left_apply(const int *__restrict__ list
   T *Fr = arena + d.off, *Fi = Fr + mm;
   T *Er = Fi + mm, *Ei = Er + pp;
   T *Zr = Ei + pp, *Zi = Zr + pp;
-  const int nF = d.m - K1;
-  const int ncols = mode == 0 ? K1 - (k0 + kb) : mode == 1 ? nF + d.p : d.p - k0;
+  const int ncols = mode == 0 ? K1 - (k0 + kb) : d.p - k0;
   const int col0 = blockIdx.x * 64;
   if (col0 >= ncols) return;
   __shared__ T Mr[kLuNb][kLuNb + 1], Mi[kLuNb][kLuNb + 1];
-  const T *slot = inv_slot(dinv, d, k0) + (mode == 2 ? 2 * kLuNb * kLuNb : 0);
-  for (int e = threadIdx.x; e < kLuNb * kLuNb; e += blockDim.x) {
-    Mr[e % kLuNb][e / kLuNb] = slot[e];
-    Mi[e % kLuNb][e / kLuNb] = slot[kLuNb * kLuNb + e];
-  }
+  stage_slot<256>(inv_slot(dinv, d, k0) + (mode == 2 ? 2 * kLuNb * kLuNb : 0), Mr, Mi);
   __syncthreads();
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   // row k0 of column j of the region, as (re, im) plane offsets
@@ -901,14 +995,6 @@ __global__ __launch_bounds__(256) void k_left_apply(const int *__restrict__ list
       base = static_cast<long long>(k0 + kb + j) * m + k0;
       xr = Fr + base;
       xi = Fi + base;
-    } else if (mode == 1 && j < nF) {
-      base = static_cast<long long>(K1 + j) * m + k0;
-      xr = Fr + base;
-      xi = Fi + base;
-    } else if (mode == 1) {
-      base = static_cast<long long>(j - nF) * pe + k0;
-      xr = Er + base;
-      xi = Ei + base;
     } else {
       base = static_cast<long long>(k0 + j) * pe + k0;
       xr = Zr + base;
@@ -964,6 +1050,188 @@ __global__ __launch_bounds__(256) void k_left_apply(const int *__restrict__ list
   }
 }
 
+// U rows of a whole block [K0, K1) for the trailing F columns [K1, m) and
+// all p columns of E: X <- L_blk^-1 X, where L_blk is the unit lower
+// triangle of the block's rows and columns, in one sweep over its panels:
+//   for panel j:  X_j <- L_D(j)^-1 X_j  (the slot's inverse),
+//                 X_i -= L_ij X_j       for the later panels i,
+// L_ij = F[rows of panel i, columns of panel j].  A workgroup holds the
+// block's rows of 64 columns in registers for the whole sweep (four waves,
+// 16 columns each), so each element is read and written once.
+//
+// The MFMAs compute M X with M (32 x 32, staged in LDS) as the first
+// operand: lane l supplies M[16 ib + l%16][k] and X[k][l%16] for the k of
+// its group g = l/16 in chunk s, and receives rows 16 ib + orow(g, v).  With
+// k = 16 (s/4) + orow(g, s%4) for chunk s, a result lands in the register
+// that held the same row of X, so results feed the next products without
+// moving.  Rows and columns of each M are renumbered in LDS by
+//   sigma(16 ib + orow(g, v)) = 8 g + 4 ib + v,
+// under which lane group g holds the 8 consecutive rows 8 g .. 8 g + 7 of
+// each panel: 64- or 32-byte vector loads and stores.
+template <typename T>
+__device__ inline int block_rows_pos(int r) {  // sigma^-1
+  return 16 * ((r % 8) / 4) + Mfma<T>::orow(r / 8, r % 4);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256, 2) void k_block_rows(const int *__restrict__ list, int nlist,
+                                                    const FrontDev *__restrict__ fr, int K0,
+                                                    int NB, const T *__restrict__ dinv,
+                                                    T *__restrict__ arena) {
+  using acc_t = typename Mfma<T>::acc_t;
+  typedef T v8 __attribute__((ext_vector_type(8)));
+  constexpr int NP = kLuNbOuter / kLuNb;  // panels per block
+  static_assert(NP == 4, "a block is four panels");
+  const FrontDev d = fr[list[blockIdx.y]];
+  if (K0 >= d.p) return;
+  const int K1 = min(K0 + NB, d.p);
+  const int nrows = K1 - K0, np = (nrows + kLuNb - 1) / kLuNb;
+  const long long m = d.m, pe = d.pe, mm = m * m, pp = pe * d.p;
+  T *Fr = arena + d.off, *Fi = Fr + mm;
+  T *Er = Fi + mm, *Ei = Er + pp;
+  const int nF = d.m - K1;
+  const int ncols = nF + d.p;
+  const int col0 = blockIdx.x * 64;
+  if (col0 >= ncols) return;
+  // LDS: stage 0 = L_D(j)^-1, stage t = L_{j+t, j}, renumbered by sigma
+  __shared__ T Sr[NP][kLuNb][kLuNb + 1], Si[NP][kLuNb][kLuNb + 1];
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64, g = lane / 16;
+  const int jl = col0 + wave * 16 + lane % 16;  // the lane's column
+  const bool live = jl < ncols;
+  T *cr, *ci;  // row K0 of the lane's column (the last column for dead lanes)
+  {
+    const int j = min(jl, ncols - 1);
+    const long long base =
+        j < nF ? static_cast<long long>(K1 + j) * m + K0 : static_cast<long long>(j - nF) * pe + K0;
+    cr = (j < nF ? Fr : Er) + base;
+    ci = (j < nF ? Fi : Ei) + base;
+  }
+  // x[j][ib][v]: row 32 j + 8 g + 4 ib + v of the lane's column.  Every
+  // load is issued unconditionally (rows past the block read row K0 and are
+  // zeroed): a run of 8 rows starting below K1 stays inside the column,
+  // whose length (m or pe) is a multiple of 32.
+  acc_t xr[NP][2], xi[NP][2];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int r0 = 32 * j + 8 * g;
+    const bool in = r0 < nrows;
+    const v8 a = *reinterpret_cast<const v8 *>(cr + (in ? r0 : 0));
+    const v8 b = *reinterpret_cast<const v8 *>(ci + (in ? r0 : 0));
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const bool ok = live && r0 + s < nrows;
+      xr[j][s / 4][s % 4] = ok ? a[s] : T(0);
+      xi[j][s / 4][s % 4] = ok ? b[s] : T(0);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    if (j >= np) break;
+    const int k0 = K0 + kLuNb * j;
+    {
+      // all of a thread's loads are issued before its first LDS store; the
+      // blocks L_{j+t, j} past the last panel are loaded from row k0 (inside
+      // the front: k0 + 32 <= m) and zeroed
+      constexpr int kPer = kLuNb * kLuNb / 256;  // 256 threads
+      const T *slot = inv_slot(dinv, d, k0);
+      T vr[NP][kPer], vi[NP][kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = threadIdx.x + 256 * u, r = e % kLuNb, c = e / kLuNb;
+        vr[0][u] = slot[e];
+        vi[0][u] = slot[kLuNb * kLuNb + e];
+#pragma unroll
+        for (int t = 1; t < NP; ++t) {
+          const int row = k0 + kLuNb * t + r;  // in the block's rows if < K1
+          const bool ok = row < K1;
+          const long long idx = static_cast<long long>(k0 + c) * m + (ok ? row : k0);
+          const T a = Fr[idx], b = Fi[idx];
+          vr[t][u] = ok ? a : T(0);
+          vi[t][u] = ok ? b : T(0);
+        }
+      }
+      __syncthreads();  // the previous panel's products have read the stages
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = threadIdx.x + 256 * u;
+        const int pi = block_rows_pos<T>(e % kLuNb), pk = block_rows_pos<T>(e / kLuNb);
+#pragma unroll
+        for (int t = 0; t < NP; ++t) {
+          Sr[t][pi][pk] = vr[t][u];
+          Si[t][pi][pk] = vi[t][u];
+        }
+      }
+    }
+    __syncthreads();
+    // X_j <- L_D(j)^-1 X_j
+    acc_t ur[2], ui[2];
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib) {
+      ur[ib] = acc_t{0, 0, 0, 0};
+      ui[ib] = acc_t{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int k = 16 * (s / 4) + Mfma<T>::orow(g, s % 4);
+      const T br = xr[j][s / 4][s % 4], bi = xi[j][s / 4][s % 4];
+#pragma unroll
+      for (int ib = 0; ib < 2; ++ib) {
+        const T ar = Sr[0][16 * ib + lane % 16][k], ai = Si[0][16 * ib + lane % 16][k];
+        ur[ib] = Mfma<T>::mma(ar, br, ur[ib]);
+        ur[ib] = Mfma<T>::mma(ai, -bi, ur[ib]);
+        ui[ib] = Mfma<T>::mma(ar, bi, ui[ib]);
+        ui[ib] = Mfma<T>::mma(ai, br, ui[ib]);
+      }
+    }
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib) {
+      xr[j][ib] = ur[ib];
+      xi[j][ib] = ui[ib];
+    }
+    // X_i -= L_ij X_j for the later panels
+#pragma unroll
+    for (int t = 1; t < NP; ++t) {
+      const int i = j + t;
+      if (i >= NP || i >= np) break;
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        const int k = 16 * (s / 4) + Mfma<T>::orow(g, s % 4);
+        const T br = xr[j][s / 4][s % 4], bi = xi[j][s / 4][s % 4];
+#pragma unroll
+        for (int ib = 0; ib < 2; ++ib) {
+          const T ar = Sr[t][16 * ib + lane % 16][k], ai = Si[t][16 * ib + lane % 16][k];
+          xr[i][ib] = Mfma<T>::mma(ar, -br, xr[i][ib]);
+          xr[i][ib] = Mfma<T>::mma(ai, bi, xr[i][ib]);
+          xi[i][ib] = Mfma<T>::mma(ar, -bi, xi[i][ib]);
+          xi[i][ib] = Mfma<T>::mma(ai, -br, xi[i][ib]);
+        }
+      }
+    }
+  }
+  if (!live) return;
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int r0 = 32 * j + 8 * g;
+    v8 a, b;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      a[s] = xr[j][s / 4][s % 4];
+      b[s] = xi[j][s / 4][s % 4];
+    }
+    if (r0 + 8 <= nrows) {
+      *reinterpret_cast<v8 *>(cr + r0) = a;
+      *reinterpret_cast<v8 *>(ci + r0) = b;
+    } else if (r0 < nrows) {
+#pragma unroll
+      for (int s = 0; s < 8; ++s)
+        if (r0 + s < nrows) {
+          cr[r0 + s] = a[s];
+          ci[r0 + s] = b[s];
+        }
+    }
+  }
+}
+
 // Rows [p, m) of the panel columns: X <- X U_D^-1.  Four waves per
 // workgroup, 32 rows x 32 columns per wave; the MFMAs compute
 // (X U)^T = U^T X^T: lane l supplies U[4s + l/16][16 jb + l%16] and
@@ -982,11 +1250,7 @@ __global__ __launch_bounds__(256) void k_right_apply(const int *__restrict__ lis
   const long long m = d.m;
   T *Fr = arena + d.off, *Fi = Fr + m * m;
   __shared__ T Ur[kLuNb][kLuNb + 1], Ui[kLuNb][kLuNb + 1];  // U[k][j]
-  const T *slot = inv_slot(dinv, d, k0) + 2 * kLuNb * kLuNb;
-  for (int e = threadIdx.x; e < kLuNb * kLuNb; e += blockDim.x) {
-    Ur[e % kLuNb][e / kLuNb] = slot[e];
-    Ui[e % kLuNb][e / kLuNb] = slot[kLuNb * kLuNb + e];
-  }
+  stage_slot<256>(inv_slot(dinv, d, k0) + 2 * kLuNb * kLuNb, Ur, Ui);
   __syncthreads();
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   const int rw = row0 + wave * 32;
@@ -1067,11 +1331,26 @@ __global__ __launch_bounds__(256) void k_upper_solve(const int *__restrict__ lis
   if (col0 >= d.p) return;
   __shared__ T Ur[kLuNb][kLuNb + 1], Ui[kLuNb][kLuNb + 1];  // U[i][q] = F[k0+i, k0+q]
   __shared__ T Dr[kLuNb], Di[kLuNb];                         // 1 / U[i][i]
-  for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
-    const int i = e % kb, q = e / kb;
-    const long long idx = static_cast<long long>(k0 + q) * m + k0 + i;
-    Ur[i][q] = Fr[idx];
-    Ui[i][q] = Fi[idx];
+  {
+    // all loads issued before the first LDS store (clamped addresses, zeroed
+    // outside the kb x kb block)
+    constexpr int kPer = kLuNb * kLuNb / 256;  // 256 threads
+    T vr[kPer], vi[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = threadIdx.x + 256 * u, i = e % kLuNb, q = e / kLuNb;
+      const long long idx = static_cast<long long>(k0 + min(q, kb - 1)) * m + k0 + min(i, kb - 1);
+      const bool ok = i < kb && q < kb;
+      const T a = Fr[idx], b = Fi[idx];
+      vr[u] = ok ? a : T(0);
+      vi[u] = ok ? b : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      Ur[e % kLuNb][e / kLuNb] = vr[u];
+      Ui[e % kLuNb][e / kLuNb] = vi[u];
+    }
   }
   const T *slot = inv_slot(dinv, d, k0);
   if (threadIdx.x < kb) {
@@ -1266,8 +1545,10 @@ __global__ __launch_bounds__(256) void k_block_swaps(const int *__restrict__ lis
 // of E (K = 128), and the pair's TRAILHEAD / TRAILREST / TRAIL then update
 // everything below B with K = 256 (the two blocks' L columns and U rows are
 // contiguous).
+// (Region 1 is unused: the block rows of the trailing columns are finished
+// by k_block_rows, which is not a GEMM.)
 enum GemmRegion {
-  kGemmInner = 0, kGemmBlkRows = 1, kGemmTrail = 2, kGemmZ = 3, kGemmTrailHead = 4,
+  kGemmInner = 0, kGemmTrail = 2, kGemmZ = 3, kGemmTrailHead = 4,
   kGemmTrailRest = 5, kGemmZBlk = 6, kGemmPairRows = 7
 };
 
@@ -1302,9 +1583,6 @@ __host__ __device__ inline GemmShape gemm_shape(int region, const FrontDev &d, i
   switch (region) {
     case kGemmInner:
       g = {d.m - k0 - kb, d.m - k0 - kb, K1 - k0 - kb, K1 - k0 - kb, kb};
-      break;
-    case kGemmBlkRows:
-      g = {K1 - k0 - kb, K1 - k0 - kb, d.m - K1 + d.p, d.m - K1, kb};
       break;
     case kGemmTrail:
       g = {d.m - K1, d.p - K1, d.m - K1 + d.p, d.m - K1, K1 - K0};
@@ -1371,13 +1649,6 @@ __device__ inline bool gemm_view(int region, const FrontDev &d, int k0, int nb, 
       v = {Fr + a, Fi + a, m, Fr + b, Fi + b, nullptr, nullptr, m, 0,
            Fr + c, Fi + c, nullptr, nullptr, m, 0,
            d.m - k0 - kb, d.m - k0 - kb, K1 - k0 - kb, K1 - k0 - kb, kb};
-      break;
-    }
-    case kGemmBlkRows: {
-      const long long a = k0 * m + k0 + kb, b = K1 * m + k0, c = K1 * m + k0 + kb;
-      v = {Fr + a, Fi + a, m, Fr + b, Fi + b, Er + k0, Ei + k0, m, p,
-           Fr + c, Fi + c, Er + k0 + kb, Ei + k0 + kb, m, p,
-           K1 - k0 - kb, K1 - k0 - kb, d.m - K1 + d.p, d.m - K1, kb};
       break;
     }
     case kGemmTrail: {
@@ -2395,16 +2666,8 @@ static hipError_t factor_levels(const LuPlan &P, hipStream_t s, FactorTiming *tm
     };
     // U rows of block [K0, K0 + NB) for the trailing F columns and E
     auto u_rows = [&](int K0) -> hipError_t {
-      const int Kend = std::min(K0 + NB, L.pmax);
-      for (int k0 = K0; k0 < Kend; k0 += nb) {
-        hipLaunchKernelGGL(k_left_apply<T>, dim3(cdivu(L.mmax + L.pmax, 64), L.count),
-                           dim3(256), 0, s, list, L.count, P.d_front, k0, nb, K0, NB, 1, dinv,
-                           arena);
-        if (k0 + nb < Kend) {
-          hipError_t r = gemm(NB - nb, L.mmax + L.pmax, kGemmBlkRows, k0, K0, NB, s);
-          if (r != hipSuccess) return r;
-        }
-      }
+      hipLaunchKernelGGL(k_block_rows<T>, dim3(cdivu(L.mmax + L.pmax, 64), L.count), dim3(256),
+                         0, s, list, L.count, P.d_front, K0, NB, dinv, arena);
       return hipGetLastError();
     };
     // Look-ahead: the panels of the next block run on the side stream once
