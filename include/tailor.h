@@ -244,7 +244,7 @@ int tailor_set_block_pairs(tailor_ctx *ctx, int32_t on);
  * A batch joins the fronts of `count` distinct contexts of one device (each
  * with a case loaded; any grid sizes) into one elimination forest whose
  * level l holds level l of every context (levels counted from the leaves).
- * Each launch of a level (extend-add, panels, pivot-block inverses, row
+ * Each launch of a level (front build, panels, pivot-block inverses, row
  * exchanges, frontal GEMMs, Z) covers the fronts of all contexts, so the
  * latency-bound stretches of one instance run beside the others' and a
  * batch takes as many launches as its deepest context alone.  The factors

@@ -3,7 +3,7 @@
 // A batch joins the fronts of `count` contexts of one device (any grid
 // sizes) into one plan whose level l holds level l of every context (levels
 // count from the leaves, so a parent still follows its children).  Every
-// launch of the factorization (extend-add, panels, pivot-block inverses,
+// launch of the factorization (front build, panels, pivot-block inverses,
 // row exchanges, frontal GEMMs, Z) then covers the fronts of all contexts at
 // that level: the latency-bound stretches of one instance (top-level panel
 // chains on one workgroup per front, small levels) run side by side with

@@ -84,36 +84,46 @@ __global__ void k_fill(double *__restrict__ p, long long n, double v) {
     p[k] = v;
 }
 
-// E = I and Z = I for every front (the planes were zero-filled before).
+// E = I and Z = I for every front: the four pe x p planes (Er, Ei, Zr, Zi)
+// written whole, two elements per store (pe is even, so a pair never
+// straddles a column).
 template <typename T>
 __global__ void k_init_identity(const FrontDev *__restrict__ fr, int nf, T *__restrict__ arena) {
+  typedef T t2 __attribute__((ext_vector_type(2)));
   const int f = blockIdx.y;
   if (f >= nf) return;
   const FrontDev d = fr[f];
-  T *E = arena + d.off + 2LL * d.m * d.m;
-  T *Z = E + 2LL * d.pe * d.p;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < d.p; k += gridDim.x * blockDim.x) {
-    E[static_cast<long long>(k) * d.pe + k] = T(1);
-    Z[static_cast<long long>(k) * d.pe + k] = T(1);
+  t2 *E = reinterpret_cast<t2 *>(arena + d.off + 2LL * d.m * d.m);
+  const long long plane = static_cast<long long>(d.pe) * d.p / 2;  // pairs per plane
+  for (long long k = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; k < 4 * plane;
+       k += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const int q = static_cast<int>(k / plane);                 // Er, Ei, Zr, Zi
+    const long long e = 2 * (k - q * plane);                   // element in the plane
+    const int col = static_cast<int>(e / d.pe), row = static_cast<int>(e - static_cast<long long>(col) * d.pe);
+    const bool re = q == 0 || q == 2;
+    E[k] = t2{re && row == col ? T(1) : T(0), re && row + 1 == col ? T(1) : T(0)};
   }
 }
 
-// Scatter K = A - sigma B into the fronts.  Thread per (block, entry).
-// amap[3k..3k+2] = front, local row node, local col node of BSR block k;
-// the front is fbase + amap[3k] of fr (fbase: the part's first front in a
-// batch plan, else 0) and fp the part's parameter block.
-// K is formed in FP64 and rounded once to the front precision T.
+// Add K = A - sigma B into the built fronts of one level: the nblk BSR
+// blocks listed in ablk, a thread per (block, entry).  amap[3k..3k+2] =
+// front, local row node, local col node of block k; the front is
+// fbase + amap[3k] of fr (fbase: the part's first front in a batch plan,
+// else 0) and fp the part's parameter block.  Distinct blocks land on
+// distinct entries, so the additions do not race.  K is formed in FP64 and
+// rounded once to the front precision T.
 template <typename T>
-__global__ void k_assemble_fronts(long long nnzb, int per, const double *__restrict__ avals,
+__global__ void k_assemble_fronts(long long nblk, const int *__restrict__ ablk, int per,
+                                  const double *__restrict__ avals,
                                   const int *__restrict__ colidx, const double *__restrict__ bvals,
                                   const double *__restrict__ fp, const int *__restrict__ amap,
                                   const FrontDev *__restrict__ fr, int fbase,
                                   T *__restrict__ arena) {
   const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= nnzb * 25) return;
+  if (t >= nblk * 25) return;
   const double sig_re = fp[kFpSigRe], sig_im = fp[kFpSigIm];
-  const long long k = t / 25;
-  const int e = static_cast<int>(t - 25 * k);
+  const long long k = ablk[t / 25];
+  const int e = static_cast<int>(t - 25 * (t / 25));
   const int f = fbase + amap[3 * k], lu = amap[3 * k + 1], lv = amap[3 * k + 2];
   const FrontDev d = fr[f];
   const int r = e / 5, c = e - 5 * (e / 5);
@@ -125,73 +135,95 @@ __global__ void k_assemble_fronts(long long nnzb, int per, const double *__restr
     vi -= sig_re * bi + sig_im * br;
   }
   const long long idx = static_cast<long long>(5 * lv + c) * d.m + 5 * lu + r;
-  arena[d.off + idx] = static_cast<T>(vr);
-  arena[d.off + static_cast<long long>(d.m) * d.m + idx] = static_cast<T>(vi);
+  T *fr_ = arena + d.off + idx, *fi_ = fr_ + static_cast<long long>(d.m) * d.m;
+  *fr_ += static_cast<T>(vr);
+  *fi_ += static_cast<T>(vi);
 }
 
-// F_parent[rel, rel] += F_child[p:, p:] for the children listed in `tasks`
-// (at most one child per parent in a launch).  Workgroups stride over the
-// child's update columns (the parent column is uniform), threads over its
-// rows, four rows per thread per step with every load before the stores;
-// the child's relative-index map (one entry per node, c / 5 of them) is
-// staged in LDS.
-constexpr int kRelLds = 2048;  // map entries staged in LDS (larger maps read global memory)
+// F of every front of a level, written once: the sum of its children's
+// update blocks F_child[p:, p:] at the rows and columns of the front that
+// the children's inverse maps give (front node -> child update node, -1
+// where the child has none; d_build / d_inv of the plan), zero elsewhere,
+// padding included.  K is added afterwards (k_assemble_fronts).  Workgroup:
+// kBuildRows rows (four per thread) of kBuildCols columns; the thread's row
+// maps are read once, a column's maps are uniform over the workgroup.  Each
+// child value is read once and each front entry written once, where a zero
+// fill and a read-modify-write per child would move F twice more.
+constexpr int kBuildRows = 1024, kBuildCols = 16;
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_extend_add(const int *__restrict__ tasks, int ntasks,
-                                                    const FrontDev *__restrict__ fr,
-                                                    const int *__restrict__ par,
-                                                    const int *__restrict__ rel,
-                                                    const long long *__restrict__ rel_off,
-                                                    T *__restrict__ arena) {
-  const int ti = blockIdx.y;
-  if (ti >= ntasks) return;
-  const int ch = tasks[ti];
-  const FrontDev dc = fr[ch], dp = fr[par[ch]];
-  const int c = dc.c;
-  if (c == 0) return;
-  const int *__restrict__ rl = rel + rel_off[ch];
-  const int nodes = c / 5;
-  __shared__ int srel[kRelLds];
-  for (int q = threadIdx.x; q < min(nodes, kRelLds); q += blockDim.x) srel[q] = rl[q];
-  __syncthreads();
-  auto prow = [&](int i) {
-    const int n = i / 5;
-    return 5 * (n < kRelLds ? srel[n] : rl[n]) + (i - 5 * n);
-  };
-  const long long cm = dc.m, pm = dp.m;
-  const T *Cr0 = arena + dc.off, *Ci0 = Cr0 + cm * cm;
-  T *Pr0 = arena + dp.off, *Pi0 = Pr0 + pm * pm;
-  constexpr int U = 4;
-  for (int j = blockIdx.x; j < c; j += gridDim.x) {
-    const T *__restrict__ cr = Cr0 + (dc.p + j) * cm + dc.p;
-    const T *__restrict__ ci = Ci0 + (dc.p + j) * cm + dc.p;
-    const long long pc = static_cast<long long>(prow(j)) * pm;
-    T *__restrict__ pr = Pr0 + pc;
-    T *__restrict__ pi = Pi0 + pc;
-    for (int i0 = threadIdx.x; i0 < c; i0 += U * blockDim.x) {
-      T ar[U], ai[U], br[U], bi[U];
-      int dst[U];
+__global__ __launch_bounds__(256) void k_build_fronts(const int *__restrict__ list, int nlist,
+                                                      const FrontDev *__restrict__ fr,
+                                                      const int *__restrict__ build,
+                                                      const int *__restrict__ inv, int nrc,
+                                                      T *__restrict__ arena) {
+  constexpr int U = kBuildRows / 256;
+  const int f = list[blockIdx.y];
+  const FrontDev d = fr[f];
+  const int r0 = (blockIdx.x % nrc) * kBuildRows, c0 = (blockIdx.x / nrc) * kBuildCols;
+  if (r0 >= d.m || c0 >= d.m) return;
+  const long long m = d.m;
+  T *Fr = arena + d.off, *Fi = Fr + m * m;
+  const int dofs = d.p + d.c;  // rows and columns with a node; the rest is padding
+  int ch[2];
+  const int *map[2];
+  const T *Cr[2], *Ci[2];
+  long long cm[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    ch[k] = build[kBuildInts * f + k];
+    map[k] = inv + max(build[kBuildInts * f + 2 + k], 0);
+    // the child's F22 (rows and columns [p, p + c)); the front's own F
+    // stands in for an absent child (read at index 0 and discarded)
+    const FrontDev dc = fr[max(ch[k], 0)];
+    cm[k] = ch[k] >= 0 ? dc.m : m;
+    Cr[k] = ch[k] >= 0 ? arena + dc.off + dc.p * (dc.m + 1LL) : Fr;
+    Ci[k] = ch[k] >= 0 ? Cr[k] + static_cast<long long>(dc.m) * dc.m : Fr;
+  }
+  // child rows of the thread's rows (-1: none)
+  int src[2][U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int r = r0 + threadIdx.x + 256 * u;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int n = ch[k] >= 0 && r < dofs ? map[k][r / 5] : -1;
+      src[k][u] = n >= 0 ? 5 * n + r % 5 : -1;
+    }
+  }
+  const int c1 = min(c0 + kBuildCols, d.m);
+  for (int j = c0; j < c1; ++j) {
+    int jc[2];
+    const T *pr[2], *pi[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int n = ch[k] >= 0 && j < dofs ? map[k][j / 5] : -1;
+      jc[k] = n >= 0 ? 5 * n + j % 5 : -1;
+      pr[k] = Cr[k] + max(jc[k], 0) * cm[k];
+      pi[k] = Ci[k] + max(jc[k], 0) * cm[k];
+    }
+    T vr[2][U], vi[2][U];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int i = i0 + u * blockDim.x;
-        dst[u] = i < c ? prow(i) : 0;
-        ar[u] = i < c ? cr[i] : T(0);
-        ai[u] = i < c ? ci[i] : T(0);
+        const int q = jc[k] >= 0 ? max(src[k][u], 0) : 0;
+        vr[k][u] = pr[k][q];
+        vi[k][u] = pi[k][q];
       }
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int i = i0 + u * blockDim.x;
-        br[u] = i < c ? pr[dst[u]] : T(0);
-        bi[u] = i < c ? pi[dst[u]] : T(0);
-      }
+    for (int u = 0; u < U; ++u) {
+      const int r = r0 + threadIdx.x + 256 * u;
+      if (r >= d.m) continue;
+      T ar = T(0), ai = T(0);
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (i0 + u * blockDim.x < c) {
-          pr[dst[u]] = br[u] + ar[u];
-          pi[dst[u]] = bi[u] + ai[u];
+      for (int k = 0; k < 2; ++k)
+        if (jc[k] >= 0 && src[k][u] >= 0) {
+          ar += vr[k][u];
+          ai += vi[k][u];
         }
-      }
+      Fr[j * m + r] = ar;
+      Fi[j * m + r] = ai;
     }
   }
 }
@@ -2541,35 +2573,29 @@ hipError_t lu_fill(hipStream_t s, double *p, long long n, double v) {
   return hipGetLastError();
 }
 
-// The zeroed fronts of one plan `part` and the scatter of its K into them:
-// fr is the front table the factorization runs on (the part's own, or a
-// batch's, where the part's fronts start at fbase), fp the part's parameter
-// block and arena the table's base.
-template <typename T>
-static hipError_t factor_scatter(const LuPlan &part, const LuInputs &in, const FrontDev *fr,
-                                 int fbase, const double *fp, T *arena, hipStream_t s) {
-  hipError_t e;
-  // zero the planes in use (the arena is sized for FP64 fronts)
-  const long long used = (part.arena_doubles * static_cast<long long>(sizeof(T)) + 7) / 8;
-  if ((e = lu_fill(s, part.arena, used, 0.0)) != hipSuccess) return e;
-  const long long threads = in.nnzb * 25;
-  hipLaunchKernelGGL(k_assemble_fronts<T>, dim3(static_cast<unsigned>((threads + 255) / 256)),
-                     dim3(256), 0, s, in.nnzb, in.per, in.avals, in.colidx, in.bvals, fp,
-                     part.d_amap, fr, fbase, arena);
-  return hipGetLastError();
-}
+// The K of one plan `part` for the factorization: fbase is where the part's
+// fronts start in the front table the factorization runs on (0 for the
+// part's own plan, else its place in a batch plan), fp its parameter block.
+struct ScatterJob {
+  const LuPlan *part;
+  LuInputs in;
+  int fbase;
+  const double *fp;
+};
 
-// E = I, Z = I and the pivot counters, then the blocked LU of every front of
-// P level by level (P: a context's plan or a batch plan; the fronts hold K).
+// E = I, Z = I and the pivot counters, then level by level: the level's
+// fronts built from their children, K of every job added, and the blocked
+// LU of the fronts (P: a context's plan or a batch plan).
 template <typename T>
-static hipError_t factor_levels(const LuPlan &P, hipStream_t s, FactorTiming *tm) {
+static hipError_t factor_levels(const LuPlan &P, const std::vector<ScatterJob> &jobs,
+                                hipStream_t s, FactorTiming *tm) {
   hipError_t e;
   std::vector<hipEvent_t> events;  // start/stop pairs around GEMM launches when timing
   std::vector<int> event_level;    // level of each pair
   std::vector<int> event_region;   // update region of each pair
   if (tm) tm->level_gemm_flops.assign(P.levels.size(), 0.0);
   T *arena = reinterpret_cast<T *>(P.arena);
-  hipLaunchKernelGGL(k_init_identity<T>, dim3(8, P.nsn), dim3(256), 0, s, P.d_front, P.nsn, arena);
+  hipLaunchKernelGGL(k_init_identity<T>, dim3(64, P.nsn), dim3(256), 0, s, P.d_front, P.nsn, arena);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, reinterpret_cast<double *>(P.d_counters),
                      static_cast<long long>((P.instances + 1) / 2), 0.0);
@@ -2577,15 +2603,24 @@ static hipError_t factor_levels(const LuPlan &P, hipStream_t s, FactorTiming *tm
   T *dinv = reinterpret_cast<T *>(P.dinv);
   for (size_t lev = 0; lev < P.levels.size(); ++lev) {
     const LevelPlan &L = P.levels[lev];
-    for (int slot = 0; slot < 2; ++slot) {
-      const int nt = L.child_slot_count[slot];
-      if (nt == 0) continue;
-      hipLaunchKernelGGL(k_extend_add<T>, dim3(256, nt), dim3(256), 0, s,
-                         P.d_child_slots + L.child_slot_off[slot], nt, P.d_front, P.d_parent,
-                         P.d_rel, P.d_rel_off, arena);
-      if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
     const int *list = P.d_level_list + L.list_off;
+    {
+      const int nrc = static_cast<int>(cdivu(L.mmax, kBuildRows));
+      hipLaunchKernelGGL(k_build_fronts<T>, dim3(nrc * cdivu(L.mmax, kBuildCols), L.count),
+                         dim3(256), 0, s, list, L.count, P.d_front, P.d_build, P.d_inv, nrc, arena);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      for (const ScatterJob &J : jobs) {
+        if (lev >= J.part->levels.size()) continue;
+        const LevelPlan &Lp = J.part->levels[lev];
+        if (Lp.ablk_count == 0) continue;
+        const long long threads = Lp.ablk_count * 25;
+        hipLaunchKernelGGL(k_assemble_fronts<T>, dim3(static_cast<unsigned>((threads + 255) / 256)),
+                           dim3(256), 0, s, Lp.ablk_count, J.part->d_ablk + Lp.ablk_off, J.in.per,
+                           J.in.avals, J.in.colidx, J.in.bvals, J.fp, J.part->d_amap, P.d_front,
+                           J.fbase, arena);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+      }
+    }
     const int order = lev < P.gemm_order.size() ? P.gemm_order[lev] : kOrderLinear;
     const int shape = lev < P.gemm_shape.size() ? P.gemm_shape[lev] : kShape64x64;
     // one batched GEMM of `region`; W: the K-block width of the trailing
@@ -2757,10 +2792,7 @@ static hipError_t factor_levels(const LuPlan &P, hipStream_t s, FactorTiming *tm
 template <typename T>
 static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, hipStream_t s,
                               FactorTiming *tm) {
-  hipError_t e = factor_scatter<T>(P, in, P.d_front, 0, P.d_fparams,
-                                   reinterpret_cast<T *>(P.arena), s);
-  if (e != hipSuccess) return e;
-  return factor_levels<T>(P, s, tm);
+  return factor_levels<T>(P, {ScatterJob{&P, in, 0, P.d_fparams}}, s, tm);
 }
 
 hipError_t lu_factor(const LuPlan &P, const LuInputs &in, bool fp32, hipStream_t s,
@@ -2800,14 +2832,14 @@ static hipError_t factor_batch_impl(const LuPlan &U, const std::vector<BatchPart
                                     const std::vector<LuInputs> &in,
                                     const std::vector<double *> &partials, bool fp32,
                                     hipStream_t s, FactorTiming *tm) {
-  T *arena = reinterpret_cast<T *>(U.arena);
+  std::vector<ScatterJob> jobs;
   for (size_t b = 0; b < parts.size(); ++b) {
     double *fp = U.d_fparams + b * kFpCount;
     hipError_t e = factor_params_at(fp, in[b], fp32, partials[b], s);
-    if (e == hipSuccess) e = factor_scatter<T>(*parts[b].plan, in[b], U.d_front, fbase[b], fp, arena, s);
     if (e != hipSuccess) return e;
+    jobs.push_back({parts[b].plan, in[b], fbase[b], fp});
   }
-  return factor_levels<T>(U, s, tm);
+  return factor_levels<T>(U, jobs, s, tm);
 }
 
 hipError_t lu_factor_batch(const LuPlan &U, const std::vector<BatchPart> &parts,

@@ -25,8 +25,7 @@ struct FrontDev {
 struct LevelPlan {
   int list_off = 0, count = 0;
   int pmax = 0, cmax = 0, mmax = 0;
-  int child_slot_off[2] = {0, 0};
-  int child_slot_count[2] = {0, 0};
+  long long ablk_off = 0, ablk_count = 0;  // the level's BSR blocks in d_ablk (context plans)
 };
 
 // Everything the numeric phase needs, resident on the device.
@@ -45,7 +44,13 @@ struct LuPlan {
   std::vector<int> h_level_list;
   FrontDev *d_front = nullptr;
   int *d_level_list = nullptr;
-  int *d_child_slots = nullptr;
+  // front build: per front kBuildInts ints (children, -1 where absent, and the
+  // offsets of their maps in d_inv); d_inv maps a front's nodes to the
+  // child's update nodes (-1 where the child has none)
+  int *d_build = nullptr;
+  int *d_inv = nullptr;
+  std::vector<int> h_build, h_inv;  // host copies (joined into batch plans)
+  int *d_ablk = nullptr;            // BSR blocks ordered by the level of their front
   int *d_parent = nullptr;
   int *d_rel = nullptr;
   long long *d_rel_off = nullptr;
@@ -89,11 +94,12 @@ constexpr int kLuNb = 32;        // panel width
 // Front orders are rounded up to, and front offsets aligned to, multiples of
 // kFrontAlign elements, so that every column of F starts on a 128-byte line
 // (FP32 fronts; 256 bytes with FP64).  The extra rows and columns belong to
-// the update part (rows and columns [p + c, m)): they start as zeros, no
-// pivot, extend-add, scatter or solve reaches them, and the updates keep
-// them zero.  The p x p planes E and Z are stored with leading dimension pe
+// the update part (rows and columns [p + c, m)): the front build writes them
+// as zeros, no pivot, child update, scatter or solve reaches them, and the
+// updates keep them zero.  The p x p planes E and Z are stored with leading dimension pe
 // (p rounded up likewise); their rows [p, pe) stay zero and unused.
 constexpr int kFrontAlign = 32;
+constexpr int kBuildInts = 4;  // child 0, child 1, their map offsets
 constexpr int kLuNbOuter = 128;  // outer block width (K of the trailing update)
 constexpr int kLuInvSlot = 4 * kLuNb * kLuNb;      // inverted pivot blocks of one panel
 constexpr int kLuPermStride = 1 + 4 * kLuNbOuter;  // one front's block row permutation

@@ -55,8 +55,8 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
   P.arena_doubles = off;
   P.w_len = woff;
 
-  // level lists and child slots
-  std::vector<int> level_list, child_slots;
+  // level lists
+  std::vector<int> level_list;
   P.levels.assign(S.levels.size(), {});
   for (size_t l = 0; l < S.levels.size(); ++l) {
     LevelPlan &L = P.levels[l];
@@ -67,12 +67,6 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
       L.pmax = std::max(L.pmax, S.p[f]);
       L.cmax = std::max(L.cmax, S.c[f]);
       L.mmax = std::max(L.mmax, fr[f].m);
-    }
-    for (int slot = 0; slot < 2; ++slot) {
-      L.child_slot_off[slot] = static_cast<int>(child_slots.size());
-      for (int f : S.levels[l])
-        if (static_cast<int>(S.children[f].size()) > slot) child_slots.push_back(S.children[f][slot]);
-      L.child_slot_count[slot] = static_cast<int>(child_slots.size()) - L.child_slot_off[slot];
     }
     // inverse workspace: the level's fronts take consecutive panel slots
     long long slots = 0;
@@ -85,8 +79,27 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
   }
   std::vector<int> parent = S.parent, rel, child_ptr(S.nsn + 1, 0), child_list, upd_pos;
   std::vector<long long> rel_off(S.nsn, 0), upd_off(S.nsn, 0);
+  // front build: for each front and each of its (at most two) children, the
+  // child's update node at each node of the front (the inverse of rel)
+  P.h_build.assign(static_cast<size_t>(kBuildInts) * S.nsn, -1);
+  P.h_inv.clear();
   for (int f = 0; f < S.nsn; ++f) {
     if (S.children[f].size() > 2) return hipErrorInvalidValue;
+    const int nodes = S.m[f] / 5;
+    for (size_t k = 0; k < S.children[f].size(); ++k) {
+      const int ch = S.children[f][k];
+      P.h_build[kBuildInts * f + k] = ch;
+      P.h_build[kBuildInts * f + 2 + k] = static_cast<int>(P.h_inv.size());
+      const size_t base = P.h_inv.size();
+      P.h_inv.resize(base + nodes, -1);
+      for (size_t n = 0; n < S.rel[ch].size(); ++n) {
+        const int q = S.rel[ch][n];
+        if (q < 0 || q >= nodes) return hipErrorInvalidValue;
+        P.h_inv[base + q] = static_cast<int>(n);
+      }
+    }
+  }
+  for (int f = 0; f < S.nsn; ++f) {
     rel_off[f] = static_cast<long long>(rel.size());
     rel.insert(rel.end(), S.rel[f].begin(), S.rel[f].end());
     child_ptr[f + 1] = child_ptr[f] + static_cast<int>(S.children[f].size());
@@ -110,6 +123,25 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
     amap[3 * k + 1] = lu;
     amap[3 * k + 2] = lv;
   }
+  // the blocks grouped by the level of their front (scattered level by
+  // level, after the level's fronts are built)
+  std::vector<int> ablk;
+  ablk.reserve(nnzb);
+  {
+    std::vector<int> level_of(S.nsn, 0);
+    for (size_t l = 0; l < S.levels.size(); ++l)
+      for (int f : S.levels[l]) level_of[f] = static_cast<int>(l);
+    std::vector<long long> count(S.levels.size() + 1, 0);
+    for (long long k = 0; k < nnzb; ++k) ++count[level_of[amap[3 * k]] + 1];
+    for (size_t l = 0; l < S.levels.size(); ++l) count[l + 1] += count[l];
+    for (size_t l = 0; l < S.levels.size(); ++l) {
+      P.levels[l].ablk_off = count[l];
+      P.levels[l].ablk_count = count[l + 1] - count[l];
+    }
+    ablk.assign(nnzb, 0);
+    std::vector<long long> next(count.begin(), count.end() - 1);
+    for (long long k = 0; k < nnzb; ++k) ablk[next[level_of[amap[3 * k]]]++] = static_cast<int>(k);
+  }
   std::vector<int> perm(P.n);
   for (int q = 0; q < S.nodes; ++q)
     for (int dof = 0; dof < 5; ++dof) perm[5 * q + dof] = 5 * S.order[q] + dof;
@@ -119,7 +151,9 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
   P.h_front = fr;
   P.h_level_list = level_list;
   if ((e = upload(&P.d_level_list, level_list, s)) != hipSuccess) return e;
-  if ((e = upload(&P.d_child_slots, child_slots, s)) != hipSuccess) return e;
+  if ((e = upload(&P.d_build, P.h_build, s)) != hipSuccess) return e;
+  if ((e = upload(&P.d_inv, P.h_inv, s)) != hipSuccess) return e;
+  if ((e = upload(&P.d_ablk, ablk, s)) != hipSuccess) return e;
   if ((e = upload(&P.d_parent, parent, s)) != hipSuccess) return e;
   if ((e = upload(&P.d_rel, rel, s)) != hipSuccess) return e;
   if ((e = upload(&P.d_rel_off, rel_off, s)) != hipSuccess) return e;
@@ -180,6 +214,8 @@ hipError_t lu_plan_batch(const std::vector<BatchPart> &parts, bool fp32, LuPlan 
   std::vector<int> parent, rel;
   std::vector<long long> rel_off;
   fbase.assign(parts.size(), 0);
+  U.h_build.clear();
+  U.h_inv.clear();
   int poff = 0;
   for (size_t b = 0; b < parts.size(); ++b) {
     const LuPlan &P = *parts[b].plan;
@@ -199,10 +235,19 @@ hipError_t lu_plan_batch(const std::vector<BatchPart> &parts, bool fp32, LuPlan 
       rel_off.push_back(static_cast<long long>(rel.size()));
       rel.insert(rel.end(), S.rel[f].begin(), S.rel[f].end());
     }
+    // the part's build records with batch front indices and map offsets
+    if (P.h_build.size() != static_cast<size_t>(kBuildInts) * P.nsn) return hipErrorInvalidValue;
+    const int ib = static_cast<int>(U.h_inv.size());
+    for (int f = 0; f < P.nsn; ++f) {
+      const int *rec = &P.h_build[kBuildInts * f];
+      for (int k = 0; k < 2; ++k) U.h_build.push_back(rec[k] < 0 ? -1 : rec[k] + fb);
+      for (int k = 0; k < 2; ++k) U.h_build.push_back(rec[k] < 0 ? -1 : rec[2 + k] + ib);
+    }
+    U.h_inv.insert(U.h_inv.end(), P.h_inv.begin(), P.h_inv.end());
     for (int f = 0; f < P.nsn; ++f) poff += P.h_front[f].p;
   }
   // level l of the batch: level l of every part, part by part
-  std::vector<int> level_list, child_slots;
+  std::vector<int> level_list;
   U.levels.assign(nlev, {});
   for (size_t l = 0; l < nlev; ++l) {
     LevelPlan &L = U.levels[l];
@@ -224,18 +269,6 @@ hipError_t lu_plan_batch(const std::vector<BatchPart> &parts, bool fp32, LuPlan 
       d.dinv_off = static_cast<int>(slots);
       slots += (d.p + kLuNb - 1) / kLuNb;
     }
-    for (int slot = 0; slot < 2; ++slot) {
-      L.child_slot_off[slot] = static_cast<int>(child_slots.size());
-      for (size_t b = 0; b < parts.size(); ++b) {
-        const LuPlan &P = *parts[b].plan;
-        const Symbolic &S = *parts[b].sym;
-        if (l >= P.levels.size()) continue;
-        for (int f : S.levels[l])
-          if (static_cast<int>(S.children[f].size()) > slot)
-            child_slots.push_back(S.children[f][slot] + fbase[b]);
-      }
-      L.child_slot_count[slot] = static_cast<int>(child_slots.size()) - L.child_slot_off[slot];
-    }
     U.dinv_slots = std::max(U.dinv_slots, slots);
     U.count_max = std::max(U.count_max, L.count);
   }
@@ -254,7 +287,8 @@ hipError_t lu_plan_batch(const std::vector<BatchPart> &parts, bool fp32, LuPlan 
   hipError_t e;
   if ((e = upload(&U.d_front, fr, s)) != hipSuccess) return e;
   if ((e = upload(&U.d_level_list, level_list, s)) != hipSuccess) return e;
-  if ((e = upload(&U.d_child_slots, child_slots, s)) != hipSuccess) return e;
+  if ((e = upload(&U.d_build, U.h_build, s)) != hipSuccess) return e;
+  if ((e = upload(&U.d_inv, U.h_inv, s)) != hipSuccess) return e;
   if ((e = upload(&U.d_parent, parent, s)) != hipSuccess) return e;
   if ((e = upload(&U.d_rel, rel, s)) != hipSuccess) return e;
   if ((e = upload(&U.d_rel_off, rel_off, s)) != hipSuccess) return e;
@@ -288,7 +322,9 @@ void lu_plan_destroy(LuPlan &P) {
   P.ev_head = P.ev_panel = nullptr;
   release(P.d_front);
   release(P.d_level_list);
-  release(P.d_child_slots);
+  release(P.d_build);
+  release(P.d_inv);
+  release(P.d_ablk);
   release(P.d_parent);
   release(P.d_rel);
   release(P.d_rel_off);
