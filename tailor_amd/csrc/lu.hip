@@ -145,11 +145,12 @@ __global__ void k_assemble_fronts(long long nblk, const int *__restrict__ ablk, 
 // the children's inverse maps give (front node -> child update node, -1
 // where the child has none; d_build / d_inv of the plan), zero elsewhere,
 // padding included.  K is added afterwards (k_assemble_fronts).  Workgroup:
-// kBuildRows rows (four per thread) of kBuildCols columns; the thread's row
-// maps are read once, a column's maps are uniform over the workgroup.  Each
-// child value is read once and each front entry written once, where a zero
-// fill and a read-modify-write per child would move F twice more.
-constexpr int kBuildRows = 1024, kBuildCols = 16;
+// kBuildRows rows of kBuildCols columns, each wave a quarter of the columns
+// with four rows per lane (64 apart); the lane's row maps are read once, a
+// column's maps are uniform over the wave.  Each child value is read once
+// and each front entry written once, where a zero fill and a
+// read-modify-write per child would move F twice more.
+constexpr int kBuildRows = 256, kBuildCols = 16;
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_build_fronts(const int *__restrict__ list, int nlist,
@@ -157,11 +158,12 @@ __global__ __launch_bounds__(256) void k_build_fronts(const int *__restrict__ li
                                                       const int *__restrict__ build,
                                                       const int *__restrict__ inv, int nrc,
                                                       T *__restrict__ arena) {
-  constexpr int U = kBuildRows / 256;
+  constexpr int U = kBuildRows / 64;
   const int f = list[blockIdx.y];
   const FrontDev d = fr[f];
   const int r0 = (blockIdx.x % nrc) * kBuildRows, c0 = (blockIdx.x / nrc) * kBuildCols;
   if (r0 >= d.m || c0 >= d.m) return;
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   const long long m = d.m;
   T *Fr = arena + d.off, *Fi = Fr + m * m;
   const int dofs = d.p + d.c;  // rows and columns with a node; the rest is padding
@@ -180,11 +182,11 @@ __global__ __launch_bounds__(256) void k_build_fronts(const int *__restrict__ li
     Cr[k] = ch[k] >= 0 ? arena + dc.off + dc.p * (dc.m + 1LL) : Fr;
     Ci[k] = ch[k] >= 0 ? Cr[k] + static_cast<long long>(dc.m) * dc.m : Fr;
   }
-  // child rows of the thread's rows (-1: none)
+  // child rows of the lane's rows (-1: none)
   int src[2][U];
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const int r = r0 + threadIdx.x + 256 * u;
+    const int r = r0 + lane + 64 * u;
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       const int n = ch[k] >= 0 && r < dofs ? map[k][r / 5] : -1;
@@ -192,7 +194,7 @@ __global__ __launch_bounds__(256) void k_build_fronts(const int *__restrict__ li
     }
   }
   const int c1 = min(c0 + kBuildCols, d.m);
-  for (int j = c0; j < c1; ++j) {
+  for (int j = c0 + wave; j < c1; j += 4) {
     int jc[2];
     const T *pr[2], *pi[2];
 #pragma unroll
@@ -213,7 +215,7 @@ __global__ __launch_bounds__(256) void k_build_fronts(const int *__restrict__ li
       }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int r = r0 + threadIdx.x + 256 * u;
+      const int r = r0 + lane + 64 * u;
       if (r >= d.m) continue;
       T ar = T(0), ai = T(0);
 #pragma unroll
