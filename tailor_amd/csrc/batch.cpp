@@ -161,8 +161,9 @@ int set_shifts(tailor_batch *bt, const tailor::cplx *sig) {
 // factors at its shift.
 int mark_factored(tailor_batch *bt, const tailor::cplx *sig, bool fp32, double ms) {
   std::vector<int> counters(std::max<size_t>(8, bt->ctxs.size()), 0);
-  BATCH_OK(hipMemcpy(counters.data(), bt->plan.d_counters, bt->ctxs.size() * sizeof(int),
-                     hipMemcpyDeviceToHost));
+  BATCH_OK(hipMemcpyAsync(counters.data(), bt->plan.d_counters, bt->ctxs.size() * sizeof(int),
+                          hipMemcpyDeviceToHost, bt->stream));
+  BATCH_OK(hipStreamSynchronize(bt->stream));
   for (size_t k = 0; k < bt->ctxs.size(); ++k)
     tailor::solver_mark_factored(bt->ctxs[k], sig[k], fp32, counters[k], ms);
   return TAILOR_OK;

@@ -432,7 +432,8 @@ hipError_t enqueue_assembly(tailor_ctx *ctx, bool constrained, hipStream_t s) {
 
 int finish_assembly(tailor_ctx *ctx, bool constrained, DeviceDiag *host_diag) {
   DeviceDiag g{};
-  HIP_TRY(ctx, hipMemcpy(&g, ctx->d_diag, sizeof g, hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpyAsync(&g, ctx->d_diag, sizeof g, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   if (host_diag) *host_diag = g;
   ctx->vals_kind = constrained ? TAILOR_OP_A : TAILOR_OP_L;
   ctx->mvals_kind = constrained ? TAILOR_OP_B : TAILOR_OP_M;
@@ -586,7 +587,8 @@ int tailor_time_assembly(tailor_ctx *ctx, int reps, double *write_kernel_ms,
   ctx->vals_kind = TAILOR_OP_A;
   ctx->mvals_kind = TAILOR_OP_B;
   tailor::DeviceDiag g{};
-  HIP_TRY(ctx, hipMemcpy(&g, ctx->d_diag, sizeof g, hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpyAsync(&g, ctx->d_diag, sizeof g, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return check_assembly_flags(ctx, g);
 }
 

@@ -103,12 +103,16 @@ int solver_prepare(tailor_ctx *ctx) {
   std::vector<int> order(S.sym.order), pos(S.sym.pos);
   HIP_OK(hipMalloc(reinterpret_cast<void **>(&S.d_order), order.size() * sizeof(int)));
   HIP_OK(hipMalloc(reinterpret_cast<void **>(&S.d_pos), pos.size() * sizeof(int)));
-  HIP_OK(hipMemcpy(S.d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(S.d_pos, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpyAsync(S.d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice,
+                       ctx->stream));
+  HIP_OK(hipMemcpyAsync(S.d_pos, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice,
+                       ctx->stream));
   std::vector<int> xcol(ctx->h_colidx.size());
   for (size_t k = 0; k < xcol.size(); ++k) xcol[k] = pos[ctx->h_colidx[k]];
   HIP_OK(hipMalloc(reinterpret_cast<void **>(&S.d_xcol), std::max<size_t>(1, xcol.size()) * sizeof(int)));
-  HIP_OK(hipMemcpy(S.d_xcol, xcol.data(), xcol.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpyAsync(S.d_xcol, xcol.data(), xcol.size() * sizeof(int), hipMemcpyHostToDevice,
+                       ctx->stream));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
   S.n = 5LL * d.nodes;
   S.symbolic_ms = now_ms() - t0;
   S.plan_id = g_next_plan_id.fetch_add(1);
@@ -146,7 +150,9 @@ int solver_factor(tailor_ctx *ctx, cplx sigma, FactorTiming *timing) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   int counters[8];
-  HIP_OK(hipMemcpy(counters, S.plan.d_counters, sizeof counters, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpyAsync(counters, S.plan.d_counters, sizeof counters, hipMemcpyDeviceToHost,
+                       ctx->stream));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
   S.perturbed = counters[0];
   S.factor_ms = ms;
   S.sigma = sigma;
@@ -216,7 +222,9 @@ int factor_checks(tailor_ctx *ctx) {
 int factor_done(tailor_ctx *ctx, cplx sigma, double ms) {
   SolverState &S = *ctx->solver;
   int counters[8];
-  HIP_OK(hipMemcpy(counters, S.plan.d_counters, sizeof counters, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpyAsync(counters, S.plan.d_counters, sizeof counters, hipMemcpyDeviceToHost,
+                       ctx->stream));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
   S.perturbed = counters[0];
   S.factor_ms = ms;
   S.sigma = sigma;

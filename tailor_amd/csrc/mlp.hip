@@ -247,11 +247,15 @@ int tailor_mlp_create(tailor_mlp **out, int device, uint64_t seed) {
   fill(oW1, kH, kIn);
   fill(oW2, kH, kH);
   fill(oW3, kOut, kH);
-  if ((e = hipMemcpy(m->params, h.data(), kParams * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
-    return bad(e);
   std::vector<float> z(kParams, 0.0f);
-  if ((e = hipMemcpy(m->m, z.data(), kParams * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) return bad(e);
-  if ((e = hipMemcpy(m->v, z.data(), kParams * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) return bad(e);
+  if ((e = hipMemcpyAsync(m->params, h.data(), kParams * sizeof(float), hipMemcpyHostToDevice,
+                          m->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(m->m, z.data(), kParams * sizeof(float), hipMemcpyHostToDevice,
+                          m->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(m->v, z.data(), kParams * sizeof(float), hipMemcpyHostToDevice,
+                          m->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(m->stream)) != hipSuccess)
+    return bad(e);
   *out = m;
   return TAILOR_OK;
 }
@@ -284,8 +288,9 @@ int tailor_mlp_set_params(tailor_mlp *m, const float *in) {
   MLP_HIP(m, hipStreamSynchronize(m->stream));
   m->step = 0;  // a new parameter set restarts the optimiser
   std::vector<float> z(kParams, 0.0f);
-  MLP_HIP(m, hipMemcpy(m->m, z.data(), kParams * sizeof(float), hipMemcpyHostToDevice));
-  MLP_HIP(m, hipMemcpy(m->v, z.data(), kParams * sizeof(float), hipMemcpyHostToDevice));
+  MLP_HIP(m, hipMemcpyAsync(m->m, z.data(), kParams * sizeof(float), hipMemcpyHostToDevice, m->stream));
+  MLP_HIP(m, hipMemcpyAsync(m->v, z.data(), kParams * sizeof(float), hipMemcpyHostToDevice, m->stream));
+  MLP_HIP(m, hipStreamSynchronize(m->stream));
   return TAILOR_OK;
 }
 
