@@ -194,10 +194,10 @@ int tailor_profile_factor(tailor_ctx *ctx, double sigma_re, double sigma_im, dou
  * level and by update region, timed as in tailor_profile_factor.
  * level_ms / level_flops receive levels 0 .. max_levels - 1 (zero past the
  * plan's levels; *levels gets the plan's count), region_ms / region_flops
- * (8 entries each, may be NULL) the regions 0 INNER, 2 TRAIL, 3 ZIN,
- * 4 TRAILHEAD, 5 TRAILREST, 6 ZBLK, 7 PAIRROWS (entry 1 is always zero: the
- * block rows are finished by a triangular sweep, not a GEMM); flops count 8
- * per complex multiply-add.  No reference counterpart (profiling only). */
+ * (8 entries each, may be NULL) the regions 0 INNER, 2 TRAIL,
+ * 4 TRAILHEAD, 5 TRAILREST, 6 ZBLK, 7 PAIRROWS (entries 1 and 3 are always
+ * zero: the block rows of the trailing columns and of Z are finished by
+ * triangular sweeps, not GEMMs); flops count 8 per complex multiply-add.  No reference counterpart (profiling only). */
 int tailor_profile_gemm(tailor_ctx *ctx, double sigma_re, double sigma_im, int32_t max_levels,
                         int32_t *levels, double *level_ms, double *level_flops,
                         double *region_ms, double *region_flops);

@@ -308,8 +308,10 @@ def _profile_factor(self, sigma: complex) -> dict:
             "gemm_launches": n.value}
 
 
-# index 1 is unused (the block rows are finished by a triangular sweep)
-GEMM_REGIONS = ("inner", "unused", "trail", "zin", "trailhead", "trailrest", "zblk", "pairrows")
+# indices 1 and 3 are unused (the block rows of the trailing columns and of
+# Z are finished by triangular sweeps)
+GEMM_REGIONS = ("inner", "unused1", "trail", "unused3", "trailhead", "trailrest", "zblk",
+                "pairrows")
 
 
 def _profile_gemm(self, sigma: complex, max_levels: int = 64) -> dict:

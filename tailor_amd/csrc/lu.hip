@@ -250,9 +250,10 @@ __global__ __launch_bounds__(256) void k_build_fronts(const int *__restrict__ li
 //                      X <- L_blk^-1 X, all panels in one sweep
 //     k_gemm TRAIL     F[K1:m, K1:m] -= L21_blk U12_blk and
 //                      E[K1:p, :]    -= L21_blk E[K0:K1, :]   (K = K1 - K0)
-//   then Z = U11^-1 block by block from the last: per panel k_upper_solve
-//   (back substitution on the panel rows of Z) and k_gemm ZIN (the block's
-//   rows above the panel), then k_gemm ZBLK (all rows above the block).
+//   then Z = U11^-1 block by block from the last: k_block_z (the block's
+//   rows: per panel from the last, back substitution on its rows and their
+//   update of the block's rows above), then k_gemm ZBLK (all rows above the
+//   block).
 //
 // Row exchanges of a block are applied to the block's own columns at once
 // but never to earlier blocks: L11 is stored block by block in the row order
@@ -1266,6 +1267,205 @@ __global__ __launch_bounds__(256, 2) void k_block_rows(const int *__restrict__ l
   }
 }
 
+// Z rows of a whole block [K0, K1), columns [K0, p): Z = U11^-1 by the
+// block's panels from the last,
+//   X_j <- U_jj^-1 X_j          back substitution,
+//   X_i -= U_ij X_j             for the earlier panels i,
+// with the block's rows of 64 columns in registers for the whole sweep,
+// in the layout of k_block_rows (lane group g holds rows 8 g .. 8 g + 7 of
+// each panel, column l%16).  The substitution scales each row by the
+// reciprocal of U[r][r] (the diagonal of the slot's U_D^-1) and subtracts
+// U[i][r] x_r from the rows above, eight rows per lane group at a time (see
+// below): Z is used as an explicit inverse by the solves, so its panel rows
+// are solved with U_jj itself rather than multiplied by the inverted pivot
+// block, which is less accurate where U_jj is ill-conditioned.  The updates
+// are MFMA products with the sigma-renumbered U_ij in LDS.  Columns
+// [K0, k0) of panel j's rows are zero in Z, so the sweep may run over them.
+template <typename T>
+__global__ __launch_bounds__(256, 2) void k_block_z(const int *__restrict__ list, int nlist,
+                                                    const FrontDev *__restrict__ fr, int K0,
+                                                    int NB, const T *__restrict__ dinv,
+                                                    T *__restrict__ arena) {
+  using acc_t = typename Mfma<T>::acc_t;
+  typedef T v8 __attribute__((ext_vector_type(8)));
+  constexpr int NP = kLuNbOuter / kLuNb;
+  static_assert(NP == 4, "a block is four panels");
+  const FrontDev d = fr[list[blockIdx.y]];
+  if (K0 >= d.p) return;
+  const int K1 = min(K0 + NB, d.p);
+  const int nrows = K1 - K0, np = (nrows + kLuNb - 1) / kLuNb;
+  const long long m = d.m, pe = d.pe, mm = m * m, pp = pe * d.p;
+  const T *Fr = arena + d.off, *Fi = Fr + mm;
+  T *Zr = arena + d.off + 2 * mm + 2 * pp, *Zi = Zr + pp;
+  const int ncols = d.p - K0;
+  const int col0 = blockIdx.x * 64;
+  if (col0 >= ncols) return;
+  // stage 0: U_jj as it stands (row i, column r); stage t: U_{j-t, j}
+  // renumbered by sigma; D: 1 / U[r][r]
+  __shared__ T Sr[NP][kLuNb][kLuNb + 1], Si[NP][kLuNb][kLuNb + 1];
+  __shared__ T Dr[kLuNb], Di[kLuNb];
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64, g = lane / 16;
+  const int jl = col0 + wave * 16 + lane % 16;  // the lane's column - K0
+  const bool live = jl < ncols;
+  T *cr = Zr + static_cast<long long>(K0 + min(jl, ncols - 1)) * pe + K0;
+  T *ci = Zi + static_cast<long long>(K0 + min(jl, ncols - 1)) * pe + K0;
+  // x[j][ib][v]: row 32 j + 8 g + 4 ib + v; loads as in k_block_rows (a run
+  // of 8 rows starting below K1 stays inside the column of pe rows)
+  acc_t xr[NP][2], xi[NP][2];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int r0 = 32 * j + 8 * g;
+    const bool in = r0 < nrows;
+    const v8 a = *reinterpret_cast<const v8 *>(cr + (in ? r0 : 0));
+    const v8 b = *reinterpret_cast<const v8 *>(ci + (in ? r0 : 0));
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const bool ok = live && r0 + s < nrows;
+      xr[j][s / 4][s % 4] = ok ? a[s] : T(0);
+      xi[j][s / 4][s % 4] = ok ? b[s] : T(0);
+    }
+  }
+#pragma unroll
+  for (int j = NP - 1; j >= 0; --j) {
+    if (j >= np) continue;
+    const int k0 = K0 + kLuNb * j;
+    const int kb = min(kLuNb, K1 - k0);
+    {
+      // every load issued before the first LDS store; entries outside the
+      // panel's kb columns (and blocks above the first panel) are zeroed,
+      // their addresses clamped into the front
+      constexpr int kPer = kLuNb * kLuNb / 256;  // 256 threads
+      const T *slot = inv_slot(dinv, d, k0);
+      T vr[NP][kPer], vi[NP][kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = threadIdx.x + 256 * u, r = e % kLuNb, c = e / kLuNb;
+        const bool cin = c < kb;
+#pragma unroll
+        for (int t = 0; t < NP; ++t) {
+          const int i = j - t;                                   // panel of the rows
+          const bool ok = cin && i >= 0 && (t > 0 || r < kb);
+          const int row = K0 + kLuNb * max(i, 0) + r;             // < K1 when ok
+          const long long idx = static_cast<long long>(k0 + c) * m + (ok ? row : k0);
+          const T a = Fr[idx], b = Fi[idx];
+          vr[t][u] = ok ? a : T(0);
+          vi[t][u] = ok ? b : T(0);
+        }
+      }
+      const T dr = threadIdx.x < kLuNb ? slot[2 * kLuNb * kLuNb + threadIdx.x * (kLuNb + 1)] : T(0);
+      const T di = threadIdx.x < kLuNb ? slot[3 * kLuNb * kLuNb + threadIdx.x * (kLuNb + 1)] : T(0);
+      __syncthreads();  // the previous panel's steps have read the stages
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = threadIdx.x + 256 * u, r = e % kLuNb, c = e / kLuNb;
+        Sr[0][r][c] = vr[0][u];
+        Si[0][r][c] = vi[0][u];
+        const int pi = block_rows_pos<T>(r), pk = block_rows_pos<T>(c);
+#pragma unroll
+        for (int t = 1; t < NP; ++t) {
+          Sr[t][pi][pk] = vr[t][u];
+          Si[t][pi][pk] = vi[t][u];
+        }
+      }
+      if (threadIdx.x < kLuNb) {
+        Dr[threadIdx.x] = dr;
+        Di[threadIdx.x] = di;
+      }
+    }
+    __syncthreads();
+    // X_j <- U_jj^-1 X_j by back substitution over the panel's kb rows, in
+    // four steps of 8 rows from the last: the lane group G holding rows
+    // 8 G .. 8 G + 7 solves them in its own registers, passes them to the
+    // other lane groups of the column by shuffles, and the groups above
+    // subtract their products with U[i][8 G .. 8 G + 7]
+#pragma unroll
+    for (int G = 3; G >= 0; --G) {
+      if (8 * G >= kb) continue;
+      if (g == G) {
+#pragma unroll
+        for (int s = 7; s >= 0; --s) {
+          const int r = 8 * G + s;
+          if (r >= kb) continue;
+          const T d_r = Dr[r], d_i = Di[r];
+          const T x_r = xr[j][s / 4][s % 4], x_i = xi[j][s / 4][s % 4];
+          const T yr = x_r * d_r - x_i * d_i, yi = x_r * d_i + x_i * d_r;
+          xr[j][s / 4][s % 4] = yr;
+          xi[j][s / 4][s % 4] = yi;
+#pragma unroll
+          for (int q = 0; q < s; ++q) {
+            const T ur = Sr[0][8 * G + q][r], ui = Si[0][8 * G + q][r];
+            xr[j][q / 4][q % 4] -= ur * yr - ui * yi;
+            xi[j][q / 4][q % 4] -= ur * yi + ui * yr;
+          }
+        }
+      }
+      if (G == 0) break;
+      const int src = (lane & 15) + 16 * G;
+      T br[8], bi[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        br[s] = __shfl(xr[j][s / 4][s % 4], src);
+        bi[s] = __shfl(xi[j][s / 4][s % 4], src);
+      }
+      if (g < G) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          T ar = xr[j][q / 4][q % 4], ai = xi[j][q / 4][q % 4];
+#pragma unroll
+          for (int s = 0; s < 8; ++s) {
+            const T ur = Sr[0][8 * g + q][8 * G + s], ui = Si[0][8 * g + q][8 * G + s];
+            ar -= ur * br[s] - ui * bi[s];
+            ai -= ur * bi[s] + ui * br[s];
+          }
+          xr[j][q / 4][q % 4] = ar;
+          xi[j][q / 4][q % 4] = ai;
+        }
+      }
+    }
+    // X_i -= U_ij X_j for the earlier panels
+#pragma unroll
+    for (int t = 1; t < NP; ++t) {
+      const int i = j - t;
+      if (i < 0) break;
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        const int k = 16 * (s / 4) + Mfma<T>::orow(g, s % 4);
+        const T br = xr[j][s / 4][s % 4], bi = xi[j][s / 4][s % 4];
+#pragma unroll
+        for (int ib = 0; ib < 2; ++ib) {
+          const T ar = Sr[t][16 * ib + lane % 16][k], ai = Si[t][16 * ib + lane % 16][k];
+          xr[i][ib] = Mfma<T>::mma(ar, -br, xr[i][ib]);
+          xr[i][ib] = Mfma<T>::mma(ai, bi, xr[i][ib]);
+          xi[i][ib] = Mfma<T>::mma(ar, -bi, xi[i][ib]);
+          xi[i][ib] = Mfma<T>::mma(ai, -br, xi[i][ib]);
+        }
+      }
+    }
+  }
+  if (!live) return;
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int r0 = 32 * j + 8 * g;
+    v8 a, b;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      a[s] = xr[j][s / 4][s % 4];
+      b[s] = xi[j][s / 4][s % 4];
+    }
+    if (r0 + 8 <= nrows) {
+      *reinterpret_cast<v8 *>(cr + r0) = a;
+      *reinterpret_cast<v8 *>(ci + r0) = b;
+    } else if (r0 < nrows) {
+#pragma unroll
+      for (int s = 0; s < 8; ++s)
+        if (r0 + s < nrows) {
+          cr[r0 + s] = a[s];
+          ci[r0 + s] = b[s];
+        }
+    }
+  }
+}
+
 // Rows [p, m) of the panel columns: X <- X U_D^-1.  Four waves per
 // workgroup, 32 rows x 32 columns per wave; the MFMAs compute
 // (X U)^T = U^T X^T: lane l supplies U[4s + l/16][16 jb + l%16] and
@@ -1341,87 +1541,6 @@ __global__ __launch_bounds__(256) void k_right_apply(const int *__restrict__ lis
         }
       }
     }
-}
-
-// Rows [k0, k0+kb) of Z columns [k0, p): X <- U_D^-1 X by back
-// substitution (Z = U11^-1 is used by the solves as an explicit inverse, so
-// its blocks are built by substitution rather than by multiplying with the
-// inverted pivot block).  Two columns per wave: lanes 32 h + i hold row i of
-// column h; step r finalises x_r with the reciprocal of U[r][r] (the
-// diagonal of U_D^-1 in the panel's slot), broadcasts it by readlane, and the
-// rows above subtract U[i][r] x_r.
-template <typename T>
-__global__ __launch_bounds__(256) void k_upper_solve(const int *__restrict__ list, int nlist,
-                                                     const FrontDev *__restrict__ fr, int k0,
-                                                     int nb, const T *__restrict__ dinv,
-                                                     T *__restrict__ arena) {
-  const FrontDev d = fr[list[blockIdx.y]];
-  if (k0 >= d.p) return;
-  const int kb = min(nb, d.p - k0);
-  const long long m = d.m, pe = d.pe, pp = pe * d.p;
-  const T *Fr = arena + d.off, *Fi = Fr + m * m;
-  T *Zr = arena + d.off + 2 * m * m + 2 * pp, *Zi = Zr + pp;
-  const int col0 = k0 + blockIdx.x * 8;  // 8 columns per workgroup
-  if (col0 >= d.p) return;
-  __shared__ T Ur[kLuNb][kLuNb + 1], Ui[kLuNb][kLuNb + 1];  // U[i][q] = F[k0+i, k0+q]
-  __shared__ T Dr[kLuNb], Di[kLuNb];                         // 1 / U[i][i]
-  {
-    // all loads issued before the first LDS store (clamped addresses, zeroed
-    // outside the kb x kb block)
-    constexpr int kPer = kLuNb * kLuNb / 256;  // 256 threads
-    T vr[kPer], vi[kPer];
-#pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-      const int e = threadIdx.x + 256 * u, i = e % kLuNb, q = e / kLuNb;
-      const long long idx = static_cast<long long>(k0 + min(q, kb - 1)) * m + k0 + min(i, kb - 1);
-      const bool ok = i < kb && q < kb;
-      const T a = Fr[idx], b = Fi[idx];
-      vr[u] = ok ? a : T(0);
-      vi[u] = ok ? b : T(0);
-    }
-#pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-      const int e = threadIdx.x + 256 * u;
-      Ur[e % kLuNb][e / kLuNb] = vr[u];
-      Ui[e % kLuNb][e / kLuNb] = vi[u];
-    }
-  }
-  const T *slot = inv_slot(dinv, d, k0);
-  if (threadIdx.x < kb) {
-    Dr[threadIdx.x] = slot[2 * kLuNb * kLuNb + threadIdx.x * (kLuNb + 1)];
-    Di[threadIdx.x] = slot[3 * kLuNb * kLuNb + threadIdx.x * (kLuNb + 1)];
-  }
-  __syncthreads();
-  const int lane = threadIdx.x % 64, h = lane / 32, i = lane % 32;
-  const int col = col0 + 2 * (threadIdx.x / 64) + h;
-  const bool live = col < d.p && i < kb;
-  T *xr_p = Zr + static_cast<long long>(min(col, d.p - 1)) * pe + k0;
-  T *xi_p = Zi + static_cast<long long>(min(col, d.p - 1)) * pe + k0;
-  T xr = T(0), xi = T(0);
-  if (live) {
-    xr = xr_p[i];
-    xi = xi_p[i];
-  }
-  for (int r = kb - 1; r >= 0; --r) {
-    if (i == r) {
-      const T dr = Dr[r], di = Di[r];
-      const T yr = xr * dr - xi * di, yi = xr * di + xi * dr;
-      xr = yr;
-      xi = yi;
-    }
-    const T b0r = readlane_t(xr, r), b0i = readlane_t(xi, r);
-    const T b1r = readlane_t(xr, 32 + r), b1i = readlane_t(xi, 32 + r);
-    const T br = h ? b1r : b0r, bi = h ? b1i : b0i;
-    if (i < r) {
-      const T ur = Ur[i][r], ui = Ui[i][r];
-      xr -= ur * br - ui * bi;
-      xi -= ur * bi + ui * br;
-    }
-  }
-  if (live) {
-    xr_p[i] = xr;
-    xi_p[i] = xi;
-  }
 }
 
 // ---------------------------------------------- row exchanges of a block
@@ -1570,19 +1689,19 @@ __global__ __launch_bounds__(256) void k_block_swaps(const int *__restrict__ lis
 // TRAIL = TRAILHEAD (the next block's fully summed columns of F) + TRAILREST
 // (the remaining columns of F and all of E); the split lets the next
 // block's panels start while the rest of the update runs.  Z = U11^-1 is
-// built block by block from the last: within block [K0, K1) each panel's
-// rows are finished and ZIN updates the block's rows above the panel
-// (K = 32); ZBLK then updates all rows above the block at once (K = 128).
+// built block by block from the last: k_block_z finishes the block's rows
+// [K0, K1), and ZBLK then updates all rows above the block at once
+// (K = 128).
 // With paired blocks (two-level blocking, LuPlan::pair_blocks), block A's
 // update of the rows below it is deferred until block B = [K1, K1n) is
 // factored: PAIRROWS applies it to B's rows of the columns right of B and
 // of E (K = 128), and the pair's TRAILHEAD / TRAILREST / TRAIL then update
 // everything below B with K = 256 (the two blocks' L columns and U rows are
 // contiguous).
-// (Region 1 is unused: the block rows of the trailing columns are finished
-// by k_block_rows, which is not a GEMM.)
+// (Regions 1 and 3 are unused: the block rows of the trailing columns and
+// of Z are finished by the triangular sweeps k_block_rows and k_block_z.)
 enum GemmRegion {
-  kGemmInner = 0, kGemmTrail = 2, kGemmZ = 3, kGemmTrailHead = 4,
+  kGemmInner = 0, kGemmTrail = 2, kGemmTrailHead = 4,
   kGemmTrailRest = 5, kGemmZBlk = 6, kGemmPairRows = 7
 };
 
@@ -1633,8 +1752,7 @@ __host__ __device__ inline GemmShape gemm_shape(int region, const FrontDev &d, i
     case kGemmZBlk:
       g = {K0, K0, d.p - K0, d.p - K0, K1 - K0};
       break;
-    default:  // ZIN
-      g = {k0 - K0, k0 - K0, d.p - k0, d.p - k0, kb};
+    default:
       break;
   }
   return g;
@@ -1699,13 +1817,8 @@ __device__ inline bool gemm_view(int region, const FrontDev &d, int k0, int nb, 
            K0, K0, d.p - K0, d.p - K0, K1 - K0};
       break;
     }
-    default: {  // ZIN
-      const long long a = k0 * m + K0, b = k0 * p + k0, c = k0 * p + K0;
-      v = {Fr + a, Fi + a, m, Zr + b, Zi + b, nullptr, nullptr, p, 0,
-           Zr + c, Zi + c, nullptr, nullptr, p, 0,
-           k0 - K0, k0 - K0, d.p - k0, d.p - k0, kb};
-      break;
-    }
+    default:
+      return false;
   }
   return v.M > 0 && v.N > 0 && v.K > 0;
 }
@@ -2760,18 +2873,12 @@ static hipError_t factor_levels(const LuPlan &P, const std::vector<ScatterJob> &
       if ((e = hipGetLastError()) != hipSuccess) return e;
       K0 += W;
     }
-    // Z = U11^-1 by backward block-Jordan steps: within each block the
-    // panels from the last (panel rows by back substitution, ZIN for the
-    // block's rows above), then ZBLK for every row above the block
+    // Z = U11^-1 by backward block-Jordan steps: the block's rows in one
+    // sweep (k_block_z), then ZBLK for every row above the block
     for (int K0 = ((L.pmax - 1) / NB) * NB; K0 >= 0; K0 -= NB) {
-      const int Kend = std::min(K0 + NB, L.pmax);
-      for (int k0 = K0 + ((Kend - 1 - K0) / nb) * nb; k0 >= K0; k0 -= nb) {
-        hipLaunchKernelGGL(k_upper_solve<T>, dim3(cdivu(L.pmax, 8), L.count), dim3(256), 0, s,
-                           list, L.count, P.d_front, k0, nb, dinv, arena);
-        if (k0 > K0)
-          if ((e = gemm(NB - nb, L.pmax, kGemmZ, k0, K0, NB, s)) != hipSuccess) return e;
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-      }
+      hipLaunchKernelGGL(k_block_z<T>, dim3(cdivu(L.pmax - K0, 64), L.count), dim3(256), 0, s,
+                         list, L.count, P.d_front, K0, NB, dinv, arena);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
       if (K0 > 0)
         if ((e = gemm(L.pmax, L.pmax, kGemmZBlk, K0, K0, NB, s)) != hipSuccess) return e;
     }
