@@ -2820,12 +2820,16 @@ static hipError_t factor_levels(const LuPlan &P, const std::vector<ScatterJob> &
                          0, s, list, L.count, P.d_front, K0, NB, dinv, arena);
       return hipGetLastError();
     };
-    // Look-ahead: the panels of the next block run on the side stream once
-    // the TRAILHEAD update has produced their columns, while the main stream
-    // applies TRAILREST (later columns and E).  The two touch disjoint
-    // columns; the main stream waits for the panels before the next block's
-    // row exchanges and U rows.  A timed factorization runs the same launches
-    // in order on one stream, so each kernel's duration is its own.
+    // Look-ahead: once the TRAILHEAD update has produced the next block's
+    // columns, TRAILREST (later columns and E) runs on the side stream while
+    // the main stream factors the next block's panels.  The two touch
+    // disjoint columns; the main stream waits for TRAILREST before the next
+    // block's row exchanges and U rows.  The side stream is confined to all
+    // but kSideReservedCus compute units (LuPlan::side), so the panel chain
+    // -- a dozen short launches per panel, one workgroup per front -- always
+    // finds free CUs instead of waiting for TRAILREST's workgroups to retire.
+    // A timed factorization runs the same launches in order on one stream,
+    // so each kernel's duration is its own.
     //
     // Paired blocks (P.pair_blocks): blocks A = [K0, K0 + NB) and
     // B = [K0 + NB, K0 + 2 NB) share one update of the rows below B
@@ -2857,10 +2861,11 @@ static hipError_t factor_levels(const LuPlan &P, const std::vector<ScatterJob> &
         if (lookahead) {
           if ((e = hipEventRecord(P.ev_head, s)) != hipSuccess) return e;
           if ((e = hipStreamWaitEvent(P.side, P.ev_head, 0)) != hipSuccess) return e;
-          if ((e = panel_phase(K0 + W, P.side)) != hipSuccess) return e;
-          if ((e = hipEventRecord(P.ev_panel, P.side)) != hipSuccess) return e;
-          if ((e = gemm(L.mmax, L.mmax + L.pmax, kGemmTrailRest, K0, K0, W, s)) != hipSuccess)
+          if ((e = gemm(L.mmax, L.mmax + L.pmax, kGemmTrailRest, K0, K0, W, P.side)) !=
+              hipSuccess)
             return e;
+          if ((e = hipEventRecord(P.ev_panel, P.side)) != hipSuccess) return e;
+          if ((e = panel_phase(K0 + W, s)) != hipSuccess) return e;
           if ((e = hipStreamWaitEvent(s, P.ev_panel, 0)) != hipSuccess) return e;
         } else {
           if ((e = panel_phase(K0 + W, s)) != hipSuccess) return e;

@@ -66,7 +66,7 @@ struct LuPlan {
   double *arena = nullptr;
   bool owns_arena = true;       // false for a batch plan, whose fronts live in its parts' arenas
   int instances = 1;            // parameter blocks / pivot counters (one per batch part)
-  hipStream_t side = nullptr;   // look-ahead panels (default priority)
+  hipStream_t side = nullptr;   // look-ahead TRAILREST (default priority, CU-masked)
   hipEvent_t ev_head = nullptr, ev_panel = nullptr;
   double *dinv = nullptr;       // inverted pivot blocks: dinv_slots panel slots
   long long dinv_slots = 0;     // most panels of any level
@@ -101,6 +101,11 @@ constexpr int kLuNb = 32;        // panel width
 constexpr int kFrontAlign = 32;
 constexpr int kBuildInts = 4;  // child 0, child 1, their map offsets
 constexpr int kLuNbOuter = 128;  // outer block width (K of the trailing update)
+// Compute units kept free of the look-ahead stream's TRAILREST updates (the
+// highest-numbered ones), so that the panel chain on the main stream is
+// dispatched at once: 16 of 256 measured 134.0 -> 130.0 ms per 200x100
+// factorization and +2.6 % on the eight-context sweep; 32 gave the same.
+constexpr int kSideReservedCus = 16;
 constexpr int kLuInvSlot = 4 * kLuNb * kLuNb;      // inverted pivot blocks of one panel
 constexpr int kLuPermStride = 1 + 4 * kLuNbOuter;  // one front's block row permutation
 

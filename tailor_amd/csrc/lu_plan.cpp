@@ -24,6 +24,24 @@ void release(T *&p) {
   p = nullptr;
 }
 
+// The look-ahead stream: non-blocking, on every compute unit but the last
+// kSideReservedCus; a plain non-blocking stream where the device has too few
+// CUs or refuses the mask.
+hipError_t side_stream(hipStream_t *s) {
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      ncu > 4 * kSideReservedCus) {
+    std::vector<uint32_t> mask((ncu + 31) / 32, 0);
+    for (int c = 0; c < ncu - kSideReservedCus; ++c) mask[c / 32] |= 1u << (c % 32);
+    if (hipExtStreamCreateWithCUMask(s, static_cast<uint32_t>(mask.size()), mask.data()) ==
+        hipSuccess)
+      return hipSuccess;
+    (void)hipGetLastError();
+  }
+  return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+
 }  // namespace
 
 hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int per, LuPlan &P,
@@ -185,7 +203,7 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
     return e;
   if ((e = hipMalloc(reinterpret_cast<void **>(&P.Y2), 2 * P.n * sizeof(double))) != hipSuccess)
     return e;
-  if ((e = hipStreamCreateWithFlags(&P.side, hipStreamNonBlocking)) != hipSuccess) return e;
+  if ((e = side_stream(&P.side)) != hipSuccess) return e;
   if ((e = hipEventCreateWithFlags(&P.ev_head, hipEventDisableTiming)) != hipSuccess) return e;
   if ((e = hipEventCreateWithFlags(&P.ev_panel, hipEventDisableTiming)) != hipSuccess) return e;
   return hipStreamSynchronize(s);
@@ -307,7 +325,7 @@ hipError_t lu_plan_batch(const std::vector<BatchPart> &parts, bool fp32, LuPlan 
                      std::max(1, U.count_max) * static_cast<size_t>(kLuPermStride) * sizeof(int))) !=
       hipSuccess)
     return e;
-  if ((e = hipStreamCreateWithFlags(&U.side, hipStreamNonBlocking)) != hipSuccess) return e;
+  if ((e = side_stream(&U.side)) != hipSuccess) return e;
   if ((e = hipEventCreateWithFlags(&U.ev_head, hipEventDisableTiming)) != hipSuccess) return e;
   if ((e = hipEventCreateWithFlags(&U.ev_panel, hipEventDisableTiming)) != hipSuccess) return e;
   return hipStreamSynchronize(s);
