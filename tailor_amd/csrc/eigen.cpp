@@ -188,6 +188,8 @@ int graph_ready(tailor_ctx *ctx) {
     return TAILOR_OK;
   release_graphs(&S);
   const LuInputs in = factor_inputs(ctx);
+  // the look-ahead stream is made outside the capture
+  HIP_OK(lu_plan_side(S.plan));
   HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   hipError_t e = lu_factor_params(S.plan, in, fp32, ctx->d_partials, ctx->stream);
   if (e == hipSuccess) e = lu_factor(S.plan, in, fp32, ctx->stream, nullptr);

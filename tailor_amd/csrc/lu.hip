@@ -2911,6 +2911,10 @@ static hipError_t factor_impl(const LuPlan &P, const LuInputs &in, hipStream_t s
 
 hipError_t lu_factor(const LuPlan &P, const LuInputs &in, bool fp32, hipStream_t s,
                      FactorTiming *tm) {
+  if (!tm) {
+    const hipError_t e = lu_plan_side(P);
+    if (e != hipSuccess) return e;
+  }
   return fp32 ? factor_impl<float>(P, in, s, tm) : factor_impl<double>(P, in, s, tm);
 }
 

@@ -66,7 +66,11 @@ struct LuPlan {
   double *arena = nullptr;
   bool owns_arena = true;       // false for a batch plan, whose fronts live in its parts' arenas
   int instances = 1;            // parameter blocks / pivot counters (one per batch part)
-  hipStream_t side = nullptr;   // look-ahead TRAILREST (default priority, CU-masked)
+  // look-ahead TRAILREST (default priority): a batch plan's is a plain
+  // stream made with the plan; a context plan's is CU-masked and made at its
+  // first factorization (lu_plan_side), so that a context whose factorizations
+  // only run in batches holds no queue of its own for it
+  mutable hipStream_t side = nullptr;
   hipEvent_t ev_head = nullptr, ev_panel = nullptr;
   double *dinv = nullptr;       // inverted pivot blocks: dinv_slots panel slots
   long long dinv_slots = 0;     // most panels of any level
@@ -131,6 +135,9 @@ hipError_t lu_plan_batch(const std::vector<BatchPart> &parts, bool fp32, LuPlan 
                          std::vector<int> &fbase, hipStream_t s);
 
 hipError_t lu_fill(hipStream_t s, double *p, long long n, double v);
+// Make P's look-ahead stream if it has none yet (before a graph capture of
+// its factorization; lu_factor calls it too).
+hipError_t lu_plan_side(const LuPlan &P);
 // Per-kernel timing of one factorization (HIP events around every frontal
 // GEMM launch) and the algorithmic flops of those launches (8 M N K per
 // front and launch: a complex multiply-add is 8 real flops).
@@ -168,10 +175,11 @@ hipError_t lu_factor(const LuPlan &P, const LuInputs &in, bool fp32, hipStream_t
                      FactorTiming *timing = nullptr);
 // One factorization of every part of a batch plan U (lu_plan_batch with the
 // same fp32) at the shifts already in U's parameter blocks: per part the
-// norms of A_bc and B_bc, the pivot thresholds, the zeroed fronts and the
-// scatter of K into them (in[b], partials[b]: 1024 doubles of scratch per
-// part), then one launch sequence over the levels of U.  As lu_factor, a
-// pure launch sequence on s and U's side stream without `timing`.
+// norms of A_bc and B_bc and the pivot thresholds (in[b], partials[b]: 1024
+// doubles of scratch per part), then one launch sequence over the levels of
+// U, each level's fronts built from their children with every part's K
+// added.  As lu_factor, a pure launch sequence on s and U's side stream
+// without `timing`.
 hipError_t lu_factor_batch(const LuPlan &U, const std::vector<BatchPart> &parts,
                            const std::vector<int> &fbase, const std::vector<LuInputs> &in,
                            const std::vector<double *> &partials, bool fp32, hipStream_t s,

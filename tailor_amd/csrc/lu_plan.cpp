@@ -203,7 +203,6 @@ hipError_t lu_plan_create(const Symbolic &S, const std::vector<int> &colidx, int
     return e;
   if ((e = hipMalloc(reinterpret_cast<void **>(&P.Y2), 2 * P.n * sizeof(double))) != hipSuccess)
     return e;
-  if ((e = side_stream(&P.side)) != hipSuccess) return e;
   if ((e = hipEventCreateWithFlags(&P.ev_head, hipEventDisableTiming)) != hipSuccess) return e;
   if ((e = hipEventCreateWithFlags(&P.ev_panel, hipEventDisableTiming)) != hipSuccess) return e;
   return hipStreamSynchronize(s);
@@ -325,10 +324,16 @@ hipError_t lu_plan_batch(const std::vector<BatchPart> &parts, bool fp32, LuPlan 
                      std::max(1, U.count_max) * static_cast<size_t>(kLuPermStride) * sizeof(int))) !=
       hipSuccess)
     return e;
-  if ((e = side_stream(&U.side)) != hipSuccess) return e;
+  // (batch factorizations replay as HIP graphs, which do not keep a
+  // stream's CU mask: a plain stream)
+  if ((e = hipStreamCreateWithFlags(&U.side, hipStreamNonBlocking)) != hipSuccess) return e;
   if ((e = hipEventCreateWithFlags(&U.ev_head, hipEventDisableTiming)) != hipSuccess) return e;
   if ((e = hipEventCreateWithFlags(&U.ev_panel, hipEventDisableTiming)) != hipSuccess) return e;
   return hipStreamSynchronize(s);
+}
+
+hipError_t lu_plan_side(const LuPlan &P) {
+  return P.side ? hipSuccess : side_stream(&P.side);
 }
 
 void lu_plan_destroy(LuPlan &P) {
