@@ -39,6 +39,41 @@ def test_learned_shifts_cut_solves():
                 assert d < 1e-9, (w, d)
 
 
+def test_network_gain_over_the_same_search_with_a_fixed_shift():
+    """The network alone: the same sweeper (subspace 40, early pass ends from
+    step 28, warm starts in wavenumber order, per-instance base flows) with
+    the network's shift centres ("learned") and with the fixed shift omega_s
+    ("fixed").  The learned form delivers every pair with fewer solves and a
+    higher rate; pairs both forms find agree."""
+    from tailor_amd import sweep as sw
+    from tailor_amd.shift_model import ShiftModel
+    from tailor_amd.train_shift_model import MODEL
+    model = ShiftModel(0).load(MODEL)
+    insts = sw.sweep_instances(4, seed=5, sizes=[(100, 50)], reynolds=1500.0, mach=0.7,
+                               per_instance_seeds=True)
+    cases = sw.prepare(insts)
+    alpha = np.array([i.alpha for i in insts])
+    shifts = model.shift_centres(alpha, np.full(4, 1500.0), np.full(4, 0.7))
+    order = sw.alpha_order(insts)
+    runner = sw.Sweeper(ncv=40, check_from=28)
+    runner.run(cases, "learned", shifts=shifts, order=order[:1])  # plans and tuning, untimed
+    learned = runner.run(cases, "learned", shifts=shifts, order=order)
+    fixed = runner.run(cases, "fixed", order=order)
+    runner.close()
+    tl, tf = learned.totals(), fixed.totals()
+    print(f"learned {tl}\nfixed   {tf}", flush=True)
+    assert tl["good_pairs"] == 8 * len(insts)
+    assert tl["solves"] < tf["solves"]
+    assert learned.eigenpairs_per_s > fixed.eigenpairs_per_s
+    by = {id(r.instance): r for r in fixed.records}
+    for r in learned.records:
+        f = by[id(r.instance)]
+        for w in r.omegas:
+            d = np.min(np.abs(f.omegas - w)) / abs(w) if len(f.omegas) else 1.0
+            if d < 1e-6:  # found by both forms
+                assert d < 1e-9, (w, d)
+
+
 @pytest.mark.parametrize("contexts", [2, 3])
 def test_pipelined_learned_sweep_matches_sequential(contexts):
     """Contexts in a pipeline (the next instances' assembly and factorization
