@@ -940,86 +940,89 @@ __global__ __launch_bounds__(64 * NPW) void k_operator(
     return;
   }
 
-  // wave w streams block row `first + w` out.  Lanes 0..49 each own one
-  // entry e = (r, c) of a 5x5 block and walk the row two blocks at a time
-  // (lanes 0..24 the even blocks, 25..49 the odd ones), so each store
-  // instruction writes 50 x 16 B contiguous bytes and the entry's coefficient
-  // values stay in registers for the whole row.
+  // The workgroup's block rows are one contiguous span of nrows * PER * 25
+  // complex entries; with NPW rows per workgroup of PER * 400 bytes each the
+  // span starts on a 128-byte line whenever NPW * PER * 25 is a multiple of 8
+  // (NPW = 8).  Thread k of the workgroup writes entries k, k + 64 * NPW, ...
+  // of the span, so each store instruction of a wave writes 64 x 16 B = 1 KB
+  // of consecutive bytes (a wave per row, 50 lanes storing 800 B at a time
+  // from registers, stored at 4.3 TB/s against 6.5 TB/s for a plain fill).
+  // Each entry reads its row's coefficients from the record in LDS.
   {
-    const int w = tid >> 6, lane = tid & 63;
-    const long long node = first + w;
-    if (node < d.nodes && lane < 50) {
+    const int per_row = PER * 25;
+    const long long nrows = min(static_cast<long long>(NPW), d.nodes - first);
+    const int span = static_cast<int>(nrows) * per_row;
+    double2v *out = reinterpret_cast<double2v *>(vals) + first * per_row;
+    // L entry e = (r, c) of block (t, s) of a row, accumulation order K0,
+    // xi, eta, mixed (GlobalOperator.cpp:323-365)
+    auto l_entry = [&](const NodeRec<SSX, SSE> &R, int t, int s, int e, double &vr,
+                       double &vi) {
+      vr = 0.0;
+      vi = 0.0;
+      if (t == R.t0) {
+        if (s == R.s0) {
+          vr = vr + R.K0[e].re;
+          vi = vi + R.K0[e].im;
+        }
+        const double wv = R.w1x[s];
+        double ar = R.Kxi[e].re * wv;
+        const double ai = R.Kxi[e].im * wv;
+        ar = ar + R.Vxx[e] * R.mw2x[s];
+        vr = vr + ar;
+        vi = vi + ai;
+      }
+      if (s == R.s0) {
+        const double wv = R.w1e[t];
+        double ar = R.Keta[e].re * wv;
+        const double ai = R.Keta[e].im * wv;
+        ar = ar + R.Vee[e] * R.mw2e[t];
+        vr = vr + ar;
+        vi = vi + ai;
+      }
+      vr = vr + R.Vxe[e] * (-R.w1e[t] * R.w1x[s]);
+    };
+    for (int k = tid; k < span; k += 64 * NPW) {
+      const int w = k / per_row, rem = k - per_row * w;
+      const int b = rem / 25, e = rem - 25 * b;
+      const int r = e / 5, c = e - 5 * r;
       const NodeRec<SSX, SSE> &R = rec[w];
-      const int half = lane >= 25 ? 1 : 0;
-      const int e = lane - 25 * half;
-      const int r = e / 5, c = e - 5 * (e / 5);
-      const dcx K0 = R.K0[e], Kx = R.Kxi[e], Ke = R.Keta[e];
-      const double Vxx = R.Vxx[e], Vxe = R.Vxe[e], Vee = R.Vee[e];
-      const int kind = constrained ? R.kind : 0;
-      const int t0 = R.t0, s0 = R.s0;
-      double2v *out = reinterpret_cast<double2v *>(vals) + node * PER * 25;
-      for (int k = 0; k < (PER + 1) / 2; ++k) {
-        const int braw = 2 * k + half;
-        const bool valid = braw < PER;
-        const int b = valid ? braw : PER - 1;
-        const int tr = b / NSX, s = b - NSX * (b / NSX);
-        const int t = R.slot_of_rank[tr];
-        // L entry, accumulation order K0, xi, eta, mixed (GlobalOperator.cpp:323-365)
-        double vr = 0.0, vi = 0.0;
-        if (t == t0) {
-          if (s == s0) {
-            vr = vr + K0.re;
-            vi = vi + K0.im;
-          }
-          const double wv = R.w1x[s];
-          double ar = Kx.re * wv;
-          const double ai = Kx.im * wv;
-          ar = ar + Vxx * R.mw2x[s];
-          vr = vr + ar;
-          vi = vi + ai;
-        }
-        if (s == s0) {
-          const double wv = R.w1e[t];
-          double ar = Ke.re * wv;
-          const double ai = Ke.im * wv;
-          ar = ar + Vee * R.mw2e[t];
-          vr = vr + ar;
-          vi = vi + ai;
-        }
-        vr = vr + Vxe * (-R.w1e[t] * R.w1x[s]);
-        double2v val;
-        if (!constrained) {
-          val = double2v{vr, vi};
-        } else {
-          const double ar = vr * -1.0, ai = vi * -1.0;  // A_bc = -L
-          const bool diagblk = t == t0 && s == s0;
-          if (kind == 1 && r >= 1) {
-            // isothermal no-slip wall: u, v, w, T rows become identity rows
-            val = double2v{(diagblk && r == c) ? 1.0 : 0.0, 0.0};
-          } else if (kind == 2) {
-            // far field: row r = P[r,:] (rows of -L) unless mode r is incoming
+      const int tr = b / NSX, s = b - NSX * tr;
+      const int t = R.slot_of_rank[tr];
+      double vr, vi;
+      l_entry(R, t, s, e, vr, vi);
+      double2v val;
+      if (!constrained) {
+        val = double2v{vr, vi};
+      } else {
+        const double ar = vr * -1.0, ai = vi * -1.0;  // A_bc = -L
+        const bool diagblk = t == R.t0 && s == R.s0;
+        if (R.kind == 1 && r >= 1) {
+          // isothermal no-slip wall: u, v, w, T rows become identity rows
+          val = double2v{(diagblk && r == c) ? 1.0 : 0.0, 0.0};
+        } else if (R.kind == 2) {
+          // far field: row r = P[r,:] (rows of -L) unless mode r is incoming
+          if (R.incoming[r]) {
+            const dcx ri = R.Ri[r * 5 + c];
+            val = diagblk ? double2v{ri.re, ri.im} : double2v{0.0, 0.0};
+          } else {
             double sr = 0.0, si = 0.0;
             for (int q = 0; q < 5; ++q) {
-              const int src = half * 25 + q * 5 + c;
-              const double xr = __shfl(ar, src), xi = __shfl(ai, src);
               const dcx pk = R.P[r * 5 + q];
               if (pk.re == 0.0 && pk.im == 0.0) continue;
+              double xr, xi;
+              l_entry(R, t, s, q * 5 + c, xr, xi);
+              xr = xr * -1.0;
+              xi = xi * -1.0;
               sr += pk.re * xr - pk.im * xi;
               si += pk.re * xi + pk.im * xr;
             }
-            if (R.incoming[r]) {
-              const dcx ri = R.Ri[r * 5 + c];
-              val = diagblk ? double2v{ri.re, ri.im} : double2v{0.0, 0.0};
-            } else {
-              val = double2v{sr, si};
-            }
-          } else {
-            val = double2v{ar, ai};
+            val = double2v{sr, si};
           }
+        } else {
+          val = double2v{ar, ai};
         }
-        // regular stores: nontemporal ones measured ~20 % slower on MI355X
-        if (valid) out[b * 25 + e] = val;
       }
+      out[k] = val;
     }
   }
 
