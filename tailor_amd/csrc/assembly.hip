@@ -743,7 +743,12 @@ __global__ __launch_bounds__(kThreads) void k_coefficients(
   __shared__ int st[kCoefNodes];
   const int tid = threadIdx.x;
   const long long first = static_cast<long long>(blockIdx.x) * kCoefNodes;
-  // six threads per node evaluate the physical blocks
+  if (tid < kCoefNodes) st[tid] = 0;
+  __syncthreads();
+  // six threads per node evaluate the physical blocks; every thread that
+  // writes blocks checks their finiteness (NonFiniteOutput of evaluate).  All
+  // six see the same point status s0: a failed point is recorded by one of
+  // them, and only nodes with s0 == 0 reach the atomic.
   for (int t6 = tid; t6 < 6 * kCoefNodes; t6 += kThreads) {
     const int n = t6 / 6, part = t6 - 6 * (t6 / 6);
     const long long node = first + n;
@@ -751,26 +756,20 @@ __global__ __launch_bounds__(kThreads) void k_coefficients(
       Point p;
       const int s0 = load_point(base, deriv, node, p);
       if (part == 5) {
-        st[n] = s0;
+        if (s0 != 0) st[n] = s0;
       } else if (s0 == 0) {
         const Transport q = transport(recipe, p.T);
         physical_part(part, p, q, phys[n]);
-      }
-    }
-  }
-  __syncthreads();
-  for (int t5 = tid; t5 < 5 * kCoefNodes; t5 += kThreads) {
-    // finiteness of the physical blocks (NonFiniteOutput of evaluate)
-    const int n = t5 / 5, part = t5 - 5 * (t5 / 5);
-    if (first + n < d.nodes && st[n] == 0) {
-      const int b0 = part == 0 ? 0 : part, b1 = part == 0 ? 1 : part + 1;
-      bool ok = true;
-      for (int b = b0; b < b1; ++b)
-        for (int k = 0; k < 25; ++k) ok = ok && finite(phys[n][b * 25 + k]);
-      if (part == 0)
-        for (int b = 5; b < 11; ++b)
+        // part 0 wrote Gamma and the six viscous blocks, part k block k
+        const int b0 = part == 0 ? 0 : part, b1 = part == 0 ? 1 : part + 1;
+        bool ok = true;
+        for (int b = b0; b < b1; ++b)
           for (int k = 0; k < 25; ++k) ok = ok && finite(phys[n][b * 25 + k]);
-      if (!ok) atomicMax(&st[n], 4);
+        if (part == 0)
+          for (int b = 5; b < 11; ++b)
+            for (int k = 0; k < 25; ++k) ok = ok && finite(phys[n][b * 25 + k]);
+        if (!ok) atomicMax(&st[n], 4);
+      }
     }
   }
   __syncthreads();
